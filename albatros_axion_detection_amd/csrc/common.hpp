@@ -1,0 +1,80 @@
+// Shared helpers for the ALBATROS MI355X kernels: error reporting, complex
+// arithmetic, launch checks and the per-device twiddle-table registry.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdarg.h>
+#include <string.h>
+
+#define ALB_OK 0
+#define ALB_EINVAL 1
+#define ALB_EHIP 2
+#define ALB_ENOTSUP 3
+
+namespace alb {
+
+// Last error message of the calling host thread (read through alb_last_error()).
+void set_error(const char* fmt, ...);
+const char* last_error();
+
+inline int hip_fail(hipError_t e, const char* what) {
+  set_error("%s: %s", what, hipGetErrorString(e));
+  return ALB_EHIP;
+}
+
+#define ALB_REQUIRE(cond, ...)            \
+  do {                                    \
+    if (!(cond)) {                        \
+      ::alb::set_error(__VA_ARGS__);      \
+      return ALB_EINVAL;                  \
+    }                                     \
+  } while (0)
+
+#define ALB_HIP(call)                                       \
+  do {                                                      \
+    hipError_t e_ = (call);                                 \
+    if (e_ != hipSuccess) return ::alb::hip_fail(e_, #call); \
+  } while (0)
+
+#define ALB_LAUNCH_CHECK(name)                                   \
+  do {                                                           \
+    hipError_t e_ = hipGetLastError();                           \
+    if (e_ != hipSuccess) return ::alb::hip_fail(e_, name);      \
+  } while (0)
+
+static inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+static inline bool is_pow2(int64_t n) { return n > 0 && (n & (n - 1)) == 0; }
+static inline int ilog2(int64_t n) {
+  int l = 0;
+  while ((int64_t(1) << l) < n) ++l;
+  return l;
+}
+
+// ---------------------------------------------------------------- complex
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+
+// -------------------------------------------------------- twiddle tables
+// Two-level table of W_N^m = exp(-2 pi i m / N):  W_N^m = A[m >> lb] * B[m & (2^lb - 1)]
+// with A[i] = W_N^(i << lb) (N >> lb entries) and B[i] = W_N^i (2^lb entries),
+// both rounded from float64.  One table per (device, N), built on first use.
+struct Twiddle {
+  const float2* A;
+  const float2* B;
+  int lb;     // bits of the low table
+  int nA;     // entries of A
+  int nB;     // entries of B
+};
+int get_twiddle(int64_t N, Twiddle* out);
+
+}  // namespace alb

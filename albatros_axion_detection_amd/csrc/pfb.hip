@@ -1,0 +1,752 @@
+// Polyphase filter bank kernels.
+//
+//  matft_filter   W[k,f] = sum_j w[j L + k] exp(-2 pi i f j / nslice) and the
+//                 Wiener filter W (1+t^2)/(|W|^2+t^2) in float64
+//                 (pfb_gpu_utils.py:29-36, :123-137).
+//  irfft_rows     row inverse real FFT (N = 2(nchan-1)), 1/N scaling, DC and
+//                 Nyquist imaginary parts ignored (pycufft.py:139-161).  Loaders:
+//                 dense complex rows, or 4-bit packed bytes with the spec_num
+//                 zero fill and channel scatter fused in
+//                 (baseband_data_classes.py:48-73, unpacking.c:155-205).
+//  rfft_rows      row forward real FFT (pycufft.py:120-137), optionally with a
+//                 polyphase window fused in front (pfb_gpu_utils.py:105-121).
+//  ipfb_cols      column pass of the inverse PFB: along time, forward real FFT,
+//                 multiply by the filter, inverse real FFT, 1/nslice scaling,
+//                 rows [cut, nslice-cut) written to the output stream
+//                 (pfb_gpu_utils.py:84-91, helper_gpu_stream.py:199).
+//  repfb_stage1   re-PFB, four-step first half: polyphase window + R1-point real
+//                 DFT down the R1 rows of each block, per column.
+//  repfb_stage2   re-PFB second half: twiddle, 4096-point FFT along columns,
+//                 only the requested fine channels written.
+#include "fft.hpp"
+
+namespace alb {
+
+constexpr int kTPB = 256;
+
+// ----------------------------------------------------------- matft / filter
+__global__ void matft_filter_kernel(const float* __restrict__ win, int ntap, int64_t lblock, int64_t nslice, int64_t nf,
+                                    double thresh, float2* __restrict__ matft, float2* __restrict__ filt) {
+  const int64_t n = lblock * nf;
+  for (int64_t idx = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; idx < n; idx += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t k = idx / nf, f = idx - k * nf;
+    double wr = 0.0, wi = 0.0;
+    for (int j = 0; j < ntap; ++j) {
+      const double w = double(win[int64_t(j) * lblock + k]);
+      const int64_t m = (f * j) % nslice;
+      double s, c;
+      sincospi(-2.0 * double(m) / double(nslice), &s, &c);
+      wr += w * c;
+      wi += w * s;
+    }
+    if (matft) matft[idx] = make_float2(float(wr), float(wi));
+    if (filt) {
+      const double a = wr * wr + wi * wi;
+      double sc;
+      if (thresh > 0.0) sc = (1.0 + thresh * thresh) / (a + thresh * thresh);
+      else sc = 1.0 / a;
+      filt[idx] = make_float2(float(wr * sc), float(wi * sc));
+    }
+  }
+}
+
+// ------------------------------------------------------------- row loaders
+// Dense complex rows: X[r][c] for c <= M.
+struct DenseRows {
+  const float2* x;
+  int64_t ld;
+  __device__ __forceinline__ void load(int64_t row, float2* X, int M, int tid, int nthr) const {
+    const float2* p = x + row * ld;
+    for (int c = tid; c <= M; c += nthr) X[c] = p[c];
+  }
+};
+
+// 4-bit packed rows of an iPFB tile.  Tile t, row r < 2*cut comes from chunk
+// t-1 row acclen-2*cut+r (zeros for t == 0), row r >= 2*cut from chunk t row
+// r-2*cut.  rowmap[chunk][row] = packed row index or -1 (zero-filled spectrum).
+// Packed byte of (chunk, p, col) at bytes[(chunk*acclen + p)*ncols + col];
+// value goes to channel number chan[col].
+struct Packed4Rows {
+  const uint8_t* bytes;
+  const int* rowmap;
+  const int* chan;
+  int64_t ncols, acclen, cut2, tile0;
+};
+
+__device__ __forceinline__ int nib4f(int v) { return v > 8 ? v - 16 : v; }
+
+// -------------------------------------------------------- irfft along rows
+// F rows per workgroup, N = 2M real outputs per row.
+template <int M, int F, int MODE>
+__global__ __launch_bounds__(kTPB) void irfft_rows_kernel(DenseRows dense, Packed4Rows pk, int64_t nrows, int64_t nslice,
+                                                          float* __restrict__ out, int64_t ldo, Twiddle tw) {
+  constexpr int N = 2 * M;
+  constexpr int MP = Padded<M>::value;
+  extern __shared__ float2 smem[];
+  float2* X = smem;                       // F x (M+1)
+  float2* Z = X + F * (M + 1);            // F x MP
+  float2* TA = Z + F * MP;
+  float2* TB = TA + tw.nA;
+  const int tid = threadIdx.x;
+  load_twiddle(tw, TA, TB, tid, kTPB);
+  const int64_t row0 = int64_t(blockIdx.x) * F;
+  // ---- load X
+  for (int f = 0; f < F; ++f) {
+    const int64_t row = row0 + f;
+    float2* Xf = X + f * (M + 1);
+    if (row >= nrows) {
+      for (int c = tid; c <= M; c += kTPB) Xf[c] = make_float2(0.f, 0.f);
+      continue;
+    }
+    if constexpr (MODE == 0) {
+      dense.load(row, Xf, M, tid, kTPB);
+    } else {
+      for (int c = tid; c <= M; c += kTPB) Xf[c] = make_float2(0.f, 0.f);
+      __syncthreads();
+      const int64_t tile = row / nslice, r = row - tile * nslice;
+      int64_t chunk, crow;
+      if (r < pk.cut2) { chunk = pk.tile0 + tile - 1; crow = pk.acclen - pk.cut2 + r; }
+      else { chunk = pk.tile0 + tile; crow = r - pk.cut2; }
+      if (chunk >= 0) {
+        const int p = pk.rowmap[chunk * pk.acclen + crow];
+        if (p >= 0) {
+          const uint8_t* src = pk.bytes + (chunk * pk.acclen + p) * pk.ncols;
+          for (int64_t c = tid; c < pk.ncols; c += kTPB) {
+            const int b = src[c];
+            Xf[pk.chan[c]] = make_float2(float(nib4f(b >> 4)), float(nib4f(b & 15)));
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // ---- Z[m] = (a + conj b) + i conj(W^m)(a - conj b), stored conjugated
+  for (int idx = tid; idx < F * (M / 2 + 1); idx += kTPB) {
+    const int f = idx / (M / 2 + 1), m = idx - f * (M / 2 + 1);
+    const float2* Xf = X + f * (M + 1);
+    float2* Zf = Z + f * MP;
+    float2 a = Xf[m], b = Xf[M - m];
+    if (m == 0) { a.y = 0.f; b.y = 0.f; }
+    const float2 w = tw_lookup(TA, TB, tw.lb, m);           // W_N^m
+    const float2 s = make_float2(a.x + b.x, a.y - b.y);     // a + conj b
+    const float2 d = make_float2(a.x - b.x, a.y + b.y);     // a - conj b
+    const float2 cd = cmulc(d, w);                          // d * conj(W^m)
+    const float2 zm = make_float2(s.x - cd.y, s.y + cd.x);  // s + i cd
+    Zf[pad16(m)] = cconj(zm);
+    if (m != 0 && 2 * m != M) {
+      const float2 s2 = make_float2(b.x + a.x, b.y - a.y);  // b + conj a
+      const float2 d2 = make_float2(b.x - a.x, b.y + a.y);  // b - conj a
+      const float2 wd = cmul(d2, w);                        // W^m d2
+      const float2 zk = make_float2(s2.x + wd.y, s2.y - wd.x);  // s2 - i W^m d2
+      Zf[pad16(M - m)] = cconj(zk);
+    }
+  }
+  __syncthreads();
+  fft_lds<M, F, kTPB>(Z, TA, TB, tw.lb, 1, tid);
+  // ---- y[2n] + i y[2n+1] = conj(FFT) / N
+  const float sc = 1.0f / float(N);
+  for (int idx = tid; idx < F * M; idx += kTPB) {
+    const int f = idx / M, n = idx - f * M;
+    const int64_t row = row0 + f;
+    if (row >= nrows) continue;
+    const float2 v = Z[f * MP + pad16(n)];
+    reinterpret_cast<float2*>(out + row * ldo)[n] = make_float2(v.x * sc, -v.y * sc);
+  }
+}
+
+// --------------------------------------------------------- rfft along rows
+// Optional polyphase front end: y[k] = sum_j ts[(row+j) L + k] w[j L + k].
+// Output columns [c0, c1) of the N/2+1 bins, row stride ldo (complex).
+template <int M, int F>
+__global__ __launch_bounds__(kTPB) void rfft_rows_kernel(const float* __restrict__ x, int64_t ldx, int64_t nrows,
+                                                         const float* __restrict__ win, int ntap, int64_t c0, int64_t c1,
+                                                         float2* __restrict__ out, int64_t ldo, Twiddle tw) {
+  constexpr int N = 2 * M;
+  constexpr int MP = Padded<M>::value;
+  extern __shared__ float2 smem[];
+  float2* Z = smem;
+  float2* TA = Z + F * MP;
+  float2* TB = TA + tw.nA;
+  const int tid = threadIdx.x;
+  load_twiddle(tw, TA, TB, tid, kTPB);
+  const int64_t row0 = int64_t(blockIdx.x) * F;
+  for (int idx = tid; idx < F * M; idx += kTPB) {
+    const int f = idx / M, n = idx - f * M;
+    const int64_t row = row0 + f;
+    float2 v = make_float2(0.f, 0.f);
+    if (row < nrows) {
+      if (win) {
+        float ax = 0.f, ay = 0.f;
+        for (int j = 0; j < ntap; ++j) {
+          const float2 t = reinterpret_cast<const float2*>(x + (row + j) * ldx)[n];
+          const float2 w = reinterpret_cast<const float2*>(win + int64_t(j) * N)[n];
+          ax = fmaf(t.x, w.x, ax);
+          ay = fmaf(t.y, w.y, ay);
+        }
+        v = make_float2(ax, ay);
+      } else {
+        v = reinterpret_cast<const float2*>(x + row * ldx)[n];
+      }
+    }
+    Z[f * MP + pad16(n)] = v;
+  }
+  __syncthreads();
+  fft_lds<M, F, kTPB>(Z, TA, TB, tw.lb, 1, tid);
+  const int64_t nc = c1 - c0;
+  for (int64_t idx = tid; idx < F * nc; idx += kTPB) {
+    const int f = int(idx / nc);
+    const int64_t m = c0 + (idx - f * nc);
+    const int64_t row = row0 + f;
+    if (row >= nrows) continue;
+    const float2 a = Z[f * MP + pad16(int(m % M))];
+    const float2 b = Z[f * MP + pad16(int((M - m % M) % M))];
+    const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));   // (a + conj b)/2
+    const float2 d = make_float2(0.5f * (a.x - b.x), 0.5f * (a.y + b.y));   // (a - conj b)/2
+    const float2 o = make_float2(d.y, -d.x);                                // d / i
+    float2 w;
+    if (m == M) w = make_float2(-1.f, 0.f);
+    else w = tw_lookup(TA, TB, tw.lb, int(m));
+    const float2 wo = cmul(w, o);
+    out[row * ldo + (m - c0)] = make_float2(e.x + wo.x, e.y + wo.y);
+  }
+}
+
+// ------------------------------------------------------ iPFB column pass
+// One workgroup = CW adjacent columns of one tile (all nslice rows) held in LDS
+// as CW/2 complex sequences u = y[:, k] + i y[:, k+1].
+template <int NSL, int CW>
+__global__ __launch_bounds__(kTPB) void ipfb_cols_kernel(const float* __restrict__ y, int64_t ldy, int64_t tile_stride,
+                                                         const float2* __restrict__ filt, int64_t ldf, int64_t ncols,
+                                                         int cut, float* __restrict__ out, int64_t ldo,
+                                                         int64_t out_tile_stride, Twiddle tw) {
+  constexpr int NP = CW / 2;
+  constexpr int PADN = Padded<NSL>::value;
+  extern __shared__ float2 smem[];
+  float2* U = smem;
+  float2* TA = U + NP * PADN;
+  float2* TB = TA + tw.nA;
+  const int tid = threadIdx.x;
+  load_twiddle(tw, TA, TB, tid, kTPB);
+  // XCD-aware strip order: consecutive strips on one XCD share cache lines.
+  const int64_t nstrips = ncols / CW;
+  int64_t sidx = blockIdx.x;
+  if (nstrips % 8 == 0) sidx = (blockIdx.x % 8) * (nstrips / 8) + blockIdx.x / 8;
+  const int64_t k0 = sidx * CW;
+  const int64_t tile = blockIdx.y;
+  const float* yt = y + tile * tile_stride;
+  for (int idx = tid; idx < NSL * NP; idx += kTPB) {
+    const int i = idx / NP, p = idx - i * NP;
+    U[p * PADN + pad16(i)] = reinterpret_cast<const float2*>(yt + i * ldy + k0)[p];
+  }
+  __syncthreads();
+  fft_lds<NSL, NP, kTPB>(U, TA, TB, tw.lb, 0, tid);
+  // filter on the real-pair spectra, stored conjugated for the inverse
+  const float sc = 1.0f / float(NSL);
+  for (int idx = tid; idx < NP * (NSL / 2 + 1); idx += kTPB) {
+    const int p = idx / (NSL / 2 + 1), f = idx - p * (NSL / 2 + 1);
+    float2* Up = U + p * PADN;
+    const float2 a = Up[pad16(f)];
+    const float2 b = Up[pad16((NSL - f) % NSL)];
+    const float2 y0 = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));   // (a + conj b)/2
+    const float2 dd = make_float2(0.5f * (a.x - b.x), 0.5f * (a.y + b.y));   // (a - conj b)/2
+    const float2 y1 = make_float2(dd.y, -dd.x);                              // -i dd
+    const float2 h0 = filt[(k0 + 2 * p) * ldf + f];
+    const float2 h1 = filt[(k0 + 2 * p + 1) * ldf + f];
+    float2 g0 = cscale(cmul(h0, y0), sc);
+    float2 g1 = cscale(cmul(h1, y1), sc);
+    if (f == 0 || 2 * f == NSL) { g0.y = 0.f; g1.y = 0.f; }
+    const float2 uf = make_float2(g0.x - g1.y, g0.y + g1.x);     // g0 + i g1
+    Up[pad16(f)] = cconj(uf);
+    if (f != 0 && 2 * f != NSL) {
+      const float2 ub = make_float2(g0.x + g1.y, -g0.y + g1.x);  // conj g0 + i conj g1
+      Up[pad16(NSL - f)] = cconj(ub);
+    }
+  }
+  __syncthreads();
+  fft_lds<NSL, NP, kTPB>(U, TA, TB, tw.lb, 0, tid);
+  float* ot = out + tile * out_tile_stride;
+  const int nkeep = NSL - 2 * cut;
+  for (int idx = tid; idx < nkeep * NP; idx += kTPB) {
+    const int i = idx / NP, p = idx - i * NP;
+    const float2 v = U[p * PADN + pad16(i + cut)];
+    reinterpret_cast<float2*>(ot + int64_t(i) * ldo + k0)[p] = make_float2(v.x, -v.y);
+  }
+}
+
+// ------------------------------------------------------- re-PFB stage 1
+// For block b and column k (row width 4096): y[r] = sum_j ts[(b+j)R1 + r][k] w[(j R1 + r) 4096 + k],
+// then the R1-point real DFT down r -> A[b][q][k], q = 0..R1/2.
+template <int R1>
+__global__ __launch_bounds__(kTPB) void repfb_stage1_kernel(const float* __restrict__ ts, const float* __restrict__ win,
+                                                            int ntap, int64_t nblock, int64_t blocks_per_wg,
+                                                            float2* __restrict__ A) {
+  constexpr int KW = 64;
+  constexpr int G = kTPB / KW;
+  constexpr int H = R1 / 2;
+  constexpr int64_t C = 4096;
+  extern __shared__ float wl[];  // [ntap*R1][KW]
+  const int tid = threadIdx.x;
+  const int kk = tid % KW, g = tid / KW;
+  const int64_t k = int64_t(blockIdx.x) * KW + kk;
+  for (int idx = tid; idx < ntap * R1 * KW; idx += kTPB) {
+    const int jr = idx / KW, c = idx - jr * KW;
+    wl[idx] = win[int64_t(jr) * C + int64_t(blockIdx.x) * KW + c];
+  }
+  __syncthreads();
+  const int64_t b0 = int64_t(blockIdx.y) * blocks_per_wg;
+  int64_t b1 = b0 + blocks_per_wg;
+  if (b1 > nblock) b1 = nblock;
+  for (int64_t b = b0 + g; b < b1; b += G) {
+    float y[R1];
+#pragma unroll
+    for (int r = 0; r < R1; ++r) y[r] = 0.f;
+    for (int j = 0; j < ntap; ++j) {
+      const float* src = ts + ((b + j) * R1) * C + k;
+      const float* wj = wl + (j * R1) * KW + kk;
+#pragma unroll
+      for (int r = 0; r < R1; ++r) y[r] = fmaf(src[int64_t(r) * C], wj[r * KW], y[r]);
+    }
+    if constexpr (R1 == 1) {
+      A[(b * (H + 1)) * C + k] = make_float2(y[0], 0.f);
+    } else {
+      float2 z[H];
+#pragma unroll
+      for (int n = 0; n < H; ++n) z[n] = make_float2(y[2 * n], y[2 * n + 1]);
+      DFT<H>::run(z);
+      float2* dst = A + (b * (H + 1)) * C + k;
+      static_for<0, H + 1>([&](auto Q) {
+        constexpr int q = decltype(Q)::value;
+        const float2 a = z[q % H];
+        const float2 bb = z[(H - q) % H];
+        const float2 e = make_float2(0.5f * (a.x + bb.x), 0.5f * (a.y - bb.y));
+        const float2 d = make_float2(0.5f * (a.x - bb.x), 0.5f * (a.y + bb.y));
+        const float2 o = make_float2(d.y, -d.x);
+        const float2 wo = twc<R1, q>(o);
+        dst[int64_t(q) * C] = make_float2(e.x + wo.x, e.y + wo.y);
+      });
+    }
+  }
+}
+
+// ------------------------------------------------------- re-PFB stage 2
+// Workgroup (q1, b): B[k] = A[b][q1 or R1-q1 conj][k] * W_L^(q1 k), 4096-point
+// FFT over k, fine channel q = q1 + R1 q2 in [c_lo, c_hi) written to
+// out[(b/bpj)*sj + (b%bpj)*sb + (q - c_lo)*sc] (complex elements; bpj = blocks per job).
+template <int R1>
+__global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __restrict__ A, int64_t c_lo, int64_t c_hi,
+                                                            float2* __restrict__ out, int64_t sb, int64_t sc, int64_t bpj,
+                                                            int64_t sj, Twiddle tw4, Twiddle twL) {
+  constexpr int N = 4096;
+  constexpr int H = R1 / 2;
+  constexpr int NPAD = Padded<N>::value;
+  extern __shared__ float2 smem[];
+  float2* Z = smem;
+  float2* TA = Z + NPAD;
+  float2* TB = TA + tw4.nA;
+  float2* LA = TB + tw4.nB;
+  float2* LB = LA + twL.nA;
+  const int tid = threadIdx.x;
+  load_twiddle(tw4, TA, TB, tid, kTPB);
+  load_twiddle(twL, LA, LB, tid, kTPB);
+  const int q1 = blockIdx.x;
+  const int64_t b = blockIdx.y;
+  const bool mirror = q1 > H;
+  const int qs = mirror ? R1 - q1 : q1;
+  const float2* src = A + (b * (H + 1) + qs) * N;
+  __syncthreads();
+  for (int k = tid; k < N; k += kTPB) {
+    float2 v = src[k];
+    if (mirror) v.y = -v.y;
+    if (q1 != 0) v = cmul(v, tw_lookup(LA, LB, twL.lb, q1 * k));
+    Z[pad16(k)] = v;
+  }
+  __syncthreads();
+  fft_lds<N, 1, kTPB>(Z, TA, TB, tw4.lb, 0, tid);
+  // q2 range with q1 + R1 q2 in [c_lo, c_hi)
+  int64_t q2lo = (c_lo - q1 + R1 - 1) / R1;
+  if (c_lo - q1 < 0) q2lo = 0;
+  const int64_t q2hi = (c_hi - 1 - q1) >= 0 ? (c_hi - 1 - q1) / R1 : -1;
+  for (int64_t q2 = q2lo + tid; q2 <= q2hi && q2 < N; q2 += kTPB) {
+    const int64_t q = q1 + int64_t(R1) * q2;
+    out[(b / bpj) * sj + (b % bpj) * sb + (q - c_lo) * sc] = Z[pad16(int(q2))];
+  }
+}
+
+// ------------------------------------------------------------ c2c rows
+// direction -1: forward; +1: inverse with 1/N (pycufft.py:92-112).
+template <int N, int F>
+__global__ __launch_bounds__(kTPB) void fft_rows_kernel(const float2* __restrict__ x, int64_t ldx, int64_t nrows,
+                                                        float2* __restrict__ out, int64_t ldo, int inverse, Twiddle tw) {
+  constexpr int NPD = Padded<N>::value;
+  extern __shared__ float2 smem[];
+  float2* Z = smem;
+  float2* TA = Z + F * NPD;
+  float2* TB = TA + tw.nA;
+  const int tid = threadIdx.x;
+  load_twiddle(tw, TA, TB, tid, kTPB);
+  const int64_t row0 = int64_t(blockIdx.x) * F;
+  for (int idx = tid; idx < F * N; idx += kTPB) {
+    const int f = idx / N, n = idx - f * N;
+    const int64_t row = row0 + f;
+    float2 v = row < nrows ? x[row * ldx + n] : make_float2(0.f, 0.f);
+    if (inverse) v.y = -v.y;
+    Z[f * NPD + pad16(n)] = v;
+  }
+  __syncthreads();
+  fft_lds<N, F, kTPB>(Z, TA, TB, tw.lb, 0, tid);
+  const float sc = inverse ? 1.0f / float(N) : 1.0f;
+  for (int idx = tid; idx < F * N; idx += kTPB) {
+    const int f = idx / N, n = idx - f * N;
+    const int64_t row = row0 + f;
+    if (row >= nrows) continue;
+    float2 v = Z[f * NPD + pad16(n)];
+    if (inverse) v = make_float2(v.x * sc, -v.y * sc);
+    out[row * ldo + n] = v;
+  }
+}
+
+// filt = 1/conj(W) [* |W|^2/(t^2+|W|^2)(1+t^2)] from a given W, in float64.
+__global__ void filter_from_matft_kernel(const float2* __restrict__ W, int64_t n, double thresh, float2* __restrict__ filt) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const double wr = W[i].x, wi = W[i].y;
+    const double a = wr * wr + wi * wi;
+    const double sc = thresh > 0.0 ? (1.0 + thresh * thresh) / (a + thresh * thresh) : 1.0 / a;
+    filt[i] = make_float2(float(wr * sc), float(wi * sc));
+  }
+}
+
+__global__ void rowmap_fill_kernel(int* rowmap, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) rowmap[i] = -1;
+}
+__global__ void rowmap_scatter_kernel(const int64_t* __restrict__ specnum, int64_t npresent, int64_t start, int64_t acclen,
+                                      int* __restrict__ rowmap) {
+  const int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (p >= npresent) return;
+  int64_t r = (specnum[p] - start) % acclen;
+  if (r < 0) r += acclen;
+  rowmap[r] = int(p);
+}
+
+}  // namespace alb
+
+using namespace alb;
+
+// ---------------------------------------------------------------- host side
+namespace {
+
+template <int M, int F, int MODE>
+int launch_irfft(DenseRows d, Packed4Rows pk, int64_t nrows, int64_t nslice, float* out, int64_t ldo, hipStream_t s) {
+  Twiddle tw;
+  int rc = get_twiddle(2 * M, &tw);
+  if (rc) return rc;
+  const size_t lds = sizeof(float2) * (size_t(F) * (M + 1) + size_t(F) * Padded<M>::value + tw.nA + tw.nB);
+  const unsigned grid = unsigned((nrows + F - 1) / F);
+  hipLaunchKernelGGL((irfft_rows_kernel<M, F, MODE>), dim3(grid), dim3(kTPB), lds, s, d, pk, nrows, nslice, out, ldo, tw);
+  ALB_LAUNCH_CHECK("irfft_rows_kernel");
+  return ALB_OK;
+}
+
+template <int MODE>
+int irfft_dispatch(int64_t n, DenseRows d, Packed4Rows pk, int64_t nrows, int64_t nslice, float* out, int64_t ldo,
+                   hipStream_t s) {
+  switch (n) {
+    case 64: return launch_irfft<32, 64, MODE>(d, pk, nrows, nslice, out, ldo, s);
+    case 128: return launch_irfft<64, 32, MODE>(d, pk, nrows, nslice, out, ldo, s);
+    case 256: return launch_irfft<128, 16, MODE>(d, pk, nrows, nslice, out, ldo, s);
+    case 512: return launch_irfft<256, 8, MODE>(d, pk, nrows, nslice, out, ldo, s);
+    case 1024: return launch_irfft<512, 4, MODE>(d, pk, nrows, nslice, out, ldo, s);
+    case 2048: return launch_irfft<1024, 2, MODE>(d, pk, nrows, nslice, out, ldo, s);
+    case 4096: return launch_irfft<2048, 2, MODE>(d, pk, nrows, nslice, out, ldo, s);
+    case 8192: return launch_irfft<4096, 1, MODE>(d, pk, nrows, nslice, out, ldo, s);
+    case 16384: return launch_irfft<8192, 1, MODE>(d, pk, nrows, nslice, out, ldo, s);
+    default:
+      set_error("irfft: length %lld not supported (power of two 64..16384)", (long long)n);
+      return ALB_ENOTSUP;
+  }
+}
+
+template <int M, int F>
+int launch_rfft(const float* x, int64_t ldx, int64_t nrows, const float* win, int ntap, int64_t c0, int64_t c1,
+                float2* out, int64_t ldo, hipStream_t s) {
+  Twiddle tw;
+  int rc = get_twiddle(2 * M, &tw);
+  if (rc) return rc;
+  const size_t lds = sizeof(float2) * (size_t(F) * Padded<M>::value + tw.nA + tw.nB);
+  const unsigned grid = unsigned((nrows + F - 1) / F);
+  hipLaunchKernelGGL((rfft_rows_kernel<M, F>), dim3(grid), dim3(kTPB), lds, s, x, ldx, nrows, win, ntap, c0, c1, out,
+                     ldo, tw);
+  ALB_LAUNCH_CHECK("rfft_rows_kernel");
+  return ALB_OK;
+}
+
+int rfft_dispatch(int64_t n, const float* x, int64_t ldx, int64_t nrows, const float* win, int ntap, int64_t c0,
+                  int64_t c1, float2* out, int64_t ldo, hipStream_t s) {
+  switch (n) {
+    case 64: return launch_rfft<32, 64>(x, ldx, nrows, win, ntap, c0, c1, out, ldo, s);
+    case 128: return launch_rfft<64, 32>(x, ldx, nrows, win, ntap, c0, c1, out, ldo, s);
+    case 256: return launch_rfft<128, 16>(x, ldx, nrows, win, ntap, c0, c1, out, ldo, s);
+    case 512: return launch_rfft<256, 8>(x, ldx, nrows, win, ntap, c0, c1, out, ldo, s);
+    case 1024: return launch_rfft<512, 4>(x, ldx, nrows, win, ntap, c0, c1, out, ldo, s);
+    case 2048: return launch_rfft<1024, 2>(x, ldx, nrows, win, ntap, c0, c1, out, ldo, s);
+    case 4096: return launch_rfft<2048, 2>(x, ldx, nrows, win, ntap, c0, c1, out, ldo, s);
+    case 8192: return launch_rfft<4096, 1>(x, ldx, nrows, win, ntap, c0, c1, out, ldo, s);
+    case 16384: return launch_rfft<8192, 1>(x, ldx, nrows, win, ntap, c0, c1, out, ldo, s);
+    default:
+      set_error("rfft: length %lld not supported (power of two 64..16384)", (long long)n);
+      return ALB_ENOTSUP;
+  }
+}
+
+template <int NSL, int CW>
+int launch_cols(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, const float2* filt, int64_t ldf,
+                int64_t ncols, int cut, float* out, int64_t ldo, int64_t out_tile_stride, hipStream_t s) {
+  Twiddle tw;
+  int rc = get_twiddle(NSL, &tw);
+  if (rc) return rc;
+  const size_t lds = sizeof(float2) * (size_t(CW / 2) * Padded<NSL>::value + tw.nA + tw.nB);
+  ALB_REQUIRE(ncols % CW == 0, "ipfb column pass: width %lld not a multiple of %d", (long long)ncols, CW);
+  hipLaunchKernelGGL((ipfb_cols_kernel<NSL, CW>), dim3(unsigned(ncols / CW), unsigned(ntiles)), dim3(kTPB), lds, s, y,
+                     ldy, tile_stride, filt, ldf, ncols, cut, out, ldo, out_tile_stride, tw);
+  ALB_LAUNCH_CHECK("ipfb_cols_kernel");
+  return ALB_OK;
+}
+
+int cols_dispatch(int64_t nslice, const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, const float2* filt,
+                  int64_t ldf, int64_t ncols, int cut, float* out, int64_t ldo, int64_t out_tile_stride,
+                  hipStream_t s) {
+  switch (nslice) {
+#define CASE(n, w) case n: return launch_cols<n, w>(y, ldy, tile_stride, ntiles, filt, ldf, ncols, cut, out, ldo, out_tile_stride, s);
+    CASE(16, 8) CASE(32, 8) CASE(64, 8) CASE(128, 8) CASE(256, 8) CASE(512, 8) CASE(1024, 8) CASE(2048, 8)
+    CASE(4096, 8) CASE(8192, 4) CASE(16384, 2)
+#undef CASE
+    default:
+      set_error("ipfb: nslice %lld not supported (power of two 16..16384)", (long long)nslice);
+      return ALB_ENOTSUP;
+  }
+}
+
+template <int R1>
+int launch_repfb_stage1(const float* ts, const float* win, int ntap, int64_t nblock, float2* A, hipStream_t s) {
+  const int64_t bpw = 16;
+  const size_t lds1 = sizeof(float) * size_t(ntap) * R1 * 64;
+  ALB_REQUIRE(lds1 <= 160 * 1024, "re-PFB: window strip does not fit in LDS");
+  hipLaunchKernelGGL((repfb_stage1_kernel<R1>), dim3(4096 / 64, unsigned((nblock + bpw - 1) / bpw)), dim3(kTPB), lds1, s,
+                     ts, win, ntap, nblock, bpw, A);
+  ALB_LAUNCH_CHECK("repfb_stage1_kernel");
+  return ALB_OK;
+}
+
+template <int R1>
+int launch_repfb_stage2(int64_t nblock, const float2* A, int64_t c_lo, int64_t c_hi, float2* out, int64_t sb, int64_t sc,
+                        int64_t bpj, int64_t sj, hipStream_t s) {
+  Twiddle tw4, twL;
+  int rc = get_twiddle(4096, &tw4);
+  if (rc) return rc;
+  rc = get_twiddle(int64_t(4096) * R1, &twL);
+  if (rc) return rc;
+  const size_t lds2 = sizeof(float2) * (size_t(Padded<4096>::value) + tw4.nA + tw4.nB + twL.nA + twL.nB);
+  hipLaunchKernelGGL((repfb_stage2_kernel<R1>), dim3(R1, unsigned(nblock)), dim3(kTPB), lds2, s, A, c_lo, c_hi, out, sb,
+                     sc, bpj, sj, tw4, twL);
+  ALB_LAUNCH_CHECK("repfb_stage2_kernel");
+  return ALB_OK;
+}
+
+template <int N, int F>
+int launch_fft(const float2* x, int64_t ldx, int64_t nrows, float2* out, int64_t ldo, int inverse, hipStream_t s) {
+  Twiddle tw;
+  int rc = get_twiddle(N, &tw);
+  if (rc) return rc;
+  const size_t lds = sizeof(float2) * (size_t(F) * Padded<N>::value + tw.nA + tw.nB);
+  hipLaunchKernelGGL((fft_rows_kernel<N, F>), dim3(unsigned((nrows + F - 1) / F)), dim3(kTPB), lds, s, x, ldx, nrows,
+                     out, ldo, inverse, tw);
+  ALB_LAUNCH_CHECK("fft_rows_kernel");
+  return ALB_OK;
+}
+
+int fft_dispatch(int64_t n, const float2* x, int64_t ldx, int64_t nrows, float2* out, int64_t ldo, int inverse,
+                 hipStream_t s) {
+  switch (n) {
+    case 16: return launch_fft<16, 64>(x, ldx, nrows, out, ldo, inverse, s);
+    case 32: return launch_fft<32, 64>(x, ldx, nrows, out, ldo, inverse, s);
+    case 64: return launch_fft<64, 64>(x, ldx, nrows, out, ldo, inverse, s);
+    case 128: return launch_fft<128, 32>(x, ldx, nrows, out, ldo, inverse, s);
+    case 256: return launch_fft<256, 16>(x, ldx, nrows, out, ldo, inverse, s);
+    case 512: return launch_fft<512, 8>(x, ldx, nrows, out, ldo, inverse, s);
+    case 1024: return launch_fft<1024, 4>(x, ldx, nrows, out, ldo, inverse, s);
+    case 2048: return launch_fft<2048, 2>(x, ldx, nrows, out, ldo, inverse, s);
+    case 4096: return launch_fft<4096, 1>(x, ldx, nrows, out, ldo, inverse, s);
+    case 8192: return launch_fft<8192, 1>(x, ldx, nrows, out, ldo, inverse, s);
+    default:
+      set_error("fft: length %lld not supported (power of two 16..8192)", (long long)n);
+      return ALB_ENOTSUP;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// pfb_gpu_utils.py:123-137 get_matft + :29-36 calculate_filter, in float64.
+// win: float32 [ntap*lblock]; outputs [lblock][nslice/2+1] complex64 (either may be null).
+int alb_matft_filter(const float* win, int ntap, int64_t lblock, int64_t nslice, double thresh, void* matft, void* filt,
+                     void* stream) {
+  ALB_REQUIRE(win && (matft || filt), "alb_matft_filter: null pointer");
+  ALB_REQUIRE(ntap > 0 && lblock > 0 && nslice > 0, "alb_matft_filter: bad sizes");
+  const int64_t nf = nslice / 2 + 1;
+  const int64_t n = lblock * nf;
+  int64_t g = (n + 255) / 256;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(matft_filter_kernel, dim3(unsigned(g)), dim3(256), 0, as_stream(stream), win, ntap, lblock, nslice,
+                     nf, thresh, (float2*)matft, (float2*)filt);
+  ALB_LAUNCH_CHECK("matft_filter_kernel");
+  return ALB_OK;
+}
+
+// pycufft.py:139-161 irfft along rows: in complex [nrows][ldi] (n/2+1 used), out float [nrows][ldo], 1/n scaling.
+int alb_irfft_rows(const void* in, int64_t ldi, float* out, int64_t ldo, int64_t nrows, int64_t n, void* stream) {
+  ALB_REQUIRE(in && out, "alb_irfft_rows: null pointer");
+  ALB_REQUIRE(ldi >= n / 2 + 1 && ldo >= n && nrows >= 0, "alb_irfft_rows: bad strides");
+  if (nrows == 0) return ALB_OK;
+  DenseRows d{(const float2*)in, ldi};
+  Packed4Rows pk{};
+  return irfft_dispatch<0>(n, d, pk, nrows, 1, out, ldo, as_stream(stream));
+}
+
+// pycufft.py:120-137 rfft along rows (unnormalised), bins [c0, c1) written.
+int alb_rfft_rows(const float* in, int64_t ldi, void* out, int64_t ldo, int64_t nrows, int64_t n, int64_t c0,
+                  int64_t c1, void* stream) {
+  ALB_REQUIRE(in && out, "alb_rfft_rows: null pointer");
+  ALB_REQUIRE(ldi >= n && c0 >= 0 && c1 <= n / 2 + 1 && c1 > c0 && ldo >= c1 - c0, "alb_rfft_rows: bad sizes");
+  if (nrows == 0) return ALB_OK;
+  return rfft_dispatch(n, in, ldi, nrows, nullptr, 1, c0, c1, (float2*)out, ldo, as_stream(stream));
+}
+
+// pfb_gpu_utils.py:105-121 cupy_pfb for lblock <= 16384: polyphase + rfft, channels [c0, c1).
+int alb_pfb_small(const float* ts, int64_t nblock, int64_t lblock, const float* win, int ntap, int64_t c0, int64_t c1,
+                  void* out, int64_t ldo, void* stream) {
+  ALB_REQUIRE(ts && win && out, "alb_pfb_small: null pointer");
+  ALB_REQUIRE(nblock >= 0 && ntap >= 1 && c0 >= 0 && c1 <= lblock / 2 + 1 && c1 > c0, "alb_pfb_small: bad sizes");
+  if (nblock == 0) return ALB_OK;
+  return rfft_dispatch(lblock, ts, lblock, nblock, win, ntap, c0, c1, (float2*)out, ldo, as_stream(stream));
+}
+
+// Column pass of the inverse PFB over ntiles tiles of [nslice][ncols] float rows
+// (row stride ldy, tile stride tile_stride); filt [ncols][ldf] complex64; kept rows
+// [cut, nslice-cut) of tile t go to out + t*out_tile_stride (row stride ldo).
+// In place (out == y, cut == 0) is allowed.
+int alb_ipfb_columns(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, int64_t nslice, int64_t ncols,
+                     const void* filt, int64_t ldf, int cut, float* out, int64_t ldo, int64_t out_tile_stride,
+                     void* stream) {
+  ALB_REQUIRE(y && filt && out, "alb_ipfb_columns: null pointer");
+  ALB_REQUIRE(ldy >= ncols && ldo >= ncols && ldf >= nslice / 2 + 1 && cut >= 0 && 2 * cut < nslice && ntiles >= 0,
+              "alb_ipfb_columns: bad sizes");
+  if (ntiles == 0) return ALB_OK;
+  return cols_dispatch(nslice, y, ldy, tile_stride, ntiles, (const float2*)filt, ldf, ncols, cut, out, ldo,
+                       out_tile_stride, as_stream(stream));
+}
+
+// Row pass of the fused 4-bit iPFB: unpack + spec_num zero fill + channel scatter
+// + irfft (n = 4096) for ntiles tiles of nslice rows, tiles tile0.. of the chunk
+// sequence.  bytes [nchunk][acclen][ncols] packed per-pol 4-bit rows (present rows
+// first), rowmap [nchunk][acclen] (-1 = missing), chan[ncols] channel numbers.
+int alb_ipfb_rows_packed4(const uint8_t* bytes, const int* rowmap, const int* chan, int64_t ncols, int64_t acclen,
+                          int cut, int64_t tile0, int64_t ntiles, float* out, void* stream) {
+  ALB_REQUIRE(bytes && rowmap && chan && out, "alb_ipfb_rows_packed4: null pointer");
+  const int64_t nslice = acclen + 2 * cut;
+  ALB_REQUIRE(acclen >= 2 * cut && ncols > 0 && ncols <= 2049, "alb_ipfb_rows_packed4: bad sizes");
+  Packed4Rows pk{bytes, rowmap, chan, ncols, acclen, 2 * int64_t(cut), tile0};
+  DenseRows d{};
+  return irfft_dispatch<1>(4096, d, pk, nslice * ntiles, nslice, out, 4096, as_stream(stream));
+}
+
+// Re-PFB (pfb_gpu_utils.py:105-121 with lblock = 4096*osamp) of nblock output
+// spectra from the row-major stream ts [(nblock+ntap-1)*osamp][4096]; fine
+// channels [c_lo, c_hi) of spectrum b to out[(b/bpj)*sj + (b%bpj)*sb + (c-c_lo)*sc];
+// A is workspace of nblock*(osamp/2+1)*4096 complex64.
+int alb_repfb_stage1(const float* ts, const float* win, int ntap, int64_t nblock, int64_t osamp, void* A,
+                     void* stream) {
+  ALB_REQUIRE(ts && win && A, "alb_repfb_stage1: null pointer");
+  ALB_REQUIRE(nblock >= 0 && ntap >= 1, "alb_repfb_stage1: bad sizes");
+  if (nblock == 0) return ALB_OK;
+  ALB_REQUIRE(nblock <= 65535 * 16, "alb_repfb_stage1: too many blocks per call");
+  hipStream_t s = as_stream(stream);
+  switch (osamp) {
+#define CASE(r) case r: return launch_repfb_stage1<r>(ts, win, ntap, nblock, (float2*)A, s);
+    CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) CASE(64)
+#undef CASE
+    default:
+      set_error("alb_repfb: osamp %lld not supported (power of two 1..64)", (long long)osamp);
+      return ALB_ENOTSUP;
+  }
+}
+
+int alb_repfb_stage2(const void* A, int64_t nblock, int64_t osamp, int64_t c_lo, int64_t c_hi, void* out, int64_t sb,
+                     int64_t sc, int64_t bpj, int64_t sj, void* stream) {
+  ALB_REQUIRE(A && out, "alb_repfb_stage2: null pointer");
+  ALB_REQUIRE(nblock >= 0 && c_lo >= 0 && c_hi > c_lo && c_hi <= 2048 * osamp + 1 && bpj >= 1,
+              "alb_repfb_stage2: bad sizes");
+  if (nblock == 0) return ALB_OK;
+  ALB_REQUIRE(nblock <= 65535, "alb_repfb: at most 65535 blocks per call");
+  hipStream_t s = as_stream(stream);
+  switch (osamp) {
+#define CASE(r) case r: return launch_repfb_stage2<r>(nblock, (const float2*)A, c_lo, c_hi, (float2*)out, sb, sc, bpj, sj, s);
+    CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) CASE(64)
+#undef CASE
+    default:
+      set_error("alb_repfb: osamp %lld not supported (power of two 1..64)", (long long)osamp);
+      return ALB_ENOTSUP;
+  }
+}
+
+// Re-PFB (pfb_gpu_utils.py:105-121 with lblock = 4096*osamp) of nblock output
+// spectra from the row-major stream ts [(nblock+ntap-1)*osamp][4096]; fine
+// channels [c_lo, c_hi) of spectrum b to out[(b/bpj)*sj + (b%bpj)*sb + (c-c_lo)*sc];
+// A is workspace of nblock*(osamp/2+1)*4096 complex64.
+int alb_repfb(const float* ts, const float* win, int ntap, int64_t nblock, int64_t osamp, void* A, int64_t c_lo,
+              int64_t c_hi, void* out, int64_t sb, int64_t sc, int64_t bpj, int64_t sj, void* stream) {
+  ALB_REQUIRE(ts && win && A && out, "alb_repfb: null pointer");
+  int rc = alb_repfb_stage1(ts, win, ntap, nblock, osamp, A, stream);
+  if (rc) return rc;
+  return alb_repfb_stage2(A, nblock, osamp, c_lo, c_hi, out, sb, sc, bpj, sj, stream);
+}
+
+int64_t alb_repfb_workspace_bytes(int64_t nblock, int64_t osamp) {
+  return nblock * (osamp / 2 + 1) * 4096 * int64_t(sizeof(float2));
+}
+
+int alb_fft_rows(const void* in, int64_t ldi, void* out, int64_t ldo, int64_t nrows, int64_t n, int direction,
+                 void* stream) {
+  ALB_REQUIRE(in && out, "alb_fft_rows: null pointer");
+  ALB_REQUIRE(direction == -1 || direction == 1, "alb_fft_rows: direction must be -1 or +1");
+  ALB_REQUIRE(ldi >= n && ldo >= n && nrows >= 0, "alb_fft_rows: bad strides");
+  if (nrows == 0) return ALB_OK;
+  return fft_dispatch(n, (const float2*)in, ldi, nrows, (float2*)out, ldo, direction == 1, as_stream(stream));
+}
+
+int alb_filter_from_matft(const void* matft, int64_t n, double thresh, void* filt, void* stream) {
+  ALB_REQUIRE(matft && filt, "alb_filter_from_matft: null pointer");
+  if (n <= 0) return ALB_OK;
+  int64_t g = (n + 255) / 256;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(filter_from_matft_kernel, dim3(unsigned(g)), dim3(256), 0, as_stream(stream), (const float2*)matft,
+                     n, thresh, (float2*)filt);
+  ALB_LAUNCH_CHECK("filter_from_matft_kernel");
+  return ALB_OK;
+}
+
+int alb_build_rowmap(const int64_t* specnum, int64_t npresent, int64_t start, int64_t acclen, int* rowmap,
+                     void* stream) {
+  ALB_REQUIRE(rowmap && (npresent == 0 || specnum), "alb_build_rowmap: null pointer");
+  ALB_REQUIRE(acclen > 0 && npresent >= 0 && npresent <= acclen, "alb_build_rowmap: bad sizes");
+  hipStream_t s = as_stream(stream);
+  int64_t g = (acclen + 255) / 256;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(rowmap_fill_kernel, dim3(unsigned(g)), dim3(256), 0, s, rowmap, acclen);
+  if (npresent > 0)
+    hipLaunchKernelGGL(rowmap_scatter_kernel, dim3(unsigned((npresent + 255) / 256)), dim3(256), 0, s, specnum, npresent,
+                       start, acclen, rowmap);
+  ALB_LAUNCH_CHECK("rowmap kernels");
+  return ALB_OK;
+}
+
+}  // extern "C"

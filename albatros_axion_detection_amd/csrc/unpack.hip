@@ -1,0 +1,295 @@
+// Baseband unpack / repack kernels: 4-bit and 1-bit decode to complex64,
+// per-pol repack (sortpols), raw-value histograms, spec_num zero-fill scatter
+// and the dropped-packet zero fill.  All memory-bound; grid-stride loops with
+// 64-bit indexing, every thread owning whole output elements.
+#include "common.hpp"
+
+namespace alb {
+
+// Signed 4-bit rule of the reference: v > 8 ? v - 16 : v   (8 -> +8).
+__device__ __forceinline__ int nib(int v) { return v > 8 ? v - 16 : v; }
+
+__global__ void fill_u8(uint8_t* p, int64_t n, uint8_t v) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+__global__ void fill_u64(uint64_t* p, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) p[i] = 0;
+}
+__global__ void fill_c64(float2* p, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    p[i] = make_float2(0.f, 0.f);
+}
+
+static inline dim3 grid_for(int64_t n, int bs = 256, int64_t cap = 65536) {
+  int64_t g = (n + bs - 1) / bs;
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  return dim3(unsigned(g));
+}
+
+// unpacking.c:155-205: (rowend-rowstart, ncols) complex64 per pol, column k from
+// byte 2*channels[k] (+1 for pol1) of spectra of 2*nchan bytes.
+__global__ void unpack4_kernel(const uint8_t* __restrict__ data, float2* __restrict__ pol0, float2* __restrict__ pol1,
+                               int64_t rowstart, int64_t nrows, const int64_t* __restrict__ channels, int64_t ncols,
+                               int64_t nchan) {
+  const int64_t n = nrows * ncols;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t r = i / ncols, k = i - r * ncols;
+    const uint8_t* p = data + (rowstart + r) * 2 * nchan + 2 * channels[k];
+    const int b0 = p[0], b1 = p[1];
+    pol0[i] = make_float2(float(nib(b0 >> 4)), float(nib(b0 & 15)));
+    pol1[i] = make_float2(float(nib(b1 >> 4)), float(nib(b1 & 15)));
+  }
+}
+
+// unpacking.c:207-246: byte j of a spectrum = channels 2j, 2j+1 of both pols,
+// bits r0c0 i0c0 r1c0 i1c0 r0c1 i0c1 r1c1 i1c1; 1 -> +1, 0 -> -1.
+__global__ void unpack1_kernel(const uint8_t* __restrict__ data, float2* __restrict__ pol0, float2* __restrict__ pol1,
+                               int64_t rowstart, int64_t nrows, int64_t chanstart, int64_t nbytes_out, int64_t nchan) {
+  const int64_t n = nrows * nbytes_out;
+  const int64_t bps = nchan / 2;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t r = i / nbytes_out, j = i - r * nbytes_out;
+    const int b = data[(r + rowstart) * bps + chanstart / 2 + j];
+    auto pm = [&](int s) { return float(2 * ((b >> s) & 1) - 1); };
+    const int64_t o = r * 2 * nbytes_out + 2 * j;
+    pol0[o] = make_float2(pm(7), pm(6));
+    pol0[o + 1] = make_float2(pm(3), pm(2));
+    pol1[o] = make_float2(pm(5), pm(4));
+    pol1[o + 1] = make_float2(pm(1), pm(0));
+  }
+}
+
+// unpacking.c:263-275: 4-bit per-pol byte split.
+__global__ void sortpols4_kernel(const uint8_t* __restrict__ data, uint8_t* __restrict__ pol0, uint8_t* __restrict__ pol1,
+                                 int64_t rowstart, int64_t nrows, int64_t ncols, int64_t nchan, int64_t chanstart) {
+  const int64_t n = nrows * ncols;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t r = i / ncols, k = i - r * ncols;
+    const uint8_t* p = data + (r + rowstart) * 2 * nchan + 2 * (k + chanstart);
+    pol0[i] = p[0];
+    pol1[i] = p[1];
+  }
+}
+
+// unpacking.c:281-366: 1-bit repack, 4 channels per output byte, ragged last byte.
+__global__ void sortpols1_kernel(const uint8_t* __restrict__ data, uint8_t* __restrict__ pol0, uint8_t* __restrict__ pol1,
+                                 int64_t rowstart, int64_t nrows, int64_t ncols, int64_t nchan, int64_t chanstart,
+                                 int cndn) {
+  const int64_t n = nrows * ncols;
+  const int64_t bps = nchan / 2;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t r = i / ncols, j = i - r * ncols;
+    const int64_t idx = bps * (r + rowstart) + 2 * j + chanstart / 2;
+    const bool last = (j == ncols - 1);
+    const int use = last ? cndn : 0;   // 0 = two full raw bytes
+    const int b1 = data[idx];
+    const int b2 = (use == 0 || use == 3) ? int(data[idx + 1]) : 0;
+    const int p0c0 = b1 & 192, p1c0 = b1 & 48, p0c1 = b1 & 12, p1c1 = b1 & 3;
+    const int p0c2 = b2 & 192, p1c2 = b2 & 48, p0c3 = b2 & 12, p1c3 = b2 & 3;
+    int o0, o1;
+    switch (use) {
+      case 0: o0 = p0c0 + (p0c1 << 2) + (p0c2 >> 4) + (p0c3 >> 2); o1 = (p1c0 << 2) + (p1c1 << 4) + (p1c2 >> 2) + p1c3; break;
+      case 3: o0 = p0c0 + (p0c1 << 2) + (p0c2 >> 4); o1 = (p1c0 << 2) + (p1c1 << 4) + (p1c2 >> 2); break;
+      case 2: o0 = p0c0 + (p0c1 << 2); o1 = (p1c0 << 2) + (p1c1 << 4); break;
+      default: o0 = p0c0; o1 = p1c0 << 2; break;
+    }
+    pol0[i] = uint8_t(o0);
+    pol1[i] = uint8_t(o1);
+  }
+}
+
+// unpacking.c:7-74: raw nibble histogram hist[v * nchan + j] over rows; mode 0/1/-1.
+// One thread per (channel, row chunk), register counters, 64-bit atomics.
+__global__ void hist4_kernel(const uint8_t* __restrict__ data, unsigned long long* hist, int64_t rowstart, int64_t nrows,
+                             int64_t nchan, int mode, int64_t rows_per) {
+  const int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (j >= nchan) return;
+  const int64_t r0 = blockIdx.y * rows_per;
+  const int64_t r1 = r0 + rows_per < nrows ? r0 + rows_per : nrows;
+  unsigned cnt[16];
+#pragma unroll
+  for (int v = 0; v < 16; ++v) cnt[v] = 0;
+  for (int64_t r = r0; r < r1; ++r) {
+    const uint8_t* p = data + (r + rowstart) * 2 * nchan + 2 * j;
+    if (mode == 0 || mode == -1) { const int b = p[0]; cnt[b >> 4]++; cnt[b & 15]++; }
+    if (mode == 1 || mode == -1) { const int b = p[1]; cnt[b >> 4]++; cnt[b & 15]++; }
+  }
+#pragma unroll
+  for (int v = 0; v < 16; ++v)
+    if (cnt[v]) atomicAdd(&hist[v * nchan + j], (unsigned long long)cnt[v]);
+}
+
+// unpacking.c:76-153: 1-bit histogram, two bins per channel.
+__global__ void hist1_kernel(const uint8_t* __restrict__ data, unsigned long long* hist, int64_t rowstart, int64_t nrows,
+                             int64_t nchan, int mode, int64_t rows_per) {
+  const int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;  // byte column
+  const int64_t ncols = nchan / 2;
+  if (j >= ncols) return;
+  const int64_t r0 = blockIdx.y * rows_per;
+  const int64_t r1 = r0 + rows_per < nrows ? r0 + rows_per : nrows;
+  unsigned c[2][2] = {{0, 0}, {0, 0}};  // [channel 2j / 2j+1][bin]
+  for (int64_t r = r0; r < r1; ++r) {
+    const int b = data[(r + rowstart) * ncols + j];
+    if (mode == 0 || mode == -1) {
+      c[0][(b >> 7) & 1]++; c[0][(b >> 6) & 1]++; c[1][(b >> 3) & 1]++; c[1][(b >> 2) & 1]++;
+    }
+    if (mode == 1 || mode == -1) {
+      c[0][(b >> 5) & 1]++; c[0][(b >> 4) & 1]++; c[1][(b >> 1) & 1]++; c[1][b & 1]++;
+    }
+  }
+  for (int ch = 0; ch < 2; ++ch)
+    for (int v = 0; v < 2; ++v)
+      if (c[ch][v]) atomicAdd(&hist[v * nchan + 2 * j + ch], (unsigned long long)c[ch][v]);
+}
+
+// baseband_data_classes.py:48-73: out[(specnum[p]-start) % nspec, channels[k]] = spec[p, k].
+__global__ void scatter_kernel(const float2* __restrict__ spec, int64_t spec_ld, const int64_t* __restrict__ specnum,
+                               int64_t npresent, int64_t start, int64_t nspec, const int64_t* __restrict__ channels,
+                               int64_t ncols, float2* __restrict__ out, int64_t nchans) {
+  const int64_t n = npresent * ncols;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t p = i / ncols, k = i - p * ncols;
+    int64_t row = (specnum[p] - start) % nspec;
+    if (row < 0) row += nspec;
+    out[row * nchans + channels[k]] = spec[p * spec_ld + k];
+  }
+}
+
+// unpacking_gpu.cu:6-21: zero packet i's bytes when spec_num[i+1]-spec_num[i] != spectra_per_packet.
+__global__ void dropped_kernel(uint8_t* data, const int64_t* spec_num, int64_t npackets, int64_t spp, int64_t bpp,
+                               unsigned long long* ndropped) {
+  const int64_t i = blockIdx.y;  // packet
+  if (i >= npackets - 1) return;
+  if (spec_num[i + 1] - spec_num[i] == spp) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(ndropped, 1ull);
+  for (int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; j < bpp; j += int64_t(gridDim.x) * blockDim.x)
+    data[i * bpp + j] = 0;
+}
+
+}  // namespace alb
+
+using namespace alb;
+
+extern "C" {
+
+int alb_unpack_4bit_float(const uint8_t* data, void* pol0, void* pol1, int64_t rowstart, int64_t rowend,
+                          const int64_t* channels, int64_t ncols, int64_t nchan, void* stream) {
+  ALB_REQUIRE(data && pol0 && pol1 && channels, "alb_unpack_4bit_float: null pointer");
+  ALB_REQUIRE(rowend >= rowstart && rowstart >= 0 && ncols >= 0 && nchan > 0, "alb_unpack_4bit_float: bad sizes");
+  const int64_t nrows = rowend - rowstart;
+  if (nrows * ncols == 0) return ALB_OK;
+  hipLaunchKernelGGL(unpack4_kernel, grid_for(nrows * ncols), dim3(256), 0, as_stream(stream), data, (float2*)pol0,
+                     (float2*)pol1, rowstart, nrows, channels, ncols, nchan);
+  ALB_LAUNCH_CHECK("unpack4_kernel");
+  return ALB_OK;
+}
+
+int alb_unpack_1bit_float(const uint8_t* data, void* pol0, void* pol1, int64_t rowstart, int64_t rowend,
+                          int64_t chanstart, int64_t chanend, int64_t nchan, void* stream) {
+  ALB_REQUIRE(data && pol0 && pol1, "alb_unpack_1bit_float: null pointer");
+  ALB_REQUIRE(rowend >= rowstart && rowstart >= 0 && chanend >= chanstart && nchan > 0,
+              "alb_unpack_1bit_float: bad sizes");
+  ALB_REQUIRE(chanstart % 2 == 0 && (chanend - chanstart) % 2 == 0,
+              "alb_unpack_1bit_float: channel range must start even and hold an even count");
+  const int64_t nrows = rowend - rowstart, nb = (chanend - chanstart) / 2;
+  if (nrows * nb == 0) return ALB_OK;
+  hipLaunchKernelGGL(unpack1_kernel, grid_for(nrows * nb), dim3(256), 0, as_stream(stream), data, (float2*)pol0,
+                     (float2*)pol1, rowstart, nrows, chanstart, nb, nchan);
+  ALB_LAUNCH_CHECK("unpack1_kernel");
+  return ALB_OK;
+}
+
+int alb_sortpols(const uint8_t* data, uint8_t* pol0, uint8_t* pol1, int64_t rowstart, int64_t rowend, int64_t ncols,
+                 int64_t nchan, int bit_depth, int64_t chanstart, int64_t chanend, void* stream) {
+  ALB_REQUIRE(data && pol0 && pol1, "alb_sortpols: null pointer");
+  ALB_REQUIRE(chanend <= nchan, "ERROR: Trying to unpack more channels than present. Check chanend.");
+  ALB_REQUIRE(rowend >= rowstart && rowstart >= 0 && chanend > chanstart && chanstart >= 0, "alb_sortpols: bad sizes");
+  const int64_t nrows = rowend - rowstart;
+  if (nrows == 0) return ALB_OK;
+  if (bit_depth == 4) {
+    ALB_REQUIRE(ncols == chanend - chanstart, "alb_sortpols: 4-bit ncols must equal chanend-chanstart");
+    hipLaunchKernelGGL(sortpols4_kernel, grid_for(nrows * ncols), dim3(256), 0, as_stream(stream), data, pol0, pol1,
+                       rowstart, nrows, ncols, nchan, chanstart);
+  } else if (bit_depth == 1) {
+    ALB_REQUIRE(chanstart % 2 == 0, "ERROR: Start channel index must be even.");
+    ALB_REQUIRE(ncols == (chanend - chanstart + 3) / 4, "alb_sortpols: 1-bit ncols must be ceil((chanend-chanstart)/4)");
+    hipLaunchKernelGGL(sortpols1_kernel, grid_for(nrows * ncols), dim3(256), 0, as_stream(stream), data, pol0, pol1,
+                       rowstart, nrows, ncols, nchan, chanstart, int((chanend - chanstart) % 4));
+  } else {
+    ALB_REQUIRE(false, "alb_sortpols: bit_depth must be 1 or 4 (got %d)", bit_depth);
+  }
+  ALB_LAUNCH_CHECK("sortpols_kernel");
+  return ALB_OK;
+}
+
+int alb_hist(const uint8_t* data, uint64_t* hist, int64_t rowstart, int64_t rowend, int64_t nchan, int bit_depth,
+             int mode, void* stream) {
+  ALB_REQUIRE(data && hist, "alb_hist: null pointer");
+  ALB_REQUIRE(bit_depth == 1 || bit_depth == 4, "bit_depth set to %d, only takes values 1 or 4", bit_depth);
+  ALB_REQUIRE(mode == 0 || mode == 1 || mode == -1, "mode set to %d, only takes values 0,1,-1", mode);
+  ALB_REQUIRE(rowend >= rowstart && rowstart >= 0 && nchan > 0, "alb_hist: bad sizes");
+  const int64_t nbins = bit_depth == 4 ? 16 : 2;
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(fill_u64, grid_for(nbins * nchan), dim3(256), 0, s, hist, nbins * nchan);
+  const int64_t nrows = rowend - rowstart;
+  if (nrows > 0) {
+    const int64_t rows_per = 256;
+    const unsigned gy = unsigned((nrows + rows_per - 1) / rows_per);
+    if (bit_depth == 4) {
+      hipLaunchKernelGGL(hist4_kernel, dim3(unsigned((nchan + 127) / 128), gy), dim3(128), 0, s, data,
+                         (unsigned long long*)hist, rowstart, nrows, nchan, mode, rows_per);
+    } else {
+      ALB_REQUIRE(nchan % 2 == 0, "alb_hist: 1-bit nchan must be even");
+      hipLaunchKernelGGL(hist1_kernel, dim3(unsigned((nchan / 2 + 127) / 128), gy), dim3(128), 0, s, data,
+                         (unsigned long long*)hist, rowstart, nrows, nchan, mode, rows_per);
+    }
+  }
+  ALB_LAUNCH_CHECK("hist_kernel");
+  return ALB_OK;
+}
+
+int alb_make_continuous(const void* spec, int64_t spec_ld, const int64_t* specnum, int64_t npresent, int64_t start,
+                        int64_t nspec, const int64_t* channels, int64_t ncols, void* out, int64_t nchans, void* stream) {
+  ALB_REQUIRE(out && (npresent == 0 || (spec && specnum && channels)), "alb_make_continuous: null pointer");
+  ALB_REQUIRE(nspec > 0 && nchans > 0 && npresent >= 0 && ncols >= 0 && spec_ld >= ncols,
+              "alb_make_continuous: bad sizes");
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(fill_c64, grid_for(nspec * nchans), dim3(256), 0, s, (float2*)out, nspec * nchans);
+  if (npresent * ncols > 0)
+    hipLaunchKernelGGL(scatter_kernel, grid_for(npresent * ncols), dim3(256), 0, s, (const float2*)spec, spec_ld,
+                       specnum, npresent, start, nspec, channels, ncols, (float2*)out, nchans);
+  ALB_LAUNCH_CHECK("scatter_kernel");
+  return ALB_OK;
+}
+
+int alb_dropped_packets(uint8_t* data, const int64_t* spec_num, int64_t num_packets, int64_t spectra_per_packet,
+                        int64_t nchan, int bit_depth, uint64_t* num_dropped, void* stream) {
+  ALB_REQUIRE(data && spec_num && num_dropped, "alb_dropped_packets: null pointer");
+  int64_t bpp;
+  if (bit_depth == 4) bpp = 2 * nchan * spectra_per_packet;
+  else if (bit_depth == 2) bpp = nchan * spectra_per_packet;
+  else if (bit_depth == 1) bpp = nchan * spectra_per_packet / 2;
+  else { set_error("alb_dropped_packets: bit_depth must be 1, 2 or 4"); return ALB_EINVAL; }
+  hipStream_t s = as_stream(stream);
+  hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, s, num_dropped, int64_t(1));
+  if (num_packets >= 2) {
+    ALB_REQUIRE(num_packets - 1 <= 65535, "alb_dropped_packets: at most 65536 packets per call");
+    hipLaunchKernelGGL(dropped_kernel, dim3(unsigned((bpp + 255) / 256 < 64 ? (bpp + 255) / 256 : 64),
+                                            unsigned(num_packets - 1)),
+                       dim3(256), 0, s, data, spec_num, num_packets, spectra_per_packet, bpp,
+                       (unsigned long long*)num_dropped);
+  }
+  ALB_LAUNCH_CHECK("dropped_kernel");
+  return ALB_OK;
+}
+
+int alb_fill_zero(void* p, int64_t nbytes, void* stream) {
+  ALB_REQUIRE(p || nbytes == 0, "alb_fill_zero: null pointer");
+  if (nbytes > 0)
+    hipLaunchKernelGGL(fill_u8, grid_for(nbytes), dim3(256), 0, as_stream(stream), (uint8_t*)p, nbytes, uint8_t(0));
+  ALB_LAUNCH_CHECK("fill_u8");
+  return ALB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,110 @@
+// Per-channel cross-antenna correlation out[i,j,f] = sum_t x[i,t,f] conj(x[j,t,f]) * (1/K)
+// (correlations_gpu.py:95-121, cgemm_batch.cu:8-54).  x and out are column-major
+// (F-order): x[(m, t, f)] at m + M*(t + K*f), out[(i, j, f)] at i + M*(j + M*f).
+//
+// M <= 8: the realified Gram G = Y Y^T of Y = [Re x; Im x] (16 x K) is built on the
+// FP32 matrix cores with v_mfma_f32_16x16x4f32, one wavefront per channel.  Lane l
+// feeds Y[l % 16][t0 + l / 16] to both operands (A = Y, B = Y^T share registers).
+// Then Re = G[i][j] + G[i+8][j+8], Im = G[i+8][j] - G[i][j+8].
+// Other M: a VALU kernel with one thread per (channel, baseline).
+#include "common.hpp"
+
+namespace alb {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+template <int M>
+__global__ __launch_bounds__(256) void xcorr_mfma16_kernel(const float* __restrict__ x, int64_t K, int64_t F,
+                                                           float2* __restrict__ out, float scale) {
+  __shared__ float G[4][16 * 16];
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int64_t f = int64_t(blockIdx.x) * 4 + w;
+  const bool valid = f < F;
+  const int r = lane % 16, slot = lane / 16;
+  const int m = r % 8, part = r / 8;
+  const bool row_ok = valid && m < M;
+  const float* base = x + (valid ? f : 0) * int64_t(2 * M) * K + 2 * m + part;
+  floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0, acc2 = acc0, acc3 = acc0;
+  int64_t t0 = 0;
+  for (; t0 + 16 <= K; t0 += 16) {
+    const float a0 = row_ok ? base[int64_t(2 * M) * (t0 + slot)] : 0.f;
+    const float a1 = row_ok ? base[int64_t(2 * M) * (t0 + 4 + slot)] : 0.f;
+    const float a2 = row_ok ? base[int64_t(2 * M) * (t0 + 8 + slot)] : 0.f;
+    const float a3 = row_ok ? base[int64_t(2 * M) * (t0 + 12 + slot)] : 0.f;
+    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, a0, acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, a1, acc1, 0, 0, 0);
+    acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, a2, acc2, 0, 0, 0);
+    acc3 = __builtin_amdgcn_mfma_f32_16x16x4f32(a3, a3, acc3, 0, 0, 0);
+  }
+  for (; t0 < K; t0 += 4) {
+    const int64_t t = t0 + slot;
+    const float a0 = (row_ok && t < K) ? base[int64_t(2 * M) * t] : 0.f;
+    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, a0, acc0, 0, 0, 0);
+  }
+  // D layout: lane l holds column l % 16, rows 4*(l/16) + v.
+#pragma unroll
+  for (int v = 0; v < 4; ++v) G[w][(4 * slot + v) * 16 + r] = (acc0[v] + acc1[v]) + (acc2[v] + acc3[v]);
+  __syncthreads();
+  if (valid && lane < M * M) {
+    const int i = lane % M, j = lane / M;
+    const float* g = G[w];
+    const float re = g[i * 16 + j] + g[(i + 8) * 16 + (j + 8)];
+    const float im = g[(i + 8) * 16 + j] - g[i * 16 + (j + 8)];
+    out[f * M * M + int64_t(j) * M + i] = make_float2(re * scale, im * scale);
+  }
+}
+
+// Generic: workgroup per channel, thread per baseline i <= j.
+__global__ __launch_bounds__(256) void xcorr_valu_kernel(const float2* __restrict__ x, int M, int64_t K, int64_t F,
+                                                         float2* __restrict__ out, float scale) {
+  const int64_t f = blockIdx.x;
+  const int npair = M * (M + 1) / 2;
+  const float2* xf = x + f * int64_t(M) * K;
+  for (int p = threadIdx.x; p < npair; p += blockDim.x) {
+    int i = 0, rem = p;
+    while (rem >= M - i) { rem -= M - i; ++i; }
+    const int j = i + rem;
+    double sr = 0.0, si = 0.0;
+    float cr = 0.f, ci = 0.f;
+    for (int64_t t = 0; t < K; ++t) {
+      const float2 a = xf[t * M + i], b = xf[t * M + j];
+      cr = fmaf(a.x, b.x, fmaf(a.y, b.y, cr));
+      ci = fmaf(a.y, b.x, fmaf(-a.x, b.y, ci));
+      if ((t & 255) == 255) { sr += cr; si += ci; cr = 0.f; ci = 0.f; }
+    }
+    sr += cr;
+    si += ci;
+    const float re = float(sr), im = float(si);
+    out[f * M * M + int64_t(j) * M + i] = make_float2(re * scale, im * scale);
+    if (i != j) out[f * M * M + int64_t(i) * M + j] = make_float2(re * scale, -im * scale);
+  }
+}
+
+}  // namespace alb
+
+using namespace alb;
+
+extern "C" {
+
+// correlations_gpu.py:95-121 avg_xcorr_all_ant_gpu (split = 1): out = x x^H * scale
+// per channel; scale is normally fl32(1/K).
+int alb_xcorr_all_ant(const void* x, int64_t M, int64_t K, int64_t F, void* out, float scale, void* stream) {
+  ALB_REQUIRE(x && out, "alb_xcorr_all_ant: null pointer");
+  ALB_REQUIRE(M >= 1 && K >= 0 && F >= 0, "alb_xcorr_all_ant: bad sizes");
+  if (F == 0) return ALB_OK;
+  hipStream_t s = as_stream(stream);
+  switch (M) {
+#define CASE(m) case m: hipLaunchKernelGGL(xcorr_mfma16_kernel<m>, dim3(unsigned((F + 3) / 4)), dim3(256), 0, s, (const float*)x, K, F, (float2*)out, scale); break;
+    CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+    default:
+      ALB_REQUIRE(M * (M + 1) / 2 <= 4096, "alb_xcorr_all_ant: M too large");
+      ALB_REQUIRE(F <= 0x7fffffff, "alb_xcorr_all_ant: too many channels");
+      hipLaunchKernelGGL(xcorr_valu_kernel, dim3(unsigned(F)), dim3(256), 0, s, (const float2*)x, int(M), K, F,
+                         (float2*)out, scale);
+  }
+  ALB_LAUNCH_CHECK("xcorr_all_ant");
+  return ALB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,272 @@
+"""Streaming re-PFB + cross-correlation -- the call surface of
+scripts/xcorr/helper_gpu_stream.py (``repfb_xcorr_avg``, ``get_chunkier``).
+
+The reference loops job -> antenna -> pol -> chunk with a handful of CuPy /
+cuFFT / cuBLAS launches per chunk and a host sync per job.  Here the whole
+stream of one input goes through four launches:
+
+1. ``alb_ipfb_rows_packed4``: 4-bit unpack + spec_num zero fill + channel
+   scatter + row inverse FFT for a batch of iPFB tiles (the ``2*cut`` rows
+   carried from the previous chunk are read in place, helper_gpu_stream.py:195-197);
+2. ``alb_ipfb_columns``: column FFT, Wiener filter, inverse column FFT; rows
+   ``[cut, nslice-cut)`` land at their place in the input's time stream
+   (helper_gpu_stream.py:199-214);
+3. ``alb_repfb``: all re-PFB jobs of the stream at once (job s covers stream
+   samples ``[s*nblock*lblock, s*nblock*lblock + (nblock+ntap-1)*lblock)``,
+   the zero-padded tail included, helper_gpu_stream.py:148-150, 217-222),
+   writing the kept fine channels straight into the correlator layout;
+4. ``alb_xcorr_all_ant``: every job's ``M x M`` visibilities in one launch.
+
+Row ``ant*npol + pol`` of the correlator input follows
+helper_gpu_stream_clean.py:375 (helper_gpu_stream.py:222 indexes ``j*nant+k``).
+"""
+import numpy as np
+
+from . import _lib
+from . import pfb as _pfb
+
+
+def get_chunkier(nchunks, lchunk, nblock, lblock, ntap):
+    """helper_gpu_stream.py:256-284: which iPFB chunks each re-PFB job consumes."""
+    ranges = []
+    stride_sz = nblock * lblock
+    total_needed = stride_sz + (ntap - 1) * lblock
+    remainder = 0
+    cur_chunk = 0
+    while True:
+        start_chunk = cur_chunk
+        while remainder < total_needed and cur_chunk < nchunks:
+            remainder += lchunk
+            cur_chunk += 1
+        if remainder >= total_needed:
+            ranges.append((start_chunk, cur_chunk))
+            remainder -= stride_sz
+        else:
+            if remainder > 0:
+                ranges.append((start_chunk, None))
+            break
+    return ranges
+
+
+def missing_fraction(nspec_present, acclen, nblock, lblock, lchunk, ntap, jobs):
+    """Missing-data percentage per (antenna, job): helper_gpu_stream.py:157-233
+    bookkeeping with one list per antenna (helper_gpu_stream_clean.py:204).
+    ``nspec_present[j][c]`` = spectra present in chunk c of antenna j."""
+    nant = len(nspec_present)
+    nchunks = len(nspec_present[0]) if nant else 0
+    szblock = (nblock + ntap - 1) * lblock
+    out = np.zeros((nant, len(jobs)), dtype=np.float64)
+    for j in range(nant):
+        lst = []
+        pfb_idx = 0
+        rem_idx = 0
+        for s, (c0, c1) in enumerate(jobs):
+            if s > 0:
+                pfb_idx = (ntap - 1) * lblock
+            sub = nspec_present[j][c0:c1] if c1 is not None else nspec_present[j][c0:nchunks]
+            if pfb_idx + rem_idx > szblock:
+                rem_idx -= szblock - pfb_idx
+                pfb_idx = szblock
+            else:
+                pfb_idx += rem_idx
+                rem_idx = 0
+                for npres in sub:
+                    final_idx = pfb_idx + lchunk
+                    lst.append([(1 - npres / acclen) * 100, (final_idx // lblock) % nblock + 1])
+                    if final_idx > szblock:
+                        rem_idx = final_idx - szblock
+                        pfb_idx = szblock
+                    else:
+                        rem_idx = 0
+                        pfb_idx = final_idx
+            out[j, s] = (sum(m[0] for m in lst) / len(lst)) if lst else np.nan
+            for m in lst:
+                m[1] -= 1
+            lst = [m for m in lst if m[1] > 0]
+    return out
+
+
+class RePFBPipeline:
+    """Device-resident unpack -> iPFB -> re-PFB -> correlate engine.
+
+    Parameters mirror helper_gpu_stream.py:62: ``acclen`` spectra per chunk,
+    ``cut`` carried/dropped rows, ``osamp`` upchannelization, ``nblock`` re-PFB
+    spectra per job, fine channels ``[repfb_chanstart, repfb_chanend)``,
+    ``chan_numbers`` = coarse channel number of each packed column
+    (helper_gpu_stream.py:196).
+    """
+
+    def __init__(self, ninput, chan_numbers, acclen, cut, osamp, nblock, repfb_chanstart, repfb_chanend,
+                 filt_thresh=0.45, ntap=4, filt=None, win=None, tile_batch=4, nchan=2049):
+        t = _lib.torch()
+        self.ninput = ninput
+        self.acclen = int(acclen)
+        self.cut = int(cut)
+        self.nslice = self.acclen + 2 * self.cut
+        self.osamp = int(osamp)
+        self.nblock = int(nblock)
+        self.ntap = int(ntap)
+        self.lblock = 4096 * self.osamp
+        self.c_lo = int(repfb_chanstart)
+        self.c_hi = int(repfb_chanend)
+        self.nfine = self.c_hi - self.c_lo
+        self.tile_batch = int(tile_batch)
+        if nchan != 2049:
+            raise NotImplementedError("the fused iPFB handles 2049-channel spectra (lblock 4096)")
+        self.chan = _lib.to_device(np.asarray(chan_numbers, dtype=np.int32), t.int32)
+        self.ncols = int(self.chan.numel())
+        self.filt = _lib.to_device(filt, t.complex64) if filt is not None else _pfb.get_filter(self.nslice, filt_thresh,
+                                                                                                  ntap=ntap)
+        self.win = _lib.to_device(win, t.float32) if win is not None else _pfb.window(ntap, self.lblock)
+        self._tiles = None
+        self._ws = None
+
+    # ---------------------------------------------------------------- plan
+    def plan(self, nchunks):
+        lchunk = 4096 * self.acclen
+        jobs = get_chunkier(nchunks, lchunk, self.nblock, self.lblock, self.ntap)
+        nblocks_total = len(jobs) * self.nblock
+        rows_needed = (nblocks_total + self.ntap - 1) * self.osamp
+        rows = max(rows_needed, nchunks * self.acclen)
+        return jobs, nblocks_total, rows
+
+    def _tile_buffer(self, ntiles):
+        t = _lib.torch()
+        need = ntiles * self.nslice * 4096
+        if self._tiles is None or self._tiles.numel() < need:
+            self._tiles = t.empty(need, dtype=t.float32, device=_lib.device())
+        return self._tiles
+
+    def _workspace(self, nblk):
+        t = _lib.torch()
+        need = int(_lib.load().alb_repfb_workspace_bytes(nblk, self.osamp)) // 8
+        if self._ws is None or self._ws.numel() < need:
+            self._ws = t.empty(need, dtype=t.complex64, device=_lib.device())
+        return self._ws
+
+    # --------------------------------------------------------------- stages
+    def ipfb_stream(self, packed, rowmap, nchunks, stream_out):
+        """Fill ``stream_out`` ([rows, 4096] float32) with the iPFB time stream
+        of one input: ``packed`` uint8 [nchunks, acclen, ncols] (present rows
+        first), ``rowmap`` int32 [nchunks, acclen]."""
+        s = _lib.stream()
+        rows_valid = nchunks * self.acclen
+        total = stream_out.shape[0]
+        if total > rows_valid:
+            tail = stream_out[rows_valid:]
+            _lib.call("alb_fill_zero", _lib.ptr(tail), tail.numel() * 4, s)
+        tb = self.tile_batch
+        y = self._tile_buffer(min(tb, nchunks))
+        for t0 in range(0, nchunks, tb):
+            nt = min(tb, nchunks - t0)
+            _lib.call("alb_ipfb_rows_packed4", _lib.ptr(packed), _lib.ptr(rowmap), _lib.ptr(self.chan), self.ncols,
+                      self.acclen, self.cut, t0, nt, _lib.ptr(y), s)
+            out = stream_out[t0 * self.acclen:]
+            _lib.call("alb_ipfb_columns", _lib.ptr(y), 4096, self.nslice * 4096, nt, self.nslice, 4096,
+                      _lib.ptr(self.filt), self.filt.shape[1], self.cut, _lib.ptr(out), 4096, self.acclen * 4096, s)
+
+    def repfb(self, stream_rows, nblocks_total, out, sb, sc, bpj, sj, block_batch=512):
+        """Re-PFB all ``nblocks_total`` spectra of one input's stream into
+        ``out`` (element addressing of ``alb_repfb``)."""
+        s = _lib.stream()
+        for b0 in range(0, nblocks_total, block_batch):
+            nb = min(block_batch, nblocks_total - b0)
+            ws = self._workspace(nb)
+            ts = stream_rows[b0 * self.osamp:]
+            # the batch's first spectrum is global block b0: shift the output base
+            j0, r0 = divmod(b0, bpj)
+            base = out.view(-1)[j0 * sj + r0 * sb:] if b0 else out.view(-1)
+            if r0 != 0:
+                raise ValueError("block_batch must be a multiple of the job length")
+            _lib.call("alb_repfb", _lib.ptr(ts), _lib.ptr(self.win), self.ntap, nb, self.osamp, _lib.ptr(ws),
+                      self.c_lo, self.c_hi, _lib.ptr(base), sb, sc, bpj, sj, s)
+
+    # ------------------------------------------------------------ pipelines
+    def spectra(self, packed, rowmap, nchunks):
+        """Single input (rechannelizer): complex64 ``[nblocks_total, nfine]`` of
+        ``cupy_pfb(stream)[:, c_lo:c_hi]`` over all jobs."""
+        t = _lib.torch()
+        jobs, nbt, rows = self.plan(nchunks)
+        stream = t.empty((rows, 4096), dtype=t.float32, device=_lib.device())
+        self.ipfb_stream(packed, rowmap, nchunks, stream)
+        out = t.empty((nbt, self.nfine), dtype=t.complex64, device=stream.device)
+        self.repfb(stream, nbt, out, self.nfine, 1, nbt, 0, block_batch=nbt)
+        return out
+
+    def correlate(self, packed_list, rowmaps, nchunks, return_xin=False):
+        """All inputs: ``vis`` complex64 F-order ``(M, M, nfine, njobs)`` of
+        ``avg_xcorr_all_ant_gpu`` per job (helper_gpu_stream.py:236)."""
+        t = _lib.torch()
+        M = self.ninput
+        jobs, nbt, rows = self.plan(nchunks)
+        njobs = len(jobs)
+        K = self.nblock
+        F = self.nfine
+        # F-order (M, K, F, S) == C-order [S][F][K][M]
+        xin = t.empty((njobs, F, K, M), dtype=t.complex64, device=_lib.device())
+        stream = t.empty((rows, 4096), dtype=t.float32, device=xin.device)
+        bb = max(1, 512 // K) * K
+        for m in range(M):
+            self.ipfb_stream(packed_list[m], rowmaps[m], nchunks, stream)
+            self.repfb(stream, nbt, xin.view(-1)[m:], M, M * K, K, M * K * F, block_batch=bb)
+        vis = t.empty((njobs, F, M, M), dtype=t.complex64, device=xin.device)
+        scale = np.float32(1.0) / np.float32(K)
+        _lib.call("alb_xcorr_all_ant", _lib.ptr(xin), M, K, F * njobs, _lib.ptr(vis), float(scale), _lib.stream())
+        res = vis.permute(3, 2, 1, 0)  # (M, M, F, S) view of the F-order result
+        if return_xin:
+            return res, xin.permute(3, 2, 1, 0)
+        return res
+
+
+def rowmaps_from_specnums(specnums_list, start_specnum, acclen):
+    """Per chunk row maps ``(specnum - start) % acclen -> packed row`` on the
+    device (baseband_data_classes.py:53)."""
+    t = _lib.torch()
+    n = len(specnums_list)
+    rm = t.empty((n, acclen), dtype=t.int32, device=_lib.device())
+    s = _lib.stream()
+    for c, sp in enumerate(specnums_list):
+        sp = np.asarray(sp, dtype=np.int64)
+        d = _lib.to_device(sp if len(sp) else np.zeros(1, dtype=np.int64), t.int64)
+        _lib.call("alb_build_rowmap", _lib.ptr(d), len(sp), int(start_specnum), int(acclen), _lib.ptr(rm[c]), s)
+    return rm
+
+
+def stack_packed(chunks, key, acclen, ncols):
+    """Stack per-chunk packed rows (present rows first) into [nchunks, acclen, ncols]."""
+    out = np.zeros((len(chunks), acclen, ncols), dtype=np.uint8)
+    for c, ch in enumerate(chunks):
+        a = np.asarray(ch[key] if not hasattr(ch[key], "cpu") else ch[key].cpu().numpy(), dtype=np.uint8)
+        out[c, : a.shape[0]] = a
+    return out
+
+
+def repfb_xcorr_avg_chunks(chunks, start_specnums, chan_numbers, repfb_chanstart, repfb_chanend, acclen, nchunks,
+                           nblock, osamp, cut=10, filt_thresh=0.45, win=None, filt=None, npol=2, return_xin=False):
+    """helper_gpu_stream.py:62-254 on in-memory chunks: ``chunks[j]`` is the
+    list of packed-4-bit BasebandFileIterator outputs of antenna j (dicts with
+    uint8 ``pol0``/``pol1`` and int64 ``specnums``).  Returns
+    ``(vis, missing_fraction, freqs, win, filt)``; vis is numpy complex64
+    F-order ``(nant*npol, nant*npol, nchan, njobs)``."""
+    nant = len(chunks)
+    ncols = len(chan_numbers)
+    pipe = RePFBPipeline(nant * npol, chan_numbers, acclen, cut, osamp, nblock, repfb_chanstart, repfb_chanend,
+                         filt_thresh=filt_thresh, win=win, filt=filt)
+    packed, rmaps = [], []
+    for j in range(nant):
+        rm = rowmaps_from_specnums([c["specnums"] for c in chunks[j][:nchunks]], start_specnums[j], acclen)
+        for k in range(npol):
+            packed.append(_lib.to_device(stack_packed(chunks[j][:nchunks], "pol%d" % k, acclen, ncols)))
+            rmaps.append(rm)
+    out = pipe.correlate(packed, rmaps, nchunks, return_xin=return_xin)
+    vis = out[0] if return_xin else out
+    jobs, _, _ = pipe.plan(nchunks)
+    lblock = 4096 * osamp
+    npres = [[len(c["specnums"]) for c in chunks[j][:nchunks]] for j in range(nant)]
+    miss = missing_fraction(npres, acclen, nblock, lblock, 4096 * acclen, 4, jobs)
+    visn = np.asfortranarray(_lib.host(vis))
+    freqs = np.arange(repfb_chanstart, repfb_chanend)
+    res = (np.ma.masked_invalid(visn), miss, freqs, pipe.win, pipe.filt)
+    if return_xin:
+        return res + (out[1],)
+    return res

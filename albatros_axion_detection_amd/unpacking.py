@@ -1,0 +1,115 @@
+"""Baseband unpacking on the GPU -- the call surface of src/correlations/unpacking.py.
+
+Inputs may be numpy arrays (copied to the device) or torch tensors on the GPU.
+Outputs follow the input kind: numpy in -> numpy out, torch in -> torch out
+(device tensors, like the CuPy arrays of the reference's GPU path).
+All arithmetic runs in libalbatros_hip.
+"""
+import numpy as np
+
+from . import _lib
+
+_backend = "hip"
+
+
+def set_backend(backend):
+    """unpacking.py:8-16.  Accepted for source compatibility; the computation
+    always runs on the GPU ("numpy" only selects numpy outputs)."""
+    global _backend
+    if backend not in ("numpy", "cupy", "torch", "hip"):
+        raise ValueError("Invalid backend. Choose either 'numpy' or 'cupy'.")
+    _backend = backend
+
+
+def _out(x, like_numpy):
+    return _lib.host(x) if (like_numpy or _backend == "numpy") else x
+
+
+def unpack_4bit(data, rowstart, rowend, channels, length_channels):
+    """unpacking.py:193-245 (GPU form :140-155; C form unpacking.c:155-205).
+
+    ``data``: raw spectra, ``2*length_channels`` bytes each; ``channels``:
+    column indices.  Returns complex64 (pol0, pol1) of shape
+    ``(rowend-rowstart, len(channels))``; nibble 8 decodes to +8.
+    """
+    t = _lib.torch()
+    isnp = _lib.is_numpy(data)
+    d = _lib.to_device(data, t.uint8)
+    ch = _lib.to_device(np.asarray(channels, dtype=np.int64) if not isinstance(channels, t.Tensor) else channels,
+                        t.int64)
+    nrows = int(rowend - rowstart)
+    ncols = int(ch.numel())
+    pol0 = t.empty((nrows, ncols), dtype=t.complex64, device=d.device)
+    pol1 = t.empty((nrows, ncols), dtype=t.complex64, device=d.device)
+    _lib.call("alb_unpack_4bit_float", _lib.ptr(d), _lib.ptr(pol0), _lib.ptr(pol1), int(rowstart), int(rowend),
+              _lib.ptr(ch), ncols, int(length_channels), _lib.stream())
+    return _out(pol0, isnp), _out(pol1, isnp)
+
+
+def unpack_1bit(data, rowstart, rowend, channels, length_channels):
+    """unpacking.py:248-302 (C form unpacking.c:207-246).
+
+    ``channels``: a continuous range of channel indices starting at an even
+    index with an even count (baseband_data_classes.py:334-341).
+    """
+    t = _lib.torch()
+    ch = np.asarray(channels, dtype=np.int64)
+    chanstart = int(ch[0])
+    chanend = int(ch[-1]) + 1
+    if np.any(np.diff(ch) != 1):
+        raise ValueError("A continuous range of channels must be provided in 1-bit mode.")
+    isnp = _lib.is_numpy(data)
+    d = _lib.to_device(data, t.uint8)
+    nrows = int(rowend - rowstart)
+    ncols = chanend - chanstart
+    pol0 = t.empty((nrows, ncols), dtype=t.complex64, device=d.device)
+    pol1 = t.empty((nrows, ncols), dtype=t.complex64, device=d.device)
+    _lib.call("alb_unpack_1bit_float", _lib.ptr(d), _lib.ptr(pol0), _lib.ptr(pol1), int(rowstart), int(rowend),
+              chanstart, chanend, int(length_channels), _lib.stream())
+    return _out(pol0, isnp), _out(pol1, isnp)
+
+
+def sortpols(data, length_channels, bit_mode, rowstart, rowend, chanstart, chanend):
+    """unpacking.py:305-377 / unpacking.c:248-368.
+
+    4-bit: per-pol bytes ``(nrows, chanend-chanstart)``.  1-bit: per pol, four
+    channels per byte, ``ceil((chanend-chanstart)/4)`` bytes per row.
+    """
+    t = _lib.torch()
+    if chanend is None:
+        chanstart, chanend = 0, length_channels
+    elif chanend > length_channels:
+        raise ValueError("ERROR: Trying to unpack more channels than present. Check chanend.")
+    nrows = int(rowend - rowstart)
+    if bit_mode == 4:
+        ncols = chanend - chanstart
+    elif bit_mode == 1:
+        if chanstart % 2 > 0:
+            raise ValueError("ERROR: Start channel index must be even.")
+        ncols = int(np.ceil((chanend - chanstart) / 4))
+    else:
+        raise ValueError("bit_mode must be 1 or 4")
+    isnp = _lib.is_numpy(data)
+    d = _lib.to_device(data, t.uint8)
+    pol0 = t.empty((nrows, ncols), dtype=t.uint8, device=d.device)
+    pol1 = t.empty((nrows, ncols), dtype=t.uint8, device=d.device)
+    _lib.call("alb_sortpols", _lib.ptr(d), _lib.ptr(pol0), _lib.ptr(pol1), int(rowstart), int(rowend), int(ncols),
+              int(length_channels), int(bit_mode), int(chanstart), int(chanend), _lib.stream())
+    return _out(pol0, isnp), _out(pol1, isnp)
+
+
+def hist(data, rowstart, rowend, length_channels, bit_depth, mode):
+    """unpacking.py:82-138: uint64 ``(2**bit_depth, length_channels)`` counts of
+    raw quantisation levels; mode 0 = pol0, 1 = pol1, -1 = both."""
+    t = _lib.torch()
+    assert bit_depth in (1, 4), f"bit_depth set to {bit_depth}, only takes values 1 or 4"
+    assert mode in (0, 1, -1), f"mode set to {mode}, only takes values 0,1,-1"
+    isnp = _lib.is_numpy(data)
+    d = _lib.to_device(data, t.uint8)
+    nbins = 2 ** bit_depth
+    h = t.empty((nbins, int(length_channels)), dtype=t.int64, device=d.device)
+    _lib.call("alb_hist", _lib.ptr(d), _lib.ptr(h), int(rowstart), int(rowend), int(length_channels), int(bit_depth),
+              int(mode), _lib.stream())
+    if isnp or _backend == "numpy":
+        return _lib.host(h).astype(np.uint64)
+    return h

@@ -1,0 +1,322 @@
+#!/usr/bin/env python3
+"""Benchmark of the ALBATROS hot path on one MI355X.
+
+Headline workload (one "step"): the single-input rechannelizer -- 4 chunks of
+4088 coarse 4-bit spectra (100 coarse channels 64..163, nslice 4096 with
+2*cut = 8 carried rows) -> unpack + spec_num zero fill -> Wiener inverse PFB
+(thresh 0.45) -> re-PFB at 64x (ntap 4) -> the 6400 fine channels of every
+complete block.  Inputs are resident in HBM before the timed region.
+
+value = ADC-equivalent samples per second (coarse spectra x 4096 / s / 1e6).
+
+Also reported: live per-kernel HIP-event times over the timed steps, the
+roofline of the dominant kernel, the pipeline-level roofline, a CPU baseline
+(the reference CPU algorithm restated, timed on a bounded sample of the same
+workload) and secondary workloads (direct 4-bit and 1-bit correlators, the
+precision correlator).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+FP32_MFMA_TFLOPS = 157.3
+HERE = os.path.dirname(os.path.abspath(__file__))
+METRIC = BASE_METRIC = "Msamples/s unpack→iPFB→PFB→xcorr, % HBM/MFMA roofline; vs plain FP32"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def read_traffic(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE /
+    WRITE_SIZE, scaled by the copy-kernel calibration), kept as key=value
+    text under profiles/.  None when not measured."""
+    p = os.path.join(HERE, "profiles", "traffic_%s.txt" % name)
+    if not os.path.exists(p):
+        return None
+    kv = {}
+    for line in open(p):
+        if "=" in line and not line.startswith("#"):
+            k, v = line.split("=", 1)
+            kv[k.strip()] = v.strip()
+    try:
+        return float(kv["bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+class Timer:
+    """HIP events around each launch on torch's current stream (the stream
+    every entry point is given)."""
+
+    def __init__(self, torch):
+        self.t = torch
+        self.ev = {}
+        self.on = False
+
+    def __call__(self, name, fn):
+        if not self.on:
+            fn()
+            return
+        a = self.t.cuda.Event(enable_timing=True)
+        b = self.t.cuda.Event(enable_timing=True)
+        a.record()
+        fn()
+        b.record()
+        self.ev.setdefault(name, []).append((a, b))
+
+    def summary(self):
+        self.t.cuda.synchronize()
+        return {k: float(np.mean([a.elapsed_time(b) for a, b in v])) for k, v in self.ev.items()}
+
+    def counts(self):
+        return {k: len(v) for k, v in self.ev.items()}
+
+
+def rechannelizer(args, torch):
+    from albatros_axion_detection_amd import _lib, pfb, repfb, synth
+    acclen, cut, osamp, nchunks = 4088, 4, 64, args.nchunks
+    chan0, nkeep = 64, 100
+    n_coarse = acclen * nchunks
+    log("[bench] generating rechannelizer input: %d coarse spectra" % n_coarse)
+    packed_np, chans = synth.rechannelizer_input(n_coarse=n_coarse, chan0=chan0, nkeep=nkeep, seed=1)
+    nslice = acclen + 2 * cut
+    nblk = (n_coarse // osamp) - 3  # every complete re-PFB block of the stream
+    c_lo, c_hi = chan0 * osamp, (chan0 + nkeep) * osamp
+    pipe = repfb.RePFBPipeline(1, chans, acclen, cut, osamp, nblk, c_lo, c_hi, filt_thresh=0.45,
+                               tile_batch=nchunks)
+    dev = _lib.device()
+    packed = _lib.to_device(packed_np.reshape(nchunks, acclen, nkeep))
+    specnums = [_lib.to_device(np.arange(c * acclen, (c + 1) * acclen, dtype=np.int64)) for c in range(nchunks)]
+    rowmap = torch.empty((nchunks, acclen), dtype=torch.int32, device=dev)
+    rows = max((nblk + 3) * osamp, n_coarse)
+    stream = torch.zeros((rows, 4096), dtype=torch.float32, device=dev)
+    tiles = torch.empty((nchunks, nslice, 4096), dtype=torch.float32, device=dev)
+    ws = torch.empty(int(_lib.load().alb_repfb_workspace_bytes(nblk, osamp)) // 8, dtype=torch.complex64,
+                     device=dev)
+    out = torch.empty((nblk, c_hi - c_lo), dtype=torch.complex64, device=dev)
+    timer = Timer(torch)
+    s = _lib.stream
+
+    def step():
+        timer("rowmap", lambda: [
+            _lib.call("alb_build_rowmap", _lib.ptr(specnums[c]), acclen, 0, acclen, _lib.ptr(rowmap[c]), s())
+            for c in range(nchunks)])
+        timer("ipfb_rows_packed4", lambda: _lib.call(
+            "alb_ipfb_rows_packed4", _lib.ptr(packed), _lib.ptr(rowmap), _lib.ptr(pipe.chan), nkeep, acclen, cut, 0,
+            nchunks, _lib.ptr(tiles), s()))
+        timer("ipfb_columns", lambda: _lib.call(
+            "alb_ipfb_columns", _lib.ptr(tiles), 4096, nslice * 4096, nchunks, nslice, 4096, _lib.ptr(pipe.filt),
+            pipe.filt.shape[1], cut, _lib.ptr(stream), 4096, acclen * 4096, s()))
+        timer("repfb_stage1", lambda: _lib.call(
+            "alb_repfb_stage1", _lib.ptr(stream), _lib.ptr(pipe.win), 4, nblk, osamp, _lib.ptr(ws), s()))
+        timer("repfb_stage2", lambda: _lib.call(
+            "alb_repfb_stage2", _lib.ptr(ws), nblk, osamp, c_lo, c_hi, _lib.ptr(out), c_hi - c_lo, 1, nblk, 0, s()))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    timer.on = True
+    t0e = torch.cuda.Event(enable_timing=True)
+    t1e = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    t0e.record()
+    for _ in range(args.steps):
+        step()
+    t1e.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = t0e.elapsed_time(t1e) / args.steps
+    kern = timer.summary()
+    timer.on = False
+    log("[bench] rechannelizer: %.3f ms/step (wall %.3f ms/step); kernels %s" % (
+        ms, 1e3 * wall / args.steps, json.dumps({k: round(v, 4) for k, v in kern.items()})))
+
+    # algorithmic bytes per launch of each kernel (per-unit figures of the
+    # rechannelizer: raw 100 B + time stream write 16384 B + read 16384 B + fine out 800 B
+    # per coarse spectrum)
+    alg = {
+        "ipfb_rows_packed4": nslice * nchunks * nkeep,                 # raw bytes in
+        "ipfb_columns": n_coarse * 4096 * 4,                            # time stream out
+        "repfb_stage1": (nblk + 3) * osamp * 4096 * 4,                  # time stream in
+        "repfb_stage2": nblk * (c_hi - c_lo) * 8,                       # fine channels out
+    }
+    dom = max((k for k in kern if k in alg), key=lambda k: kern[k])
+    achieved = alg[dom] / (kern[dom] * 1e-3) / 1e9
+    traffic = read_traffic(dom + "_rechannelizer")
+    roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "algorithmic_bytes_per_launch": alg[dom], "avg_launch_ms": round(kern[dom], 5)}
+    pipe_bytes = 33668 * n_coarse
+    pipe_roof = {"bound": "hbm", "algorithmic_bytes_per_step": pipe_bytes,
+                 "achieved": round(pipe_bytes / (ms * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(pipe_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    value = n_coarse * 4096 / (ms * 1e-3) / 1e6
+    res = {
+        "value": value, "ms_per_step": ms, "kernels_ms": {k: round(v, 5) for k, v in kern.items()},
+        "roofline": roof, "pipeline_roofline": pipe_roof,
+        "config": {"workload": "rechannelizer: 1 input, %d coarse 4-bit spectra (%d chunks x %d, nslice %d, cut %d), "
+                               "coarse ch %d..%d -> %d fine ch, osamp %d, ntap 4, Wiener thresh 0.45, %d re-PFB "
+                               "spectra" % (n_coarse, nchunks, acclen, nslice, cut, chan0, chan0 + nkeep - 1,
+                                            c_hi - c_lo, osamp, nblk),
+                   "n_coarse": n_coarse, "acclen": acclen, "cut": cut, "osamp": osamp, "nchan_fine": c_hi - c_lo,
+                   "nblock": nblk},
+        "ref_unit_msps": n_coarse * 2049 * 8 / (ms * 1e-3) / 1e6,
+    }
+    ctx = dict(packed_np=packed_np, acclen=acclen, cut=cut, osamp=osamp, chan0=chan0, nkeep=nkeep,
+               filt=pipe.filt, win=pipe.win)
+    return res, ctx
+
+
+def cpu_baseline(ctx, budget_s):
+    """The reference CPU rechannelizer (pfb_cpu_utils.py cpu_ipfb / cpu_pfb with
+    scipy.fft workers = cores) on one chunk of the same workload."""
+    from albatros_axion_detection_amd import _lib
+    from oracle import cpu_port
+    acclen, cut, osamp = ctx["acclen"], ctx["cut"], ctx["osamp"]
+    nkeep, chan0 = ctx["nkeep"], ctx["chan0"]
+    sample_chunks = 1
+    raw = ctx["packed_np"][: sample_chunks * acclen]
+    filt = _lib.host(ctx["filt"]).astype(np.complex64)
+    win = _lib.host(ctx["win"]).astype(np.float32)
+    cores = cpu_port.ncores()
+
+    def run():
+        hi = (raw >> 4).astype(np.int8)
+        lo = (raw & 15).astype(np.int8)
+        hi = np.where(hi > 8, hi - 16, hi)
+        lo = np.where(lo > 8, lo - 16, lo)
+        full = np.zeros((sample_chunks * acclen, 2049), np.complex64)  # make_continuous zero fill
+        full[:, chan0:chan0 + nkeep] = hi + 1j * lo
+        chunks = [full[c * acclen:(c + 1) * acclen] for c in range(sample_chunks)]
+        return cpu_port.rechannelize(chunks, filt, win, acclen, cut, osamp, chan0 * osamp,
+                                     (chan0 + nkeep) * osamp)
+
+    run()
+    times = []
+    t_all = time.perf_counter()
+    while len(times) < 5 and (time.perf_counter() - t_all) < budget_s:
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    nspec = sample_chunks * acclen
+    return {"value": nspec * 4096 / t / 1e6, "unit": "Msamples/s", "cores": cores, "kind": "port",
+            "sample": "1 chunk of the same workload (%d coarse spectra): unpack + zero fill, cpu_ipfb, cpu_pfb "
+                      "(scipy.fft float32, workers=%d), median of %d runs" % (nspec, cores, len(times))}
+
+
+def secondary(args, torch):
+    """Direct 4-bit (2 ant x 2 pol, 128 ch, 2^17 spectra, 4096-blocks), precision
+    correlator (8 x 1024 x 6400, FP32 on MFMA) and direct 1-bit correlator."""
+    from albatros_axion_detection_amd import _lib, synth
+    out = {}
+    dev = _lib.device()
+
+    def timeit(fn, n=10):
+        fn()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(n):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / n
+
+    # direct 4-bit
+    inp = _lib.to_device(synth.direct4(T=2 ** 17, nchan=128, seed=0))
+    o4 = torch.empty((32, 10, 128, 2), dtype=torch.int64, device=dev)
+    ptrs = (_lib._vp * 4)(*[inp[i].data_ptr() for i in range(4)])
+    ms = timeit(lambda: _lib.call("alb_xcorr_4bit_allpairs", ptrs, 4, 2 ** 17, 128, 4096, _lib.ptr(o4), _lib.stream()))
+    b = 68091904
+    out["direct4"] = {"workload": "4 inputs x 2^17 spectra x 128 ch, 10 pairs, 4096-blocks", "ms": ms,
+                      "Msamples_s": 4 * 2 ** 17 * 128 / (ms * 1e-3) / 1e6,
+                      "hbm_frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    del inp, o4
+    # precision correlator, FP32 on MFMA
+    x = synth.precision_xcorr(M=8, T=1024, F=6400, seed=2, device=dev)
+    xd = x.permute(2, 1, 0)
+    vis = torch.empty((6400, 8, 8), dtype=torch.complex64, device=dev)
+    ms = timeit(lambda: _lib.call("alb_xcorr_all_ant", _lib.ptr(xd), 8, 1024, 6400, _lib.ptr(vis),
+                                  float(np.float32(1) / np.float32(1024)), _lib.stream()))
+    b = 422707200
+    out["xcorr_fp32_mfma"] = {"workload": "8 inputs x 1024 x 6400 ch complex64", "ms": ms,
+                              "Msamples_s": 8 * 1024 * 6400 / (ms * 1e-3) / 1e6,
+                              "hbm_frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    del x, xd, vis
+    # direct 1-bit (reduced T when asked)
+    T1 = args.onebit_T
+    inp = synth.direct1(ninput=16, T=T1, nchan=1024, seed=3, device=dev)
+    o1 = torch.empty((T1 // 8192, 136, 1024, 2), dtype=torch.int64, device=dev)
+    ptrs = (_lib._vp * 16)(*[inp[i].data_ptr() for i in range(16)])
+    ms = timeit(lambda: _lib.call("alb_xcorr_1bit_allpairs", ptrs, 16, T1, 256, 1024, 8192, _lib.ptr(o1),
+                                  _lib.stream()), n=2)
+    b = 16 * T1 * 256 + (T1 // 8192) * 136 * 1024 * 2 * 8
+    out["direct1_xor"] = {"workload": "16 inputs x %d spectra x 1024 ch, 136 pairs, 8192-blocks" % T1, "ms": ms,
+                          "Msamples_s": 16 * T1 * 1024 / (ms * 1e-3) / 1e6,
+                          "hbm_frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nchunks", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=30.0)
+    ap.add_argument("--onebit-T", type=int, default=2 ** 17)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X (asked for %d)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    sys.path.insert(0, HERE)
+    res, ctx = rechannelizer(args, torch)
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline ...")
+        cpu = cpu_baseline(ctx, args.cpu_budget)
+        log("[bench] CPU baseline: %.1f Msamples/s on %d cores" % (cpu["value"], cpu["cores"]))
+    sec = None
+    if not args.no_secondary:
+        log("[bench] secondary workloads ...")
+        sec = secondary(args, torch)
+    line = {
+        "metric": METRIC,
+        "value": round(res["value"], 2),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded noise + tone, coarse-PFB'd and 4-bit quantised)",
+        "config": res["config"],
+        "roofline": res["roofline"],
+        "pipeline_roofline": res["pipeline_roofline"],
+        "cpu_baseline": cpu,
+        "kernels_ms": res["kernels_ms"],
+        "ref_unit_msps": round(res["ref_unit_msps"], 1),
+        "secondary": sec,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,139 @@
+/*
+ * albatros_hip.h -- C ABI of libalbatros_hip.so, the MI355X (gfx950) hot path of
+ * the ALBATROS baseband pipeline: 4/1-bit unpack, Wiener-filtered inverse PFB,
+ * forward re-PFB (upchannelization) and per-channel cross-antenna correlation.
+ *
+ * Conventions (all entry points):
+ *   - every data pointer is a DEVICE pointer (hipMalloc / torch CUDA storage);
+ *   - sizes and strides are int64_t element counts; complex data is interleaved
+ *     float32 (re, im) -- numpy complex64 / torch.complex64;
+ *   - `stream` is a hipStream_t (NULL = default stream); work is stream ordered,
+ *     nothing synchronises the host unless stated;
+ *   - return 0 on success, 1 invalid argument, 2 HIP runtime error, 3 size not
+ *     supported; alb_last_error() gives the message of the calling thread;
+ *   - no entry point keeps a pointer to caller memory after it returns; the only
+ *     library-owned allocations are read-only twiddle tables built in float64 on
+ *     first use of a transform size (one per device and size).
+ *
+ * Each declaration cites the reference interface it replaces
+ * (paths relative to albatros_analysis/ of PhilJ0ly/albatros_axion_detection).
+ */
+#ifndef ALBATROS_HIP_H
+#define ALBATROS_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- housekeeping ------------------------------------------------------ */
+const char* alb_last_error(void);
+int alb_version(void);
+int alb_stream_synchronize(void* stream);
+/* Zero nbytes of device memory with a kernel (safe inside graph capture). */
+int alb_fill_zero(void* p, int64_t nbytes, void* stream);
+
+/* ---- unpack (src/correlations/unpacking.c, unpacking.py) --------------- */
+/* unpacking.c:155 unpack_4bit_float / unpacking.py:140 _unpack_4bit_float_gpu.
+ * data: raw spectra of 2*nchan bytes; pol0/pol1: complex64 [rowend-rowstart][ncols];
+ * channels: int64 column indices (device). */
+int alb_unpack_4bit_float(const uint8_t* data, void* pol0, void* pol1, int64_t rowstart, int64_t rowend,
+                          const int64_t* channels, int64_t ncols, int64_t nchan, void* stream);
+/* unpacking.c:207 unpack_1bit_float / unpacking.py:165 _unpack_1bit_float_gpu. */
+int alb_unpack_1bit_float(const uint8_t* data, void* pol0, void* pol1, int64_t rowstart, int64_t rowend,
+                          int64_t chanstart, int64_t chanend, int64_t nchan, void* stream);
+/* unpacking.c:248 sortpols (bit_depth 4 or 1). */
+int alb_sortpols(const uint8_t* data, uint8_t* pol0, uint8_t* pol1, int64_t rowstart, int64_t rowend, int64_t ncols,
+                 int64_t nchan, int bit_depth, int64_t chanstart, int64_t chanend, void* stream);
+/* unpacking.c:7 hist_4bit / :76 hist_1bit -> uint64 [2^bit_depth][nchan] (no write past the end). */
+int alb_hist(const uint8_t* data, uint64_t* hist, int64_t rowstart, int64_t rowend, int64_t nchan, int bit_depth,
+             int mode, void* stream);
+/* baseband_data_classes.py:48 make_continuous_gpu: zero-filled [nspec][nchans] complex64 with
+ * out[(specnum[p]-start) % nspec][channels[k]] = spec[p*spec_ld + k]. */
+int alb_make_continuous(const void* spec, int64_t spec_ld, const int64_t* specnum, int64_t npresent, int64_t start,
+                        int64_t nspec, const int64_t* channels, int64_t ncols, void* out, int64_t nchans,
+                        void* stream);
+/* unpacking_gpu.cu:24 dropped_packets: zero packet i when spec_num[i+1]-spec_num[i] != spectra_per_packet;
+ * *num_dropped (device uint64) receives the count. */
+int alb_dropped_packets(uint8_t* data, const int64_t* spec_num, int64_t num_packets, int64_t spectra_per_packet,
+                        int64_t nchan, int bit_depth, uint64_t* num_dropped, void* stream);
+
+/* ---- integer correlators (src/correlations/correlations_cpu.c) --------- */
+/* correlations_cpu.c:39 avg_autocorr_4bit (autocorr=1) and :128 avg_xcorr_4bit (autocorr=0);
+ * with match != NULL (from alb_match_rows) :188 avg_xcorr_4bit_2ant.  out: int64 [ncol][2] exact sums. */
+int alb_xcorr_4bit_pair(const uint8_t* data0, const uint8_t* data1, int64_t nrows, int64_t ncol, const int* match,
+                        int autocorr, int64_t* out, void* stream);
+/* correlations_cpu.c:201-214 spec_num merge: match[i] = row of list 1 or -1, *count = matches (device int). */
+int alb_match_rows(const int64_t* specnum0, const int64_t* specnum1, int64_t n0, int64_t n1, int64_t idxstart0,
+                   int64_t idxstart1, int* match, int* count, void* stream);
+/* correlations_cpu.c:273 avg_xcorr_1bit (vanvleck=0 -> int64 [nchan][2]) and :435 / :533
+ * avg_xcorr_1bit_vanvleck(_2ant) (vanvleck=1 -> int64 [4][nchan] = R0, R1, IM0, IM1). */
+int alb_xcorr_1bit_pair(const uint8_t* data0, const uint8_t* data1, int64_t nrows, int64_t ncol, int64_t nchan,
+                        const int* match, int vanvleck, int64_t* out, void* stream);
+/* All baselines i <= j of ninput per-pol 4-bit inputs [nrows][ncol] in one pass (the per-pair
+ * correlations_cpu.c:128 applied to every pair, as scripts/autocorravg.py:114-116 does pair by pair);
+ * out int64 [nrows/blocklen][npair][ncol][2].  inputs: HOST array of ninput device pointers. */
+int alb_xcorr_4bit_allpairs(const uint8_t* const* inputs, int ninput, int64_t nrows, int64_t ncol, int64_t blocklen,
+                            int64_t* out, void* stream);
+/* Same for sortpols 1-bit inputs (correlations_cpu.c:273 on every pair); out int64 [nblock][npair][nchan][2]. */
+int alb_xcorr_1bit_allpairs(const uint8_t* const* inputs, int ninput, int64_t nrows, int64_t ncol, int64_t nchan,
+                            int64_t blocklen, int64_t* out, void* stream);
+
+/* ---- float correlator (src/correlations/correlations_gpu.py, cgemm_batch.cu) -- */
+/* correlations_gpu.py:95 avg_xcorr_all_ant_gpu / cgemm_batch.cu:8 cgemm_strided_batched (C = A B^H):
+ * x complex64 F-order (M, K, F); out complex64 F-order (M, M, F) = x x^H * scale. */
+int alb_xcorr_all_ant(const void* x, int64_t M, int64_t K, int64_t F, void* out, float scale, void* stream);
+
+/* ---- PFB / inverse PFB (src/utils/pfb_gpu_utils.py, pycufft.py/.cu) ---- */
+/* pfb_gpu_utils.py:123 get_matft + :29 calculate_filter in float64: win float32 [ntap*lblock];
+ * matft / filt complex64 [lblock][nslice/2+1] (either may be NULL). */
+int alb_matft_filter(const float* win, int ntap, int64_t lblock, int64_t nslice, double thresh, void* matft,
+                     void* filt, void* stream);
+/* pfb_gpu_utils.py:29 calculate_filter from a given complex64 matft (float64 arithmetic). */
+int alb_filter_from_matft(const void* matft, int64_t n, double thresh, void* filt, void* stream);
+/* pycufft.py:139 irfft / pycufft.cu:159 cufft_c2r_mohan (+ the /n of pycufft.py:153), along rows. */
+int alb_irfft_rows(const void* in, int64_t ldi, float* out, int64_t ldo, int64_t nrows, int64_t n, void* stream);
+/* pycufft.py:120 rfft / pycufft.cu:68 cufft_r2c_mohan along rows; bins [c0, c1) written. */
+int alb_rfft_rows(const float* in, int64_t ldi, void* out, int64_t ldo, int64_t nrows, int64_t n, int64_t c0,
+                  int64_t c1, void* stream);
+/* pycufft.py:92 _fft / pycufft.cu:122 cufft_c2c along rows; direction -1 forward, +1 inverse (with 1/n). */
+int alb_fft_rows(const void* in, int64_t ldi, void* out, int64_t ldo, int64_t nrows, int64_t n, int direction,
+                 void* stream);
+/* pfb_gpu_utils.py:105 cupy_pfb for lblock <= 16384 (coarse PFB): channels [c0, c1) of
+ * (nblock, lblock/2+1) written with row stride ldo. */
+int alb_pfb_small(const float* ts, int64_t nblock, int64_t lblock, const float* win, int ntap, int64_t c0, int64_t c1,
+                  void* out, int64_t ldo, void* stream);
+/* pfb_gpu_utils.py:84-91 (the column half of cupy_ipfb) over ntiles tiles; kept rows
+ * [cut, nslice-cut) written (helper_gpu_stream.py:199). */
+int alb_ipfb_columns(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, int64_t nslice, int64_t ncols,
+                     const void* filt, int64_t ldf, int cut, float* out, int64_t ldo, int64_t out_tile_stride,
+                     void* stream);
+/* Row half of the streaming iPFB with the 4-bit unpack and spec_num zero fill fused in
+ * (unpacking.py:140 + baseband_data_classes.py:48 + pfb_gpu_utils.py:80, helper_gpu_stream.py:195-199). */
+int alb_ipfb_rows_packed4(const uint8_t* bytes, const int* rowmap, const int* chan, int64_t ncols, int64_t acclen,
+                          int cut, int64_t tile0, int64_t ntiles, float* out, void* stream);
+/* pfb_gpu_utils.py:105 cupy_pfb with lblock = 4096*osamp (re-PFB, helper_gpu_stream.py:222):
+ * fine channels [c_lo, c_hi) of nblock spectra; spectrum b, channel c goes to
+ * out[(b/bpj)*sj + (b%bpj)*sb + (c-c_lo)*sc] (bpj = blocks per re-PFB job, so all jobs of a
+ * stream land in the correlator's F-order (M, nblock, nchan, njobs) layout); A is workspace of
+ * alb_repfb_workspace_bytes(nblock, osamp) bytes. */
+int alb_repfb(const float* ts, const float* win, int ntap, int64_t nblock, int64_t osamp, void* A, int64_t c_lo,
+              int64_t c_hi, void* out, int64_t sb, int64_t sc, int64_t bpj, int64_t sj, void* stream);
+int64_t alb_repfb_workspace_bytes(int64_t nblock, int64_t osamp);
+/* The two halves of alb_repfb (same arguments): polyphase window + osamp-point DFT down each
+ * column into A, then twiddle + 4096-point FFT along the columns into the fine channels. */
+int alb_repfb_stage1(const float* ts, const float* win, int ntap, int64_t nblock, int64_t osamp, void* A,
+                     void* stream);
+int alb_repfb_stage2(const void* A, int64_t nblock, int64_t osamp, int64_t c_lo, int64_t c_hi, void* out, int64_t sb,
+                     int64_t sc, int64_t bpj, int64_t sj, void* stream);
+
+/* ---- streaming helpers (scripts/xcorr/helper_gpu_stream.py) ------------ */
+/* Row map of one chunk for the fused iPFB: rowmap[(specnum[p]-start) % acclen] = p, -1 elsewhere
+ * (baseband_data_classes.py:53 modular row of make_continuous_gpu). */
+int alb_build_rowmap(const int64_t* specnum, int64_t npresent, int64_t start, int64_t acclen, int* rowmap,
+                     void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ALBATROS_HIP_H */

@@ -1,0 +1,179 @@
+"""GPU parity of the FFTs, window spectrum / Wiener filter, inverse PFB,
+forward PFB / re-PFB and the streaming pipeline against the float64 oracle.
+
+Tolerance (stated per test): ``max|gpu - ref| / rms(ref)`` for transforms,
+with the float32 FFT error budget ~ log2(n) * 6e-8; 1e-5 is the bar of the
+path (the reference's own CPU-vs-GPU float32 envelope is 1e-7..2e-4 absolute,
+src/utils/test_results/out_comparison_test.out:78-186).
+"""
+import numpy as np
+import pytest
+
+from oracle import pfb as opfb
+from oracle import stream as ostream
+from tests.conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def pu():
+    from albatros_axion_detection_amd import pfb
+    return pfb
+
+
+@pytest.mark.parametrize("n", [64, 512, 2048, 4096, 16384])
+def test_rfft_irfft_rows(pu, rng, n):
+    x = rng.standard_normal((33, n)).astype(np.float32)
+    got = pu.rfft(x, axis=1)
+    ref = np.fft.rfft(x.astype(np.float64), axis=1)
+    assert got.shape == ref.shape and rel_err(got, ref) < TOL
+    X = (rng.standard_normal((33, n // 2 + 1)) + 1j * rng.standard_normal((33, n // 2 + 1))).astype(np.complex64)
+    got = pu.irfft(X, axis=1)
+    ref = np.fft.irfft(X.astype(np.complex128), axis=1)  # ignores Im of DC / Nyquist
+    assert got.shape == (33, n) and rel_err(got, ref) < TOL
+
+
+def test_rfft_axis0_and_c2c(pu, rng):
+    x = rng.standard_normal((256, 7)).astype(np.float32)
+    got = pu.rfft(x, axis=0)
+    assert rel_err(got, np.fft.rfft(x.astype(np.float64), axis=0)) < TOL
+    z = (rng.standard_normal((9, 1024)) + 1j * rng.standard_normal((9, 1024))).astype(np.complex64)
+    assert rel_err(pu.fft(z), np.fft.fft(z.astype(np.complex128), axis=1)) < TOL
+    assert rel_err(pu.ifft(z), np.fft.ifft(z.astype(np.complex128), axis=1)) < TOL
+    # brute-force DFT pins the sign convention at a tiny size
+    assert rel_err(pu.fft(z[:, :16]), opfb.dft_brute(z[:, :16])) < TOL
+
+
+def test_matft_and_filter(pu):
+    for nslice in (64, 4096):
+        got = pu.get_matft(nslice).cpu().numpy()
+        ref = opfb.matft_fp64(nslice)
+        assert got.shape == ref.shape and np.max(np.abs(got - ref)) < 1e-6
+        f = pu.get_filter(nslice, 0.45).cpu().numpy()
+        fref = opfb.wiener_filter(ref, 0.45)
+        assert np.max(np.abs(f - fref)) < 1e-6
+        f2 = pu.calculate_filter(pu.get_matft(nslice), 0.45).cpu().numpy()
+        assert np.max(np.abs(f2 - fref)) < 1e-5
+    f0 = pu.calculate_filter(opfb.matft_fp64(64).astype(np.complex64), 0.0)
+    assert np.allclose(f0 * np.conj(opfb.matft_fp64(64)), 1.0, atol=1e-5)
+
+
+@pytest.mark.parametrize("nslice", [64, 1024, 4096])
+def test_ipfb(pu, rng, nslice):
+    dat = (rng.standard_normal((nslice, 2049)) + 1j * rng.standard_normal((nslice, 2049))).astype(np.complex64)
+    filt = opfb.wiener_filter(opfb.matft_fp64(nslice), 0.45)
+    got = pu.cupy_ipfb(dat, filt.astype(np.complex64))
+    ref = opfb.ipfb(dat, filt)
+    assert got.shape == (nslice, 4096) and rel_err(got, ref) < TOL
+    # stale (dat, matft, thresh) form of scripts/ipfb/upchannelize.py:94
+    got2 = pu.cupy_ipfb(dat, pu.get_matft(nslice), thresh=0.45)
+    assert rel_err(got2, ref) < 3 * TOL
+
+
+def test_ipfb_inverts_pfb(pu, rng):
+    # thresh 0 on a circulant stream: exact inversion of the forward PFB
+    nslice = 256
+    ts = rng.standard_normal(nslice * 4096)
+    ext = np.concatenate([ts, ts[: 3 * 4096]]).astype(np.float32)
+    win = opfb.window_f32(4, 4096)
+    spec = pu.cupy_pfb(ext, win)
+    assert spec.shape == (nslice, 2049)
+    assert rel_err(spec, opfb.pfb(ext, win)) < TOL
+    rec = pu.cupy_ipfb(spec, pu.get_filter(nslice, 0.0))
+    assert rel_err(rec.ravel(), ts) < 1e-4
+
+
+@pytest.mark.parametrize("osamp", [1, 4, 64])
+def test_repfb_channels(pu, rng, osamp):
+    L = 4096 * osamp
+    nblock = 5
+    ts = rng.standard_normal((nblock + 3) * L).astype(np.float32)
+    win = opfb.window_f32(4, L)
+    nchan = 2048 * osamp + 1
+    c0, c1 = 64 * osamp, 164 * osamp
+    ref = opfb.pfb(ts, win, nchan, 4)[:, c0:c1]
+    for method in (("repfb", "small") if osamp <= 4 else ("repfb",)):
+        got = pu.pfb_channels(ts, win, nchan, 4, c0, c1, method=method).cpu().numpy()
+        assert got.shape == ref.shape and rel_err(got, ref) < TOL, method
+    full = pu.cupy_pfb(ts, win, nchan=nchan)
+    assert rel_err(full, opfb.pfb(ts, win, nchan, 4)) < TOL
+
+
+def _packed_chunks(rng, nchunks, acclen, ncols, loss=0.1, start=1000):
+    """Per-chunk packed 4-bit rows with a fraction of spectra missing."""
+    chunks = []
+    for c in range(nchunks):
+        keep = np.sort(rng.choice(acclen, int(acclen * (1 - loss)), replace=False))
+        raw = rng.integers(0, 256, size=(acclen, ncols), dtype=np.uint8)
+        raw[len(keep):] = 0
+        chunks.append({"pol0": raw, "specnums": start + c * acclen + keep})
+    return chunks
+
+
+def _decode(chunks):
+    from oracle import unpack as ounpack
+    out = []
+    for ch in chunks:
+        r, i = ounpack.decode_4bit_bytes(ch["pol0"])
+        out.append({"pol0": r + 1j * i, "specnums": ch["specnums"]})
+    return out
+
+
+def test_stream_spectra_single_input(rng):
+    # fused unpack + zero fill + iPFB over chunks + re-PFB vs the oracle stream
+    from albatros_axion_detection_amd import repfb
+    acclen, cut, osamp, nblock = 120, 4, 4, 6
+    chan = np.arange(64, 164)
+    nchunks = 5
+    chunks = _packed_chunks(rng, nchunks, acclen, len(chan))
+    pipe = repfb.RePFBPipeline(1, chan, acclen, cut, osamp, nblock, 64 * osamp, 164 * osamp)
+    rm = repfb.rowmaps_from_specnums([c["specnums"] for c in chunks], 1000, acclen)
+    packed = repfb.stack_packed(chunks, "pol0", acclen, len(chan))
+    from albatros_axion_detection_amd import _lib
+    got = pipe.spectra(_lib.to_device(packed), rm, nchunks).cpu().numpy()
+    filt = opfb.wiener_filter(opfb.matft_fp64(acclen + 2 * cut), 0.45)
+    st = ostream.ipfb_stream(_decode(chunks), 1000, chan, acclen, cut, filt, 0)
+    jobs = ostream.get_chunkier(nchunks, 4096 * acclen, nblock, 4096 * osamp, 4)
+    need = (len(jobs) * nblock + 3) * 4096 * osamp
+    buf = np.zeros(max(need, st.size))
+    buf[: st.size] = st
+    win = opfb.window_f32(4, 4096 * osamp)
+    ref = opfb.pfb(buf[:need], win, 2048 * osamp + 1, 4)[:, 64 * osamp:164 * osamp]
+    assert got.shape == ref.shape and rel_err(got, ref) < TOL
+
+
+def test_stream_xcorr_two_antennas(rng):
+    from albatros_axion_detection_amd import repfb
+    acclen, cut, osamp, nblock = 96, 4, 2, 4
+    chan = np.arange(200, 230)
+    nchunks = 4
+    chunks = []
+    for j in range(2):
+        cj = _packed_chunks(rng, nchunks, acclen, len(chan), loss=0.05 * (j + 1), start=500)
+        for c in cj:
+            r1 = rng.integers(0, 256, size=c["pol0"].shape, dtype=np.uint8)
+            r1[len(c["specnums"]):] = 0
+            c["pol1"] = r1
+        chunks.append(cj)
+    vis, miss, freqs, win, filt = repfb.repfb_xcorr_avg_chunks(chunks, [500, 500], chan, 200 * osamp, 230 * osamp,
+                                                              acclen, nchunks, nblock, osamp, cut=cut)
+    dec = []
+    for cj in chunks:
+        dd = []
+        for c in cj:
+            from oracle import unpack as ounpack
+            r0, i0 = ounpack.decode_4bit_bytes(c["pol0"])
+            r1, i1 = ounpack.decode_4bit_bytes(c["pol1"])
+            dd.append({"pol0": r0 + 1j * i0, "pol1": r1 + 1j * i1, "specnums": c["specnums"]})
+        dec.append(dd)
+    filt64 = opfb.wiener_filter(opfb.matft_fp64(acclen + 2 * cut), 0.45)
+    rvis, rmiss, rfreqs = ostream.repfb_xcorr_avg(dec, [500, 500], chan, 200 * osamp, 230 * osamp, acclen, nchunks,
+                                                  nblock, osamp, cut, filt64, opfb.window_f32(4, 4096 * osamp))
+    assert vis.shape == rvis.shape
+    d = np.sqrt(np.abs(np.einsum("iifs->ifs", rvis)))
+    norm = d[:, None] * d[None, :]
+    assert np.max(np.abs(np.asarray(vis) - rvis) / norm) < 1e-4
+    assert np.allclose(miss, rmiss, equal_nan=True)
+    assert np.array_equal(freqs, rfreqs)

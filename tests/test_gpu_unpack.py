@@ -1,0 +1,145 @@
+"""GPU parity of the unpack family against the oracle (bit exact)."""
+import numpy as np
+import pytest
+
+from oracle import ref as oref
+from oracle import stream as ostream
+from oracle import unpack as ounpack
+from tests import fixtures as fx
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def unpk():
+    from albatros_axion_detection_amd import unpacking
+    return unpacking
+
+
+def test_unpack_4bit_reference_fixture(unpk):
+    packed = fx.packed_4bit()
+    t0, t1 = fx.expected_complex_4bit()
+    p0, p1 = unpk.unpack_4bit(packed, 0, 2, np.arange(0, 10), 10)
+    assert p0.dtype == np.complex64 and p0.shape == t0.shape
+    assert np.array_equal(p0, t0) and np.array_equal(p1, t1)
+    p0, p1 = unpk.unpack_4bit(packed, 0, 2, np.arange(3, 4), 10)
+    assert np.array_equal(p0, t0[0:2, 3:4]) and np.array_equal(p1, t1[0:2, 3:4])
+    p0, p1 = unpk.unpack_4bit(packed, 0, 2, np.asarray([2, 5], dtype="int64"), 10)
+    assert np.array_equal(p0, t0[0:2, [2, 5]]) and np.array_equal(p1, t1[0:2, [2, 5]])
+
+
+def test_unpack_4bit_exhaustive_and_random(unpk, rng):
+    # every byte value in both pols, then a random block with a channel gather
+    allb = np.arange(256, dtype=np.uint8)
+    raw = np.stack([allb, allb[::-1]], axis=1).reshape(1, -1)  # 256 channels, one spectrum
+    p0, p1 = unpk.unpack_4bit(raw, 0, 1, np.arange(256), 256)
+    e0, e1 = ounpack.unpack_4bit(raw, 0, 1, np.arange(256), 256)
+    assert np.array_equal(p0, e0) and np.array_equal(p1, e1)
+    nchan = 2049
+    raw = rng.integers(0, 256, size=(333, 2 * nchan), dtype=np.uint8)
+    ch = np.sort(rng.choice(nchan, 300, replace=False)).astype(np.int64)
+    p0, p1 = unpk.unpack_4bit(raw, 17, 300, ch, nchan)
+    e0, e1 = ounpack.unpack_4bit(raw, 17, 300, ch, nchan)
+    assert np.array_equal(p0, e0) and np.array_equal(p1, e1)
+    # empty row range
+    p0, _ = unpk.unpack_4bit(raw, 5, 5, ch, nchan)
+    assert p0.shape == (0, len(ch))
+
+
+def test_unpack_1bit(unpk, rng):
+    packed = fx.packed_1bit()
+    t0, t1 = fx.expected_complex_1bit()
+    p0, p1 = unpk.unpack_1bit(packed, 0, 2, np.arange(2, 6), 8)
+    assert np.array_equal(p0, t0[:, 2:6]) and np.array_equal(p1, t1[:, 2:6])
+    p0, p1 = unpk.unpack_1bit(packed, 1, 2, np.arange(2, 4), 8)
+    assert p0[0, 1] == 1 + 1j and p1[0, 1] == -1 - 1j
+    raw = rng.integers(0, 256, size=(100, 512), dtype=np.uint8)
+    p0, p1 = unpk.unpack_1bit(raw, 3, 99, np.arange(64, 900), 1024)
+    e0, e1 = ounpack.unpack_1bit(raw, 3, 99, 64, 900, 1024)
+    assert np.array_equal(p0, e0) and np.array_equal(p1, e1)
+    with pytest.raises(ValueError):
+        unpk.unpack_1bit(raw, 0, 2, np.array([2, 5]), 1024)
+
+
+def test_sortpols_all_tails(unpk, rng):
+    packed = fx.packed_1bit()
+    p0, p1 = unpk.sortpols(packed, 8, 1, 0, 2, 2, 7)
+    assert p0[1, 0] == fx.str_to_bits("01110101") and p0[0, 0] == fx.str_to_bits("10101010")
+    assert p0[1, 1] == fx.str_to_bits("01000000") and p1[1, 0] == fx.str_to_bits("10001010")
+    assert p1[0, 0] == fx.str_to_bits("01010101") and p1[1, 1] == fx.str_to_bits("10000000")
+    a1 = fx.packed_1bit_ant(fx.ant_1bit(im_sign=1))
+    a2 = fx.packed_1bit_ant(fx.ant_1bit(im_sign=-1))
+    p0, p1 = unpk.sortpols(a1, 8, 1, 5, 6, 2, 3)
+    assert p0[0, 0] == 192 and p1[0, 0] == 192
+    p0, p1 = unpk.sortpols(a2, 8, 1, 9, 10, 4, 7)
+    assert p0[0, 0] == 168 and p1[0, 0] == 168
+    raw = rng.integers(0, 256, size=(200, 512), dtype=np.uint8)
+    for cs, ce in [(0, 1024), (2, 1023), (4, 1022), (6, 1021), (0, 1), (10, 13)]:
+        g = unpk.sortpols(raw, 1024, 1, 7, 190, cs, ce)
+        e = ounpack.sortpols(raw, 1024, 1, 7, 190, cs, ce)
+        assert np.array_equal(g[0], e[0]) and np.array_equal(g[1], e[1]), (cs, ce)
+        if oref.available():
+            r = oref.sortpols(raw, 1024, 1, 7, 190, cs, ce)
+            assert np.array_equal(g[0], r[0]) and np.array_equal(g[1], r[1])
+    raw4 = rng.integers(0, 256, size=(200, 2 * 300), dtype=np.uint8)
+    g = unpk.sortpols(raw4, 300, 4, 3, 199, 10, 250)
+    e = ounpack.sortpols(raw4, 300, 4, 3, 199, 10, 250)
+    assert np.array_equal(g[0], e[0]) and np.array_equal(g[1], e[1])
+    with pytest.raises(ValueError):
+        unpk.sortpols(raw4, 300, 4, 0, 10, 0, 301)
+    with pytest.raises(ValueError):
+        unpk.sortpols(raw, 1024, 1, 0, 10, 1, 9)
+
+
+def test_hist(unpk, rng):
+    packed = fx.packed_4bit()
+    h = unpk.hist(packed, 0, 2, 10, 4, 0)
+    assert h[0, 3] == 1 and h[4, 3] == 1
+    h = unpk.hist(packed, 0, 2, 10, 4, 1)
+    assert h[0, 3] == 1 and h[12, 3] == 1
+    p1 = fx.packed_1bit()
+    h = unpk.hist(p1, 0, 2, 8, 1, -1)
+    assert h[0, 3] == 4 and h[1, 3] == 4 and h[0, 4] == 4
+    raw = rng.integers(0, 256, size=(700, 2 * 61), dtype=np.uint8)
+    for mode in (0, 1, -1):
+        assert np.array_equal(unpk.hist(raw, 3, 650, 61, 4, mode), ounpack.hist_4bit(raw, 3, 650, 61, mode))
+    raw1 = rng.integers(0, 256, size=(700, 32), dtype=np.uint8)
+    for mode in (0, 1, -1):
+        assert np.array_equal(unpk.hist(raw1, 3, 650, 64, 1, mode), ounpack.hist_1bit(raw1, 3, 650, 64, mode))
+
+
+def test_make_continuous(rng):
+    from albatros_axion_detection_amd import baseband_data_classes as bdc
+    nspec, nchans, channels, specnum = fx.make_continuous_case()
+    spec = rng.standard_normal((len(specnum), len(channels))).astype(np.complex64)
+    out = bdc.make_continuous_gpu(spec, specnum, 0, channels, nspec, nchans)
+    assert out[10, 100] == spec[0, 0] and out[50, 200] == spec[4, 1]
+    assert out[49, 200] == 0 and out[50, 199] == 0 and out[9, 100] == 0 and out[10, 101] == 0
+    ones = np.ones((len(specnum), len(channels)), np.complex64) * (1 + 1j)
+    out = bdc.make_continuous_gpu(ones, specnum, 0, channels, nspec, nchans)
+    assert out.sum() == 10 + 10j and out.sum(axis=0)[100] == 5 + 5j
+    # modular rows and packed-at-top semantics vs the oracle
+    acclen = 64
+    sn = np.sort(rng.choice(acclen, 40, replace=False)) + 1000
+    sp = np.zeros((acclen, 30), np.complex64)
+    sp[:40] = rng.standard_normal((40, 30)) + 1j * rng.standard_normal((40, 30))
+    ch = np.arange(500, 530)
+    g = bdc.make_continuous_gpu(sp, sn, 1000 - 3 * acclen, ch, acclen, 2049)
+    e = ostream.make_continuous(sp, sn, 1000 - 3 * acclen, ch, acclen, 2049)
+    assert np.array_equal(g, e.astype(np.complex64))
+
+
+def test_dropped_packets(rng):
+    import torch
+    from albatros_axion_detection_amd import _lib
+    spp, nchan, npk = 5, 4, 7
+    data = rng.integers(1, 256, size=(npk, 2 * nchan * spp), dtype=np.uint8)
+    spec = np.array([0, 5, 10, 20, 25, 30, 35], dtype=np.int64)  # jump after packet 2
+    d = _lib.to_device(data)
+    s = _lib.to_device(spec)
+    nd = torch.empty(1, dtype=torch.int64, device=d.device)
+    _lib.call("alb_dropped_packets", _lib.ptr(d), _lib.ptr(s), npk, spp, nchan, 4, _lib.ptr(nd), _lib.stream())
+    out = _lib.host(d)
+    exp = data.copy()
+    exp[2] = 0
+    assert np.array_equal(out, exp) and int(nd.item()) == 1

@@ -1,0 +1,59 @@
+"""Host-side logic of the package against the oracle (CPU only)."""
+import os
+import struct
+
+import numpy as np
+
+from oracle import stream as ostream
+
+
+def test_chunk_planner_matches_oracle():
+    from albatros_axion_detection_amd import repfb
+    for args in [(4, 10, 2, 5, 4), (3, 100, 2, 10, 4), (64, 4096 * 4088, 3, 4096 * 64, 4), (0, 1, 1, 1, 4),
+                 (17, 4096 * 2048, 3, 4096 * 65536, 4)]:
+        assert repfb.get_chunkier(*args) == ostream.get_chunkier(*args)
+
+
+def test_missing_fraction_matches_oracle():
+    from albatros_axion_detection_amd import repfb
+    rng = np.random.default_rng(0)
+    acclen, nblock, osamp, nchunks = 50, 3, 2, 9
+    lblock = 4096 * osamp
+    lchunk = 4096 * acclen
+    chunks = [[{"specnums": np.arange(rng.integers(20, acclen + 1))} for _ in range(nchunks)] for _ in range(2)]
+    jobs = ostream.get_chunkier(nchunks, lchunk, nblock, lblock, 4)
+    ref = ostream.missing_fraction(chunks, acclen, nchunks, nblock, lblock, lchunk, 4, jobs)
+    npres = [[len(c["specnums"]) for c in cj] for cj in chunks]
+    got = repfb.missing_fraction(npres, acclen, nblock, lblock, lchunk, 4, jobs)
+    assert np.allclose(got, ref, equal_nan=True)
+
+
+def _write_raw(path, chans, spp, specnums, bits=4, missing=()):
+    """A .raw file in the format of generate_test_baseband.py:52-152."""
+    bps = len(chans)
+    bpp = bps * spp + 4
+    hdr = np.asarray([8 * 10 + 8 * len(chans), bpp, len(chans), spp, bits, 0], dtype=">Q")
+    with open(path, "wb") as f:
+        hdr.tofile(f)
+        np.asarray(chans, dtype=">Q").tofile(f)
+        np.asarray([0, 1234567], dtype=">Q").tofile(f)
+        np.asarray([1.0, 2.0, 3.0], dtype=">d").tofile(f)
+        for p, s in enumerate(specnums):
+            if p in missing:
+                continue
+            spectrum = np.ones(bps, dtype="u1")
+            spectrum[2:] = spectrum[2:] << 4
+            np.asarray((int(s) % 2 ** 32, np.tile(spectrum, spp)),
+                       dtype=[("spec_num", ">I"), ("spectra", "%dB" % (bpp - 4))]).tofile(f)
+
+
+def test_header_parse(tmp_path):
+    from albatros_axion_detection_amd import baseband_data_classes as bdc
+    chans = np.ravel(np.column_stack((np.arange(452, 454), np.arange(452, 454))))
+    p = str(tmp_path / "a.raw")
+    _write_raw(p, chans, 5, [2 ** 32 - 10 + 5 * i for i in range(7)])
+    h = bdc.get_header(p)
+    assert h["length_channels"] == 2 and h["spectra_per_packet"] == 5 and h["bit_mode"] == 4
+    assert list(h["channels"]) == [452, 453]
+    assert h["gps_timestamp"] == 1234567 and h["gps_elevation"] == 3.0
+    assert h["num_packets"] == 7
