@@ -146,10 +146,22 @@ __device__ __forceinline__ void fft_pass(float2* lds, const float2* A, const flo
     if (TOT % T == 0 || b < TOT) {
       const int f = b / NB, j = b % NB;
       if constexpr (NS > 1) {
+        // W^(r*base) for r = 1..R-1 from the looked-up powers W^base, W^2base,
+        // W^4base, W^8base (products of at most four factors)
         constexpr int STEP = N / (NS * R);
-        const int jm = j % NS;
+        const int base = ((j % NS) * STEP) << tsh;
+        float2 wp[4];
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[t][r] = cmul(v[t][r], tw_lookup(A, B, lb, (jm * r * STEP) << tsh));
+        for (int e = 0; (1 << e) < R; ++e) wp[e] = tw_lookup(A, B, lb, base << e);
+        float2 w[R];
+        w[0] = make_float2(1.f, 0.f);
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          const int hi = 31 - __builtin_clz(r);  // highest set bit (compile time after unrolling)
+          w[r] = (r == (1 << hi)) ? wp[hi] : cmul(w[r - (1 << hi)], wp[hi]);
+        }
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[t][r] = cmul(v[t][r], w[r]);
       }
       DFT<R>::run(v[t]);
       const int d = (j / NS) * NS * R + (j % NS);
@@ -168,13 +180,15 @@ struct PickRadix {
 
 // Forward FFT (unnormalised, natural order) of F sequences of length N in LDS.
 // The twiddle tables may belong to a larger power-of-two size N << tsh.
+// Passes run while the sub-transform size is below NSE (NSE < N stops early:
+// the caller finishes the last pass itself, e.g. for only some output bins).
 // The caller synchronises before the call; the call ends with a barrier.
-template <int N, int F, int T, int NS = 1>
+template <int N, int F, int T, int NS = 1, int NSE = N>
 __device__ __forceinline__ void fft_lds(float2* lds, const float2* A, const float2* B, int lb, int tsh, int tid) {
-  if constexpr (NS < N) {
+  if constexpr (NS < NSE) {
     constexpr int R = PickRadix<N / NS>::value;
     fft_pass<N, R, NS, F, T>(lds, A, B, lb, tsh, tid);
-    fft_lds<N, F, T, NS * R>(lds, A, B, lb, tsh, tid);
+    fft_lds<N, F, T, NS * R, NSE>(lds, A, B, lb, tsh, tid);
   }
 }
 

@@ -55,17 +55,13 @@ __global__ void matft_filter_kernel(const float* __restrict__ win, int ntap, int
 struct DenseRows {
   const float2* x;
   int64_t ld;
-  __device__ __forceinline__ void load(int64_t row, float2* X, int M, int tid, int nthr) const {
-    const float2* p = x + row * ld;
-    for (int c = tid; c <= M; c += nthr) X[c] = p[c];
-  }
 };
 
 // 4-bit packed rows of an iPFB tile.  Tile t, row r < 2*cut comes from chunk
 // t-1 row acclen-2*cut+r (zeros for t == 0), row r >= 2*cut from chunk t row
-// r-2*cut.  rowmap[chunk][row] = packed row index or -1 (zero-filled spectrum).
-// Packed byte of (chunk, p, col) at bytes[(chunk*acclen + p)*ncols + col];
-// value goes to channel number chan[col].
+// r-2*cut.  rowmap[chunk][row] = row p of the packed byte array [rows][ncols]
+// holding that spectrum, or -1 (zero-filled spectrum); byte (p, col) goes to
+// channel number chan[col].
 struct Packed4Rows {
   const uint8_t* bytes;
   const int* rowmap;
@@ -76,69 +72,81 @@ struct Packed4Rows {
 __device__ __forceinline__ int nib4f(int v) { return v > 8 ? v - 16 : v; }
 
 // -------------------------------------------------------- irfft along rows
-// F rows per workgroup, N = 2M real outputs per row.
+// F rows per workgroup, N = 2M real outputs per row.  The packed half spectrum
+// Z[m] = (X[m] + conj X[M-m]) + i conj(W_N^m) (X[m] - conj X[M-m]) is formed
+// straight into LDS (stored conjugated, so a forward FFT gives the inverse),
+// then y[2n] + i y[2n+1] = IFFT_M(Z)[n] / N.
 template <int M, int F, int MODE>
 __global__ __launch_bounds__(kTPB) void irfft_rows_kernel(DenseRows dense, Packed4Rows pk, int64_t nrows, int64_t nslice,
                                                           float* __restrict__ out, int64_t ldo, Twiddle tw) {
   constexpr int N = 2 * M;
   constexpr int MP = Padded<M>::value;
   extern __shared__ float2 smem[];
-  float2* X = smem;                       // F x (M+1)
-  float2* Z = X + F * (M + 1);            // F x MP
+  float2* Z = smem;                       // F x MP
   float2* TA = Z + F * MP;
   float2* TB = TA + tw.nA;
   const int tid = threadIdx.x;
   load_twiddle(tw, TA, TB, tid, kTPB);
   const int64_t row0 = int64_t(blockIdx.x) * F;
-  // ---- load X
-  for (int f = 0; f < F; ++f) {
-    const int64_t row = row0 + f;
-    float2* Xf = X + f * (M + 1);
-    if (row >= nrows) {
-      for (int c = tid; c <= M; c += kTPB) Xf[c] = make_float2(0.f, 0.f);
-      continue;
+  if constexpr (MODE == 0) {
+    __syncthreads();
+    for (int idx = tid; idx < F * (M / 2 + 1); idx += kTPB) {
+      const int f = idx / (M / 2 + 1), m = idx - f * (M / 2 + 1);
+      const int64_t row = row0 + f;
+      float2* Zf = Z + f * MP;
+      float2 a = make_float2(0.f, 0.f), b = a;
+      if (row < nrows) {
+        const float2* X = dense.x + row * dense.ld;
+        a = X[m];
+        b = X[M - m];
+      }
+      if (m == 0) { a.y = 0.f; b.y = 0.f; }
+      const float2 w = tw_lookup(TA, TB, tw.lb, m);           // W_N^m
+      const float2 s = make_float2(a.x + b.x, a.y - b.y);     // a + conj b
+      const float2 d = make_float2(a.x - b.x, a.y + b.y);     // a - conj b
+      const float2 cd = cmulc(d, w);                          // d * conj(W^m)
+      Zf[pad16(m)] = make_float2(s.x - cd.y, -(s.y + cd.x));  // conj(s + i cd)
+      if (m != 0 && 2 * m != M) {
+        const float2 s2 = make_float2(b.x + a.x, b.y - a.y);  // b + conj a
+        const float2 d2 = make_float2(b.x - a.x, b.y + a.y);  // b - conj a
+        const float2 wd = cmul(d2, w);                        // W^m d2
+        Zf[pad16(M - m)] = make_float2(s2.x + wd.y, -(s2.y - wd.x));  // conj(s2 - i W^m d2)
+      }
     }
-    if constexpr (MODE == 0) {
-      dense.load(row, Xf, M, tid, kTPB);
-    } else {
-      for (int c = tid; c <= M; c += kTPB) Xf[c] = make_float2(0.f, 0.f);
-      __syncthreads();
+  } else {
+    for (int idx = tid; idx < F * MP; idx += kTPB) Z[idx] = make_float2(0.f, 0.f);
+    __syncthreads();
+    // channel ch contributes x (1 + i conj W^ch) to Z[ch] and conj(x) (1 + i W^ch) to Z[M-ch]
+    for (int64_t idx = tid; idx < F * pk.ncols; idx += kTPB) {
+      const int f = int(idx / pk.ncols);
+      const int64_t c = idx - int64_t(f) * pk.ncols;
+      const int64_t row = row0 + f;
+      if (row >= nrows) continue;
       const int64_t tile = row / nslice, r = row - tile * nslice;
       int64_t chunk, crow;
       if (r < pk.cut2) { chunk = pk.tile0 + tile - 1; crow = pk.acclen - pk.cut2 + r; }
       else { chunk = pk.tile0 + tile; crow = r - pk.cut2; }
-      if (chunk >= 0) {
-        const int p = pk.rowmap[chunk * pk.acclen + crow];
-        if (p >= 0) {
-          const uint8_t* src = pk.bytes + (chunk * pk.acclen + p) * pk.ncols;
-          for (int64_t c = tid; c < pk.ncols; c += kTPB) {
-            const int b = src[c];
-            Xf[pk.chan[c]] = make_float2(float(nib4f(b >> 4)), float(nib4f(b & 15)));
-          }
-        }
+      if (chunk < 0) continue;
+      const int p = pk.rowmap[chunk * pk.acclen + crow];
+      if (p < 0) continue;
+      const int byte = pk.bytes[int64_t(p) * pk.ncols + c];
+      const int ch = pk.chan[c];
+      float2 x = make_float2(float(nib4f(byte >> 4)), float(nib4f(byte & 15)));
+      if (ch == 0 || ch == M) x.y = 0.f;
+      const float2 w = tw_lookup(TA, TB, tw.lb, ch);
+      float* Zf = reinterpret_cast<float*>(Z + f * MP);
+      if (ch < M) {
+        const float2 g = make_float2(1.f + w.y, w.x);             // 1 + i conj(w)
+        const float2 v = cmul(x, g);
+        atomicAdd(&Zf[2 * pad16(ch)], v.x);
+        atomicAdd(&Zf[2 * pad16(ch) + 1], -v.y);
       }
-    }
-  }
-  __syncthreads();
-  // ---- Z[m] = (a + conj b) + i conj(W^m)(a - conj b), stored conjugated
-  for (int idx = tid; idx < F * (M / 2 + 1); idx += kTPB) {
-    const int f = idx / (M / 2 + 1), m = idx - f * (M / 2 + 1);
-    const float2* Xf = X + f * (M + 1);
-    float2* Zf = Z + f * MP;
-    float2 a = Xf[m], b = Xf[M - m];
-    if (m == 0) { a.y = 0.f; b.y = 0.f; }
-    const float2 w = tw_lookup(TA, TB, tw.lb, m);           // W_N^m
-    const float2 s = make_float2(a.x + b.x, a.y - b.y);     // a + conj b
-    const float2 d = make_float2(a.x - b.x, a.y + b.y);     // a - conj b
-    const float2 cd = cmulc(d, w);                          // d * conj(W^m)
-    const float2 zm = make_float2(s.x - cd.y, s.y + cd.x);  // s + i cd
-    Zf[pad16(m)] = cconj(zm);
-    if (m != 0 && 2 * m != M) {
-      const float2 s2 = make_float2(b.x + a.x, b.y - a.y);  // b + conj a
-      const float2 d2 = make_float2(b.x - a.x, b.y + a.y);  // b - conj a
-      const float2 wd = cmul(d2, w);                        // W^m d2
-      const float2 zk = make_float2(s2.x + wd.y, s2.y - wd.x);  // s2 - i W^m d2
-      Zf[pad16(M - m)] = cconj(zk);
+      if (ch > 0) {
+        const float2 g = make_float2(1.f - w.y, w.x);             // 1 + i w
+        const float2 v = cmul(cconj(x), g);
+        atomicAdd(&Zf[2 * pad16(M - ch)], v.x);
+        atomicAdd(&Zf[2 * pad16(M - ch) + 1], -v.y);
+      }
     }
   }
   __syncthreads();
@@ -214,8 +222,8 @@ __global__ __launch_bounds__(kTPB) void rfft_rows_kernel(const float* __restrict
 // ------------------------------------------------------ iPFB column pass
 // One workgroup = CW adjacent columns of one tile (all nslice rows) held in LDS
 // as CW/2 complex sequences u = y[:, k] + i y[:, k+1].
-template <int NSL, int CW>
-__global__ __launch_bounds__(kTPB) void ipfb_cols_kernel(const float* __restrict__ y, int64_t ldy, int64_t tile_stride,
+template <int NSL, int CW, int T>
+__global__ __launch_bounds__(T) void ipfb_cols_kernel(const float* __restrict__ y, int64_t ldy, int64_t tile_stride,
                                                          const float2* __restrict__ filt, int64_t ldf, int64_t ncols,
                                                          int cut, float* __restrict__ out, int64_t ldo,
                                                          int64_t out_tile_stride, Twiddle tw) {
@@ -226,7 +234,7 @@ __global__ __launch_bounds__(kTPB) void ipfb_cols_kernel(const float* __restrict
   float2* TA = U + NP * PADN;
   float2* TB = TA + tw.nA;
   const int tid = threadIdx.x;
-  load_twiddle(tw, TA, TB, tid, kTPB);
+  load_twiddle(tw, TA, TB, tid, T);
   // XCD-aware strip order: consecutive strips on one XCD share cache lines.
   const int64_t nstrips = ncols / CW;
   int64_t sidx = blockIdx.x;
@@ -234,15 +242,15 @@ __global__ __launch_bounds__(kTPB) void ipfb_cols_kernel(const float* __restrict
   const int64_t k0 = sidx * CW;
   const int64_t tile = blockIdx.y;
   const float* yt = y + tile * tile_stride;
-  for (int idx = tid; idx < NSL * NP; idx += kTPB) {
+  for (int idx = tid; idx < NSL * NP; idx += T) {
     const int i = idx / NP, p = idx - i * NP;
     U[p * PADN + pad16(i)] = reinterpret_cast<const float2*>(yt + i * ldy + k0)[p];
   }
   __syncthreads();
-  fft_lds<NSL, NP, kTPB>(U, TA, TB, tw.lb, 0, tid);
+  fft_lds<NSL, NP, T>(U, TA, TB, tw.lb, 0, tid);
   // filter on the real-pair spectra, stored conjugated for the inverse
   const float sc = 1.0f / float(NSL);
-  for (int idx = tid; idx < NP * (NSL / 2 + 1); idx += kTPB) {
+  for (int idx = tid; idx < NP * (NSL / 2 + 1); idx += T) {
     const int p = idx / (NSL / 2 + 1), f = idx - p * (NSL / 2 + 1);
     float2* Up = U + p * PADN;
     const float2 a = Up[pad16(f)];
@@ -263,10 +271,10 @@ __global__ __launch_bounds__(kTPB) void ipfb_cols_kernel(const float* __restrict
     }
   }
   __syncthreads();
-  fft_lds<NSL, NP, kTPB>(U, TA, TB, tw.lb, 0, tid);
+  fft_lds<NSL, NP, T>(U, TA, TB, tw.lb, 0, tid);
   float* ot = out + tile * out_tile_stride;
   const int nkeep = NSL - 2 * cut;
-  for (int idx = tid; idx < nkeep * NP; idx += kTPB) {
+  for (int idx = tid; idx < nkeep * NP; idx += T) {
     const int i = idx / NP, p = idx - i * NP;
     const float2 v = U[p * PADN + pad16(i + cut)];
     reinterpret_cast<float2*>(ot + int64_t(i) * ldo + k0)[p] = make_float2(v.x, -v.y);
@@ -328,6 +336,76 @@ __global__ __launch_bounds__(kTPB) void repfb_stage1_kernel(const float* __restr
   }
 }
 
+// Same result for R1 = 4S >= 16, with each column's R1-point DFT split over 4
+// lanes: lane t owns rows r = 4s + t (s < S), does the polyphase sum and an
+// S-point real DFT Z_t, then, after an exchange through LDS, the radix-4
+// combine  Y[q0 + S p] = sum_t (W_R1^(q0 t) Z_t[q0]) W_4^(p t).
+// A workgroup = 64 columns of one block; no spill-prone 64-register arrays.
+template <int R1>
+__global__ __launch_bounds__(kTPB) void repfb_stage1_split_kernel(const float* __restrict__ ts,
+                                                                  const float* __restrict__ win, int ntap,
+                                                                  float2* __restrict__ A) {
+  constexpr int S = R1 / 4;
+  constexpr int H = R1 / 2;
+  constexpr int ZH = S / 2 + 1;      // stored half spectrum of Z_t
+  constexpr int64_t C = 4096;
+  __shared__ float2 zx[64][4][ZH];
+  __shared__ float2 wr[R1];
+  const int tid = threadIdx.x;
+  const int t = tid & 3, cl = tid >> 2;  // lane within the column quad, column 0..63
+  const int64_t k = int64_t(blockIdx.x) * 64 + cl;
+  const int64_t b = blockIdx.y;
+  if (tid < R1) {
+    double sn, cs;
+    sincospi(-2.0 * double(tid) / double(R1), &sn, &cs);
+    wr[tid] = make_float2(float(cs), float(sn));
+  }
+  // polyphase sums of this lane's rows
+  float y[S];
+#pragma unroll
+  for (int sI = 0; sI < S; ++sI) y[sI] = 0.f;
+  for (int j = 0; j < ntap; ++j) {
+    const float* src = ts + ((b + j) * R1 + t) * C + k;
+    const float* wj = win + (int64_t(j) * R1 + t) * C + k;
+#pragma unroll
+    for (int sI = 0; sI < S; ++sI) y[sI] = fmaf(src[int64_t(4 * sI) * C], wj[int64_t(4 * sI) * C], y[sI]);
+  }
+  // S-point real DFT of y -> Z_t[q0], q0 = 0..S/2
+  {
+    float2 z[S / 2];
+#pragma unroll
+    for (int n = 0; n < S / 2; ++n) z[n] = make_float2(y[2 * n], y[2 * n + 1]);
+    DFT<S / 2>::run(z);
+    static_for<0, S / 2 + 1>([&](auto Q) {
+      constexpr int q = decltype(Q)::value;
+      const float2 a = z[q % (S / 2)];
+      const float2 bb = z[(S / 2 - q) % (S / 2)];
+      const float2 e = make_float2(0.5f * (a.x + bb.x), 0.5f * (a.y - bb.y));
+      const float2 d = make_float2(0.5f * (a.x - bb.x), 0.5f * (a.y + bb.y));
+      const float2 o = make_float2(d.y, -d.x);
+      const float2 wo = twc<S, q>(o);
+      zx[cl][t][q] = make_float2(e.x + wo.x, e.y + wo.y);
+    });
+  }
+  __syncthreads();
+  // lane u combines q0 = u, u+4, ...: outputs q = q0 (p=0), q0+S (p=1), and q = 2S = H for q0 = 0
+  float2* dst = A + (b * (H + 1)) * C + k;
+  for (int q0 = t; q0 < S; q0 += 4) {
+    float2 v[4];
+#pragma unroll
+    for (int tt = 0; tt < 4; ++tt) {
+      float2 zz = (2 * q0 <= S) ? zx[cl][tt][q0] : cconj(zx[cl][tt][S - q0]);
+      v[tt] = cmul(zz, wr[(q0 * tt) % R1]);
+    }
+    // 4-point DFT over tt: p = 0, 1, 2
+    const float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+    const float2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+    dst[int64_t(q0) * C] = cadd(s02, s13);                                     // p = 0
+    dst[int64_t(q0 + S) * C] = make_float2(d02.x + d13.y, d02.y - d13.x);      // p = 1: d02 - i d13
+    if (q0 == 0) dst[int64_t(H) * C] = csub(s02, s13);                         // p = 2
+  }
+}
+
 // ------------------------------------------------------- re-PFB stage 2
 // Workgroup (q1, b): B[k] = A[b][q1 or R1-q1 conj][k] * W_L^(q1 k), 4096-point
 // FFT over k, fine channel q = q1 + R1 q2 in [c_lo, c_hi) written to
@@ -361,14 +439,34 @@ __global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __rest
     Z[pad16(k)] = v;
   }
   __syncthreads();
-  fft_lds<N, 1, kTPB>(Z, TA, TB, tw4.lb, 0, tid);
   // q2 range with q1 + R1 q2 in [c_lo, c_hi)
   int64_t q2lo = (c_lo - q1 + R1 - 1) / R1;
   if (c_lo - q1 < 0) q2lo = 0;
   const int64_t q2hi = (c_hi - 1 - q1) >= 0 ? (c_hi - 1 - q1) / R1 : -1;
-  for (int64_t q2 = q2lo + tid; q2 <= q2hi && q2 < N; q2 += kTPB) {
-    const int64_t q = q1 + int64_t(R1) * q2;
-    out[(b / bpj) * sj + (b % bpj) * sb + (q - c_lo) * sc] = Z[pad16(int(q2))];
+  const int64_t nq = q2hi - q2lo + 1;
+  if (nq <= 512) {
+    // passes NS = 1, 16; the last radix-16 pass only for the kept bins:
+    // X[q2] = sum_m Z[(q2 % 256) + 256 m] W_4096^(m q2)
+    fft_lds<N, 1, kTPB, 1, 256>(Z, TA, TB, tw4.lb, 0, tid);
+    for (int64_t q2 = q2lo + tid; q2 <= q2hi; q2 += kTPB) {
+      const int j = int(q2 & 255);
+      float ax = 0.f, ay = 0.f;
+#pragma unroll
+      for (int m = 0; m < 16; ++m) {
+        const float2 v = Z[pad16(j + 256 * m)];
+        const float2 w = tw_lookup(TA, TB, tw4.lb, int((int64_t(m) * q2) & (N - 1)));
+        ax = fmaf(v.x, w.x, fmaf(-v.y, w.y, ax));
+        ay = fmaf(v.x, w.y, fmaf(v.y, w.x, ay));
+      }
+      const int64_t q = q1 + int64_t(R1) * q2;
+      out[(b / bpj) * sj + (b % bpj) * sb + (q - c_lo) * sc] = make_float2(ax, ay);
+    }
+  } else {
+    fft_lds<N, 1, kTPB>(Z, TA, TB, tw4.lb, 0, tid);
+    for (int64_t q2 = q2lo + tid; q2 <= q2hi && q2 < N; q2 += kTPB) {
+      const int64_t q = q1 + int64_t(R1) * q2;
+      out[(b / bpj) * sj + (b % bpj) * sb + (q - c_lo) * sc] = Z[pad16(int(q2))];
+    }
   }
 }
 
@@ -419,12 +517,12 @@ __global__ void rowmap_fill_kernel(int* rowmap, int64_t n) {
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) rowmap[i] = -1;
 }
 __global__ void rowmap_scatter_kernel(const int64_t* __restrict__ specnum, int64_t npresent, int64_t start, int64_t acclen,
-                                      int* __restrict__ rowmap) {
+                                      int64_t row_base, int* __restrict__ rowmap) {
   const int64_t p = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (p >= npresent) return;
   int64_t r = (specnum[p] - start) % acclen;
   if (r < 0) r += acclen;
-  rowmap[r] = int(p);
+  rowmap[r] = int(row_base + p);
 }
 
 }  // namespace alb
@@ -439,7 +537,7 @@ int launch_irfft(DenseRows d, Packed4Rows pk, int64_t nrows, int64_t nslice, flo
   Twiddle tw;
   int rc = get_twiddle(2 * M, &tw);
   if (rc) return rc;
-  const size_t lds = sizeof(float2) * (size_t(F) * (M + 1) + size_t(F) * Padded<M>::value + tw.nA + tw.nB);
+  const size_t lds = sizeof(float2) * (size_t(F) * Padded<M>::value + tw.nA + tw.nB);
   const unsigned grid = unsigned((nrows + F - 1) / F);
   hipLaunchKernelGGL((irfft_rows_kernel<M, F, MODE>), dim3(grid), dim3(kTPB), lds, s, d, pk, nrows, nslice, out, ldo, tw);
   ALB_LAUNCH_CHECK("irfft_rows_kernel");
@@ -497,7 +595,7 @@ int rfft_dispatch(int64_t n, const float* x, int64_t ldx, int64_t nrows, const f
   }
 }
 
-template <int NSL, int CW>
+template <int NSL, int CW, int T>
 int launch_cols(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, const float2* filt, int64_t ldf,
                 int64_t ncols, int cut, float* out, int64_t ldo, int64_t out_tile_stride, hipStream_t s) {
   Twiddle tw;
@@ -505,7 +603,7 @@ int launch_cols(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles
   if (rc) return rc;
   const size_t lds = sizeof(float2) * (size_t(CW / 2) * Padded<NSL>::value + tw.nA + tw.nB);
   ALB_REQUIRE(ncols % CW == 0, "ipfb column pass: width %lld not a multiple of %d", (long long)ncols, CW);
-  hipLaunchKernelGGL((ipfb_cols_kernel<NSL, CW>), dim3(unsigned(ncols / CW), unsigned(ntiles)), dim3(kTPB), lds, s, y,
+  hipLaunchKernelGGL((ipfb_cols_kernel<NSL, CW, T>), dim3(unsigned(ncols / CW), unsigned(ntiles)), dim3(T), lds, s, y,
                      ldy, tile_stride, filt, ldf, ncols, cut, out, ldo, out_tile_stride, tw);
   ALB_LAUNCH_CHECK("ipfb_cols_kernel");
   return ALB_OK;
@@ -515,9 +613,9 @@ int cols_dispatch(int64_t nslice, const float* y, int64_t ldy, int64_t tile_stri
                   int64_t ldf, int64_t ncols, int cut, float* out, int64_t ldo, int64_t out_tile_stride,
                   hipStream_t s) {
   switch (nslice) {
-#define CASE(n, w) case n: return launch_cols<n, w>(y, ldy, tile_stride, ntiles, filt, ldf, ncols, cut, out, ldo, out_tile_stride, s);
-    CASE(16, 8) CASE(32, 8) CASE(64, 8) CASE(128, 8) CASE(256, 8) CASE(512, 8) CASE(1024, 8) CASE(2048, 8)
-    CASE(4096, 8) CASE(8192, 4) CASE(16384, 2)
+#define CASE(n, w, t) case n: return launch_cols<n, w, t>(y, ldy, tile_stride, ntiles, filt, ldf, ncols, cut, out, ldo, out_tile_stride, s);
+    CASE(16, 8, 256) CASE(32, 8, 256) CASE(64, 8, 256) CASE(128, 8, 256) CASE(256, 8, 256) CASE(512, 8, 256)
+    CASE(1024, 8, 256) CASE(2048, 8, 512) CASE(4096, 8, 1024) CASE(8192, 4, 1024) CASE(16384, 2, 1024)
 #undef CASE
     default:
       set_error("ipfb: nslice %lld not supported (power of two 16..16384)", (long long)nslice);
@@ -527,6 +625,13 @@ int cols_dispatch(int64_t nslice, const float* y, int64_t ldy, int64_t tile_stri
 
 template <int R1>
 int launch_repfb_stage1(const float* ts, const float* win, int ntap, int64_t nblock, float2* A, hipStream_t s) {
+  if constexpr (R1 >= 16) {
+    ALB_REQUIRE(nblock <= 65535, "alb_repfb: at most 65535 blocks per call");
+    hipLaunchKernelGGL((repfb_stage1_split_kernel<R1>), dim3(4096 / 64, unsigned(nblock)), dim3(kTPB), 0, s, ts, win,
+                       ntap, A);
+    ALB_LAUNCH_CHECK("repfb_stage1_split_kernel");
+    return ALB_OK;
+  }
   const int64_t bpw = 16;
   const size_t lds1 = sizeof(float) * size_t(ntap) * R1 * 64;
   ALB_REQUIRE(lds1 <= 160 * 1024, "re-PFB: window strip does not fit in LDS");
@@ -647,8 +752,8 @@ int alb_ipfb_columns(const float* y, int64_t ldy, int64_t tile_stride, int64_t n
 
 // Row pass of the fused 4-bit iPFB: unpack + spec_num zero fill + channel scatter
 // + irfft (n = 4096) for ntiles tiles of nslice rows, tiles tile0.. of the chunk
-// sequence.  bytes [nchunk][acclen][ncols] packed per-pol 4-bit rows (present rows
-// first), rowmap [nchunk][acclen] (-1 = missing), chan[ncols] channel numbers.
+// sequence.  bytes [rows][ncols] packed per-pol 4-bit spectra, rowmap
+// [nchunk][acclen] = row of bytes or -1 (missing), chan[ncols] channel numbers.
 int alb_ipfb_rows_packed4(const uint8_t* bytes, const int* rowmap, const int* chan, int64_t ncols, int64_t acclen,
                           int cut, int64_t tile0, int64_t ntiles, float* out, void* stream) {
   ALB_REQUIRE(bytes && rowmap && chan && out, "alb_ipfb_rows_packed4: null pointer");
@@ -734,17 +839,18 @@ int alb_filter_from_matft(const void* matft, int64_t n, double thresh, void* fil
   return ALB_OK;
 }
 
-int alb_build_rowmap(const int64_t* specnum, int64_t npresent, int64_t start, int64_t acclen, int* rowmap,
-                     void* stream) {
+int alb_build_rowmap(const int64_t* specnum, int64_t npresent, int64_t start, int64_t acclen, int64_t row_base,
+                     int* rowmap, void* stream) {
   ALB_REQUIRE(rowmap && (npresent == 0 || specnum), "alb_build_rowmap: null pointer");
-  ALB_REQUIRE(acclen > 0 && npresent >= 0 && npresent <= acclen, "alb_build_rowmap: bad sizes");
+  ALB_REQUIRE(acclen > 0 && npresent >= 0 && npresent <= acclen && row_base >= 0 && row_base + npresent <= 0x7fffffff,
+              "alb_build_rowmap: bad sizes");
   hipStream_t s = as_stream(stream);
   int64_t g = (acclen + 255) / 256;
   if (g > 65536) g = 65536;
   hipLaunchKernelGGL(rowmap_fill_kernel, dim3(unsigned(g)), dim3(256), 0, s, rowmap, acclen);
   if (npresent > 0)
     hipLaunchKernelGGL(rowmap_scatter_kernel, dim3(unsigned((npresent + 255) / 256)), dim3(256), 0, s, specnum, npresent,
-                       start, acclen, rowmap);
+                       start, acclen, row_base, rowmap);
   ALB_LAUNCH_CHECK("rowmap kernels");
   return ALB_OK;
 }

@@ -175,10 +175,10 @@ extern "C" {
 
 int alb_unpack_4bit_float(const uint8_t* data, void* pol0, void* pol1, int64_t rowstart, int64_t rowend,
                           const int64_t* channels, int64_t ncols, int64_t nchan, void* stream) {
-  ALB_REQUIRE(data && pol0 && pol1 && channels, "alb_unpack_4bit_float: null pointer");
   ALB_REQUIRE(rowend >= rowstart && rowstart >= 0 && ncols >= 0 && nchan > 0, "alb_unpack_4bit_float: bad sizes");
   const int64_t nrows = rowend - rowstart;
   if (nrows * ncols == 0) return ALB_OK;
+  ALB_REQUIRE(data && pol0 && pol1 && channels, "alb_unpack_4bit_float: null pointer");
   hipLaunchKernelGGL(unpack4_kernel, grid_for(nrows * ncols), dim3(256), 0, as_stream(stream), data, (float2*)pol0,
                      (float2*)pol1, rowstart, nrows, channels, ncols, nchan);
   ALB_LAUNCH_CHECK("unpack4_kernel");
@@ -187,9 +187,10 @@ int alb_unpack_4bit_float(const uint8_t* data, void* pol0, void* pol1, int64_t r
 
 int alb_unpack_1bit_float(const uint8_t* data, void* pol0, void* pol1, int64_t rowstart, int64_t rowend,
                           int64_t chanstart, int64_t chanend, int64_t nchan, void* stream) {
-  ALB_REQUIRE(data && pol0 && pol1, "alb_unpack_1bit_float: null pointer");
   ALB_REQUIRE(rowend >= rowstart && rowstart >= 0 && chanend >= chanstart && nchan > 0,
               "alb_unpack_1bit_float: bad sizes");
+  if ((rowend - rowstart) * (chanend - chanstart) == 0) return ALB_OK;
+  ALB_REQUIRE(data && pol0 && pol1, "alb_unpack_1bit_float: null pointer");
   ALB_REQUIRE(chanstart % 2 == 0 && (chanend - chanstart) % 2 == 0,
               "alb_unpack_1bit_float: channel range must start even and hold an even count");
   const int64_t nrows = rowend - rowstart, nb = (chanend - chanstart) / 2;
@@ -202,11 +203,11 @@ int alb_unpack_1bit_float(const uint8_t* data, void* pol0, void* pol1, int64_t r
 
 int alb_sortpols(const uint8_t* data, uint8_t* pol0, uint8_t* pol1, int64_t rowstart, int64_t rowend, int64_t ncols,
                  int64_t nchan, int bit_depth, int64_t chanstart, int64_t chanend, void* stream) {
-  ALB_REQUIRE(data && pol0 && pol1, "alb_sortpols: null pointer");
   ALB_REQUIRE(chanend <= nchan, "ERROR: Trying to unpack more channels than present. Check chanend.");
   ALB_REQUIRE(rowend >= rowstart && rowstart >= 0 && chanend > chanstart && chanstart >= 0, "alb_sortpols: bad sizes");
   const int64_t nrows = rowend - rowstart;
   if (nrows == 0) return ALB_OK;
+  ALB_REQUIRE(data && pol0 && pol1, "alb_sortpols: null pointer");
   if (bit_depth == 4) {
     ALB_REQUIRE(ncols == chanend - chanstart, "alb_sortpols: 4-bit ncols must equal chanend-chanstart");
     hipLaunchKernelGGL(sortpols4_kernel, grid_for(nrows * ncols), dim3(256), 0, as_stream(stream), data, pol0, pol1,

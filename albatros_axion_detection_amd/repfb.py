@@ -113,7 +113,10 @@ class RePFBPipeline:
         self.tile_batch = int(tile_batch)
         if nchan != 2049:
             raise NotImplementedError("the fused iPFB handles 2049-channel spectra (lblock 4096)")
-        self.chan = _lib.to_device(np.asarray(chan_numbers, dtype=np.int32), t.int32)
+        chan_numbers = np.asarray(chan_numbers)
+        if chan_numbers.size == 0 or chan_numbers.min() < 0 or chan_numbers.max() > 2048:
+            raise ValueError("coarse channel numbers must lie in [0, 2048]")
+        self.chan = _lib.to_device(chan_numbers.astype(np.int32), t.int32)
         self.ncols = int(self.chan.numel())
         self.filt = _lib.to_device(filt, t.complex64) if filt is not None else _pfb.get_filter(self.nslice, filt_thresh,
                                                                                                   ntap=ntap)
@@ -218,9 +221,11 @@ class RePFBPipeline:
         return res
 
 
-def rowmaps_from_specnums(specnums_list, start_specnum, acclen):
+def rowmaps_from_specnums(specnums_list, start_specnum, acclen, row_bases=None):
     """Per chunk row maps ``(specnum - start) % acclen -> packed row`` on the
-    device (baseband_data_classes.py:53)."""
+    device (baseband_data_classes.py:53).  Chunk c's present spectra are
+    packed rows ``row_bases[c] + p`` (default ``c*acclen``: the stacked
+    BasebandFileIterator layout)."""
     t = _lib.torch()
     n = len(specnums_list)
     rm = t.empty((n, acclen), dtype=t.int32, device=_lib.device())
@@ -228,7 +233,26 @@ def rowmaps_from_specnums(specnums_list, start_specnum, acclen):
     for c, sp in enumerate(specnums_list):
         sp = np.asarray(sp, dtype=np.int64)
         d = _lib.to_device(sp if len(sp) else np.zeros(1, dtype=np.int64), t.int64)
-        _lib.call("alb_build_rowmap", _lib.ptr(d), len(sp), int(start_specnum), int(acclen), _lib.ptr(rm[c]), s)
+        base = c * acclen if row_bases is None else int(row_bases[c])
+        _lib.call("alb_build_rowmap", _lib.ptr(d), len(sp), int(start_specnum), int(acclen), base, _lib.ptr(rm[c]), s)
+    return rm
+
+
+def rowmap_stream(specnums, start_specnum, nchunks, acclen):
+    """Row map of a whole stream whose present spectra are stored
+    consecutively (packed row p holds spec_num ``specnums[p]``): re-blocks
+    arrival chunks of any length into ``nchunks`` fixed chunks of ``acclen``
+    spectra, missing spec_nums zero-filled.  ``specnums``: sorted int64
+    (numpy or device).  Returns int32 ``[nchunks, acclen]`` on the device."""
+    t = _lib.torch()
+    sp = np.asarray(specnums if _lib.is_numpy(specnums) else specnums.cpu().numpy(), dtype=np.int64)
+    total = nchunks * acclen
+    lo = np.searchsorted(sp, start_specnum, side="left")
+    hi = np.searchsorted(sp, start_specnum + total, side="left")
+    rm = t.empty((nchunks, acclen), dtype=t.int32, device=_lib.device())
+    d = _lib.to_device(sp[lo:hi] if hi > lo else np.zeros(1, dtype=np.int64), t.int64)
+    _lib.call("alb_build_rowmap", _lib.ptr(d), int(hi - lo), int(start_specnum), int(total), int(lo), _lib.ptr(rm),
+              _lib.stream())
     return rm
 
 

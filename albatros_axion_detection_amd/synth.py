@@ -78,6 +78,44 @@ def precision_xcorr(M=8, T=1024, F=6400, seed=2, sky=0.4, device=None):
     return np.asfortranarray((n + sky * s).astype(np.complex64))
 
 
+def batched_stream(nant=2, nchan=300, nchunks=64, lmin=2 ** 10, lmax=2 ** 14, loss=0.005, run=5, seed=4,
+                   sigma=2.5):
+    """Uneven arrival chunks of 4-bit data with packet loss.
+
+    ``nchunks`` arrival chunks with seeded lengths uniform in [lmin, lmax]
+    spectra; missing spectra come in runs of ``run`` consecutive spec_num
+    values (about ``loss`` of the total).  Antenna b = 0.3 a + 0.95 noise per
+    pol (as the direct 4-bit workload).  Returns
+    ``(packed [nant*2][P][nchan] uint8, specnums [P] int64, lengths)`` where
+    the P present spectra are stored consecutively (every input has the same
+    gaps) and specnums start at 0.
+    """
+    rng = np.random.default_rng(seed)
+    lengths = rng.integers(lmin, lmax + 1, size=nchunks)
+    total = int(lengths.sum())
+    nruns = rng.poisson(loss * total / run)
+    starts = np.sort(rng.choice(max(1, total - run), size=nruns, replace=False))
+    present = np.ones(total, dtype=bool)
+    for s in starts:
+        present[s:s + run] = False
+    specnums = np.nonzero(present)[0].astype(np.int64)
+    P = len(specnums)
+    packed = np.empty((2 * nant, P, nchan), dtype=np.uint8)
+    step = 1 << 16
+    for p0 in range(0, P, step):
+        n = min(step, P - p0)
+        for pol in range(2):
+            a = sigma * (rng.standard_normal((n, nchan)) + 1j * rng.standard_normal((n, nchan)))
+            for ant in range(nant):
+                if ant == 0:
+                    x = a
+                else:
+                    x = 0.3 * a + 0.95 * sigma * (rng.standard_normal((n, nchan))
+                                                  + 1j * rng.standard_normal((n, nchan)))
+                packed[ant * 2 + pol, p0:p0 + n] = quantize_4bit(x.real, x.imag)
+    return packed, specnums, lengths
+
+
 def pack_1bit(re_bits, im_bits):
     """Sign bits [..., nchan] (1 = positive) -> sortpols 1-bit bytes
     [..., ceil(nchan/4)]: channel 4j+c at bits (7-2c, 6-2c) of byte j."""

@@ -106,7 +106,7 @@ def rechannelizer(args, torch):
 
     def step():
         timer("rowmap", lambda: [
-            _lib.call("alb_build_rowmap", _lib.ptr(specnums[c]), acclen, 0, acclen, _lib.ptr(rowmap[c]), s())
+            _lib.call("alb_build_rowmap", _lib.ptr(specnums[c]), acclen, 0, acclen, c * acclen, _lib.ptr(rowmap[c]), s())
             for c in range(nchunks)])
         timer("ipfb_rows_packed4", lambda: _lib.call(
             "alb_ipfb_rows_packed4", _lib.ptr(packed), _lib.ptr(rowmap), _lib.ptr(pipe.chan), nkeep, acclen, cut, 0,

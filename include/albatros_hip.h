@@ -109,7 +109,10 @@ int alb_ipfb_columns(const float* y, int64_t ldy, int64_t tile_stride, int64_t n
                      const void* filt, int64_t ldf, int cut, float* out, int64_t ldo, int64_t out_tile_stride,
                      void* stream);
 /* Row half of the streaming iPFB with the 4-bit unpack and spec_num zero fill fused in
- * (unpacking.py:140 + baseband_data_classes.py:48 + pfb_gpu_utils.py:80, helper_gpu_stream.py:195-199). */
+ * (unpacking.py:140 + baseband_data_classes.py:48 + pfb_gpu_utils.py:80, helper_gpu_stream.py:195-199):
+ * bytes [rows][ncols] packed 4-bit spectra of one pol, rowmap [nchunk][acclen] from alb_build_rowmap,
+ * chan[ncols] coarse channel numbers (< 2049); tiles tile0 .. tile0+ntiles-1 of nslice = acclen+2*cut
+ * rows each -> out [ntiles][nslice][4096] float32. */
 int alb_ipfb_rows_packed4(const uint8_t* bytes, const int* rowmap, const int* chan, int64_t ncols, int64_t acclen,
                           int cut, int64_t tile0, int64_t ntiles, float* out, void* stream);
 /* pfb_gpu_utils.py:105 cupy_pfb with lblock = 4096*osamp (re-PFB, helper_gpu_stream.py:222):
@@ -128,10 +131,11 @@ int alb_repfb_stage2(const void* A, int64_t nblock, int64_t osamp, int64_t c_lo,
                      int64_t sc, int64_t bpj, int64_t sj, void* stream);
 
 /* ---- streaming helpers (scripts/xcorr/helper_gpu_stream.py) ------------ */
-/* Row map of one chunk for the fused iPFB: rowmap[(specnum[p]-start) % acclen] = p, -1 elsewhere
- * (baseband_data_classes.py:53 modular row of make_continuous_gpu). */
-int alb_build_rowmap(const int64_t* specnum, int64_t npresent, int64_t start, int64_t acclen, int* rowmap,
-                     void* stream);
+/* Row map of one chunk for the fused iPFB: rowmap[(specnum[p]-start) % acclen] = row_base + p
+ * (the packed row holding spectrum p), -1 elsewhere (baseband_data_classes.py:53 modular row of
+ * make_continuous_gpu). */
+int alb_build_rowmap(const int64_t* specnum, int64_t npresent, int64_t start, int64_t acclen, int64_t row_base,
+                     int* rowmap, void* stream);
 
 #ifdef __cplusplus
 }

@@ -81,8 +81,13 @@ def test_ipfb_inverts_pfb(pu, rng):
     spec = pu.cupy_pfb(ext, win)
     assert spec.shape == (nslice, 2049)
     assert rel_err(spec, opfb.pfb(ext, win)) < TOL
-    rec = pu.cupy_ipfb(spec, pu.get_filter(nslice, 0.0))
-    assert rel_err(rec.ravel(), ts) < 1e-4
+    # parity on the same (GPU) spectra, then the inversion itself: at thresh 0
+    # the inverse window gain max|1/W| (~70 here) amplifies the float32 error of
+    # the forward spectra, so the round trip is held to 1e-3
+    filt0 = opfb.wiener_filter(opfb.matft_fp64(nslice), 0.0)
+    rec = pu.cupy_ipfb(spec, filt0.astype(np.complex64))
+    assert rel_err(rec, opfb.ipfb(spec, filt0)) < TOL * np.max(np.abs(filt0))
+    assert rel_err(rec.ravel(), ts) < 1e-3
 
 
 @pytest.mark.parametrize("osamp", [1, 4, 64])
@@ -146,7 +151,7 @@ def test_stream_spectra_single_input(rng):
 
 def test_stream_xcorr_two_antennas(rng):
     from albatros_axion_detection_amd import repfb
-    acclen, cut, osamp, nblock = 96, 4, 2, 4
+    acclen, cut, osamp, nblock = 120, 4, 2, 4
     chan = np.arange(200, 230)
     nchunks = 4
     chunks = []
