@@ -43,6 +43,8 @@ SIGNATURES = {
     "alb_fft_rows": (_i32, [_vp, _i64, _vp, _i64, _i64, _i64, _i32, _vp]),
     "alb_pfb_small": (_i32, [_vp, _i64, _i64, _vp, _i32, _i64, _i64, _vp, _i64, _vp]),
     "alb_ipfb_columns": (_i32, [_vp, _i64, _i64, _i64, _i64, _i64, _vp, _i64, _i32, _vp, _i64, _i64, _vp]),
+    "alb_ipfb_columns_any": (_i32, [_vp, _i64, _i64, _i64, _i64, _i64, _vp, _i64, _i32, _vp, _i64, _i64, _vp, _vp,
+                                     _i64, _vp]),
     "alb_ipfb_rows_packed4": (_i32, [_vp, _vp, _vp, _i64, _i64, _i32, _i64, _i64, _vp, _vp]),
     "alb_repfb": (_i32, [_vp, _vp, _i32, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _vp]),
     "alb_repfb_workspace_bytes": (_i64, [_i64, _i64]),

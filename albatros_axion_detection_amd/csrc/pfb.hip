@@ -281,6 +281,75 @@ __global__ __launch_bounds__(T) void ipfb_cols_kernel(const float* __restrict__ 
   }
 }
 
+// ---------------------------------------- iPFB column pass, any even nslice
+// Chirp-z (Bluestein) form for lengths that are not powers of two: with
+// c_m = exp(-i pi m^2 / n),  DFT_n(x)[k] = c_k sum_m (x_m c_m) conj(c_{k-m}),
+// a circular convolution done with P-point FFTs (P >= 2n-1, power of two).
+// chirp[m] = c_m (m < n) and bhat = FFT_P(b), b[m] = conj(c_m) for m < n,
+// b[P-m] = conj(c_m) for 0 < m < n, are float64-derived tables from the caller.
+// One workgroup = 2 adjacent columns (one complex sequence) of one tile.
+template <int P>
+__device__ __forceinline__ void bluestein_dft(float2* U, int n, const float2* __restrict__ chirp,
+                                              const float2* __restrict__ bhat, const float2* TA, const float2* TB,
+                                              int lb, int tid) {
+  // U[0..n) in natural (padded) order -> DFT_n(U) in U[0..n)
+  for (int m = tid; m < P; m += kTPB) {
+    float2 v = make_float2(0.f, 0.f);
+    if (m < n) v = cmul(U[pad16(m)], chirp[m]);
+    U[pad16(m)] = v;
+  }
+  __syncthreads();
+  fft_lds<P, 1, kTPB>(U, TA, TB, lb, 0, tid);
+  const float sp = 1.0f / float(P);
+  for (int m = tid; m < P; m += kTPB) U[pad16(m)] = cconj(cscale(cmul(U[pad16(m)], bhat[m]), sp));
+  __syncthreads();
+  fft_lds<P, 1, kTPB>(U, TA, TB, lb, 0, tid);  // conj(IFFT(.)) * P
+  for (int k = tid; k < n; k += kTPB) U[pad16(k)] = cmul(cconj(U[pad16(k)]), chirp[k]);
+  __syncthreads();
+}
+
+template <int P>
+__global__ __launch_bounds__(kTPB) void ipfb_cols_bluestein_kernel(const float* __restrict__ y, int64_t ldy,
+                                                                   int64_t tile_stride, const float2* __restrict__ filt,
+                                                                   int64_t ldf, int n, int cut, float* __restrict__ out,
+                                                                   int64_t ldo, int64_t out_tile_stride,
+                                                                   const float2* __restrict__ chirp,
+                                                                   const float2* __restrict__ bhat, Twiddle tw) {
+  extern __shared__ float2 smem[];
+  float2* U = smem;
+  float2* TA = U + Padded<P>::value;
+  float2* TB = TA + tw.nA;
+  const int tid = threadIdx.x;
+  load_twiddle(tw, TA, TB, tid, kTPB);
+  const int64_t k0 = int64_t(blockIdx.x) * 2;
+  const int64_t tile = blockIdx.y;
+  const float* yt = y + tile * tile_stride;
+  for (int i = tid; i < n; i += kTPB) U[pad16(i)] = reinterpret_cast<const float2*>(yt + int64_t(i) * ldy + k0)[0];
+  __syncthreads();
+  bluestein_dft<P>(U, n, chirp, bhat, TA, TB, tw.lb, tid);
+  // filter the real pair, then conj(U') so that a forward DFT gives the inverse
+  const float sc = 1.0f / float(n);
+  for (int f = tid; f <= n / 2; f += kTPB) {
+    const float2 a = U[pad16(f)];
+    const float2 b = U[pad16((n - f) % n)];
+    const float2 y0 = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
+    const float2 dd = make_float2(0.5f * (a.x - b.x), 0.5f * (a.y + b.y));
+    const float2 y1 = make_float2(dd.y, -dd.x);
+    float2 g0 = cscale(cmul(filt[k0 * ldf + f], y0), sc);
+    float2 g1 = cscale(cmul(filt[(k0 + 1) * ldf + f], y1), sc);
+    if (f == 0 || 2 * f == n) { g0.y = 0.f; g1.y = 0.f; }
+    U[pad16(f)] = cconj(make_float2(g0.x - g1.y, g0.y + g1.x));
+    if (f != 0 && 2 * f != n) U[pad16(n - f)] = cconj(make_float2(g0.x + g1.y, -g0.y + g1.x));
+  }
+  __syncthreads();
+  bluestein_dft<P>(U, n, chirp, bhat, TA, TB, tw.lb, tid);
+  float* ot = out + tile * out_tile_stride;
+  for (int i = tid; i < n - 2 * cut; i += kTPB) {
+    const float2 v = U[pad16(i + cut)];
+    reinterpret_cast<float2*>(ot + int64_t(i) * ldo + k0)[0] = make_float2(v.x, -v.y);
+  }
+}
+
 // ------------------------------------------------------- re-PFB stage 1
 // For block b and column k (row width 4096): y[r] = sum_j ts[(b+j)R1 + r][k] w[(j R1 + r) 4096 + k],
 // then the R1-point real DFT down r -> A[b][q][k], q = 0..R1/2.
@@ -687,6 +756,20 @@ int fft_dispatch(int64_t n, const float2* x, int64_t ldx, int64_t nrows, float2*
   }
 }
 
+template <int P>
+int launch_cols_bluestein(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, int n, const float2* filt,
+                          int64_t ldf, int64_t ncols, int cut, float* out, int64_t ldo, int64_t out_tile_stride,
+                          const float2* chirp, const float2* bhat, hipStream_t s) {
+  Twiddle tw;
+  int rc = get_twiddle(P, &tw);
+  if (rc) return rc;
+  const size_t lds = sizeof(float2) * (size_t(Padded<P>::value) + tw.nA + tw.nB);
+  hipLaunchKernelGGL((ipfb_cols_bluestein_kernel<P>), dim3(unsigned(ncols / 2), unsigned(ntiles)), dim3(kTPB), lds, s, y,
+                     ldy, tile_stride, filt, ldf, n, cut, out, ldo, out_tile_stride, chirp, bhat, tw);
+  ALB_LAUNCH_CHECK("ipfb_cols_bluestein_kernel");
+  return ALB_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -853,6 +936,43 @@ int alb_build_rowmap(const int64_t* specnum, int64_t npresent, int64_t start, in
                        start, acclen, row_base, rowmap);
   ALB_LAUNCH_CHECK("rowmap kernels");
   return ALB_OK;
+}
+
+// Column pass of the inverse PFB for any even nslice (chirp-z form; the
+// power-of-two alb_ipfb_columns is the fast path).  chirp: complex64 [nslice]
+// exp(-i pi m^2/nslice); bhat: complex64 [P] = FFT_P of the conjugate chirp
+// filter, P = the power of two >= 2*nslice-1 (P <= 16384).
+int alb_ipfb_columns_any(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, int64_t nslice,
+                         int64_t ncols, const void* filt, int64_t ldf, int cut, float* out, int64_t ldo,
+                         int64_t out_tile_stride, const void* chirp, const void* bhat, int64_t P, void* stream) {
+  ALB_REQUIRE(y && filt && out && chirp && bhat, "alb_ipfb_columns_any: null pointer");
+  ALB_REQUIRE(nslice >= 2 && nslice % 2 == 0, "alb_ipfb_columns_any: nslice must be even (got %lld)",
+              (long long)nslice);
+  ALB_REQUIRE(P >= 2 * nslice - 1 && is_pow2(P), "alb_ipfb_columns_any: P must be a power of two >= 2*nslice-1");
+  ALB_REQUIRE(ncols % 2 == 0 && ldy >= ncols && ldo >= ncols && ldf >= nslice / 2 + 1 && cut >= 0 && 2 * cut < nslice,
+              "alb_ipfb_columns_any: bad sizes");
+  if (ntiles == 0) return ALB_OK;
+  hipStream_t s = as_stream(stream);
+  const int n = int(nslice);
+#define ARGS y, ldy, tile_stride, ntiles, n, (const float2*)filt, ldf, ncols, cut, out, ldo, out_tile_stride, \
+             (const float2*)chirp, (const float2*)bhat, s
+  switch (P) {
+    case 16: return launch_cols_bluestein<16>(ARGS);
+    case 32: return launch_cols_bluestein<32>(ARGS);
+    case 64: return launch_cols_bluestein<64>(ARGS);
+    case 128: return launch_cols_bluestein<128>(ARGS);
+    case 256: return launch_cols_bluestein<256>(ARGS);
+    case 512: return launch_cols_bluestein<512>(ARGS);
+    case 1024: return launch_cols_bluestein<1024>(ARGS);
+    case 2048: return launch_cols_bluestein<2048>(ARGS);
+    case 4096: return launch_cols_bluestein<4096>(ARGS);
+    case 8192: return launch_cols_bluestein<8192>(ARGS);
+    case 16384: return launch_cols_bluestein<16384>(ARGS);
+    default:
+      set_error("alb_ipfb_columns_any: P %lld not supported (<= 16384)", (long long)P);
+      return ALB_ENOTSUP;
+  }
+#undef ARGS
 }
 
 }  // extern "C"

@@ -135,6 +135,45 @@ def ifft(inp, axis=1, use_cache=True, copy=False):
 
 
 # ------------------------------------------------------------ inverse PFB
+_BLUESTEIN = {}
+
+
+def bluestein_tables(n):
+    """Chirp-z tables of an n-point DFT for the column pass at non power-of-two
+    nslice: chirp ``exp(-i pi m^2 / n)`` and the P-point FFT of its conjugate
+    filter (float64 on the host, stored complex64 on the device)."""
+    dev = _lib.device()
+    key = (n, str(dev))
+    if key not in _BLUESTEIN:
+        P = 1
+        while P < 2 * n - 1:
+            P *= 2
+        m = np.arange(n, dtype=np.int64)
+        ang = -np.pi * ((m * m) % (2 * n)).astype(np.float64) / n
+        chirp = np.exp(1j * ang)
+        b = np.zeros(P, dtype=np.complex128)
+        b[:n] = np.conj(chirp)
+        b[P - n + 1:] = np.conj(chirp[1:])[::-1]
+        bhat = np.fft.fft(b)
+        _BLUESTEIN[key] = (_lib.to_device(chirp.astype(np.complex64)), _lib.to_device(bhat.astype(np.complex64)), P)
+    return _BLUESTEIN[key]
+
+
+def ipfb_columns(y, ldy, tile_stride, ntiles, nslice, ncols, filt, cut, out, ldo, out_tile_stride):
+    """Column pass of the inverse PFB (pfb_gpu_utils.py:84-91) over ``ntiles``
+    tiles: power-of-two nslice on the radix-16 path, any even nslice on the
+    chirp-z path."""
+    n = int(nslice)
+    if n & (n - 1) == 0:
+        _lib.call("alb_ipfb_columns", _lib.ptr(y), ldy, tile_stride, ntiles, n, ncols, _lib.ptr(filt), filt.shape[1],
+                  int(cut), _lib.ptr(out), ldo, out_tile_stride, _lib.stream())
+    else:
+        chirp, bhat, P = bluestein_tables(n)
+        _lib.call("alb_ipfb_columns_any", _lib.ptr(y), ldy, tile_stride, ntiles, n, ncols, _lib.ptr(filt),
+                  filt.shape[1], int(cut), _lib.ptr(out), ldo, out_tile_stride, _lib.ptr(chirp), _lib.ptr(bhat), P,
+                  _lib.stream())
+
+
 def cupy_ipfb(dat, filt, thresh=None):
     """pfb_gpu_utils.py:56-91: ``irfft`` rows -> ``rfft`` along time ->
     ``* filt`` -> ``irfft`` along time; returns float32 ``(nslice, 2(nchan-1))``.
@@ -153,10 +192,12 @@ def cupy_ipfb(dat, filt, thresh=None):
     n = 2 * (nchan - 1)
     if tuple(f.shape) != (n, nslice // 2 + 1):
         raise ValueError("filter shape %s does not match (%d, %d)" % (tuple(f.shape), n, nslice // 2 + 1))
+    if nslice % 2:
+        raise ValueError("nslice must be even (the inverse real FFT along time returns 2*(nslice//2) rows, "
+                         "pycufft.py:146)")
     y = t.empty((nslice, n), dtype=t.float32, device=d.device)
     _lib.call("alb_irfft_rows", _lib.ptr(d), nchan, _lib.ptr(y), n, nslice, n, _lib.stream())
-    _lib.call("alb_ipfb_columns", _lib.ptr(y), n, 0, 1, nslice, n, _lib.ptr(f), f.shape[1], 0, _lib.ptr(y), n, 0,
-              _lib.stream())
+    ipfb_columns(y, n, 0, 1, nslice, n, f, 0, y, n, 0)
     return _lib.host(y) if isnp else y
 
 

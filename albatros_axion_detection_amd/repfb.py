@@ -165,8 +165,8 @@ class RePFBPipeline:
             _lib.call("alb_ipfb_rows_packed4", _lib.ptr(packed), _lib.ptr(rowmap), _lib.ptr(self.chan), self.ncols,
                       self.acclen, self.cut, t0, nt, _lib.ptr(y), s)
             out = stream_out[t0 * self.acclen:]
-            _lib.call("alb_ipfb_columns", _lib.ptr(y), 4096, self.nslice * 4096, nt, self.nslice, 4096,
-                      _lib.ptr(self.filt), self.filt.shape[1], self.cut, _lib.ptr(out), 4096, self.acclen * 4096, s)
+            _pfb.ipfb_columns(y, 4096, self.nslice * 4096, nt, self.nslice, 4096, self.filt, self.cut, out, 4096,
+                              self.acclen * 4096)
 
     def repfb(self, stream_rows, nblocks_total, out, sb, sc, bpj, sj, block_batch=512):
         """Re-PFB all ``nblocks_total`` spectra of one input's stream into

@@ -108,6 +108,12 @@ int alb_pfb_small(const float* ts, int64_t nblock, int64_t lblock, const float* 
 int alb_ipfb_columns(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, int64_t nslice, int64_t ncols,
                      const void* filt, int64_t ldf, int cut, float* out, int64_t ldo, int64_t out_tile_stride,
                      void* stream);
+/* The same column pass for any even nslice (chirp-z form with P-point FFTs, P the power of two
+ * >= 2*nslice-1, P <= 16384): chirp complex64 [nslice] = exp(-i pi m^2/nslice), bhat complex64 [P]
+ * = FFT_P of b (b[m] = conj(chirp[m]) for m < nslice, b[P-m] = conj(chirp[m]) for 0 < m < nslice). */
+int alb_ipfb_columns_any(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, int64_t nslice,
+                         int64_t ncols, const void* filt, int64_t ldf, int cut, float* out, int64_t ldo,
+                         int64_t out_tile_stride, const void* chirp, const void* bhat, int64_t P, void* stream);
 /* Row half of the streaming iPFB with the 4-bit unpack and spec_num zero fill fused in
  * (unpacking.py:140 + baseband_data_classes.py:48 + pfb_gpu_utils.py:80, helper_gpu_stream.py:195-199):
  * bytes [rows][ncols] packed 4-bit spectra of one pol, rowmap [nchunk][acclen] from alb_build_rowmap,

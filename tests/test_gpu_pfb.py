@@ -60,8 +60,10 @@ def test_matft_and_filter(pu):
     assert np.allclose(f0 * np.conj(opfb.matft_fp64(64)), 1.0, atol=1e-5)
 
 
-@pytest.mark.parametrize("nslice", [64, 1024, 4096])
+@pytest.mark.parametrize("nslice", [64, 1024, 4096, 2068, 4116, 120])
 def test_ipfb(pu, rng, nslice):
+    # powers of two on the radix-16 path; 2068 = 2048 + 2*10 (the reference's
+    # default cut) and 4116 on the chirp-z path
     dat = (rng.standard_normal((nslice, 2049)) + 1j * rng.standard_normal((nslice, 2049))).astype(np.complex64)
     filt = opfb.wiener_filter(opfb.matft_fp64(nslice), 0.45)
     got = pu.cupy_ipfb(dat, filt.astype(np.complex64))
@@ -126,10 +128,12 @@ def _decode(chunks):
     return out
 
 
-def test_stream_spectra_single_input(rng):
-    # fused unpack + zero fill + iPFB over chunks + re-PFB vs the oracle stream
+@pytest.mark.parametrize("acclen,cut", [(120, 4), (100, 10)])
+def test_stream_spectra_single_input(rng, acclen, cut):
+    # fused unpack + zero fill + iPFB over chunks + re-PFB vs the oracle stream;
+    # (100, 10) is the reference's default cut with a non power-of-two nslice
     from albatros_axion_detection_amd import repfb
-    acclen, cut, osamp, nblock = 120, 4, 4, 6
+    osamp, nblock = 4, 6
     chan = np.arange(64, 164)
     nchunks = 5
     chunks = _packed_chunks(rng, nchunks, acclen, len(chan))
@@ -164,6 +168,7 @@ def test_stream_xcorr_two_antennas(rng):
         chunks.append(cj)
     vis, miss, freqs, win, filt = repfb.repfb_xcorr_avg_chunks(chunks, [500, 500], chan, 200 * osamp, 230 * osamp,
                                                               acclen, nchunks, nblock, osamp, cut=cut)
+    # nslice = 128 here; the chirp-z column pass is covered by test_ipfb
     dec = []
     for cj in chunks:
         dd = []
