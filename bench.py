@@ -267,6 +267,25 @@ def secondary(args, torch):
     return out
 
 
+def copy_calibration(torch, nbytes=1 << 30, reps=5):
+    """Plain device-to-device copies of a known size: the FETCH_SIZE /
+    WRITE_SIZE counters read for these dispatches give the factor that turns
+    the counters into HBM bytes (expected nbytes read and nbytes written)."""
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / reps
+    print(json.dumps({"copy_bytes": nbytes, "reps": reps, "ms": round(ms, 5),
+                      "GBps_read_plus_write": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,6 +296,8 @@ def main():
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--onebit-T", type=int, default=2 ** 17)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 plain device copies of 1 GiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X (asked for %d)" % args.gpus,
@@ -284,6 +305,9 @@ def main():
         sys.exit(2)
     import torch
     sys.path.insert(0, HERE)
+    if args.copy_calibration:
+        copy_calibration(torch)
+        return
     res, ctx = rechannelizer(args, torch)
     cpu = None
     if not args.no_cpu:
