@@ -50,6 +50,10 @@ SIGNATURES = {
     "alb_repfb_workspace_bytes": (_i64, [_i64, _i64]),
     "alb_repfb_stage1": (_i32, [_vp, _vp, _i32, _i64, _i64, _vp, _vp]),
     "alb_repfb_stage2": (_i32, [_vp, _i64, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _vp]),
+    "alb_ipfb_columns_repfb": (_i32, [_vp, _i64, _i64, _i64, _i64, _vp, _i64, _i32, _vp, _i64, _i64, _vp, _i32, _i64,
+                                      _i64, _vp, _vp]),
+    "alb_repfb_interior": (_i32, [_i64, _i64, _i64, _i32, _i64, _vp, _vp]),
+    "alb_repfb_stage1_list": (_i32, [_vp, _vp, _i32, _vp, _i64, _i64, _vp, _vp]),
     "alb_build_rowmap": (_i32, [_vp, _i64, _i64, _i64, _i64, _vp, _vp]),
 }
 

@@ -23,6 +23,7 @@
 namespace alb {
 
 constexpr int kTPB = 256;
+constexpr int kBlockListCap = 64;
 
 // ----------------------------------------------------------- matft / filter
 __global__ void matft_filter_kernel(const float* __restrict__ win, int ntap, int64_t lblock, int64_t nslice, int64_t nf,
@@ -222,32 +223,42 @@ __global__ __launch_bounds__(kTPB) void rfft_rows_kernel(const float* __restrict
 // ------------------------------------------------------ iPFB column pass
 // One workgroup = CW adjacent columns of one tile (all nslice rows) held in LDS
 // as CW/2 complex sequences u = y[:, k] + i y[:, k+1].
+//
+// Workgroup order (workgroup i runs on XCD i % 8): each XCD owns a contiguous
+// range of strips and walks it in groups of G strips, all tiles of a group
+// back to back.  Workgroups in flight together read neighbouring columns of
+// the same rows (whole cache lines, long DRAM bursts) and a group's filter
+// columns are re-read by the next tile while still cached.
+__device__ __forceinline__ void cols_item(int64_t nstrips, int64_t* sidx, int64_t* tile) {
+  const int64_t ntiles = gridDim.x / nstrips;
+  if (nstrips % 8 == 0) {
+    const int64_t P = nstrips / 8;                 // strips per XCD
+    const int64_t G = (P % 32 == 0) ? 32 : P;      // strips per group
+    const int64_t slot = blockIdx.x / 8;
+    const int64_t grp = slot / (G * ntiles), rem = slot - grp * (G * ntiles);
+    *tile = rem / G;
+    *sidx = (blockIdx.x % 8) * P + grp * G + (rem - *tile * G);
+  } else {
+    *tile = blockIdx.x / nstrips;
+    *sidx = blockIdx.x - *tile * nstrips;
+  }
+}
+
+// Load the strip, forward FFT along time, Wiener filter with 1/nslice, inverse
+// FFT.  On return (after a barrier) time row i of column k0 + 2p is
+// U[p][i].x and of column k0 + 2p + 1 is -U[p][i].y.
 template <int NSL, int CW, int T>
-__global__ __launch_bounds__(T) void ipfb_cols_kernel(const float* __restrict__ y, int64_t ldy, int64_t tile_stride,
-                                                         const float2* __restrict__ filt, int64_t ldf, int64_t ncols,
-                                                         int cut, float* __restrict__ out, int64_t ldo,
-                                                         int64_t out_tile_stride, Twiddle tw) {
+__device__ __forceinline__ void cols_core(const float* __restrict__ yt, int64_t ldy, const float2* __restrict__ filt,
+                                          int64_t ldf, int64_t k0, float2* U, const float2* TA, const float2* TB,
+                                          int lb, int tid) {
   constexpr int NP = CW / 2;
   constexpr int PADN = Padded<NSL>::value;
-  extern __shared__ float2 smem[];
-  float2* U = smem;
-  float2* TA = U + NP * PADN;
-  float2* TB = TA + tw.nA;
-  const int tid = threadIdx.x;
-  load_twiddle(tw, TA, TB, tid, T);
-  // XCD-aware strip order: consecutive strips on one XCD share cache lines.
-  const int64_t nstrips = ncols / CW;
-  int64_t sidx = blockIdx.x;
-  if (nstrips % 8 == 0) sidx = (blockIdx.x % 8) * (nstrips / 8) + blockIdx.x / 8;
-  const int64_t k0 = sidx * CW;
-  const int64_t tile = blockIdx.y;
-  const float* yt = y + tile * tile_stride;
   for (int idx = tid; idx < NSL * NP; idx += T) {
     const int i = idx / NP, p = idx - i * NP;
     U[p * PADN + pad16(i)] = reinterpret_cast<const float2*>(yt + i * ldy + k0)[p];
   }
   __syncthreads();
-  fft_lds<NSL, NP, T>(U, TA, TB, tw.lb, 0, tid);
+  fft_lds<NSL, NP, T>(U, TA, TB, lb, 0, tid);
   // filter on the real-pair spectra, stored conjugated for the inverse
   const float sc = 1.0f / float(NSL);
   for (int idx = tid; idx < NP * (NSL / 2 + 1); idx += T) {
@@ -271,13 +282,177 @@ __global__ __launch_bounds__(T) void ipfb_cols_kernel(const float* __restrict__ 
     }
   }
   __syncthreads();
-  fft_lds<NSL, NP, T>(U, TA, TB, tw.lb, 0, tid);
+  fft_lds<NSL, NP, T>(U, TA, TB, lb, 0, tid);
+}
+
+template <int NSL, int CW, int T>
+__global__ __launch_bounds__(T) void ipfb_cols_kernel(const float* __restrict__ y, int64_t ldy, int64_t tile_stride,
+                                                         const float2* __restrict__ filt, int64_t ldf, int64_t ncols,
+                                                         int cut, float* __restrict__ out, int64_t ldo,
+                                                         int64_t out_tile_stride, Twiddle tw) {
+  constexpr int NP = CW / 2;
+  constexpr int PADN = Padded<NSL>::value;
+  extern __shared__ float2 smem[];
+  float2* U = smem;
+  float2* TA = U + NP * PADN;
+  float2* TB = TA + tw.nA;
+  const int tid = threadIdx.x;
+  load_twiddle(tw, TA, TB, tid, T);
+  int64_t sidx, tile;
+  cols_item(ncols / CW, &sidx, &tile);
+  const int64_t k0 = sidx * CW;
+  cols_core<NSL, CW, T>(y + tile * tile_stride, ldy, filt, ldf, k0, U, TA, TB, tw.lb, tid);
   float* ot = out + tile * out_tile_stride;
   const int nkeep = NSL - 2 * cut;
   for (int idx = tid; idx < nkeep * NP; idx += T) {
     const int i = idx / NP, p = idx - i * NP;
     const float2 v = U[p * PADN + pad16(i + cut)];
     reinterpret_cast<float2*>(ot + int64_t(i) * ldo + k0)[p] = make_float2(v.x, -v.y);
+  }
+}
+
+// ------------------------------------ iPFB column pass fused with re-PFB stage 1
+// Stream row g = row0 + tile * rows_per_tile + (i - cut) holds kept row i of a
+// tile.  Re-PFB block b (R1 rows per block row, ntap taps) reads stream rows
+// [R1 b, R1 (b + ntap)).  The blocks whose rows all lie in this tile,
+// [bi0, bi1) below, get their stage-1 result A[b][q][k] (q <= R1/2) straight
+// from LDS; only rows that other blocks read go to the stream buffer.
+__host__ __device__ inline void repfb_interior(int64_t g0, int64_t nkeep, int64_t R1, int64_t ntap, int64_t nblocks,
+                                               int64_t* bi0, int64_t* bi1) {
+  const int64_t lo = (g0 + R1 - 1) / R1;
+  int64_t hi = (g0 + nkeep) / R1 - ntap + 1;
+  if (hi > nblocks) hi = nblocks;
+  *bi0 = lo;
+  *bi1 = hi > lo ? hi : lo;
+}
+
+// Stage 1 for both columns of a pair at once: z[r] = y0[r] + i y1[r] (y0, y1
+// the polyphase sums of columns k0+2p and k0+2p+1), R1-point DFT split over 4
+// lanes as in the split kernel below (lane t: rows 4s + t, S-point DFT, radix-4
+// combine through LDS), then Y0[q] = (Z[q] + conj Z[-q]) / 2 and
+// Y1[q] = (Z[q] - conj Z[-q]) / 2i.
+template <int NSL, int CW, int T, int R1>
+__global__ __launch_bounds__(T) void ipfb_cols_repfb_kernel(const float* __restrict__ y, int64_t ldy,
+                                                               int64_t tile_stride, const float2* __restrict__ filt,
+                                                               int64_t ldf, int cut, float* __restrict__ ts,
+                                                               int64_t row0, int64_t rows_per_tile,
+                                                               const float* __restrict__ win, int ntap,
+                                                               int64_t nblocks, float2* __restrict__ A, Twiddle tw) {
+  constexpr int NP = CW / 2;
+  constexpr int PADN = Padded<NSL>::value;
+  constexpr int S = R1 / 4;
+  constexpr int H = R1 / 2;
+  constexpr int SP = S >= 16 ? S + 1 : S;   // row of the exchange buffer (padded when it fits)
+  constexpr int64_t C = 4096;
+  extern __shared__ float2 smem[];
+  float2* U = smem;
+  float2* TA = U + NP * PADN;
+  float2* TB = TA + tw.nA;
+  float2* WR = TB + tw.nB;                                  // W_R1^m, m < R1
+  float* WIN = reinterpret_cast<float*>(WR + R1);           // window strip [ntap*R1][CW]
+  const int tid = threadIdx.x;
+  load_twiddle(tw, TA, TB, tid, T);
+  int64_t sidx, tile;
+  cols_item(C / CW, &sidx, &tile);
+  const int64_t k0 = sidx * CW;
+  if (tid < R1) {
+    double sn, cs;
+    sincospi(-2.0 * double(tid) / double(R1), &sn, &cs);
+    WR[tid] = make_float2(float(cs), float(sn));
+  }
+  for (int idx = tid; idx < ntap * R1 * CW; idx += T) {
+    const int r = idx / CW, c = idx - r * CW;
+    WIN[idx] = win[int64_t(r) * C + k0 + c];
+  }
+  cols_core<NSL, CW, T>(y + tile * tile_stride, ldy, filt, ldf, k0, U, TA, TB, tw.lb, tid);
+
+  const int nkeep = NSL - 2 * cut;
+  const int64_t g0 = row0 + tile * rows_per_tile;
+  int64_t bi0, bi1;
+  repfb_interior(g0, nkeep, R1, ntap, nblocks, &bi0, &bi1);
+  const int nint = int(bi1 - bi0);
+  // rows read by blocks outside [bi0, bi1) go to the stream buffer
+  const int64_t wlo = nint > 0 ? R1 * (bi0 + ntap - 1) : g0 + nkeep;   // rows [g0, wlo) are written
+  const int64_t whi = nint > 0 ? R1 * bi1 : g0 + nkeep;                // and rows [whi, g0 + nkeep)
+  for (int idx = tid; idx < nkeep * NP; idx += T) {
+    const int i = idx / NP, p = idx - i * NP;
+    const int64_t g = g0 + i;
+    if (g < wlo || g >= whi) {
+      const float2 v = U[p * PADN + pad16(i + cut)];
+      reinterpret_cast<float2*>(ts + g * C + k0)[p] = make_float2(v.x, -v.y);
+    }
+  }
+  if (nint == 0) return;
+
+  // polyphase sums and S-point DFTs: lane u = task * 4 + t, task = (b - bi0) + nint * p
+  const int nlanes = nint * NP * 4;
+  constexpr int PER = 2;   // lane slots per thread (nint * NP * 4 <= PER * T is checked by the host)
+  float2 z[PER][S];
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const int u = tid + e * T;
+    if (u < nlanes) {
+      const int t = u & 3, task = u >> 2;
+      const int p = task / nint, bl = task - p * nint;
+      const int64_t b = bi0 + bl;
+      const float2* Up = U + p * PADN;
+#pragma unroll
+      for (int sI = 0; sI < S; ++sI) z[e][sI] = make_float2(0.f, 0.f);
+      for (int j = 0; j < ntap; ++j) {
+        const int ib = int(R1 * (b + j) - g0) + cut + t;   // local row of s = 0
+        const float2* wj = reinterpret_cast<const float2*>(WIN + (j * R1 + t) * CW) + p;
+#pragma unroll
+        for (int sI = 0; sI < S; ++sI) {
+          const float2 v = Up[pad16(ib + 4 * sI)];
+          const float2 w = wj[sI * 4 * NP];
+          z[e][sI].x = fmaf(v.x, w.x, z[e][sI].x);
+          z[e][sI].y = fmaf(-v.y, w.y, z[e][sI].y);
+        }
+      }
+      DFT<S>::run(z[e]);
+    }
+  }
+  __syncthreads();  // every lane is done reading U: reuse it as the exchange buffer
+  float2* X = U;
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const int u = tid + e * T;
+    if (u < nlanes) {
+#pragma unroll
+      for (int q0 = 0; q0 < S; ++q0) X[u * SP + q0] = z[e][q0];
+    }
+  }
+  __syncthreads();
+  // radix-4 combine: lane (task, t) owns q0 = t, t+4, ...; Y[q0 + S pp] goes to slot (pp, q0)
+  for (int u = tid; u < nlanes; u += T) {
+    const int t = u & 3, task = u >> 2;
+    float2* Xt = X + task * 4 * SP;
+    for (int q0 = t; q0 < S; q0 += 4) {
+      float2 v[4];
+#pragma unroll
+      for (int tt = 0; tt < 4; ++tt) v[tt] = cmul(Xt[tt * SP + q0], WR[(q0 * tt) % R1]);
+      const float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+      const float2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+      Xt[0 * SP + q0] = cadd(s02, s13);                                    // pp = 0
+      Xt[1 * SP + q0] = make_float2(d02.x + d13.y, d02.y - d13.x);         // pp = 1: d02 - i d13
+      Xt[2 * SP + q0] = csub(s02, s13);                                    // pp = 2
+      Xt[3 * SP + q0] = make_float2(d02.x - d13.y, d02.y + d13.x);         // pp = 3: d02 + i d13
+    }
+  }
+  __syncthreads();
+  // separate the two real columns and store A[b][q][k0 + 2p .. + 1], q <= H
+  const int nout = nint * (H + 1) * NP;
+  for (int it = tid; it < nout; it += T) {
+    const int p = it % NP, r = it / NP;
+    const int q = r % (H + 1), bl = r / (H + 1);
+    const float2* Xt = X + (bl + nint * p) * 4 * SP;
+    const int qm = (R1 - q) % R1;
+    const float2 a = Xt[(q / S) * SP + (q % S)];
+    const float2 c = Xt[(qm / S) * SP + (qm % S)];
+    const float2 y0 = make_float2(0.5f * (a.x + c.x), 0.5f * (a.y - c.y));     // (a + conj c) / 2
+    const float2 y1 = make_float2(0.5f * (a.y + c.y), -0.5f * (a.x - c.x));    // (a - conj c) / 2i
+    float4* dst = reinterpret_cast<float4*>(A + ((bi0 + bl) * (H + 1) + q) * C + k0) + p;
+    *dst = make_float4(y0.x, y0.y, y1.x, y1.y);
   }
 }
 
@@ -409,21 +584,29 @@ __global__ __launch_bounds__(kTPB) void repfb_stage1_kernel(const float* __restr
 // lanes: lane t owns rows r = 4s + t (s < S), does the polyphase sum and an
 // S-point real DFT Z_t, then, after an exchange through LDS, the radix-4
 // combine  Y[q0 + S p] = sum_t (W_R1^(q0 t) Z_t[q0]) W_4^(p t).
-// A workgroup = 64 columns of one block; no spill-prone 64-register arrays.
+// A workgroup = 64 columns of one block; wave t holds lane t of all 64
+// columns, so every global load and store of a wave is 64 consecutive
+// columns; no spill-prone 64-register arrays.
+// Block b of workgroup row y: y itself, or list.b[y] when a list is given.
+struct BlockList {
+  int n;
+  int b[kBlockListCap];
+};
+
 template <int R1>
 __global__ __launch_bounds__(kTPB) void repfb_stage1_split_kernel(const float* __restrict__ ts,
                                                                   const float* __restrict__ win, int ntap,
-                                                                  float2* __restrict__ A) {
+                                                                  float2* __restrict__ A, BlockList list) {
   constexpr int S = R1 / 4;
   constexpr int H = R1 / 2;
   constexpr int ZH = S / 2 + 1;      // stored half spectrum of Z_t
   constexpr int64_t C = 4096;
-  __shared__ float2 zx[64][4][ZH];
+  __shared__ float2 zx[4][ZH][64];
   __shared__ float2 wr[R1];
   const int tid = threadIdx.x;
-  const int t = tid & 3, cl = tid >> 2;  // lane within the column quad, column 0..63
+  const int t = tid >> 6, cl = tid & 63;  // lane (= wave) within the column quad, column 0..63
   const int64_t k = int64_t(blockIdx.x) * 64 + cl;
-  const int64_t b = blockIdx.y;
+  const int64_t b = list.n ? list.b[blockIdx.y] : int64_t(blockIdx.y);
   if (tid < R1) {
     double sn, cs;
     sincospi(-2.0 * double(tid) / double(R1), &sn, &cs);
@@ -453,7 +636,7 @@ __global__ __launch_bounds__(kTPB) void repfb_stage1_split_kernel(const float* _
       const float2 d = make_float2(0.5f * (a.x - bb.x), 0.5f * (a.y + bb.y));
       const float2 o = make_float2(d.y, -d.x);
       const float2 wo = twc<S, q>(o);
-      zx[cl][t][q] = make_float2(e.x + wo.x, e.y + wo.y);
+      zx[t][q][cl] = make_float2(e.x + wo.x, e.y + wo.y);
     });
   }
   __syncthreads();
@@ -463,7 +646,7 @@ __global__ __launch_bounds__(kTPB) void repfb_stage1_split_kernel(const float* _
     float2 v[4];
 #pragma unroll
     for (int tt = 0; tt < 4; ++tt) {
-      float2 zz = (2 * q0 <= S) ? zx[cl][tt][q0] : cconj(zx[cl][tt][S - q0]);
+      float2 zz = (2 * q0 <= S) ? zx[tt][q0][cl] : cconj(zx[tt][S - q0][cl]);
       v[tt] = cmul(zz, wr[(q0 * tt) % R1]);
     }
     // 4-point DFT over tt: p = 0, 1, 2
@@ -476,9 +659,20 @@ __global__ __launch_bounds__(kTPB) void repfb_stage1_split_kernel(const float* _
 }
 
 // ------------------------------------------------------- re-PFB stage 2
-// Workgroup (q1, b): B[k] = A[b][q1 or R1-q1 conj][k] * W_L^(q1 k), 4096-point
-// FFT over k, fine channel q = q1 + R1 q2 in [c_lo, c_hi) written to
+// Workgroup (q1 <= R1/2, b): u[k] = A[b][q1][k] W_L^(q1 k), U = FFT_4096(u).
+// Fine channel q = q1 + R1 q2 is U[q2].  Because the block is real, channel
+// q' = (R1 - q1) + R1 q2' is conj(X[L - q']) = conj(U[4095 - q2']), so one
+// transform serves both q1 and its mirror R1 - q1 and A is read once.
+// Kept channels q in [c_lo, c_hi) go to
 // out[(b/bpj)*sj + (b%bpj)*sb + (q - c_lo)*sc] (complex elements; bpj = blocks per job).
+__device__ __forceinline__ void q2_range(int64_t q1, int64_t R1, int64_t c_lo, int64_t c_hi, int64_t* lo, int64_t* hi) {
+  int64_t a = (c_lo - q1 + R1 - 1) / R1;
+  if (c_lo - q1 < 0) a = 0;
+  const int64_t z = (c_hi - 1 - q1) >= 0 ? (c_hi - 1 - q1) / R1 : -1;
+  *lo = a;
+  *hi = z < 4095 ? z : 4095;
+}
+
 template <int R1>
 __global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __restrict__ A, int64_t c_lo, int64_t c_hi,
                                                             float2* __restrict__ out, int64_t sb, int64_t sc, int64_t bpj,
@@ -497,44 +691,49 @@ __global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __rest
   load_twiddle(twL, LA, LB, tid, kTPB);
   const int q1 = blockIdx.x;
   const int64_t b = blockIdx.y;
-  const bool mirror = q1 > H;
-  const int qs = mirror ? R1 - q1 : q1;
-  const float2* src = A + (b * (H + 1) + qs) * N;
+  const float2* src = A + (b * (H + 1) + q1) * N;
   __syncthreads();
   for (int k = tid; k < N; k += kTPB) {
     float2 v = src[k];
-    if (mirror) v.y = -v.y;
     if (q1 != 0) v = cmul(v, tw_lookup(LA, LB, twL.lb, q1 * k));
     Z[pad16(k)] = v;
   }
   __syncthreads();
-  // q2 range with q1 + R1 q2 in [c_lo, c_hi)
-  int64_t q2lo = (c_lo - q1 + R1 - 1) / R1;
-  if (c_lo - q1 < 0) q2lo = 0;
-  const int64_t q2hi = (c_hi - 1 - q1) >= 0 ? (c_hi - 1 - q1) / R1 : -1;
-  const int64_t nq = q2hi - q2lo + 1;
-  if (nq <= 512) {
-    // passes NS = 1, 16; the last radix-16 pass only for the kept bins:
-    // X[q2] = sum_m Z[(q2 % 256) + 256 m] W_4096^(m q2)
+  int64_t alo, ahi, mlo = 0, mhi = -1;
+  q2_range(q1, R1, c_lo, c_hi, &alo, &ahi);
+  const bool mirror = q1 > 0 && q1 < H;
+  if (mirror) q2_range(R1 - q1, R1, c_lo, c_hi, &mlo, &mhi);
+  const int64_t na = ahi >= alo ? ahi - alo + 1 : 0;
+  const int64_t nm = mhi >= mlo ? mhi - mlo + 1 : 0;
+  float2* ob = out + (b / bpj) * sj + (b % bpj) * sb;
+  if (na + nm <= 512) {
+    // passes NS = 1, 16; the last radix-16 pass only for the wanted bins:
+    // U[j] = sum_m Z[(j % 256) + 256 m] W_4096^(m j)
     fft_lds<N, 1, kTPB, 1, 256>(Z, TA, TB, tw4.lb, 0, tid);
-    for (int64_t q2 = q2lo + tid; q2 <= q2hi; q2 += kTPB) {
-      const int j = int(q2 & 255);
+    for (int64_t t = tid; t < na + nm; t += kTPB) {
+      const bool dir = t < na;
+      const int64_t q2 = dir ? alo + t : mlo + (t - na);
+      const int j = dir ? int(q2) : int(N - 1 - q2);
+      const int jl = j & 255;
       float ax = 0.f, ay = 0.f;
 #pragma unroll
       for (int m = 0; m < 16; ++m) {
-        const float2 v = Z[pad16(j + 256 * m)];
-        const float2 w = tw_lookup(TA, TB, tw4.lb, int((int64_t(m) * q2) & (N - 1)));
+        const float2 v = Z[pad16(jl + 256 * m)];
+        const float2 w = tw_lookup(TA, TB, tw4.lb, (m * j) & (N - 1));
         ax = fmaf(v.x, w.x, fmaf(-v.y, w.y, ax));
         ay = fmaf(v.x, w.y, fmaf(v.y, w.x, ay));
       }
-      const int64_t q = q1 + int64_t(R1) * q2;
-      out[(b / bpj) * sj + (b % bpj) * sb + (q - c_lo) * sc] = make_float2(ax, ay);
+      const int64_t q = dir ? q1 + int64_t(R1) * q2 : (R1 - q1) + int64_t(R1) * q2;
+      ob[(q - c_lo) * sc] = dir ? make_float2(ax, ay) : make_float2(ax, -ay);
     }
   } else {
     fft_lds<N, 1, kTPB>(Z, TA, TB, tw4.lb, 0, tid);
-    for (int64_t q2 = q2lo + tid; q2 <= q2hi && q2 < N; q2 += kTPB) {
-      const int64_t q = q1 + int64_t(R1) * q2;
-      out[(b / bpj) * sj + (b % bpj) * sb + (q - c_lo) * sc] = Z[pad16(int(q2))];
+    for (int64_t t = tid; t < na + nm; t += kTPB) {
+      const bool dir = t < na;
+      const int64_t q2 = dir ? alo + t : mlo + (t - na);
+      const float2 v = Z[pad16(dir ? int(q2) : int(N - 1 - q2))];
+      const int64_t q = dir ? q1 + int64_t(R1) * q2 : (R1 - q1) + int64_t(R1) * q2;
+      ob[(q - c_lo) * sc] = dir ? v : cconj(v);
     }
   }
 }
@@ -672,7 +871,8 @@ int launch_cols(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles
   if (rc) return rc;
   const size_t lds = sizeof(float2) * (size_t(CW / 2) * Padded<NSL>::value + tw.nA + tw.nB);
   ALB_REQUIRE(ncols % CW == 0, "ipfb column pass: width %lld not a multiple of %d", (long long)ncols, CW);
-  hipLaunchKernelGGL((ipfb_cols_kernel<NSL, CW, T>), dim3(unsigned(ncols / CW), unsigned(ntiles)), dim3(T), lds, s, y,
+  ALB_REQUIRE((ncols / CW) * ntiles < (int64_t(1) << 31), "ipfb column pass: too many tiles");
+  hipLaunchKernelGGL((ipfb_cols_kernel<NSL, CW, T>), dim3(unsigned((ncols / CW) * ntiles)), dim3(T), lds, s, y,
                      ldy, tile_stride, filt, ldf, ncols, cut, out, ldo, out_tile_stride, tw);
   ALB_LAUNCH_CHECK("ipfb_cols_kernel");
   return ALB_OK;
@@ -696,8 +896,9 @@ template <int R1>
 int launch_repfb_stage1(const float* ts, const float* win, int ntap, int64_t nblock, float2* A, hipStream_t s) {
   if constexpr (R1 >= 16) {
     ALB_REQUIRE(nblock <= 65535, "alb_repfb: at most 65535 blocks per call");
+    BlockList none{};
     hipLaunchKernelGGL((repfb_stage1_split_kernel<R1>), dim3(4096 / 64, unsigned(nblock)), dim3(kTPB), 0, s, ts, win,
-                       ntap, A);
+                       ntap, A, none);
     ALB_LAUNCH_CHECK("repfb_stage1_split_kernel");
     return ALB_OK;
   }
@@ -711,6 +912,65 @@ int launch_repfb_stage1(const float* ts, const float* win, int ntap, int64_t nbl
 }
 
 template <int R1>
+int launch_repfb_stage1_list(const float* ts, const float* win, int ntap, const int* blocks, int64_t n, float2* A,
+                             hipStream_t s) {
+  if constexpr (R1 >= 16) {
+    for (int64_t i0 = 0; i0 < n; i0 += kBlockListCap) {
+      BlockList bl{};
+      bl.n = int(n - i0 < kBlockListCap ? n - i0 : kBlockListCap);
+      for (int i = 0; i < bl.n; ++i) bl.b[i] = blocks[i0 + i];
+      hipLaunchKernelGGL((repfb_stage1_split_kernel<R1>), dim3(4096 / 64, unsigned(bl.n)), dim3(kTPB), 0, s, ts, win,
+                         ntap, A, bl);
+      ALB_LAUNCH_CHECK("repfb_stage1_split_kernel");
+    }
+    return ALB_OK;
+  } else {
+    set_error("alb_repfb_stage1_list: osamp %d not supported (16..64)", R1);
+    return ALB_ENOTSUP;
+  }
+}
+
+// Shape limits of the fused kernel: lane slots (2 per thread), the exchange
+// buffer (reuses the strip's LDS) and the window strip.
+template <int NSL, int CW, int T, int R1>
+int check_cols_repfb(int cut, int ntap) {
+  constexpr int NP = CW / 2, S = R1 / 4;
+  constexpr int SP = S >= 16 ? S + 1 : S;
+  const int64_t nkeep = NSL - 2 * int64_t(cut);
+  const int64_t nint_max = nkeep / R1 - ntap + 1;
+  if (ntap < 1 || ntap * R1 > 512) {
+    set_error("fused iPFB/re-PFB: ntap * osamp must be <= 512");
+    return ALB_ENOTSUP;
+  }
+  if (nint_max * NP * 4 > 2 * T || nint_max * NP * 4 * SP > NP * Padded<NSL>::value) {
+    set_error("fused iPFB/re-PFB: tile too long for osamp %d", R1);
+    return ALB_ENOTSUP;
+  }
+  return ALB_OK;
+}
+
+template <int NSL, int CW, int T, int R1>
+int launch_cols_repfb(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, const float2* filt, int64_t ldf,
+                      int cut, float* ts, int64_t row0, int64_t rows_per_tile, const float* win, int ntap,
+                      int64_t nblocks, float2* A, hipStream_t s) {
+  constexpr int NP = CW / 2;
+  int rc = check_cols_repfb<NSL, CW, T, R1>(cut, ntap);
+  if (rc || ntiles == 0) return rc;
+  ALB_REQUIRE(y && filt && ts && win && A, "alb_ipfb_columns_repfb: null pointer");
+  ALB_REQUIRE((4096 / CW) * ntiles < (int64_t(1) << 31), "fused iPFB/re-PFB: too many tiles");
+  Twiddle tw;
+  rc = get_twiddle(NSL, &tw);
+  if (rc) return rc;
+  const size_t lds = sizeof(float2) * (size_t(NP) * Padded<NSL>::value + tw.nA + tw.nB + R1) +
+                     sizeof(float) * size_t(ntap) * R1 * CW;
+  ALB_REQUIRE(lds <= 160 * 1024, "fused iPFB/re-PFB: LDS budget exceeded");
+  hipLaunchKernelGGL((ipfb_cols_repfb_kernel<NSL, CW, T, R1>), dim3(unsigned((4096 / CW) * ntiles)), dim3(T), lds, s, y,
+                     ldy, tile_stride, filt, ldf, cut, ts, row0, rows_per_tile, win, ntap, nblocks, A, tw);
+  ALB_LAUNCH_CHECK("ipfb_cols_repfb_kernel");
+  return ALB_OK;
+}
+
+template <int R1>
 int launch_repfb_stage2(int64_t nblock, const float2* A, int64_t c_lo, int64_t c_hi, float2* out, int64_t sb, int64_t sc,
                         int64_t bpj, int64_t sj, hipStream_t s) {
   Twiddle tw4, twL;
@@ -719,7 +979,7 @@ int launch_repfb_stage2(int64_t nblock, const float2* A, int64_t c_lo, int64_t c
   rc = get_twiddle(int64_t(4096) * R1, &twL);
   if (rc) return rc;
   const size_t lds2 = sizeof(float2) * (size_t(Padded<4096>::value) + tw4.nA + tw4.nB + twL.nA + twL.nB);
-  hipLaunchKernelGGL((repfb_stage2_kernel<R1>), dim3(R1, unsigned(nblock)), dim3(kTPB), lds2, s, A, c_lo, c_hi, out, sb,
+  hipLaunchKernelGGL((repfb_stage2_kernel<R1>), dim3(R1 / 2 + 1, unsigned(nblock)), dim3(kTPB), lds2, s, A, c_lo, c_hi, out, sb,
                      sc, bpj, sj, tw4, twL);
   ALB_LAUNCH_CHECK("repfb_stage2_kernel");
   return ALB_OK;
@@ -896,6 +1156,57 @@ int alb_repfb(const float* ts, const float* win, int ntap, int64_t nblock, int64
   int rc = alb_repfb_stage1(ts, win, ntap, nblock, osamp, A, stream);
   if (rc) return rc;
   return alb_repfb_stage2(A, nblock, osamp, c_lo, c_hi, out, sb, sc, bpj, sj, stream);
+}
+
+// Stage 1 of the re-PFB for a list of blocks (host array of n block indices):
+// block b reads ts rows [osamp*b, osamp*(b+ntap)) and writes A[b].
+int alb_repfb_stage1_list(const float* ts, const float* win, int ntap, const int* blocks, int64_t n, int64_t osamp,
+                          void* A, void* stream) {
+  ALB_REQUIRE(n >= 0, "alb_repfb_stage1_list: bad sizes");
+  if (n == 0) return ALB_OK;
+  ALB_REQUIRE(ts && win && A && blocks, "alb_repfb_stage1_list: null pointer");
+  ALB_REQUIRE(ntap >= 1, "alb_repfb_stage1_list: bad ntap");
+  for (int64_t i = 0; i < n; ++i)
+    ALB_REQUIRE(blocks[i] >= 0, "alb_repfb_stage1_list: negative block index");
+  hipStream_t s = as_stream(stream);
+  switch (osamp) {
+#define CASE(r) case r: return launch_repfb_stage1_list<r>(ts, win, ntap, blocks, n, (float2*)A, s);
+    CASE(16) CASE(32) CASE(64)
+#undef CASE
+    default:
+      set_error("alb_repfb_stage1_list: osamp %lld not supported (16, 32, 64)", (long long)osamp);
+      return ALB_ENOTSUP;
+  }
+}
+
+// Interior re-PFB blocks of a tile whose kept rows are stream rows
+// [g0, g0 + nkeep): [*lo, *hi) (empty when *hi == *lo).
+int alb_repfb_interior(int64_t g0, int64_t nkeep, int64_t osamp, int ntap, int64_t nblocks, int64_t* lo, int64_t* hi) {
+  ALB_REQUIRE(lo && hi && g0 >= 0 && nkeep >= 0 && osamp >= 1 && ntap >= 1 && nblocks >= 0,
+              "alb_repfb_interior: bad arguments");
+  repfb_interior(g0, nkeep, osamp, ntap, nblocks, lo, hi);
+  return ALB_OK;
+}
+
+// Inverse-PFB column pass fused with re-PFB stage 1 (see ipfb_cols_repfb_kernel).
+int alb_ipfb_columns_repfb(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, int64_t nslice,
+                           const void* filt, int64_t ldf, int cut, float* ts, int64_t row0, int64_t rows_per_tile,
+                           const float* win, int ntap, int64_t osamp, int64_t nblocks, void* A, void* stream) {
+  ALB_REQUIRE(ntiles >= 0 && nblocks >= 0 && row0 >= 0 && rows_per_tile >= 0, "alb_ipfb_columns_repfb: bad sizes");
+  ALB_REQUIRE(cut >= 0 && 2 * cut < nslice, "alb_ipfb_columns_repfb: bad cut");
+  ALB_REQUIRE(ntiles == 0 || (ldy >= 4096 && ldf >= nslice / 2 + 1), "alb_ipfb_columns_repfb: bad strides");
+  hipStream_t s = as_stream(stream);
+  if (nslice == 4096) {
+    switch (osamp) {
+#define CASE(r) case r: return launch_cols_repfb<4096, 8, 1024, r>(y, ldy, tile_stride, ntiles, (const float2*)filt, ldf, cut, ts, row0, rows_per_tile, win, ntap, nblocks, (float2*)A, s);
+      CASE(32) CASE(64)
+#undef CASE
+      default: break;
+    }
+  }
+  set_error("alb_ipfb_columns_repfb: nslice %lld with osamp %lld not supported (nslice 4096, osamp 32 or 64)",
+            (long long)nslice, (long long)osamp);
+  return ALB_ENOTSUP;
 }
 
 int64_t alb_repfb_workspace_bytes(int64_t nblock, int64_t osamp) {

@@ -20,10 +20,15 @@ stream of one input goes through four launches:
 Row ``ant*npol + pol`` of the correlator input follows
 helper_gpu_stream_clean.py:375 (helper_gpu_stream.py:222 indexes ``j*nant+k``).
 """
+import ctypes
+
 import numpy as np
 
 from . import _lib
 from . import pfb as _pfb
+
+# largest re-PFB workspace (all blocks of one input at once) the fused path allocates
+FUSED_WORKSPACE_LIMIT = 16 << 30
 
 
 def get_chunkier(nchunks, lchunk, nblock, lblock, ntap):
@@ -184,19 +189,84 @@ class RePFBPipeline:
             _lib.call("alb_repfb", _lib.ptr(ts), _lib.ptr(self.win), self.ntap, nb, self.osamp, _lib.ptr(ws),
                       self.c_lo, self.c_hi, _lib.ptr(base), sb, sc, bpj, sj, s)
 
+    # ----------------------------------------------------- fused iPFB/re-PFB
+    def fused_supported(self, nblocks_total):
+        """True when the column pass can carry re-PFB stage 1 (nslice 4096,
+        osamp 32 or 64) and the all-blocks workspace stays within
+        ``FUSED_WORKSPACE_LIMIT``."""
+        lib = _lib.load()
+        rc = lib.alb_ipfb_columns_repfb(None, 4096, 0, 0, self.nslice, None, self.nslice // 2 + 1, self.cut, None, 0,
+                                        self.acclen, None, self.ntap, self.osamp, 0, None, None)
+        ws = int(lib.alb_repfb_workspace_bytes(nblocks_total, self.osamp))
+        return rc == 0 and ws <= FUSED_WORKSPACE_LIMIT
+
+    def interior(self, g0, nblocks_total):
+        """Re-PFB blocks ``[lo, hi)`` whose rows all lie in the tile whose kept
+        rows start at stream row ``g0``."""
+        lo, hi = ctypes.c_int64(), ctypes.c_int64()
+        _lib.call("alb_repfb_interior", int(g0), self.acclen, self.osamp, self.ntap, int(nblocks_total),
+                  ctypes.byref(lo), ctypes.byref(hi))
+        return lo.value, hi.value
+
+    def edge_blocks(self, nchunks, nblocks_total):
+        """Blocks not interior to any of the ``nchunks`` tiles (sorted int32)."""
+        mask = np.ones(nblocks_total, dtype=bool)
+        for c in range(nchunks):
+            lo, hi = self.interior(c * self.acclen, nblocks_total)
+            mask[lo:hi] = False
+        return np.nonzero(mask)[0].astype(np.int32)
+
+    def stream_repfb(self, packed, rowmap, nchunks, stream_out, nblocks_total, out, sb, sc, bpj, sj):
+        """unpack -> iPFB -> re-PFB of one input into ``out`` (addressing of
+        ``alb_repfb``).  Fused form: the column pass produces re-PFB stage 1 of
+        every block inside one tile and writes only the stream rows the other
+        (edge) blocks read; those run from the stream buffer once all tiles are
+        done, then stage 2 runs over all blocks.  Other shapes: the separate
+        column pass and re-PFB."""
+        if not self.fused_supported(nblocks_total):
+            self.ipfb_stream(packed, rowmap, nchunks, stream_out)
+            self.repfb(stream_out, nblocks_total, out, sb, sc, bpj, sj, block_batch=max(1, 512 // bpj) * bpj)
+            return
+        if stream_out.shape[0] < max((nblocks_total + self.ntap - 1) * self.osamp, nchunks * self.acclen):
+            raise ValueError("stream buffer too short for the re-PFB blocks")
+        s = _lib.stream()
+        rows_valid = nchunks * self.acclen
+        total = stream_out.shape[0]
+        if total > rows_valid:
+            tail = stream_out[rows_valid:]
+            _lib.call("alb_fill_zero", _lib.ptr(tail), tail.numel() * 4, s)
+        ws = self._workspace(nblocks_total)
+        tb = self.tile_batch
+        y = self._tile_buffer(min(tb, nchunks))
+        for t0 in range(0, nchunks, tb):
+            nt = min(tb, nchunks - t0)
+            _lib.call("alb_ipfb_rows_packed4", _lib.ptr(packed), _lib.ptr(rowmap), _lib.ptr(self.chan), self.ncols,
+                      self.acclen, self.cut, t0, nt, _lib.ptr(y), s)
+            _lib.call("alb_ipfb_columns_repfb", _lib.ptr(y), 4096, self.nslice * 4096, nt, self.nslice,
+                      _lib.ptr(self.filt), self.filt.shape[1], self.cut, _lib.ptr(stream_out), t0 * self.acclen,
+                      self.acclen, _lib.ptr(self.win), self.ntap, self.osamp, nblocks_total, _lib.ptr(ws), s)
+        edges = self.edge_blocks(nchunks, nblocks_total)
+        _lib.call("alb_repfb_stage1_list", _lib.ptr(stream_out), _lib.ptr(self.win), self.ntap,
+                  edges.ctypes.data_as(ctypes.c_void_p), len(edges), self.osamp, _lib.ptr(ws), s)
+        _lib.call("alb_repfb_stage2", _lib.ptr(ws), nblocks_total, self.osamp, self.c_lo, self.c_hi, _lib.ptr(out),
+                  sb, sc, bpj, sj, s)
+
     # ------------------------------------------------------------ pipelines
-    def spectra(self, packed, rowmap, nchunks):
+    def spectra(self, packed, rowmap, nchunks, fused=True):
         """Single input (rechannelizer): complex64 ``[nblocks_total, nfine]`` of
         ``cupy_pfb(stream)[:, c_lo:c_hi]`` over all jobs."""
         t = _lib.torch()
         jobs, nbt, rows = self.plan(nchunks)
         stream = t.empty((rows, 4096), dtype=t.float32, device=_lib.device())
-        self.ipfb_stream(packed, rowmap, nchunks, stream)
         out = t.empty((nbt, self.nfine), dtype=t.complex64, device=stream.device)
-        self.repfb(stream, nbt, out, self.nfine, 1, nbt, 0, block_batch=nbt)
+        if fused:
+            self.stream_repfb(packed, rowmap, nchunks, stream, nbt, out, self.nfine, 1, nbt, 0)
+        else:
+            self.ipfb_stream(packed, rowmap, nchunks, stream)
+            self.repfb(stream, nbt, out, self.nfine, 1, nbt, 0, block_batch=nbt)
         return out
 
-    def correlate(self, packed_list, rowmaps, nchunks, return_xin=False):
+    def correlate(self, packed_list, rowmaps, nchunks, return_xin=False, fused=True):
         """All inputs: ``vis`` complex64 F-order ``(M, M, nfine, njobs)`` of
         ``avg_xcorr_all_ant_gpu`` per job (helper_gpu_stream.py:236)."""
         t = _lib.torch()
@@ -210,8 +280,12 @@ class RePFBPipeline:
         stream = t.empty((rows, 4096), dtype=t.float32, device=xin.device)
         bb = max(1, 512 // K) * K
         for m in range(M):
-            self.ipfb_stream(packed_list[m], rowmaps[m], nchunks, stream)
-            self.repfb(stream, nbt, xin.view(-1)[m:], M, M * K, K, M * K * F, block_batch=bb)
+            if fused:
+                self.stream_repfb(packed_list[m], rowmaps[m], nchunks, stream, nbt, xin.view(-1)[m:], M, M * K, K,
+                                  M * K * F)
+            else:
+                self.ipfb_stream(packed_list[m], rowmaps[m], nchunks, stream)
+                self.repfb(stream, nbt, xin.view(-1)[m:], M, M * K, K, M * K * F, block_batch=bb)
         vis = t.empty((njobs, F, M, M), dtype=t.complex64, device=xin.device)
         scale = np.float32(1.0) / np.float32(K)
         _lib.call("alb_xcorr_all_ant", _lib.ptr(xin), M, K, F * njobs, _lib.ptr(vis), float(scale), _lib.stream())

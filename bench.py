@@ -16,6 +16,7 @@ workload) and secondary workloads (direct 4-bit and 1-bit correlators, the
 precision correlator).
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -93,7 +94,8 @@ def rechannelizer(args, torch):
                                tile_batch=nchunks)
     dev = _lib.device()
     packed = _lib.to_device(packed_np.reshape(nchunks, acclen, nkeep))
-    specnums = [_lib.to_device(np.arange(c * acclen, (c + 1) * acclen, dtype=np.int64)) for c in range(nchunks)]
+    # spec_num of every present spectrum of the nchunks chunks, in arrival order
+    specnums = _lib.to_device(np.arange(n_coarse, dtype=np.int64))
     rowmap = torch.empty((nchunks, acclen), dtype=torch.int32, device=dev)
     rows = max((nblk + 3) * osamp, n_coarse)
     stream = torch.zeros((rows, 4096), dtype=torch.float32, device=dev)
@@ -104,18 +106,34 @@ def rechannelizer(args, torch):
     timer = Timer(torch)
     s = _lib.stream
 
+    fused = not args.unfused
+    if fused and not pipe.fused_supported(nblk):
+        raise SystemExit("fused column pass unavailable for this shape")
+    edges = pipe.edge_blocks(nchunks, nblk)
+    edges_p = edges.ctypes.data_as(ctypes.c_void_p)
+
     def step():
-        timer("rowmap", lambda: [
-            _lib.call("alb_build_rowmap", _lib.ptr(specnums[c]), acclen, 0, acclen, c * acclen, _lib.ptr(rowmap[c]), s())
-            for c in range(nchunks)])
+        # the chunks are consecutive acclen-spectrum windows of one spec_num
+        # range, so one call maps all of them: row = spec_num - start
+        timer("rowmap", lambda: _lib.call(
+            "alb_build_rowmap", _lib.ptr(specnums), n_coarse, 0, n_coarse, 0, _lib.ptr(rowmap), s()))
         timer("ipfb_rows_packed4", lambda: _lib.call(
             "alb_ipfb_rows_packed4", _lib.ptr(packed), _lib.ptr(rowmap), _lib.ptr(pipe.chan), nkeep, acclen, cut, 0,
             nchunks, _lib.ptr(tiles), s()))
-        timer("ipfb_columns", lambda: _lib.call(
-            "alb_ipfb_columns", _lib.ptr(tiles), 4096, nslice * 4096, nchunks, nslice, 4096, _lib.ptr(pipe.filt),
-            pipe.filt.shape[1], cut, _lib.ptr(stream), 4096, acclen * 4096, s()))
-        timer("repfb_stage1", lambda: _lib.call(
-            "alb_repfb_stage1", _lib.ptr(stream), _lib.ptr(pipe.win), 4, nblk, osamp, _lib.ptr(ws), s()))
+        if fused:
+            timer("ipfb_columns_repfb", lambda: _lib.call(
+                "alb_ipfb_columns_repfb", _lib.ptr(tiles), 4096, nslice * 4096, nchunks, nslice, _lib.ptr(pipe.filt),
+                pipe.filt.shape[1], cut, _lib.ptr(stream), 0, acclen, _lib.ptr(pipe.win), 4, osamp, nblk,
+                _lib.ptr(ws), s()))
+            timer("repfb_stage1_edges", lambda: _lib.call(
+                "alb_repfb_stage1_list", _lib.ptr(stream), _lib.ptr(pipe.win), 4, edges_p, len(edges), osamp,
+                _lib.ptr(ws), s()))
+        else:
+            timer("ipfb_columns", lambda: _lib.call(
+                "alb_ipfb_columns", _lib.ptr(tiles), 4096, nslice * 4096, nchunks, nslice, 4096, _lib.ptr(pipe.filt),
+                pipe.filt.shape[1], cut, _lib.ptr(stream), 4096, acclen * 4096, s()))
+            timer("repfb_stage1", lambda: _lib.call(
+                "alb_repfb_stage1", _lib.ptr(stream), _lib.ptr(pipe.win), 4, nblk, osamp, _lib.ptr(ws), s()))
         timer("repfb_stage2", lambda: _lib.call(
             "alb_repfb_stage2", _lib.ptr(ws), nblk, osamp, c_lo, c_hi, _lib.ptr(out), c_hi - c_lo, 1, nblk, 0, s()))
 
@@ -139,13 +157,18 @@ def rechannelizer(args, torch):
     log("[bench] rechannelizer: %.3f ms/step (wall %.3f ms/step); kernels %s" % (
         ms, 1e3 * wall / args.steps, json.dumps({k: round(v, 4) for k, v in kern.items()})))
 
-    # algorithmic bytes per launch of each kernel (per-unit figures of the
-    # rechannelizer: raw 100 B + time stream write 16384 B + read 16384 B + fine out 800 B
-    # per coarse spectrum)
+    # algorithmic bytes per launch of each kernel: the per-unit figures of the
+    # rechannelizer (per coarse spectrum: raw 100 B in, time stream 16384 B
+    # written and 16384 B read, 800 B of fine channels out) times the units
+    # the launch covers; the fused column pass produces and consumes the time
+    # stream, so it carries both stream terms.  The edge blocks' re-reads of
+    # the stream are design overhead, counted as what that launch reads.
     alg = {
-        "ipfb_rows_packed4": nslice * nchunks * nkeep,                 # raw bytes in
+        "ipfb_rows_packed4": n_coarse * nkeep,                          # raw bytes in
         "ipfb_columns": n_coarse * 4096 * 4,                            # time stream out
         "repfb_stage1": (nblk + 3) * osamp * 4096 * 4,                  # time stream in
+        "ipfb_columns_repfb": 2 * n_coarse * 4096 * 4,                  # time stream out + in
+        "repfb_stage1_edges": len(edges) * osamp * 4 * 4096 * 4,        # stream rows of the edge blocks
         "repfb_stage2": nblk * (c_hi - c_lo) * 8,                       # fine channels out
     }
     dom = max((k for k in kern if k in alg), key=lambda k: kern[k])
@@ -296,6 +319,8 @@ def main():
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
     ap.add_argument("--onebit-T", type=int, default=2 ** 17)
+    ap.add_argument("--unfused", action="store_true",
+                    help="time the separate column pass + re-PFB stage 1 instead of the fused kernel")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain device copies of 1 GiB (HBM counter calibration) and exit")
     args = ap.parse_args()

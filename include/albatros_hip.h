@@ -135,6 +135,24 @@ int alb_repfb_stage1(const float* ts, const float* win, int ntap, int64_t nblock
                      void* stream);
 int alb_repfb_stage2(const void* A, int64_t nblock, int64_t osamp, int64_t c_lo, int64_t c_hi, void* out, int64_t sb,
                      int64_t sc, int64_t bpj, int64_t sj, void* stream);
+/* The iPFB column pass (alb_ipfb_columns, ncols = 4096, row stride 4096) fused with re-PFB stage 1:
+ * the time stream is the input of pfb_gpu_utils.py:105 cupy_pfb at lblock = 4096*osamp
+ * (helper_gpu_stream.py:199-222).  Kept row i of tile c is stream row
+ * g = row0 + c*rows_per_tile + (i - cut).  For every re-PFB block b < nblocks whose stream rows
+ * [osamp*b, osamp*(b+ntap)) all lie in one tile (alb_repfb_interior) A[b] is produced in the
+ * same launch; only stream rows read by the other blocks are written to ts ([rows][4096]).
+ * Those other blocks go through alb_repfb_stage1_list once every tile that feeds them is done.
+ * nslice 4096 with osamp 32 or 64 (ENOTSUP otherwise: use alb_ipfb_columns + alb_repfb_stage1).
+ * With ntiles = 0 only the shape (nslice, cut, ntap, osamp) is checked; pointers may be null. */
+int alb_ipfb_columns_repfb(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, int64_t nslice,
+                           const void* filt, int64_t ldf, int cut, float* ts, int64_t row0, int64_t rows_per_tile,
+                           const float* win, int ntap, int64_t osamp, int64_t nblocks, void* A, void* stream);
+/* Interior blocks [*lo, *hi) of a tile whose kept rows are stream rows [g0, g0 + nkeep). */
+int alb_repfb_interior(int64_t g0, int64_t nkeep, int64_t osamp, int ntap, int64_t nblocks, int64_t* lo, int64_t* hi);
+/* alb_repfb_stage1 for the n blocks listed in the host array blocks (block b reads ts rows
+ * [osamp*b, osamp*(b+ntap)), writes A[b]); osamp 16, 32 or 64. */
+int alb_repfb_stage1_list(const float* ts, const float* win, int ntap, const int* blocks, int64_t n, int64_t osamp,
+                          void* A, void* stream);
 
 /* ---- streaming helpers (scripts/xcorr/helper_gpu_stream.py) ------------ */
 /* Row map of one chunk for the fused iPFB: rowmap[(specnum[p]-start) % acclen] = row_base + p

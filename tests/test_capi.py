@@ -59,3 +59,42 @@ def test_invalid_arguments_fail_loudly_without_gpu_work():
         _lib.call("alb_sortpols", None, None, None, 0, 1, 1, 8, 4, 0, 1, None)
     with pytest.raises(ValueError):
         _lib.call("alb_xcorr_all_ant", None, 8, 16, 4, None, 1.0, None)
+
+
+def test_repfb_interior_blocks_brute_force():
+    # the interior-block range of a tile against a direct scan (host-only entry point)
+    from albatros_axion_detection_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    import numpy as np
+    rng = np.random.default_rng(7)
+    lo, hi = ctypes.c_int64(), ctypes.c_int64()
+    for _ in range(400):
+        R1 = int(rng.choice([16, 32, 64]))
+        ntap = int(rng.integers(1, 6))
+        nkeep = int(rng.integers(0, 600))
+        g0 = int(rng.integers(0, 1000))
+        nblocks = int(rng.integers(0, 40))
+        _lib.call("alb_repfb_interior", g0, nkeep, R1, ntap, nblocks, ctypes.byref(lo), ctypes.byref(hi))
+        want = [b for b in range(nblocks) if R1 * b >= g0 and R1 * (b + ntap) <= g0 + nkeep]
+        got = list(range(lo.value, hi.value))
+        assert got == want, (g0, nkeep, R1, ntap, nblocks)
+
+
+def test_fused_column_pass_shape_probe():
+    # ntiles = 0 checks the shape only (no pointers, no GPU work)
+    from albatros_axion_detection_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lib = _lib.load()
+    probe = lambda nslice, cut, ntap, osamp: lib.alb_ipfb_columns_repfb(
+        None, 4096, 0, 0, nslice, None, nslice // 2 + 1, cut, None, 0, nslice - 2 * cut, None, ntap, osamp, 0, None,
+        None)
+    assert probe(4096, 4, 4, 64) == 0
+    assert probe(4096, 4, 4, 32) == 0
+    assert probe(4096, 4, 4, 16) == 3      # ENOTSUP: falls back to the two-kernel form
+    assert probe(2048, 4, 4, 64) == 3
+    assert probe(4096, 4, 9, 64) == 3      # window strip too large
+    with pytest.raises(ValueError):
+        _lib.call("alb_ipfb_columns_repfb", None, 4096, 0, 1, 4096, None, 2049, 4, None, 0, 4088, None, 4, 64, 10,
+                  None, None)

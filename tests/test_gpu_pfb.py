@@ -153,6 +153,35 @@ def test_stream_spectra_single_input(rng, acclen, cut):
     assert got.shape == ref.shape and rel_err(got, ref) < TOL
 
 
+@pytest.mark.parametrize("osamp,nblock", [(64, 20), (32, 30)])
+def test_stream_spectra_fused(rng, osamp, nblock):
+    # nslice 4096 (acclen 4088, cut 4): the column pass carries re-PFB stage 1
+    # for blocks inside one tile, the tile-boundary blocks run from the stream
+    # buffer; checked against the oracle and against the two-kernel form
+    from albatros_axion_detection_amd import _lib, repfb
+    acclen, cut, nchunks = 4088, 4, 3
+    chan = np.arange(64, 164)
+    chunks = _packed_chunks(rng, nchunks, acclen, len(chan), loss=0.02)
+    pipe = repfb.RePFBPipeline(1, chan, acclen, cut, osamp, nblock, 64 * osamp, 164 * osamp)
+    jobs, nbt, rows = pipe.plan(nchunks)
+    assert pipe.fused_supported(nbt)
+    edges = pipe.edge_blocks(nchunks, nbt)
+    assert 0 < len(edges) < nbt
+    rm = repfb.rowmaps_from_specnums([c["specnums"] for c in chunks], 1000, acclen)
+    packed = _lib.to_device(repfb.stack_packed(chunks, "pol0", acclen, len(chan)))
+    got = pipe.spectra(packed, rm, nchunks).cpu().numpy()
+    two = pipe.spectra(packed, rm, nchunks, fused=False).cpu().numpy()
+    assert rel_err(got, two) < 2e-6
+    filt = opfb.wiener_filter(opfb.matft_fp64(acclen + 2 * cut), 0.45)
+    st = ostream.ipfb_stream(_decode(chunks), 1000, chan, acclen, cut, filt, 0)
+    need = (nbt + 3) * 4096 * osamp
+    buf = np.zeros(max(need, st.size))
+    buf[: st.size] = st
+    win = opfb.window_f32(4, 4096 * osamp)
+    ref = opfb.pfb(buf[:need], win, 2048 * osamp + 1, 4)[:, 64 * osamp:164 * osamp]
+    assert got.shape == ref.shape and rel_err(got, ref) < TOL
+
+
 def test_stream_xcorr_two_antennas(rng):
     from albatros_axion_detection_amd import repfb
     acclen, cut, osamp, nblock = 120, 4, 2, 4
