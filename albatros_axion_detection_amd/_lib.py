@@ -36,6 +36,7 @@ SIGNATURES = {
     "alb_xcorr_4bit_allpairs": (_i32, [_vp, _i32, _i64, _i64, _i64, _vp, _vp]),
     "alb_xcorr_1bit_allpairs": (_i32, [_vp, _i32, _i64, _i64, _i64, _i64, _vp, _vp]),
     "alb_xcorr_all_ant": (_i32, [_vp, _i64, _i64, _i64, _vp, _f32, _vp]),
+    "alb_xcorr_all_ant_prec": (_i32, [_vp, _i64, _i64, _i64, _vp, _f32, _i32, _vp]),
     "alb_matft_filter": (_i32, [_vp, _i32, _i64, _i64, _f64, _vp, _vp, _vp]),
     "alb_filter_from_matft": (_i32, [_vp, _i64, _f64, _vp, _vp]),
     "alb_irfft_rows": (_i32, [_vp, _i64, _vp, _i64, _i64, _i64, _vp]),
@@ -54,6 +55,8 @@ SIGNATURES = {
                                       _i64, _vp, _vp]),
     "alb_repfb_interior": (_i32, [_i64, _i64, _i64, _i32, _i64, _vp, _vp]),
     "alb_repfb_stage1_list": (_i32, [_vp, _vp, _i32, _vp, _i64, _i64, _vp, _vp]),
+    "alb_tune_set": (_i32, [ctypes.c_char_p, _i64]),
+    "alb_tune_get": (_i64, [ctypes.c_char_p]),
     "alb_build_rowmap": (_i32, [_vp, _i64, _i64, _i64, _i64, _vp, _vp]),
 }
 

@@ -148,13 +148,21 @@ def van_vleck_correction(R0, R1, I0, I1, rowcount):
     return np.sin(R0 * h) + np.sin(R1 * h) + 1j * (np.sin(I1 * h) - np.sin(I0 * h))
 
 
-def avg_xcorr_all_ant_gpu(x, nant, npol, ntime, nfreq, split=1, scratch=None, out=None):
+PRECISION_MODES = {"fp32": 0, "bf16x3": 1, "fp16": 2, "fp16x3": 3}
+
+
+def avg_xcorr_all_ant_gpu(x, nant, npol, ntime, nfreq, split=1, scratch=None, out=None, precision="fp32"):
     """correlations_gpu.py:95-121: ``out[i,j,f] = sum_k x[i,k,f] conj(x[j,k,f]) / K``.
 
     ``x``: complex64 F-order ``(nant*npol, ntime, nfreq)`` (a torch tensor whose
     memory is column-major, e.g. ``t.permute(2,1,0)`` of a C-order
     ``(nfreq, ntime, M)`` tensor, or a numpy F-order array).  Returns the full
     Hermitian ``(M, M, nfreq)`` complex64 in F-order.
+
+    ``precision`` (not in the reference, which runs a cuBLAS FP32 CGEMM):
+    ``"fp32"`` (default), ``"bf16x3"`` (BF16 hi/lo split, three products),
+    ``"fp16"`` or ``"fp16x3"`` (FP16 hi/lo split), all accumulating in FP32 on
+    the matrix cores; the reduced forms take ``nant*npol <= 8``.
     """
     t = _lib.torch()
     M = nant * npol
@@ -179,8 +187,11 @@ def avg_xcorr_all_ant_gpu(x, nant, npol, ntime, nfreq, split=1, scratch=None, ou
         o = out.permute(2, 1, 0)
         if not o.is_contiguous():
             raise ValueError("invalid out buffer")
+    if precision not in PRECISION_MODES:
+        raise ValueError("precision must be one of %s" % sorted(PRECISION_MODES))
     scale = np.float32(1.0) / np.float32(K)
-    _lib.call("alb_xcorr_all_ant", _lib.ptr(xd), M, K, nfreq, _lib.ptr(o), float(scale), _lib.stream())
+    _lib.call("alb_xcorr_all_ant_prec", _lib.ptr(xd), M, K, nfreq, _lib.ptr(o), float(scale),
+              PRECISION_MODES[precision], _lib.stream())
     res = o.permute(2, 1, 0)
     if isnp:
         return np.asfortranarray(_lib.host(o).transpose(2, 1, 0))
@@ -207,12 +218,35 @@ def pair_index(ninput):
     return [(i, j) for i in range(ninput) for j in range(i, ninput)]
 
 
-def xcorr_4bit_allpairs(inputs, blocklen):
+_INT_METHODS = {"auto": 0, "valu": 1, "mfma": 2}
+
+
+class _int_method:
+    """Select the all-pairs integer form for one call ("auto": int8 matrix
+    cores whenever the shape allows; "valu": the VALU / XOR form; "mfma")."""
+
+    def __init__(self, method):
+        if method not in _INT_METHODS:
+            raise ValueError("method must be one of %s" % sorted(_INT_METHODS))
+        self.v = _INT_METHODS[method]
+
+    def __enter__(self):
+        lib = _lib.load()
+        self.old = lib.alb_tune_get(b"xcorr_int_method")
+        _lib.call("alb_tune_set", b"xcorr_int_method", self.v)
+
+    def __exit__(self, *exc):
+        _lib.call("alb_tune_set", b"xcorr_int_method", self.old)
+
+
+def xcorr_4bit_allpairs(inputs, blocklen, method="auto"):
     """Exact block sums of every baseline of 4-bit inputs.
 
     ``inputs``: uint8 ``[ninput, T, nchan]`` per-pol bytes (sortpols layout),
     device tensor or numpy.  Returns int64 ``[T//blocklen, npair, nchan, 2]``
     (re, im of ``x_i conj(x_j)``, pairs in :func:`pair_index` order).
+    ``method``: "auto" (int8 matrix cores when ``blocklen % 64 == 0``),
+    "valu" or "mfma".
     """
     t = _lib.torch()
     isnp = _lib.is_numpy(inputs)
@@ -222,13 +256,16 @@ def xcorr_4bit_allpairs(inputs, blocklen):
     npair = n * (n + 1) // 2
     out = t.empty((nb, npair, C, 2), dtype=t.int64, device=d.device)
     ptrs = (_lib._vp * n)(*[d[i].data_ptr() for i in range(n)])
-    _lib.call("alb_xcorr_4bit_allpairs", ptrs, n, T, C, int(blocklen), _lib.ptr(out), _lib.stream())
+    with _int_method(method):
+        _lib.call("alb_xcorr_4bit_allpairs", ptrs, n, T, C, int(blocklen), _lib.ptr(out), _lib.stream())
     return _finish(out, isnp)
 
 
-def xcorr_1bit_allpairs(inputs, nchan, blocklen):
+def xcorr_1bit_allpairs(inputs, nchan, blocklen, method="auto"):
     """Exact block sums of every baseline of 1-bit inputs (sortpols 1-bit
-    layout ``[ninput, T, ceil(nchan/4)]``) -> int64 ``[T//blocklen, npair, nchan, 2]``."""
+    layout ``[ninput, T, ncol >= ceil(nchan/4)]``) -> int64
+    ``[T//blocklen, npair, nchan, 2]``.  ``method``: "auto" (int8 matrix cores
+    when ``blocklen % 64 == 0`` and ``ncol % 4 == 0``), "valu" (XOR form) or "mfma"."""
     t = _lib.torch()
     isnp = _lib.is_numpy(inputs)
     d = _lib.to_device(inputs, t.uint8)
@@ -237,7 +274,9 @@ def xcorr_1bit_allpairs(inputs, nchan, blocklen):
     npair = n * (n + 1) // 2
     out = t.empty((nb, npair, nchan, 2), dtype=t.int64, device=d.device)
     ptrs = (_lib._vp * n)(*[d[i].data_ptr() for i in range(n)])
-    _lib.call("alb_xcorr_1bit_allpairs", ptrs, n, T, ncol, int(nchan), int(blocklen), _lib.ptr(out), _lib.stream())
+    with _int_method(method):
+        _lib.call("alb_xcorr_1bit_allpairs", ptrs, n, T, ncol, int(nchan), int(blocklen), _lib.ptr(out),
+                  _lib.stream())
     return _finish(out, isnp)
 
 

@@ -77,4 +77,7 @@ struct Twiddle {
 };
 int get_twiddle(int64_t N, Twiddle* out);
 
+// Tuning knob set through alb_tune_set (0 when unset).
+int64_t tune_get(const char* key);
+
 }  // namespace alb

@@ -676,7 +676,7 @@ __device__ __forceinline__ void q2_range(int64_t q1, int64_t R1, int64_t c_lo, i
 template <int R1>
 __global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __restrict__ A, int64_t c_lo, int64_t c_hi,
                                                             float2* __restrict__ out, int64_t sb, int64_t sc, int64_t bpj,
-                                                            int64_t sj, Twiddle tw4, Twiddle twL) {
+                                                            int64_t sj, Twiddle tw4) {
   constexpr int N = 4096;
   constexpr int H = R1 / 2;
   constexpr int NPAD = Padded<N>::value;
@@ -684,18 +684,24 @@ __global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __rest
   float2* Z = smem;
   float2* TA = Z + NPAD;
   float2* TB = TA + tw4.nA;
-  float2* LA = TB + tw4.nB;
-  float2* LB = LA + twL.nA;
   const int tid = threadIdx.x;
   load_twiddle(tw4, TA, TB, tid, kTPB);
-  load_twiddle(twL, LA, LB, tid, kTPB);
   const int q1 = blockIdx.x;
   const int64_t b = blockIdx.y;
   const float2* src = A + (b * (H + 1) + q1) * N;
+  // W_L^(q1 k) = W_L^(q1 c) W_4096^(q1 a) for k = R1 a + c; c = tid mod R1 is
+  // the same for all of this thread's k (kTPB is a multiple of R1)
+  static_assert(kTPB % R1 == 0, "thread stride must keep k mod R1 fixed");
+  float2 wc = make_float2(1.f, 0.f);
+  if (q1 != 0) {
+    double sn, cs;
+    sincospi(-2.0 * double(q1 * (tid % R1)) / double(int64_t(N) * R1), &sn, &cs);
+    wc = make_float2(float(cs), float(sn));
+  }
   __syncthreads();
   for (int k = tid; k < N; k += kTPB) {
     float2 v = src[k];
-    if (q1 != 0) v = cmul(v, tw_lookup(LA, LB, twL.lb, q1 * k));
+    if (q1 != 0) v = cmul(v, cmul(wc, tw_lookup(TA, TB, tw4.lb, (q1 * (k / R1)) & (N - 1))));
     Z[pad16(k)] = v;
   }
   __syncthreads();
@@ -881,6 +887,15 @@ int launch_cols(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles
 int cols_dispatch(int64_t nslice, const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, const float2* filt,
                   int64_t ldf, int64_t ncols, int cut, float* out, int64_t ldo, int64_t out_tile_stride,
                   hipStream_t s) {
+  if (nslice == 4096) {
+    // strip width (columns per workgroup): 8 -> one 1024-thread workgroup per
+    // CU, 4 -> two of 512, 2 -> four of 256
+    switch (tune_get("ipfb_cols_width")) {
+      case 2: return launch_cols<4096, 2, 256>(y, ldy, tile_stride, ntiles, filt, ldf, ncols, cut, out, ldo, out_tile_stride, s);
+      case 4: return launch_cols<4096, 4, 512>(y, ldy, tile_stride, ntiles, filt, ldf, ncols, cut, out, ldo, out_tile_stride, s);
+      default: break;
+    }
+  }
   switch (nslice) {
 #define CASE(n, w, t) case n: return launch_cols<n, w, t>(y, ldy, tile_stride, ntiles, filt, ldf, ncols, cut, out, ldo, out_tile_stride, s);
     CASE(16, 8, 256) CASE(32, 8, 256) CASE(64, 8, 256) CASE(128, 8, 256) CASE(256, 8, 256) CASE(512, 8, 256)
@@ -973,14 +988,12 @@ int launch_cols_repfb(const float* y, int64_t ldy, int64_t tile_stride, int64_t 
 template <int R1>
 int launch_repfb_stage2(int64_t nblock, const float2* A, int64_t c_lo, int64_t c_hi, float2* out, int64_t sb, int64_t sc,
                         int64_t bpj, int64_t sj, hipStream_t s) {
-  Twiddle tw4, twL;
+  Twiddle tw4;
   int rc = get_twiddle(4096, &tw4);
   if (rc) return rc;
-  rc = get_twiddle(int64_t(4096) * R1, &twL);
-  if (rc) return rc;
-  const size_t lds2 = sizeof(float2) * (size_t(Padded<4096>::value) + tw4.nA + tw4.nB + twL.nA + twL.nB);
+  const size_t lds2 = sizeof(float2) * (size_t(Padded<4096>::value) + tw4.nA + tw4.nB);
   hipLaunchKernelGGL((repfb_stage2_kernel<R1>), dim3(R1 / 2 + 1, unsigned(nblock)), dim3(kTPB), lds2, s, A, c_lo, c_hi, out, sb,
-                     sc, bpj, sj, tw4, twL);
+                     sc, bpj, sj, tw4);
   ALB_LAUNCH_CHECK("repfb_stage2_kernel");
   return ALB_OK;
 }
@@ -1196,13 +1209,16 @@ int alb_ipfb_columns_repfb(const float* y, int64_t ldy, int64_t tile_stride, int
   ALB_REQUIRE(cut >= 0 && 2 * cut < nslice, "alb_ipfb_columns_repfb: bad cut");
   ALB_REQUIRE(ntiles == 0 || (ldy >= 4096 && ldf >= nslice / 2 + 1), "alb_ipfb_columns_repfb: bad strides");
   hipStream_t s = as_stream(stream);
-  if (nslice == 4096) {
-    switch (osamp) {
-#define CASE(r) case r: return launch_cols_repfb<4096, 8, 1024, r>(y, ldy, tile_stride, ntiles, (const float2*)filt, ldf, cut, ts, row0, rows_per_tile, win, ntap, nblocks, (float2*)A, s);
-      CASE(32) CASE(64)
+  if (nslice == 4096 && (osamp == 32 || osamp == 64)) {
+    const int64_t w = tune_get("ipfb_cols_width");
+#define CASE(W, T, r) if (w == W && osamp == r) return launch_cols_repfb<4096, W, T, r>(y, ldy, tile_stride, ntiles, (const float2*)filt, ldf, cut, ts, row0, rows_per_tile, win, ntap, nblocks, (float2*)A, s);
+    CASE(2, 256, 32) CASE(2, 256, 64) CASE(4, 512, 32) CASE(4, 512, 64)
 #undef CASE
-      default: break;
-    }
+    if (osamp == 32)
+      return launch_cols_repfb<4096, 8, 1024, 32>(y, ldy, tile_stride, ntiles, (const float2*)filt, ldf, cut, ts, row0,
+                                                  rows_per_tile, win, ntap, nblocks, (float2*)A, s);
+    return launch_cols_repfb<4096, 8, 1024, 64>(y, ldy, tile_stride, ntiles, (const float2*)filt, ldf, cut, ts, row0,
+                                                rows_per_tile, win, ntap, nblocks, (float2*)A, s);
   }
   set_error("alb_ipfb_columns_repfb: nslice %lld with osamp %lld not supported (nslice 4096, osamp 32 or 64)",
             (long long)nslice, (long long)osamp);

@@ -2,6 +2,7 @@
 // the small C-ABI housekeeping entry points (version, last error, sync).
 #include <mutex>
 #include <map>
+#include <string>
 #include <utility>
 #include "common.hpp"
 
@@ -62,9 +63,37 @@ int get_twiddle(int64_t N, Twiddle* out) {
   return ALB_OK;
 }
 
+// Tuning knobs (0 = the built-in choice).
+static std::mutex g_tune_mu;
+static std::map<std::string, int64_t> g_tune;
+
+int64_t tune_get(const char* key) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(key);
+  return it == g_tune.end() ? 0 : it->second;
+}
+
 }  // namespace alb
 
 extern "C" {
+
+int alb_tune_set(const char* key, int64_t value) {
+  ALB_REQUIRE(key, "alb_tune_set: null key");
+  const std::string k(key);
+  if (k == "ipfb_cols_width") {
+    ALB_REQUIRE(value == 0 || value == 2 || value == 4 || value == 8,
+                "alb_tune_set: ipfb_cols_width must be 0, 2, 4 or 8");
+  } else if (k == "xcorr_int_method") {
+    ALB_REQUIRE(value >= 0 && value <= 2, "alb_tune_set: xcorr_int_method must be 0, 1 or 2");
+  } else {
+    ALB_REQUIRE(false, "alb_tune_set: unknown key '%s'", key);
+  }
+  std::lock_guard<std::mutex> lk(alb::g_tune_mu);
+  alb::g_tune[k] = value;
+  return ALB_OK;
+}
+
+int64_t alb_tune_get(const char* key) { return key ? alb::tune_get(key) : 0; }
 
 const char* alb_last_error(void) { return alb::last_error(); }
 

@@ -54,6 +54,80 @@ __global__ __launch_bounds__(256) void xcorr_mfma16_kernel(const float* __restri
   }
 }
 
+// Reduced-precision forms of the same Gram product on the 16x16x32 matrix cores
+// (FP32 accumulation), one wavefront per channel:
+//   PREC_BF16X3  x = hi + lo with hi = bf16(x), lo = bf16(x - hi);
+//                G = hi hi^T + (hi lo^T + lo hi^T)   (lo lo^T dropped)
+//   PREC_FP16    x -> fp16, one product
+//   PREC_FP16X3  x = hi + 2^-11 lo with hi = fp16(x), lo = fp16(2^11 (x - hi)) (the
+//                scaled residual stays out of the fp16 subnormal range);
+//                G = hi hi^T + 2^-11 (hi lo^T + lo hi^T)
+// Lane l feeds time samples t0 + 8 (l / 16) + i (i < 8) of realified row l % 16 to
+// both operands; A and B share one k order, so any assignment of samples to k is a
+// valid Gram sum.
+enum { PREC_FP32 = 0, PREC_BF16X3 = 1, PREC_FP16 = 2, PREC_FP16X3 = 3 };
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+template <int M, int MODE>
+__global__ __launch_bounds__(256) void xcorr_mfma16_lp_kernel(const float* __restrict__ x, int64_t K, int64_t F,
+                                                              float2* __restrict__ out, float scale) {
+  __shared__ float G[4][16 * 16];
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int64_t f = int64_t(blockIdx.x) * 4 + w;
+  const bool valid = f < F;
+  const int r = lane % 16, grp = lane / 16;
+  const int m = r % 8, part = r / 8;
+  const bool row_ok = valid && m < M;
+  const float* base = x + (valid ? f : 0) * int64_t(2 * M) * K + 2 * m + part;
+  floatx4 accH = {0.f, 0.f, 0.f, 0.f}, accX = accH;
+  for (int64_t t0 = 0; t0 < K; t0 += 32) {
+    float v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int64_t t = t0 + 8 * grp + i;
+      v[i] = (row_ok && t < K) ? base[int64_t(2 * M) * t] : 0.f;
+    }
+    if constexpr (MODE == PREC_BF16X3) {
+      bf16x8 hi, lo;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        hi[i] = (__bf16)v[i];
+        lo[i] = (__bf16)(v[i] - (float)hi[i]);
+      }
+      accH = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, hi, accH, 0, 0, 0);
+      accX = __builtin_amdgcn_mfma_f32_16x16x32_bf16(hi, lo, accX, 0, 0, 0);
+      accX = __builtin_amdgcn_mfma_f32_16x16x32_bf16(lo, hi, accX, 0, 0, 0);
+    } else if constexpr (MODE == PREC_FP16) {
+      halfx8 h;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) h[i] = (_Float16)v[i];
+      accH = __builtin_amdgcn_mfma_f32_16x16x32_f16(h, h, accH, 0, 0, 0);
+    } else {
+      halfx8 hi, lo;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        hi[i] = (_Float16)v[i];
+        lo[i] = (_Float16)((v[i] - (float)hi[i]) * 2048.f);
+      }
+      accH = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi, hi, accH, 0, 0, 0);
+      accX = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi, lo, accX, 0, 0, 0);
+      accX = __builtin_amdgcn_mfma_f32_16x16x32_f16(lo, hi, accX, 0, 0, 0);
+    }
+  }
+  constexpr float xs = MODE == PREC_FP16X3 ? 1.f / 2048.f : 1.f;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) G[w][(4 * grp + v) * 16 + r] = accH[v] + accX[v] * xs;
+  __syncthreads();
+  if (valid && lane < M * M) {
+    const int i = lane % M, j = lane / M;
+    const float* g = G[w];
+    const float re = g[i * 16 + j] + g[(i + 8) * 16 + (j + 8)];
+    const float im = g[(i + 8) * 16 + j] - g[i * 16 + (j + 8)];
+    out[f * M * M + int64_t(j) * M + i] = make_float2(re * scale, im * scale);
+  }
+}
+
 // Generic: workgroup per channel, thread per baseline i <= j.
 __global__ __launch_bounds__(256) void xcorr_valu_kernel(const float2* __restrict__ x, int M, int64_t K, int64_t F,
                                                          float2* __restrict__ out, float scale) {
@@ -104,6 +178,33 @@ int alb_xcorr_all_ant(const void* x, int64_t M, int64_t K, int64_t F, void* out,
                          (float2*)out, scale);
   }
   ALB_LAUNCH_CHECK("xcorr_all_ant");
+  return ALB_OK;
+}
+
+// The same product in a chosen arithmetic: 0 FP32 (as alb_xcorr_all_ant), 1 BF16 hi/lo
+// (three products), 2 FP16, 3 FP16 hi/lo; all accumulate in FP32 on the matrix
+// cores.  Modes 1-3 take M <= 8.
+int alb_xcorr_all_ant_prec(const void* x, int64_t M, int64_t K, int64_t F, void* out, float scale, int mode,
+                           void* stream) {
+  ALB_REQUIRE(mode >= 0 && mode <= 3, "alb_xcorr_all_ant_prec: mode must be 0..3");
+  if (mode == PREC_FP32) return alb_xcorr_all_ant(x, M, K, F, out, scale, stream);
+  ALB_REQUIRE(x && out, "alb_xcorr_all_ant_prec: null pointer");
+  ALB_REQUIRE(M >= 1 && K >= 0 && F >= 0, "alb_xcorr_all_ant_prec: bad sizes");
+  if (M > 8) {
+    set_error("alb_xcorr_all_ant_prec: reduced-precision forms take M <= 8 (got %lld)", (long long)M);
+    return ALB_ENOTSUP;
+  }
+  if (F == 0) return ALB_OK;
+  hipStream_t s = as_stream(stream);
+  const dim3 grid(unsigned((F + 3) / 4));
+#define CASE(m, md) case m: hipLaunchKernelGGL((xcorr_mfma16_lp_kernel<m, md>), grid, dim3(256), 0, s, (const float*)x, K, F, (float2*)out, scale); break;
+#define MODES(md) switch (M) { CASE(1, md) CASE(2, md) CASE(3, md) CASE(4, md) CASE(5, md) CASE(6, md) CASE(7, md) CASE(8, md) default: break; }
+  if (mode == PREC_BF16X3) { MODES(PREC_BF16X3) }
+  else if (mode == PREC_FP16) { MODES(PREC_FP16) }
+  else { MODES(PREC_FP16X3) }
+#undef MODES
+#undef CASE
+  ALB_LAUNCH_CHECK("xcorr_all_ant_prec");
   return ALB_OK;
 }
 

@@ -243,6 +243,192 @@ __global__ __launch_bounds__(256) void allpairs1_kernel(InputPtrs in, int nin, i
   }
 }
 
+// ------------------------------------------------ all pairs on the int8 matrix cores
+// One wavefront = one dword column w (4 bytes of every spectrum) of one block;
+// lane l takes realified row l % 16 and the 16 spectra t0 + 16 (l / 16) + i of
+// each 64-spectrum step, so every v_mfma_i32_16x16x64_i8 adds 64 spectra to a
+// 16 x 16 Gram matrix G = Y Y^T (A and B share registers and k order).  Sums are
+// exact in int32 (|G| <= 64 * blocklen, checked on the host).
+typedef int intx4 __attribute__((ext_vector_type(4)));
+
+// byte j of x0..x3 -> bytes 0..3 (v_perm_b32; selector 0x0c gives 0x00)
+__device__ __forceinline__ uint32_t gather_byte(uint32_t x0, uint32_t x1, uint32_t x2, uint32_t x3, int j) {
+  const uint32_t s01 = uint32_t(j) | (uint32_t(4 + j) << 8) | 0x0c0c0000u;
+  const uint32_t s23 = 0x00000c0cu | (uint32_t(j) << 16) | (uint32_t(4 + j) << 24);
+  return __builtin_amdgcn_perm(x1, x0, s01) | __builtin_amdgcn_perm(x3, x2, s23);
+}
+
+// Work item of a workgroup: XCD-aware so the column groups of one block run
+// together on one XCD (workgroup i is dispatched to XCD i % 8) and share the
+// cache lines of the block's rows.
+__device__ __forceinline__ void mfma_item(int64_t ngx, int64_t nblock, int64_t* blk, int64_t* gx) {
+  const int64_t total = ngx * nblock;
+  int64_t item = blockIdx.x;
+  if (total % 8 == 0) item = (blockIdx.x % 8) * (total / 8) + blockIdx.x / 8;
+  *blk = item / ngx;
+  *gx = item - *blk * ngx;
+}
+
+// 4-bit: rows 0..7 = real parts of inputs 0..7, rows 8..15 = imaginary parts
+// (inputs >= NIN are zero); one channel (byte) per product.
+// Re_ab = G[a][b] + G[8+a][8+b],  Im_ab = G[8+a][b] - G[a][8+b]  (x_a conj x_b).
+template <int NIN>
+__global__ __launch_bounds__(256) void allpairs4_mfma_kernel(InputPtrs in, int64_t ncol, int64_t blocklen,
+                                                             int64_t nblock, long long* __restrict__ out) {
+  constexpr int NP = NIN * (NIN + 1) / 2;
+  __shared__ int S[4][4][256];
+  const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+  int64_t blk, gx;
+  mfma_item((ncol / 4 + 3) / 4, nblock, &blk, &gx);
+  const int64_t w = gx * 4 + wv;
+  const bool wok = w < ncol / 4;
+  const int r = lane % 16, g = lane / 16;
+  const int u = r % 8, part = r / 8;
+  const bool uok = wok && u < NIN;
+  const uint8_t* src = in.p[uok ? u : 0] + 4 * (wok ? w : 0);
+  intx4 acc[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc[j] = intx4{0, 0, 0, 0};
+  // control flow stays wave-uniform around the matrix instructions: rows of
+  // absent inputs load nothing and feed zeros
+  for (int64_t t0 = blk * blocklen; t0 < (blk + 1) * blocklen; t0 += 64) {
+    uint32_t x[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      x[i] = uok ? *reinterpret_cast<const uint32_t*>(src + (t0 + 16 * g + i) * ncol) : 0u;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      intx4 v;
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        const uint32_t pd = gather_byte(x[4 * d], x[4 * d + 1], x[4 * d + 2], x[4 * d + 3], j);
+        const uint32_t nib = part ? (pd & 0x0F0F0F0Fu) : ((pd >> 4) & 0x0F0F0F0Fu);
+        // v > 8 -> v - 16 per byte: set the high nibble of those bytes
+        uint32_t sgn = ((nib + 0x07070707u) >> 4) & 0x01010101u;
+        sgn <<= 4;
+        sgn |= sgn << 1;
+        sgn |= sgn << 2;
+        v[d] = int(nib | sgn);
+      }
+      acc[j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(v, v, acc[j], 0, 0, 0);
+    }
+  }
+  // D layout: lane l holds G[4 (l / 16) + v][l % 16]
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) S[wv][j][(4 * g + v) * 16 + r] = acc[j][v];
+  __syncthreads();
+  if (wok) {
+    for (int it = lane; it < NP * 4; it += 64) {
+      const int j = it % 4, p = it / 4;
+      int a = 0, rem = p;
+      while (rem >= NIN - a) { rem -= NIN - a; ++a; }
+      const int b = a + rem;
+      const int* G = S[wv][j];
+      const long long re = (long long)G[a * 16 + b] + G[(8 + a) * 16 + 8 + b];
+      const long long im = (long long)G[(8 + a) * 16 + b] - G[a * 16 + 8 + b];
+      const int64_t ch = 4 * w + j;
+      long long* o = out + ((blk * NP + p) * ncol + ch) * 2;
+      o[0] = re;
+      o[1] = im;
+    }
+  }
+}
+
+// 1-bit: bit b -> x = 2b - 1.  The products run on the bits themselves
+// (0/1 int8): with G' = B B^T,  sum x_a x_b = 4 G'_ab - 2 G'_aa - 2 G'_bb + K
+// (b^2 = b, so the diagonal holds the bit counts).  Row u = input u; three
+// products per channel: re re^T, im im^T, re im^T.  A wave takes BPW bytes
+// (4 BPW channels) of a dword column: the two waves of a dword share its loads
+// through L1 and each keeps 12 BPW accumulator tiles.
+constexpr int kBPW1 = 2;
+constexpr int kDwordsPerWG1 = 4 * kBPW1 / 4;
+
+template <int NIN>
+__global__ __launch_bounds__(256) void allpairs1_mfma_kernel(InputPtrs in, int64_t ncol, int64_t nchan,
+                                                             int64_t blocklen, int64_t nblock,
+                                                             long long* __restrict__ out) {
+  constexpr int NP = NIN * (NIN + 1) / 2;
+  constexpr int BPW = kBPW1;
+  __shared__ int S[4][4][3][256];
+  const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+  int64_t blk, gx;
+  mfma_item((ncol / 4 + kDwordsPerWG1 - 1) / kDwordsPerWG1, nblock, &blk, &gx);
+  const int64_t w = gx * kDwordsPerWG1 + wv / (4 / BPW);
+  const int j0 = (wv % (4 / BPW)) * BPW;
+  const bool wok = w < ncol / 4;
+  const int u = lane % 16, g = lane / 16;
+  const bool uok = wok && u < NIN;
+  const uint8_t* src = in.p[uok ? u : 0] + 4 * (wok ? w : 0);
+  intx4 acc[BPW][4][3];
+#pragma unroll
+  for (int j = 0; j < BPW; ++j)
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int m = 0; m < 3; ++m) acc[j][c][m] = intx4{0, 0, 0, 0};
+  for (int64_t t0 = blk * blocklen; t0 < (blk + 1) * blocklen; t0 += 64) {
+    uint32_t x[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i)
+      x[i] = uok ? *reinterpret_cast<const uint32_t*>(src + (t0 + 16 * g + i) * ncol) : 0u;
+#pragma unroll
+    for (int j = 0; j < BPW; ++j) {
+      uint32_t pd[4];
+#pragma unroll
+      for (int d = 0; d < 4; ++d)
+        pd[d] = gather_byte(x[4 * d], x[4 * d + 1], x[4 * d + 2], x[4 * d + 3], j0 + j);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        intx4 re, im;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+          re[d] = int((pd[d] >> (7 - 2 * c)) & 0x01010101u);
+          im[d] = int((pd[d] >> (6 - 2 * c)) & 0x01010101u);
+        }
+        acc[j][c][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(re, re, acc[j][c][0], 0, 0, 0);
+        acc[j][c][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(im, im, acc[j][c][1], 0, 0, 0);
+        acc[j][c][2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(re, im, acc[j][c][2], 0, 0, 0);
+      }
+    }
+  }
+  const long long K = blocklen;
+#pragma unroll
+  for (int j = 0; j < BPW; ++j) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) S[wv][c][m][(4 * g + v) * 16 + u] = acc[j][c][m][v];
+    __syncthreads();
+    if (wok) {
+      for (int it = lane; it < NP * 4; it += 64) {
+        const int c = it % 4, p = it / 4;
+        int a = 0, rem = p;
+        while (rem >= NIN - a) { rem -= NIN - a; ++a; }
+        const int b = a + rem;
+        const int* RR = S[wv][c][0];
+        const int* II = S[wv][c][1];
+        const int* RI = S[wv][c][2];
+        const long long ra = RR[a * 17], rb = RR[b * 17], ia = II[a * 17], ib = II[b * 17];
+        const long long gre = 4LL * RR[a * 16 + b] - 2 * ra - 2 * rb + K;     // sum re_a re_b
+        const long long gim = 4LL * II[a * 16 + b] - 2 * ia - 2 * ib + K;     // sum im_a im_b
+        const long long xab = 4LL * RI[a * 16 + b] - 2 * ra - 2 * ib + K;     // sum re_a im_b
+        const long long xba = 4LL * RI[b * 16 + a] - 2 * rb - 2 * ia + K;     // sum re_b im_a
+        const int64_t ch = 4 * (4 * w + j0 + j) + c;
+        if (ch < nchan) {
+          long long* o = out + ((blk * NP + p) * nchan + ch) * 2;
+          o[0] = gre + gim;
+          o[1] = xba - xab;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 static inline dim3 grid1(int64_t n, int bs = 256) { return dim3(unsigned((n + bs - 1) / bs)); }
 
 }  // namespace alb
@@ -318,6 +504,20 @@ int alb_xcorr_4bit_allpairs(const uint8_t* const* inputs, int ninput, int64_t nr
   const int64_t nblock = nrows / blocklen;
   const int64_t npair = int64_t(ninput) * (ninput + 1) / 2;
   hipStream_t s = as_stream(stream);
+  const int64_t method = tune_get("xcorr_int_method");
+  const bool mfma_ok = blocklen % 64 == 0 && blocklen <= (int64_t(1) << 24);
+  ALB_REQUIRE(method != 2 || mfma_ok, "alb_xcorr_4bit_allpairs: the matrix-core form needs blocklen %% 64 == 0");
+  if (method != 1 && mfma_ok) {
+    const int64_t total = ((ncol / 4 + 3) / 4) * nblock;
+    ALB_REQUIRE(total < (int64_t(1) << 31), "alb_xcorr_4bit_allpairs: too many blocks");
+    switch (ninput) {
+#define CASE(n) case n: hipLaunchKernelGGL(allpairs4_mfma_kernel<n>, dim3(unsigned(total)), dim3(256), 0, s, ip, ncol, blocklen, nblock, (long long*)out); break;
+      CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+    }
+    ALB_LAUNCH_CHECK("allpairs4_mfma_kernel");
+    return ALB_OK;
+  }
   hipLaunchKernelGGL(fill_i64, grid1(nblock * npair * ncol * 2), dim3(256), 0, s, out, nblock * npair * ncol * 2);
   const int nsplit = int(blocklen >= 1024 ? 8 : 1);
   ALB_REQUIRE(blocklen / nsplit <= (int64_t(1) << 24), "alb_xcorr_4bit_allpairs: block too long for int32 partials");
@@ -347,6 +547,22 @@ int alb_xcorr_1bit_allpairs(const uint8_t* const* inputs, int ninput, int64_t nr
   const int64_t nblock = nrows / blocklen;
   const int npair = ninput * (ninput + 1) / 2;
   hipStream_t s = as_stream(stream);
+  const int64_t method = tune_get("xcorr_int_method");
+  const bool mfma_ok = blocklen % 64 == 0 && ncol % 4 == 0 && blocklen <= (int64_t(1) << 28);
+  ALB_REQUIRE(method != 2 || mfma_ok,
+              "alb_xcorr_1bit_allpairs: the matrix-core form needs blocklen %% 64 == 0 and ncol %% 4 == 0");
+  if (method != 1 && mfma_ok) {
+    const int64_t total = ((ncol / 4 + kDwordsPerWG1 - 1) / kDwordsPerWG1) * nblock;
+    ALB_REQUIRE(total < (int64_t(1) << 31), "alb_xcorr_1bit_allpairs: too many blocks");
+    switch (ninput) {
+#define CASE(n) case n: hipLaunchKernelGGL(allpairs1_mfma_kernel<n>, dim3(unsigned(total)), dim3(256), 0, s, ip, ncol, nchan, blocklen, nblock, (long long*)out); break;
+      CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
+      CASE(14) CASE(15) CASE(16)
+#undef CASE
+    }
+    ALB_LAUNCH_CHECK("allpairs1_mfma_kernel");
+    return ALB_OK;
+  }
   hipLaunchKernelGGL(fill_i64, grid1(nblock * npair * nchan * 2), dim3(256), 0, s, out, nblock * npair * nchan * 2);
   const int nsplit = int(blocklen >= 2048 ? 4 : 1);
   const int pairs_per = 8;

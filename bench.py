@@ -137,6 +137,27 @@ def rechannelizer(args, torch):
         timer("repfb_stage2", lambda: _lib.call(
             "alb_repfb_stage2", _lib.ptr(ws), nblk, osamp, c_lo, c_hi, _lib.ptr(out), c_hi - c_lo, 1, nblk, 0, s()))
 
+    # launch-shape autotuning (untimed): strip width of the column pass
+    lib = _lib.load()
+    sweep = {}
+    widths = [args.cols_width] if args.cols_width else [8, 4, 2]
+    for w in widths:
+        lib.alb_tune_set(b"ipfb_cols_width", w)
+        step()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(3):
+            step()
+        b.record()
+        torch.cuda.synchronize()
+        sweep[w] = a.elapsed_time(b) / 3
+    best = min(sweep, key=sweep.get)
+    lib.alb_tune_set(b"ipfb_cols_width", best)
+    log("[bench] column-pass width sweep (ms/step): %s -> %d" % (
+        json.dumps({str(k): round(v, 4) for k, v in sweep.items()}), best))
+
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -192,6 +213,7 @@ def rechannelizer(args, torch):
                    "n_coarse": n_coarse, "acclen": acclen, "cut": cut, "osamp": osamp, "nchan_fine": c_hi - c_lo,
                    "nblock": nblk},
         "ref_unit_msps": n_coarse * 2049 * 8 / (ms * 1e-3) / 1e6,
+        "tuning": {"ipfb_cols_width": best, "sweep_ms_per_step": {str(k): round(v, 4) for k, v in sweep.items()}},
     }
     ctx = dict(packed_np=packed_np, acclen=acclen, cut=cut, osamp=osamp, chan0=chan0, nkeep=nkeep,
                filt=pipe.filt, win=pipe.win)
@@ -255,38 +277,72 @@ def secondary(args, torch):
         torch.cuda.synchronize()
         return a.elapsed_time(b) / n
 
-    # direct 4-bit
+    # direct 4-bit: 4 inputs x 2^17 spectra x 128 ch, 10 pairs, 4096-blocks
+    lib = _lib.load()
     inp = _lib.to_device(synth.direct4(T=2 ** 17, nchan=128, seed=0))
-    o4 = torch.empty((32, 10, 128, 2), dtype=torch.int64, device=dev)
     ptrs = (_lib._vp * 4)(*[inp[i].data_ptr() for i in range(4)])
-    ms = timeit(lambda: _lib.call("alb_xcorr_4bit_allpairs", ptrs, 4, 2 ** 17, 128, 4096, _lib.ptr(o4), _lib.stream()))
-    b = 68091904
-    out["direct4"] = {"workload": "4 inputs x 2^17 spectra x 128 ch, 10 pairs, 4096-blocks", "ms": ms,
-                      "Msamples_s": 4 * 2 ** 17 * 128 / (ms * 1e-3) / 1e6,
-                      "hbm_frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-    del inp, o4
-    # precision correlator, FP32 on MFMA
-    x = synth.precision_xcorr(M=8, T=1024, F=6400, seed=2, device=dev)
-    xd = x.permute(2, 1, 0)
-    vis = torch.empty((6400, 8, 8), dtype=torch.complex64, device=dev)
-    ms = timeit(lambda: _lib.call("alb_xcorr_all_ant", _lib.ptr(xd), 8, 1024, 6400, _lib.ptr(vis),
-                                  float(np.float32(1) / np.float32(1024)), _lib.stream()))
-    b = 422707200
-    out["xcorr_fp32_mfma"] = {"workload": "8 inputs x 1024 x 6400 ch complex64", "ms": ms,
-                              "Msamples_s": 8 * 1024 * 6400 / (ms * 1e-3) / 1e6,
-                              "hbm_frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-    del x, xd, vis
-    # direct 1-bit (reduced T when asked)
+    b4 = 68091904
+    forms, res4 = {}, {}
+    for name, m in (("mfma_i8", 2), ("valu", 1)):
+        o4 = torch.empty((32, 10, 128, 2), dtype=torch.int64, device=dev)
+        lib.alb_tune_set(b"xcorr_int_method", m)
+        ms = timeit(lambda: _lib.call("alb_xcorr_4bit_allpairs", ptrs, 4, 2 ** 17, 128, 4096, _lib.ptr(o4),
+                                      _lib.stream()))
+        res4[name] = o4
+        forms[name] = {"ms": round(ms, 5), "Msamples_s": round(4 * 2 ** 17 * 128 / (ms * 1e-3) / 1e6, 1),
+                       "hbm_frac": round(b4 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    forms["mfma_i8"]["mfma_frac"] = round(32 * 128 * 64 * 32768 / (forms["mfma_i8"]["ms"] * 1e-3) / 5e15, 5)
+    lib.alb_tune_set(b"xcorr_int_method", 0)
+    out["direct4"] = {"workload": "4 inputs x 2^17 spectra x 128 ch, 10 pairs, 4096-blocks, int64 sums",
+                      "forms": forms, "forms_bit_identical": bool(torch.equal(res4["mfma_i8"], res4["valu"]))}
+    del inp, res4
+    # direct 1-bit: 16 inputs x T spectra x 1024 ch (256 B), 136 pairs, 8192-blocks
     T1 = args.onebit_T
     inp = synth.direct1(ninput=16, T=T1, nchan=1024, seed=3, device=dev)
-    o1 = torch.empty((T1 // 8192, 136, 1024, 2), dtype=torch.int64, device=dev)
     ptrs = (_lib._vp * 16)(*[inp[i].data_ptr() for i in range(16)])
-    ms = timeit(lambda: _lib.call("alb_xcorr_1bit_allpairs", ptrs, 16, T1, 256, 1024, 8192, _lib.ptr(o1),
-                                  _lib.stream()), n=2)
-    b = 16 * T1 * 256 + (T1 // 8192) * 136 * 1024 * 2 * 8
-    out["direct1_xor"] = {"workload": "16 inputs x %d spectra x 1024 ch, 136 pairs, 8192-blocks" % T1, "ms": ms,
-                          "Msamples_s": 16 * T1 * 1024 / (ms * 1e-3) / 1e6,
-                          "hbm_frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    nb1 = T1 // 8192
+    b1 = 16 * T1 * 256 + nb1 * 136 * 1024 * 2 * 8
+    forms, res1 = {}, {}
+    for name, m, n in (("mfma_i8", 2, 5), ("xor_valu", 1, 2)):
+        o1 = torch.empty((nb1, 136, 1024, 2), dtype=torch.int64, device=dev)
+        lib.alb_tune_set(b"xcorr_int_method", m)
+        ms = timeit(lambda: _lib.call("alb_xcorr_1bit_allpairs", ptrs, 16, T1, 256, 1024, 8192, _lib.ptr(o1),
+                                      _lib.stream()), n=n)
+        res1[name] = o1
+        forms[name] = {"ms": round(ms, 4), "Msamples_s": round(16 * T1 * 1024 / (ms * 1e-3) / 1e6, 1),
+                       "hbm_frac": round(b1 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    forms["mfma_i8"]["mfma_frac"] = round(3 * 1024 * (T1 // 64) * 32768 / (forms["mfma_i8"]["ms"] * 1e-3) / 5e15, 4)
+    lib.alb_tune_set(b"xcorr_int_method", 0)
+    out["direct1"] = {"workload": "16 inputs x %d spectra x 1024 ch, 136 pairs, 8192-blocks, int64 sums" % T1,
+                      "forms": forms, "forms_bit_identical": bool(torch.equal(res1["mfma_i8"], res1["xor_valu"]))}
+    del inp, res1
+    # precision correlator: 8 inputs x 1024 fine spectra x 6400 channels, four
+    # arithmetic forms on the matrix cores, each against a float64 product
+    x = synth.precision_xcorr(M=8, T=1024, F=6400, seed=2, device=dev)
+    xd = x.permute(2, 1, 0)                     # C-order [F][T][M]
+    x64 = xd.to(torch.complex128)
+    ref = torch.matmul(x64.transpose(1, 2).contiguous(), x64.conj().contiguous()) / 1024.0   # [f, i, j]
+    dg = torch.sqrt(torch.diagonal(ref, dim1=1, dim2=2).real)
+    norm = dg[:, :, None] * dg[:, None, :]
+    del x64
+    vis = torch.empty((6400, 8, 8), dtype=torch.complex64, device=dev)
+    b = 422707200
+    flops = 16 * 16 * 1024 * 2 * 6400           # one realified 16x16 Gram product per channel
+    peaks = {"fp32": FP32_MFMA_TFLOPS, "bf16x3": 2500.0, "fp16": 2500.0, "fp16x3": 2500.0}
+    prec = {}
+    for mode, name in enumerate(("fp32", "bf16x3", "fp16", "fp16x3")):
+        ms = timeit(lambda: _lib.call("alb_xcorr_all_ant_prec", _lib.ptr(xd), 8, 1024, 6400, _lib.ptr(vis),
+                                      float(np.float32(1) / np.float32(1024)), mode, _lib.stream()))
+        c = vis.transpose(1, 2).to(torch.complex128)   # vis[f, j, i] = C_ij
+        err = float(torch.max(torch.abs(c - ref) / norm))
+        nprod = 3 if name.endswith("x3") else 1
+        prec[name] = {"ms": round(ms, 5), "Msamples_s": round(8 * 1024 * 6400 / (ms * 1e-3) / 1e6, 1),
+                      "max_rel_err_vs_fp64": err,
+                      "hbm_frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                      "mfma_frac": round(nprod * flops / (ms * 1e-3) / 1e12 / peaks[name], 4)}
+    out["xcorr_precision"] = {"workload": "8 inputs x 1024 x 6400 ch complex64 (F-order), full 8x8 per channel; "
+                                          "error = max |C - C64| / sqrt(C64_ii C64_jj)", "forms": prec}
+    del x, xd, vis, ref, norm
     return out
 
 
@@ -318,7 +374,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=30.0)
-    ap.add_argument("--onebit-T", type=int, default=2 ** 17)
+    ap.add_argument("--onebit-T", type=int, default=2 ** 20)
+    ap.add_argument("--cols-width", type=int, default=0, choices=[0, 2, 4, 8],
+                    help="column-pass strip width (0: pick the fastest of 8, 4, 2 before timing)")
     ap.add_argument("--unfused", action="store_true",
                     help="time the separate column pass + re-PFB stage 1 instead of the fused kernel")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -362,6 +420,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels_ms": res["kernels_ms"],
         "ref_unit_msps": round(res["ref_unit_msps"], 1),
+        "tuning": res["tuning"],
         "secondary": sec,
     }
     print(json.dumps(line), flush=True)

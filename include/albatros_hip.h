@@ -83,6 +83,11 @@ int alb_xcorr_1bit_allpairs(const uint8_t* const* inputs, int ninput, int64_t nr
 /* correlations_gpu.py:95 avg_xcorr_all_ant_gpu / cgemm_batch.cu:8 cgemm_strided_batched (C = A B^H):
  * x complex64 F-order (M, K, F); out complex64 F-order (M, M, F) = x x^H * scale. */
 int alb_xcorr_all_ant(const void* x, int64_t M, int64_t K, int64_t F, void* out, float scale, void* stream);
+/* The same product in a chosen arithmetic on the matrix cores, FP32 accumulation:
+ * mode 0 FP32 (= alb_xcorr_all_ant), 1 BF16 hi/lo split (hi hi^T + hi lo^T + lo hi^T),
+ * 2 FP16, 3 FP16 hi/lo split; modes 1-3 take M <= 8 (ENOTSUP otherwise). */
+int alb_xcorr_all_ant_prec(const void* x, int64_t M, int64_t K, int64_t F, void* out, float scale, int mode,
+                           void* stream);
 
 /* ---- PFB / inverse PFB (src/utils/pfb_gpu_utils.py, pycufft.py/.cu) ---- */
 /* pfb_gpu_utils.py:123 get_matft + :29 calculate_filter in float64: win float32 [ntap*lblock];
@@ -160,6 +165,15 @@ int alb_repfb_stage1_list(const float* ts, const float* win, int ntap, const int
  * make_continuous_gpu). */
 int alb_build_rowmap(const int64_t* specnum, int64_t npresent, int64_t start, int64_t acclen, int64_t row_base,
                      int* rowmap, void* stream);
+
+/* ---- tuning (no counterpart in the reference) ---------------------------- */
+/* Set a launch-shape knob; 0 restores the built-in choice.  Keys:
+ *   "ipfb_cols_width"  columns per workgroup of the nslice = 4096 column pass (2, 4 or 8);
+ *   "xcorr_int_method" all-pairs integer correlators: 1 VALU (XOR form for 1-bit),
+ *                      2 int8 matrix cores (blocklen % 64 == 0; 1-bit also ncol % 4 == 0),
+ *                      0 matrix cores whenever the shape allows. */
+int alb_tune_set(const char* key, int64_t value);
+int64_t alb_tune_get(const char* key);
 
 #ifdef __cplusplus
 }

@@ -77,26 +77,46 @@ def test_xcorr_1bit_and_vanvleck(cr, rng):
         assert n2 == n and all(np.array_equal(g, r) for g, r in zip(got, ref))
 
 
-def test_allpairs_4bit(cr):
+@pytest.mark.parametrize("method", ["valu", "mfma"])
+def test_allpairs_4bit(cr, rng, method):
     from albatros_axion_detection_amd import synth
     inp = synth.direct4(T=8192, nchan=128, seed=4)
-    got = cr.xcorr_4bit_allpairs(inp, 4096)
+    got = cr.xcorr_4bit_allpairs(inp, 4096, method=method)
     exp = ocorr.allpairs_4bit_block_sums(inp, 4096)
     assert got.shape == (2, 10, 128, 2) and np.array_equal(got, exp)
     # edge: nibble 8 (+8) everywhere, the largest products
     eights = np.full((3, 4096, 8), 0x88, dtype=np.uint8)
-    got = cr.xcorr_4bit_allpairs(eights, 1024)
+    got = cr.xcorr_4bit_allpairs(eights, 1024, method=method)
     assert np.all(got[..., 0] == 128 * 1024) and np.all(got[..., 1] == 0)
+    # every byte value, 8 inputs (a full 16-row product), 12 channels
+    raw = rng.integers(0, 256, size=(8, 640, 12), dtype=np.uint8)
+    got = cr.xcorr_4bit_allpairs(raw, 320, method=method)
+    assert np.array_equal(got, ocorr.allpairs_4bit_block_sums(raw, 320))
 
 
-def test_allpairs_1bit(cr):
+@pytest.mark.parametrize("method", ["valu", "mfma"])
+def test_allpairs_1bit(cr, method):
     from albatros_axion_detection_amd import synth
-    inp = synth.direct1(ninput=5, T=4096, nchan=60, seed=6)
-    got = cr.xcorr_1bit_allpairs(inp, 60, 2048)
-    exp = ocorr.allpairs_1bit_block_sums(inp, 60, 2048)
+    inp = synth.direct1(ninput=5, T=4096, nchan=64, seed=6)
+    got = cr.xcorr_1bit_allpairs(inp, 64, 2048, method=method)
+    exp = ocorr.allpairs_1bit_block_sums(inp, 64, 2048)
     assert got.shape == exp.shape and np.array_equal(got, exp)
     # autocorrelation of +-1 data is exactly 2 per spectrum
     assert np.all(got[:, 0, :, 0] == 2 * 2048) and np.all(got[:, 0, :, 1] == 0)
+    # 16 inputs (every row of the product), channel count short of the last byte
+    inp = synth.direct1(ninput=16, T=1024, nchan=64, seed=7)
+    got = cr.xcorr_1bit_allpairs(inp, 62, 512, method=method)
+    assert np.array_equal(got, ocorr.allpairs_1bit_block_sums(inp, 62, 512))
+
+
+def test_allpairs_1bit_ragged_columns(cr):
+    # ncol not a multiple of 4: the VALU form serves "auto", the matrix-core form refuses
+    from albatros_axion_detection_amd import synth
+    inp = synth.direct1(ninput=5, T=4096, nchan=60, seed=6)
+    got = cr.xcorr_1bit_allpairs(inp, 60, 2048)
+    assert np.array_equal(got, ocorr.allpairs_1bit_block_sums(inp, 60, 2048))
+    with pytest.raises(ValueError):
+        cr.xcorr_1bit_allpairs(inp, 60, 2048, method="mfma")
 
 
 def test_xcorr_all_ant_reference_constants(cr):
@@ -135,3 +155,33 @@ def test_xcorr_all_ant_torch_fortran_view(cr):
     assert isinstance(out, torch.Tensor) and tuple(out.shape) == (8, 8, 40)
     d = np.sqrt(np.abs(np.einsum("iif->if", ref)))
     assert np.max(np.abs(out.cpu().numpy() - ref) / (d[:, None] * d[None])) < 1e-5
+
+
+@pytest.mark.parametrize("M,K,F", [(8, 1024, 64), (5, 1000, 33), (2, 77, 9)])
+def test_xcorr_all_ant_precision_forms(cr, M, K, F):
+    # diagonal-normalised error |dC_ij| / sqrt(C_ii C_jj) against the float64
+    # oracle for each arithmetic; the FP16 form must show its 11-bit rounding
+    # (so it is not silently an FP32 product) and the split forms must recover
+    # close to FP32 accuracy
+    from albatros_axion_detection_amd import synth
+    x = synth.precision_xcorr(M=M, T=K, F=F, seed=3 * M + K)
+    ref = ocorr.avg_xcorr_all_ant(x, K)
+    d = np.sqrt(np.abs(np.einsum("iif->if", ref)))
+    norm = d[:, None, :] * d[None, :, :]
+    err = {}
+    for prec in ("fp32", "bf16x3", "fp16", "fp16x3"):
+        got = cr.avg_xcorr_all_ant_gpu(x, M, 1, K, F, precision=prec)
+        err[prec] = float(np.max(np.abs(got - ref) / norm))
+        assert np.max(np.abs(got - np.conj(got.transpose(1, 0, 2))) / norm) < 1e-6, prec
+    print("[gpu-case] precision errors", M, K, F, err, flush=True)
+    assert err["fp32"] < 1e-5 and err["fp16x3"] < 1e-5
+    assert err["bf16x3"] < 5e-5
+    assert 1e-5 < err["fp16"] < 5e-3
+    assert err["bf16x3"] < err["fp16"] / 10 and err["fp16x3"] < err["fp16"] / 10
+
+
+def test_xcorr_precision_rejects_large_m(cr):
+    from albatros_axion_detection_amd import synth
+    x = synth.precision_xcorr(M=10, T=16, F=2, seed=0)
+    with pytest.raises(NotImplementedError):
+        cr.avg_xcorr_all_ant_gpu(x, 10, 1, 16, 2, precision="fp16")

@@ -182,6 +182,32 @@ def test_stream_spectra_fused(rng, osamp, nblock):
     assert got.shape == ref.shape and rel_err(got, ref) < TOL
 
 
+def test_column_pass_width_invariance(rng):
+    # the strip width only changes how many columns share a workgroup: every
+    # column's arithmetic is the same, so outputs must be bit-identical
+    from albatros_axion_detection_amd import _lib, repfb
+    lib = _lib.load()
+    acclen, cut, nchunks, osamp, nblock = 4088, 4, 2, 64, 10
+    chan = np.arange(300, 400)
+    chunks = _packed_chunks(rng, nchunks, acclen, len(chan), loss=0.05)
+    pipe = repfb.RePFBPipeline(1, chan, acclen, cut, osamp, nblock, 300 * osamp, 400 * osamp)
+    rm = repfb.rowmaps_from_specnums([c["specnums"] for c in chunks], 1000, acclen)
+    packed = _lib.to_device(repfb.stack_packed(chunks, "pol0", acclen, len(chan)))
+    outs = {}
+    try:
+        for w in (8, 4, 2):
+            lib.alb_tune_set(b"ipfb_cols_width", w)
+            print("[gpu-case] width", w, flush=True)
+            outs[w] = (pipe.spectra(packed, rm, nchunks).cpu().numpy(),
+                       pipe.spectra(packed, rm, nchunks, fused=False).cpu().numpy())
+    finally:
+        lib.alb_tune_set(b"ipfb_cols_width", 0)
+    for w in (4, 2):
+        assert np.array_equal(outs[w][0], outs[8][0]), w
+        assert np.array_equal(outs[w][1], outs[8][1]), w
+    assert rel_err(outs[8][0], outs[8][1]) < 2e-6
+
+
 def test_stream_xcorr_two_antennas(rng):
     from albatros_axion_detection_amd import repfb
     acclen, cut, osamp, nblock = 120, 4, 2, 4
