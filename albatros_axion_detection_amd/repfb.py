@@ -266,18 +266,31 @@ class RePFBPipeline:
             self.repfb(stream, nbt, out, self.nfine, 1, nbt, 0, block_batch=nbt)
         return out
 
-    def correlate(self, packed_list, rowmaps, nchunks, return_xin=False, fused=True):
-        """All inputs: ``vis`` complex64 F-order ``(M, M, nfine, njobs)`` of
-        ``avg_xcorr_all_ant_gpu`` per job (helper_gpu_stream.py:236)."""
+    def buffers(self, nchunks):
+        """Device buffers of one correlate pass over ``nchunks`` chunks,
+        allocated once and reused (a captured hipGraph may not allocate).
+        Also sizes the tile and re-PFB work buffers."""
         t = _lib.torch()
-        M = self.ninput
         jobs, nbt, rows = self.plan(nchunks)
-        njobs = len(jobs)
-        K = self.nblock
-        F = self.nfine
-        # F-order (M, K, F, S) == C-order [S][F][K][M]
-        xin = t.empty((njobs, F, K, M), dtype=t.complex64, device=_lib.device())
-        stream = t.empty((rows, 4096), dtype=t.float32, device=xin.device)
+        M, K, F = self.ninput, self.nblock, self.nfine
+        dev = _lib.device()
+        bufs = {"jobs": jobs, "nbt": nbt, "rows": rows,
+                # F-order (M, K, F, S) == C-order [S][F][K][M]
+                "xin": t.empty((len(jobs), F, K, M), dtype=t.complex64, device=dev),
+                "stream": t.empty((rows, 4096), dtype=t.float32, device=dev),
+                "vis": t.empty((len(jobs), F, M, M), dtype=t.complex64, device=dev)}
+        self._tile_buffer(min(self.tile_batch, nchunks))
+        self._workspace(nbt if self.fused_supported(nbt) else max(1, 512 // K) * K)
+        return bufs
+
+    def correlate_into(self, packed_list, rowmaps, nchunks, bufs, fused=True, precision="fp32"):
+        """Launch the whole pass into ``bufs`` (no allocation, no host sync, so
+        it can be captured in a hipGraph).  Returns the ``(M, M, nfine, njobs)``
+        F-order view of ``bufs["vis"]``."""
+        from .correlations import PRECISION_MODES
+        M, K, F = self.ninput, self.nblock, self.nfine
+        xin, stream, vis, nbt = bufs["xin"], bufs["stream"], bufs["vis"], bufs["nbt"]
+        njobs = len(bufs["jobs"])
         bb = max(1, 512 // K) * K
         for m in range(M):
             if fused:
@@ -286,12 +299,49 @@ class RePFBPipeline:
             else:
                 self.ipfb_stream(packed_list[m], rowmaps[m], nchunks, stream)
                 self.repfb(stream, nbt, xin.view(-1)[m:], M, M * K, K, M * K * F, block_batch=bb)
-        vis = t.empty((njobs, F, M, M), dtype=t.complex64, device=xin.device)
         scale = np.float32(1.0) / np.float32(K)
-        _lib.call("alb_xcorr_all_ant", _lib.ptr(xin), M, K, F * njobs, _lib.ptr(vis), float(scale), _lib.stream())
-        res = vis.permute(3, 2, 1, 0)  # (M, M, F, S) view of the F-order result
+        _lib.call("alb_xcorr_all_ant_prec", _lib.ptr(xin), M, K, F * njobs, _lib.ptr(vis), float(scale),
+                  PRECISION_MODES[precision], _lib.stream())
+        return vis.permute(3, 2, 1, 0)
+
+    def correlate_per_chunk(self, packed_list, rowmaps, nchunks, bufs):
+        """The per-chunk launch form in plain FP32, following the reference's
+        loop structure (helper_gpu_stream.py:157-236): per input and chunk one
+        row pass and one column pass, per job and input one re-PFB launch, per
+        job one correlator launch.  Same result as :meth:`correlate_into`."""
+        M, K, F = self.ninput, self.nblock, self.nfine
+        xin, stream, vis = bufs["xin"], bufs["stream"], bufs["vis"]
+        njobs = len(bufs["jobs"])
+        s = _lib.stream()
+        y = self._tile_buffer(1)
+        ws = self._workspace(K)
+        rows_valid = nchunks * self.acclen
+        scale = np.float32(1.0) / np.float32(K)
+        for m in range(M):
+            if stream.shape[0] > rows_valid:
+                tail = stream[rows_valid:]
+                _lib.call("alb_fill_zero", _lib.ptr(tail), tail.numel() * 4, s)
+            for c in range(nchunks):
+                _lib.call("alb_ipfb_rows_packed4", _lib.ptr(packed_list[m]), _lib.ptr(rowmaps[m]), _lib.ptr(self.chan),
+                          self.ncols, self.acclen, self.cut, c, 1, _lib.ptr(y), s)
+                _pfb.ipfb_columns(y, 4096, self.nslice * 4096, 1, self.nslice, 4096, self.filt, self.cut,
+                                  stream[c * self.acclen:], 4096, self.acclen * 4096)
+            for j in range(njobs):
+                ts = stream[j * K * self.osamp:]
+                base = xin.view(-1)[j * M * K * F + m:]
+                _lib.call("alb_repfb", _lib.ptr(ts), _lib.ptr(self.win), self.ntap, K, self.osamp, _lib.ptr(ws),
+                          self.c_lo, self.c_hi, _lib.ptr(base), M, M * K, K, M * K * F, s)
+        for j in range(njobs):
+            _lib.call("alb_xcorr_all_ant", _lib.ptr(xin[j]), M, K, F, _lib.ptr(vis[j]), float(scale), s)
+        return vis.permute(3, 2, 1, 0)
+
+    def correlate(self, packed_list, rowmaps, nchunks, return_xin=False, fused=True):
+        """All inputs: ``vis`` complex64 F-order ``(M, M, nfine, njobs)`` of
+        ``avg_xcorr_all_ant_gpu`` per job (helper_gpu_stream.py:236)."""
+        bufs = self.buffers(nchunks)
+        res = self.correlate_into(packed_list, rowmaps, nchunks, bufs, fused=fused)
         if return_xin:
-            return res, xin.permute(3, 2, 1, 0)
+            return res, bufs["xin"].permute(3, 2, 1, 0)
         return res
 
 

@@ -78,8 +78,15 @@ def precision_xcorr(M=8, T=1024, F=6400, seed=2, sky=0.4, device=None):
     return np.asfortranarray((n + sky * s).astype(np.complex64))
 
 
+def _quantize_4bit_torch(re, im):
+    import torch
+    r = torch.clamp(torch.round(re), -7, 7).to(torch.int32)
+    i = torch.clamp(torch.round(im), -7, 7).to(torch.int32)
+    return (((r & 15) << 4) | (i & 15)).to(torch.uint8)
+
+
 def batched_stream(nant=2, nchan=300, nchunks=64, lmin=2 ** 10, lmax=2 ** 14, loss=0.005, run=5, seed=4,
-                   sigma=2.5):
+                   sigma=2.5, device=None):
     """Uneven arrival chunks of 4-bit data with packet loss.
 
     ``nchunks`` arrival chunks with seeded lengths uniform in [lmin, lmax]
@@ -100,6 +107,23 @@ def batched_stream(nant=2, nchan=300, nchunks=64, lmin=2 ** 10, lmax=2 ** 14, lo
         present[s:s + run] = False
     specnums = np.nonzero(present)[0].astype(np.int64)
     P = len(specnums)
+    if device is not None:
+        # same construction drawn with torch on the device (torch.round is
+        # half-to-even like np.rint); a different random stream than numpy
+        import torch
+        g = torch.Generator(device=device)
+        g.manual_seed(seed)
+        packed = torch.empty((2 * nant, P, nchan), dtype=torch.uint8, device=device)
+        step = 1 << 16
+        for p0 in range(0, P, step):
+            n = min(step, P - p0)
+            for pol in range(2):
+                a = sigma * torch.randn((n, nchan, 2), generator=g, device=device)
+                for ant in range(nant):
+                    x = a if ant == 0 else 0.3 * a + 0.95 * sigma * torch.randn((n, nchan, 2), generator=g,
+                                                                                device=device)
+                    packed[ant * 2 + pol, p0:p0 + n] = _quantize_4bit_torch(x[..., 0], x[..., 1])
+        return packed, specnums, lengths
     packed = np.empty((2 * nant, P, nchan), dtype=np.uint8)
     step = 1 << 16
     for p0 in range(0, P, step):
