@@ -47,6 +47,10 @@ SIGNATURES = {
     "alb_ipfb_columns_any": (_i32, [_vp, _i64, _i64, _i64, _i64, _i64, _vp, _i64, _i32, _vp, _i64, _i64, _vp, _vp,
                                      _i64, _vp]),
     "alb_ipfb_rows_packed4": (_i32, [_vp, _vp, _vp, _i64, _i64, _i32, _i64, _i64, _vp, _vp]),
+    "alb_ipfb_rows_packed4_mfma": (_i32, [_vp, _vp, _vp, _i64, _i64, _i32, _i64, _i64, _vp, _vp, _vp]),
+    "alb_ipfb_rows_packed4_blocked": (_i32, [_vp, _vp, _vp, _i64, _i64, _i32, _i64, _i64, _vp, _vp]),
+    "alb_ipfb_rows_coeff_bytes": (_i64, [_i64]),
+    "alb_ipfb_rows_coeff": (_i32, [_vp, _i64, _vp, _vp]),
     "alb_repfb": (_i32, [_vp, _vp, _i32, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _vp]),
     "alb_repfb_workspace_bytes": (_i64, [_i64, _i64]),
     "alb_repfb_stage1": (_i32, [_vp, _vp, _i32, _i64, _i64, _vp, _vp]),

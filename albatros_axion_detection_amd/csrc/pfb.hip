@@ -19,6 +19,7 @@
 //  repfb_stage2   re-PFB second half: twiddle, 4096-point FFT along columns,
 //                 only the requested fine channels written.
 #include "fft.hpp"
+#include "packed.hpp"
 
 namespace alb {
 
@@ -58,19 +59,6 @@ struct DenseRows {
   int64_t ld;
 };
 
-// 4-bit packed rows of an iPFB tile.  Tile t, row r < 2*cut comes from chunk
-// t-1 row acclen-2*cut+r (zeros for t == 0), row r >= 2*cut from chunk t row
-// r-2*cut.  rowmap[chunk][row] = row p of the packed byte array [rows][ncols]
-// holding that spectrum, or -1 (zero-filled spectrum); byte (p, col) goes to
-// channel number chan[col].
-struct Packed4Rows {
-  const uint8_t* bytes;
-  const int* rowmap;
-  const int* chan;
-  int64_t ncols, acclen, cut2, tile0;
-};
-
-__device__ __forceinline__ int nib4f(int v) { return v > 8 ? v - 16 : v; }
 
 // -------------------------------------------------------- irfft along rows
 // F rows per workgroup, N = 2M real outputs per row.  The packed half spectrum
@@ -87,8 +75,15 @@ __global__ __launch_bounds__(kTPB) void irfft_rows_kernel(DenseRows dense, Packe
   float2* TA = Z + F * MP;
   float2* TB = TA + tw.nA;
   const int tid = threadIdx.x;
+  // Workgroup i runs on XCD i % 8: consecutive row groups go to one XCD back
+  // to back, so the blocked layout's 32-byte row pieces of neighbouring
+  // workgroups fill whole cache lines in that XCD's L2.
+  const int64_t ngroups = (nrows + F - 1) / F;
+  const int64_t per = (ngroups + 7) / 8;
+  const int64_t item = int64_t(blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (item >= ngroups) return;   // whole workgroup: padding of the 8-way split
   load_twiddle(tw, TA, TB, tid, kTPB);
-  const int64_t row0 = int64_t(blockIdx.x) * F;
+  const int64_t row0 = item * F;
   if constexpr (MODE == 0) {
     __syncthreads();
     for (int idx = tid; idx < F * (M / 2 + 1); idx += kTPB) {
@@ -123,12 +118,7 @@ __global__ __launch_bounds__(kTPB) void irfft_rows_kernel(DenseRows dense, Packe
       const int64_t c = idx - int64_t(f) * pk.ncols;
       const int64_t row = row0 + f;
       if (row >= nrows) continue;
-      const int64_t tile = row / nslice, r = row - tile * nslice;
-      int64_t chunk, crow;
-      if (r < pk.cut2) { chunk = pk.tile0 + tile - 1; crow = pk.acclen - pk.cut2 + r; }
-      else { chunk = pk.tile0 + tile; crow = r - pk.cut2; }
-      if (chunk < 0) continue;
-      const int p = pk.rowmap[chunk * pk.acclen + crow];
+      const int64_t p = packed_row(pk, row, nslice);
       if (p < 0) continue;
       const int byte = pk.bytes[int64_t(p) * pk.ncols + c];
       const int ch = pk.chan[c];
@@ -154,6 +144,22 @@ __global__ __launch_bounds__(kTPB) void irfft_rows_kernel(DenseRows dense, Packe
   fft_lds<M, F, kTPB>(Z, TA, TB, tw.lb, 1, tid);
   // ---- y[2n] + i y[2n+1] = conj(FFT) / N
   const float sc = 1.0f / float(N);
+  if (ldo == 8) {
+    // blocked tile layout: element (row i of tile t, column k) at
+    // out[t][k / 8][i][k % 8] (rows of a tile nslice apart per block)
+    const int64_t t0 = row0 / nslice, i0 = row0 - t0 * nslice;
+    for (int idx = tid; idx < F * M; idx += kTPB) {
+      const int f = idx / M, n = idx - f * M;
+      const int64_t row = row0 + f;
+      if (row >= nrows) continue;
+      int64_t t = t0, i = i0 + f;
+      while (i >= nslice) { i -= nslice; ++t; }
+      const float2 v = Z[f * MP + pad16(n)];
+      reinterpret_cast<float2*>(out + t * int64_t(N) * nslice + int64_t(n / 4) * 8 * nslice + i * 8)[n % 4] =
+          make_float2(v.x * sc, -v.y * sc);
+    }
+    return;
+  }
   for (int idx = tid; idx < F * M; idx += kTPB) {
     const int f = idx / M, n = idx - f * M;
     const int64_t row = row0 + f;
@@ -229,11 +235,11 @@ __global__ __launch_bounds__(kTPB) void rfft_rows_kernel(const float* __restrict
 // back to back.  Workgroups in flight together read neighbouring columns of
 // the same rows (whole cache lines, long DRAM bursts) and a group's filter
 // columns are re-read by the next tile while still cached.
-__device__ __forceinline__ void cols_item(int64_t nstrips, int64_t* sidx, int64_t* tile) {
+__device__ __forceinline__ void cols_item(int64_t nstrips, int group, int64_t* sidx, int64_t* tile) {
   const int64_t ntiles = gridDim.x / nstrips;
   if (nstrips % 8 == 0) {
-    const int64_t P = nstrips / 8;                 // strips per XCD
-    const int64_t G = (P % 32 == 0) ? 32 : P;      // strips per group
+    const int64_t P = nstrips / 8;                        // strips per XCD
+    const int64_t G = (P % group == 0) ? group : P;       // strips per group
     const int64_t slot = blockIdx.x / 8;
     const int64_t grp = slot / (G * ntiles), rem = slot - grp * (G * ntiles);
     *tile = rem / G;
@@ -247,15 +253,19 @@ __device__ __forceinline__ void cols_item(int64_t nstrips, int64_t* sidx, int64_
 // Load the strip, forward FFT along time, Wiener filter with 1/nslice, inverse
 // FFT.  On return (after a barrier) time row i of column k0 + 2p is
 // U[p][i].x and of column k0 + 2p + 1 is -U[p][i].y.
+// Tile layout: element (i, k) at yt[(k / ldy) * ldy * NSL + i * ldy + k % ldy]:
+// row-major with row stride ldy when ldy >= the column count, or blocks of
+// ldy columns (each block's rows contiguous, ldy a multiple of CW) otherwise.
 template <int NSL, int CW, int T>
 __device__ __forceinline__ void cols_core(const float* __restrict__ yt, int64_t ldy, const float2* __restrict__ filt,
                                           int64_t ldf, int64_t k0, float2* U, const float2* TA, const float2* TB,
                                           int lb, int tid) {
   constexpr int NP = CW / 2;
   constexpr int PADN = Padded<NSL>::value;
+  const float* ys = yt + (k0 / ldy) * ldy * NSL + (k0 % ldy);
   for (int idx = tid; idx < NSL * NP; idx += T) {
     const int i = idx / NP, p = idx - i * NP;
-    U[p * PADN + pad16(i)] = reinterpret_cast<const float2*>(yt + i * ldy + k0)[p];
+    U[p * PADN + pad16(i)] = reinterpret_cast<const float2*>(ys + i * ldy)[p];
   }
   __syncthreads();
   fft_lds<NSL, NP, T>(U, TA, TB, lb, 0, tid);
@@ -289,7 +299,7 @@ template <int NSL, int CW, int T>
 __global__ __launch_bounds__(T) void ipfb_cols_kernel(const float* __restrict__ y, int64_t ldy, int64_t tile_stride,
                                                          const float2* __restrict__ filt, int64_t ldf, int64_t ncols,
                                                          int cut, float* __restrict__ out, int64_t ldo,
-                                                         int64_t out_tile_stride, Twiddle tw) {
+                                                         int64_t out_tile_stride, int group, Twiddle tw) {
   constexpr int NP = CW / 2;
   constexpr int PADN = Padded<NSL>::value;
   extern __shared__ float2 smem[];
@@ -299,7 +309,7 @@ __global__ __launch_bounds__(T) void ipfb_cols_kernel(const float* __restrict__ 
   const int tid = threadIdx.x;
   load_twiddle(tw, TA, TB, tid, T);
   int64_t sidx, tile;
-  cols_item(ncols / CW, &sidx, &tile);
+  cols_item(ncols / CW, group, &sidx, &tile);
   const int64_t k0 = sidx * CW;
   cols_core<NSL, CW, T>(y + tile * tile_stride, ldy, filt, ldf, k0, U, TA, TB, tw.lb, tid);
   float* ot = out + tile * out_tile_stride;
@@ -337,7 +347,8 @@ __global__ __launch_bounds__(T) void ipfb_cols_repfb_kernel(const float* __restr
                                                                int64_t ldf, int cut, float* __restrict__ ts,
                                                                int64_t row0, int64_t rows_per_tile,
                                                                const float* __restrict__ win, int ntap,
-                                                               int64_t nblocks, float2* __restrict__ A, Twiddle tw) {
+                                                               int64_t nblocks, float2* __restrict__ A, int group,
+                                                               Twiddle tw) {
   constexpr int NP = CW / 2;
   constexpr int PADN = Padded<NSL>::value;
   constexpr int S = R1 / 4;
@@ -353,7 +364,7 @@ __global__ __launch_bounds__(T) void ipfb_cols_repfb_kernel(const float* __restr
   const int tid = threadIdx.x;
   load_twiddle(tw, TA, TB, tid, T);
   int64_t sidx, tile;
-  cols_item(C / CW, &sidx, &tile);
+  cols_item(C / CW, group, &sidx, &tile);
   const int64_t k0 = sidx * CW;
   if (tid < R1) {
     double sn, cs;
@@ -674,9 +685,9 @@ __device__ __forceinline__ void q2_range(int64_t q1, int64_t R1, int64_t c_lo, i
 }
 
 template <int R1>
-__global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __restrict__ A, int64_t c_lo, int64_t c_hi,
-                                                            float2* __restrict__ out, int64_t sb, int64_t sc, int64_t bpj,
-                                                            int64_t sj, Twiddle tw4) {
+__global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __restrict__ A, int64_t nblock, int64_t c_lo,
+                                                            int64_t c_hi, float2* __restrict__ out, int64_t sb, int64_t sc,
+                                                            int64_t bpj, int64_t sj, Twiddle tw4) {
   constexpr int N = 4096;
   constexpr int H = R1 / 2;
   constexpr int NPAD = Padded<N>::value;
@@ -686,8 +697,14 @@ __global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __rest
   float2* TB = TA + tw4.nA;
   const int tid = threadIdx.x;
   load_twiddle(tw4, TA, TB, tid, kTPB);
-  const int q1 = blockIdx.x;
-  const int64_t b = blockIdx.y;
+  // Workgroup i runs on XCD i % 8: the H+1 workgroups of one block go to one
+  // XCD back to back, so their interleaved output channels merge in its L2.
+  const int64_t total = int64_t(H + 1) * nblock;
+  const int64_t per = (total + 7) / 8;
+  const int64_t item = int64_t(blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (item >= total) return;   // the whole workgroup: padding of the 8-way split
+  const int q1 = int(item % (H + 1));
+  const int64_t b = item / (H + 1);
   const float2* src = A + (b * (H + 1) + q1) * N;
   // W_L^(q1 k) = W_L^(q1 c) W_4096^(q1 a) for k = R1 a + c; c = tid mod R1 is
   // the same for all of this thread's k (kTPB is a multiple of R1)
@@ -806,13 +823,21 @@ using namespace alb;
 // ---------------------------------------------------------------- host side
 namespace {
 
+// strips per group of the column-pass workgroup order (tuning knob, default 32)
+int cols_group() {
+  const int64_t g = tune_get("ipfb_cols_group");
+  return g > 0 ? int(g) : 32;
+}
+
 template <int M, int F, int MODE>
 int launch_irfft(DenseRows d, Packed4Rows pk, int64_t nrows, int64_t nslice, float* out, int64_t ldo, hipStream_t s) {
   Twiddle tw;
   int rc = get_twiddle(2 * M, &tw);
   if (rc) return rc;
   const size_t lds = sizeof(float2) * (size_t(F) * Padded<M>::value + tw.nA + tw.nB);
-  const unsigned grid = unsigned((nrows + F - 1) / F);
+  const int64_t ngroups = (nrows + F - 1) / F;
+  ALB_REQUIRE(ngroups < (int64_t(1) << 31) - 8, "irfft: too many rows");
+  const unsigned grid = unsigned((ngroups + 7) / 8 * 8);
   hipLaunchKernelGGL((irfft_rows_kernel<M, F, MODE>), dim3(grid), dim3(kTPB), lds, s, d, pk, nrows, nslice, out, ldo, tw);
   ALB_LAUNCH_CHECK("irfft_rows_kernel");
   return ALB_OK;
@@ -879,7 +904,7 @@ int launch_cols(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles
   ALB_REQUIRE(ncols % CW == 0, "ipfb column pass: width %lld not a multiple of %d", (long long)ncols, CW);
   ALB_REQUIRE((ncols / CW) * ntiles < (int64_t(1) << 31), "ipfb column pass: too many tiles");
   hipLaunchKernelGGL((ipfb_cols_kernel<NSL, CW, T>), dim3(unsigned((ncols / CW) * ntiles)), dim3(T), lds, s, y,
-                     ldy, tile_stride, filt, ldf, ncols, cut, out, ldo, out_tile_stride, tw);
+                     ldy, tile_stride, filt, ldf, ncols, cut, out, ldo, out_tile_stride, cols_group(), tw);
   ALB_LAUNCH_CHECK("ipfb_cols_kernel");
   return ALB_OK;
 }
@@ -980,7 +1005,8 @@ int launch_cols_repfb(const float* y, int64_t ldy, int64_t tile_stride, int64_t 
                      sizeof(float) * size_t(ntap) * R1 * CW;
   ALB_REQUIRE(lds <= 160 * 1024, "fused iPFB/re-PFB: LDS budget exceeded");
   hipLaunchKernelGGL((ipfb_cols_repfb_kernel<NSL, CW, T, R1>), dim3(unsigned((4096 / CW) * ntiles)), dim3(T), lds, s, y,
-                     ldy, tile_stride, filt, ldf, cut, ts, row0, rows_per_tile, win, ntap, nblocks, A, tw);
+                     ldy, tile_stride, filt, ldf, cut, ts, row0, rows_per_tile, win, ntap, nblocks, A, cols_group(),
+                     tw);
   ALB_LAUNCH_CHECK("ipfb_cols_repfb_kernel");
   return ALB_OK;
 }
@@ -992,8 +1018,10 @@ int launch_repfb_stage2(int64_t nblock, const float2* A, int64_t c_lo, int64_t c
   int rc = get_twiddle(4096, &tw4);
   if (rc) return rc;
   const size_t lds2 = sizeof(float2) * (size_t(Padded<4096>::value) + tw4.nA + tw4.nB);
-  hipLaunchKernelGGL((repfb_stage2_kernel<R1>), dim3(R1 / 2 + 1, unsigned(nblock)), dim3(kTPB), lds2, s, A, c_lo, c_hi, out, sb,
-                     sc, bpj, sj, tw4);
+  const int64_t total = int64_t(R1 / 2 + 1) * nblock;
+  ALB_REQUIRE(total < (int64_t(1) << 31) - 8, "alb_repfb: too many blocks");
+  hipLaunchKernelGGL((repfb_stage2_kernel<R1>), dim3(unsigned((total + 7) / 8 * 8)), dim3(kTPB), lds2, s, A, nblock,
+                     c_lo, c_hi, out, sb, sc, bpj, sj, tw4);
   ALB_LAUNCH_CHECK("repfb_stage2_kernel");
   return ALB_OK;
 }
@@ -1099,7 +1127,8 @@ int alb_ipfb_columns(const float* y, int64_t ldy, int64_t tile_stride, int64_t n
                      const void* filt, int64_t ldf, int cut, float* out, int64_t ldo, int64_t out_tile_stride,
                      void* stream) {
   ALB_REQUIRE(y && filt && out, "alb_ipfb_columns: null pointer");
-  ALB_REQUIRE(ldy >= ncols && ldo >= ncols && ldf >= nslice / 2 + 1 && cut >= 0 && 2 * cut < nslice && ntiles >= 0,
+  ALB_REQUIRE((ldy >= ncols || (ldy == 8 && ncols % 8 == 0)) && ldo >= ncols && ldf >= nslice / 2 + 1 && cut >= 0 &&
+                  2 * cut < nslice && ntiles >= 0,
               "alb_ipfb_columns: bad sizes");
   if (ntiles == 0) return ALB_OK;
   return cols_dispatch(nslice, y, ldy, tile_stride, ntiles, (const float2*)filt, ldf, ncols, cut, out, ldo,
@@ -1118,6 +1147,18 @@ int alb_ipfb_rows_packed4(const uint8_t* bytes, const int* rowmap, const int* ch
   Packed4Rows pk{bytes, rowmap, chan, ncols, acclen, 2 * int64_t(cut), tile0};
   DenseRows d{};
   return irfft_dispatch<1>(4096, d, pk, nslice * ntiles, nslice, out, 4096, as_stream(stream));
+}
+
+// The same row pass written in the blocked tile layout out[ntiles][4096/8][nslice][8]
+// (the column passes read it with ldy = 8: one contiguous strip per workgroup).
+int alb_ipfb_rows_packed4_blocked(const uint8_t* bytes, const int* rowmap, const int* chan, int64_t ncols,
+                                  int64_t acclen, int cut, int64_t tile0, int64_t ntiles, float* out, void* stream) {
+  ALB_REQUIRE(bytes && rowmap && chan && out, "alb_ipfb_rows_packed4_blocked: null pointer");
+  const int64_t nslice = acclen + 2 * cut;
+  ALB_REQUIRE(acclen >= 2 * cut && ncols > 0 && ncols <= 2049, "alb_ipfb_rows_packed4_blocked: bad sizes");
+  Packed4Rows pk{bytes, rowmap, chan, ncols, acclen, 2 * int64_t(cut), tile0};
+  DenseRows d{};
+  return irfft_dispatch<1>(4096, d, pk, nslice * ntiles, nslice, out, 8, as_stream(stream));
 }
 
 // Re-PFB (pfb_gpu_utils.py:105-121 with lblock = 4096*osamp) of nblock output
@@ -1207,7 +1248,8 @@ int alb_ipfb_columns_repfb(const float* y, int64_t ldy, int64_t tile_stride, int
                            const float* win, int ntap, int64_t osamp, int64_t nblocks, void* A, void* stream) {
   ALB_REQUIRE(ntiles >= 0 && nblocks >= 0 && row0 >= 0 && rows_per_tile >= 0, "alb_ipfb_columns_repfb: bad sizes");
   ALB_REQUIRE(cut >= 0 && 2 * cut < nslice, "alb_ipfb_columns_repfb: bad cut");
-  ALB_REQUIRE(ntiles == 0 || (ldy >= 4096 && ldf >= nslice / 2 + 1), "alb_ipfb_columns_repfb: bad strides");
+  ALB_REQUIRE(ntiles == 0 || ((ldy >= 4096 || ldy == 8) && ldf >= nslice / 2 + 1),
+              "alb_ipfb_columns_repfb: bad strides (ldy >= 4096 row-major, or 8 for the blocked tile layout)");
   hipStream_t s = as_stream(stream);
   if (nslice == 4096 && (osamp == 32 || osamp == 64)) {
     const int64_t w = tune_get("ipfb_cols_width");

@@ -91,7 +91,8 @@ def rechannelizer(args, torch):
     nblk = (n_coarse // osamp) - 3  # every complete re-PFB block of the stream
     c_lo, c_hi = chan0 * osamp, (chan0 + nkeep) * osamp
     pipe = repfb.RePFBPipeline(1, chans, acclen, cut, osamp, nblk, c_lo, c_hi, filt_thresh=0.45,
-                               tile_batch=nchunks)
+                               tile_batch=nchunks, row_pass=args.row_pass)
+    ldy = 8 if pipe.pow2 else 4096
     dev = _lib.device()
     packed = _lib.to_device(packed_np.reshape(nchunks, acclen, nkeep))
     # spec_num of every present spectrum of the nchunks chunks, in arrival order
@@ -117,12 +118,17 @@ def rechannelizer(args, torch):
         # range, so one call maps all of them: row = spec_num - start
         timer("rowmap", lambda: _lib.call(
             "alb_build_rowmap", _lib.ptr(specnums), n_coarse, 0, n_coarse, 0, _lib.ptr(rowmap), s()))
-        timer("ipfb_rows_packed4", lambda: _lib.call(
-            "alb_ipfb_rows_packed4", _lib.ptr(packed), _lib.ptr(rowmap), _lib.ptr(pipe.chan), nkeep, acclen, cut, 0,
-            nchunks, _lib.ptr(tiles), s()))
+        if pipe.row_pass == "mfma":
+            timer("ipfb_rows_mfma", lambda: _lib.call(
+                "alb_ipfb_rows_packed4_mfma", _lib.ptr(packed), _lib.ptr(rowmap), _lib.ptr(pipe.chan), nkeep, acclen,
+                cut, 0, nchunks, _lib.ptr(pipe.coeff), _lib.ptr(tiles), s()))
+        else:
+            timer("ipfb_rows_packed4", lambda: _lib.call(
+                "alb_ipfb_rows_packed4_blocked" if pipe.pow2 else "alb_ipfb_rows_packed4", _lib.ptr(packed),
+                _lib.ptr(rowmap), _lib.ptr(pipe.chan), nkeep, acclen, cut, 0, nchunks, _lib.ptr(tiles), s()))
         if fused:
             timer("ipfb_columns_repfb", lambda: _lib.call(
-                "alb_ipfb_columns_repfb", _lib.ptr(tiles), 4096, nslice * 4096, nchunks, nslice, _lib.ptr(pipe.filt),
+                "alb_ipfb_columns_repfb", _lib.ptr(tiles), ldy, nslice * 4096, nchunks, nslice, _lib.ptr(pipe.filt),
                 pipe.filt.shape[1], cut, _lib.ptr(stream), 0, acclen, _lib.ptr(pipe.win), 4, osamp, nblk,
                 _lib.ptr(ws), s()))
             timer("repfb_stage1_edges", lambda: _lib.call(
@@ -130,7 +136,7 @@ def rechannelizer(args, torch):
                 _lib.ptr(ws), s()))
         else:
             timer("ipfb_columns", lambda: _lib.call(
-                "alb_ipfb_columns", _lib.ptr(tiles), 4096, nslice * 4096, nchunks, nslice, 4096, _lib.ptr(pipe.filt),
+                "alb_ipfb_columns", _lib.ptr(tiles), ldy, nslice * 4096, nchunks, nslice, 4096, _lib.ptr(pipe.filt),
                 pipe.filt.shape[1], cut, _lib.ptr(stream), 4096, acclen * 4096, s()))
             timer("repfb_stage1", lambda: _lib.call(
                 "alb_repfb_stage1", _lib.ptr(stream), _lib.ptr(pipe.win), 4, nblk, osamp, _lib.ptr(ws), s()))
@@ -141,22 +147,27 @@ def rechannelizer(args, torch):
     lib = _lib.load()
     sweep = {}
     widths = [args.cols_width] if args.cols_width else [8, 4, 2]
+    groups = [args.cols_group] if args.cols_group else [32, 16, 8]
     for w in widths:
-        lib.alb_tune_set(b"ipfb_cols_width", w)
-        step()
-        torch.cuda.synchronize()
-        a = torch.cuda.Event(enable_timing=True)
-        b = torch.cuda.Event(enable_timing=True)
-        a.record()
-        for _ in range(3):
+        for gsz in groups:
+            lib.alb_tune_set(b"ipfb_cols_width", w)
+            lib.alb_tune_set(b"ipfb_cols_group", gsz)
             step()
-        b.record()
-        torch.cuda.synchronize()
-        sweep[w] = a.elapsed_time(b) / 3
+            torch.cuda.synchronize()
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(3):
+                step()
+            b.record()
+            torch.cuda.synchronize()
+            sweep["w%d_g%d" % (w, gsz)] = a.elapsed_time(b) / 3
     best = min(sweep, key=sweep.get)
-    lib.alb_tune_set(b"ipfb_cols_width", best)
-    log("[bench] column-pass width sweep (ms/step): %s -> %d" % (
-        json.dumps({str(k): round(v, 4) for k, v in sweep.items()}), best))
+    bw, bg = (int(x[1:]) for x in best.split("_"))
+    lib.alb_tune_set(b"ipfb_cols_width", bw)
+    lib.alb_tune_set(b"ipfb_cols_group", bg)
+    log("[bench] column-pass launch-shape sweep (ms/step): %s -> %s" % (
+        json.dumps({k: round(v, 4) for k, v in sweep.items()}), best))
 
     for _ in range(args.warmup):
         step()
@@ -186,6 +197,7 @@ def rechannelizer(args, torch):
     # the stream are design overhead, counted as what that launch reads.
     alg = {
         "ipfb_rows_packed4": n_coarse * nkeep,                          # raw bytes in
+        "ipfb_rows_mfma": n_coarse * nkeep,                             # raw bytes in
         "ipfb_columns": n_coarse * 4096 * 4,                            # time stream out
         "repfb_stage1": (nblk + 3) * osamp * 4096 * 4,                  # time stream in
         "ipfb_columns_repfb": 2 * n_coarse * 4096 * 4,                  # time stream out + in
@@ -213,7 +225,8 @@ def rechannelizer(args, torch):
                    "n_coarse": n_coarse, "acclen": acclen, "cut": cut, "osamp": osamp, "nchan_fine": c_hi - c_lo,
                    "nblock": nblk},
         "ref_unit_msps": n_coarse * 2049 * 8 / (ms * 1e-3) / 1e6,
-        "tuning": {"ipfb_cols_width": best, "sweep_ms_per_step": {str(k): round(v, 4) for k, v in sweep.items()}},
+        "tuning": {"ipfb_cols_width": bw, "ipfb_cols_group": bg,
+                   "sweep_ms_per_step": {k: round(v, 4) for k, v in sweep.items()}},
     }
     ctx = dict(packed_np=packed_np, acclen=acclen, cut=cut, osamp=osamp, chan0=chan0, nkeep=nkeep,
                filt=pipe.filt, win=pipe.win)
@@ -449,6 +462,10 @@ def main():
     ap.add_argument("--onebit-T", type=int, default=2 ** 20)
     ap.add_argument("--cols-width", type=int, default=0, choices=[0, 2, 4, 8],
                     help="column-pass strip width (0: pick the fastest of 8, 4, 2 before timing)")
+    ap.add_argument("--cols-group", type=int, default=0, choices=[0, 4, 8, 16, 32, 64],
+                    help="column-pass strips per XCD group (0: pick the fastest of 32, 16, 8 before timing)")
+    ap.add_argument("--row-pass", default="auto", choices=["auto", "fft", "mfma"],
+                    help="iPFB row pass: FFT per spectrum or matrix-core synthesis from the kept channels")
     ap.add_argument("--unfused", action="store_true",
                     help="time the separate column pass + re-PFB stage 1 instead of the fused kernel")
     ap.add_argument("--copy-calibration", action="store_true",

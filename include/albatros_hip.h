@@ -109,7 +109,8 @@ int alb_fft_rows(const void* in, int64_t ldi, void* out, int64_t ldo, int64_t nr
 int alb_pfb_small(const float* ts, int64_t nblock, int64_t lblock, const float* win, int ntap, int64_t c0, int64_t c1,
                   void* out, int64_t ldo, void* stream);
 /* pfb_gpu_utils.py:84-91 (the column half of cupy_ipfb) over ntiles tiles; kept rows
- * [cut, nslice-cut) written (helper_gpu_stream.py:199). */
+ * [cut, nslice-cut) written (helper_gpu_stream.py:199).  Tiles are row-major with row stride
+ * ldy >= ncols, or (ldy = 8) in the blocked layout of alb_ipfb_rows_packed4_mfma. */
 int alb_ipfb_columns(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, int64_t nslice, int64_t ncols,
                      const void* filt, int64_t ldf, int cut, float* out, int64_t ldo, int64_t out_tile_stride,
                      void* stream);
@@ -126,6 +127,19 @@ int alb_ipfb_columns_any(const float* y, int64_t ldy, int64_t tile_stride, int64
  * rows each -> out [ntiles][nslice][4096] float32. */
 int alb_ipfb_rows_packed4(const uint8_t* bytes, const int* rowmap, const int* chan, int64_t ncols, int64_t acclen,
                           int cut, int64_t tile0, int64_t ntiles, float* out, void* stream);
+/* The same row pass written in the blocked tile layout out[ntiles][4096/8][nslice][8], read by
+ * the column passes with ldy = 8 (each column strip one contiguous span). */
+int alb_ipfb_rows_packed4_blocked(const uint8_t* bytes, const int* rowmap, const int* chan, int64_t ncols,
+                                  int64_t acclen, int cut, int64_t tile0, int64_t ntiles, float* out, void* stream);
+/* The same row pass as a matrix-core synthesis from the ncols non-zero channels
+ * (irfft of pycufft.py:139-161 as y = X T with a float64-built FP16 hi/lo cos/sin table T,
+ * FP32 accumulation), written in the blocked tile layout out[ntiles][4096/8][nslice][8]
+ * that the column passes take with ldy = 8.  coeff: alb_ipfb_rows_coeff_bytes(ncols) bytes
+ * filled once by alb_ipfb_rows_coeff from the channel numbers. */
+int alb_ipfb_rows_packed4_mfma(const uint8_t* bytes, const int* rowmap, const int* chan, int64_t ncols, int64_t acclen,
+                               int cut, int64_t tile0, int64_t ntiles, const void* coeff, float* out, void* stream);
+int64_t alb_ipfb_rows_coeff_bytes(int64_t ncols);
+int alb_ipfb_rows_coeff(const int* chan, int64_t ncols, void* coeff, void* stream);
 /* pfb_gpu_utils.py:105 cupy_pfb with lblock = 4096*osamp (re-PFB, helper_gpu_stream.py:222):
  * fine channels [c_lo, c_hi) of nblock spectra; spectrum b, channel c goes to
  * out[(b/bpj)*sj + (b%bpj)*sb + (c-c_lo)*sc] (bpj = blocks per re-PFB job, so all jobs of a
@@ -147,6 +161,7 @@ int alb_repfb_stage2(const void* A, int64_t nblock, int64_t osamp, int64_t c_lo,
  * [osamp*b, osamp*(b+ntap)) all lie in one tile (alb_repfb_interior) A[b] is produced in the
  * same launch; only stream rows read by the other blocks are written to ts ([rows][4096]).
  * Those other blocks go through alb_repfb_stage1_list once every tile that feeds them is done.
+ * Tiles row-major (ldy >= 4096) or blocked (ldy = 8, alb_ipfb_rows_packed4_mfma).
  * nslice 4096 with osamp 32 or 64 (ENOTSUP otherwise: use alb_ipfb_columns + alb_repfb_stage1).
  * With ntiles = 0 only the shape (nslice, cut, ntap, osamp) is checked; pointers may be null. */
 int alb_ipfb_columns_repfb(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, int64_t nslice,
@@ -169,6 +184,7 @@ int alb_build_rowmap(const int64_t* specnum, int64_t npresent, int64_t start, in
 /* ---- tuning (no counterpart in the reference) ---------------------------- */
 /* Set a launch-shape knob; 0 restores the built-in choice.  Keys:
  *   "ipfb_cols_width"  columns per workgroup of the nslice = 4096 column pass (2, 4 or 8);
+ *   "ipfb_cols_group"  column strips per XCD group of the column-pass order (4..64, default 32);
  *   "xcorr_int_method" all-pairs integer correlators: 1 VALU (XOR form for 1-bit),
  *                      2 int8 matrix cores (blocklen % 64 == 0; 1-bit also ncol % 4 == 0),
  *                      0 matrix cores whenever the shape allows. */
