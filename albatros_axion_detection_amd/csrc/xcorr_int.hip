@@ -272,30 +272,66 @@ __device__ __forceinline__ void mfma_item(int64_t ngx, int64_t nblock, int64_t* 
 // 4-bit: rows 0..7 = real parts of inputs 0..7, rows 8..15 = imaginary parts
 // (inputs >= NIN are zero); one channel (byte) per product.
 // Re_ab = G[a][b] + G[8+a][8+b],  Im_ab = G[8+a][b] - G[a][8+b]  (x_a conj x_b).
+//
+// Workgroup: 4 waves, 4 dword columns (16 channels) of one block.  Each
+// 64-spectrum step the 16-byte row pieces of the inputs are loaded once into
+// LDS (double-buffered, [spectrum][dword][input], 33-dword spectrum stride so
+// the operand reads of 4 spectrum groups land in different banks; the real and
+// imaginary rows of an input read the same word).  Wave w takes dword w.
+constexpr int kDW4 = 4;          // dword columns per workgroup
+constexpr int kTS4 = 33;         // LDS dwords per spectrum
+
+// With nsplit > 1 the spectra of a block are shared by nsplit workgroups whose
+// exact partial sums meet in the zero-filled output through int64 atomics.
 template <int NIN>
 __global__ __launch_bounds__(256) void allpairs4_mfma_kernel(InputPtrs in, int64_t ncol, int64_t blocklen,
-                                                             int64_t nblock, long long* __restrict__ out) {
+                                                             int64_t nblock, int nsplit, long long* __restrict__ out) {
   constexpr int NP = NIN * (NIN + 1) / 2;
+  __shared__ uint32_t buf[2][64 * kTS4];
   __shared__ int S[4][4][256];
-  const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
-  int64_t blk, gx;
-  mfma_item((ncol / 4 + 3) / 4, nblock, &blk, &gx);
-  const int64_t w = gx * 4 + wv;
-  const bool wok = w < ncol / 4;
+  const int tid = threadIdx.x;
+  const int wv = tid / 64, lane = tid % 64;
+  int64_t bs, gx;
+  mfma_item((ncol / 4) / kDW4, nblock * nsplit, &bs, &gx);
+  const int64_t blk = bs / nsplit, sp = bs - blk * nsplit;
   const int r = lane % 16, g = lane / 16;
   const int u = r % 8, part = r / 8;
-  const bool uok = wok && u < NIN;
-  const uint8_t* src = in.p[uok ? u : 0] + 4 * (wok ? w : 0);
+  // loader: chunks c = tid + 256 e (e < 2): input lu = c % 8, spectrum lt = c / 8
+  const int lu = tid % 8, lt = tid / 8;             // lt in [0, 32); second chunk lt + 32
+  const bool lok = lu < NIN;
+  const uint8_t* lsrc = in.p[lok ? lu : 0] + 16 * gx;
+  const int64_t nsteps = blocklen / 64 / nsplit;
+  const int64_t t_begin = blk * blocklen + sp * nsteps * 64;
+  uint4 ld0 = make_uint4(0, 0, 0, 0), ld1 = ld0;
+  if (lok) {
+    ld0 = *reinterpret_cast<const uint4*>(lsrc + (t_begin + lt) * ncol);
+    ld1 = *reinterpret_cast<const uint4*>(lsrc + (t_begin + lt + 32) * ncol);
+  }
+  auto stash = [&](uint32_t* b) {
+    b[lt * kTS4 + 0 * 8 + lu] = ld0.x;
+    b[lt * kTS4 + 1 * 8 + lu] = ld0.y;
+    b[lt * kTS4 + 2 * 8 + lu] = ld0.z;
+    b[lt * kTS4 + 3 * 8 + lu] = ld0.w;
+    b[(lt + 32) * kTS4 + 0 * 8 + lu] = ld1.x;
+    b[(lt + 32) * kTS4 + 1 * 8 + lu] = ld1.y;
+    b[(lt + 32) * kTS4 + 2 * 8 + lu] = ld1.z;
+    b[(lt + 32) * kTS4 + 3 * 8 + lu] = ld1.w;
+  };
+  stash(buf[0]);
+  __syncthreads();
   intx4 acc[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) acc[j] = intx4{0, 0, 0, 0};
-  // control flow stays wave-uniform around the matrix instructions: rows of
-  // absent inputs load nothing and feed zeros
-  for (int64_t t0 = blk * blocklen; t0 < (blk + 1) * blocklen; t0 += 64) {
+  for (int64_t st = 0; st < nsteps; ++st) {
+    if (st + 1 < nsteps && lok) {
+      const int64_t t1 = t_begin + (st + 1) * 64;
+      ld0 = *reinterpret_cast<const uint4*>(lsrc + (t1 + lt) * ncol);
+      ld1 = *reinterpret_cast<const uint4*>(lsrc + (t1 + lt + 32) * ncol);
+    }
+    const uint32_t* b = buf[st & 1];
     uint32_t x[16];
 #pragma unroll
-    for (int i = 0; i < 16; ++i)
-      x[i] = uok ? *reinterpret_cast<const uint32_t*>(src + (t0 + 16 * g + i) * ncol) : 0u;
+    for (int i = 0; i < 16; ++i) x[i] = b[(16 * g + i) * kTS4 + wv * 8 + u];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       intx4 v;
@@ -310,8 +346,11 @@ __global__ __launch_bounds__(256) void allpairs4_mfma_kernel(InputPtrs in, int64
         sgn |= sgn << 2;
         v[d] = int(nib | sgn);
       }
+      // rows of absent inputs hold zeros (their loads were skipped)
       acc[j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(v, v, acc[j], 0, 0, 0);
     }
+    if (st + 1 < nsteps) stash(buf[(st + 1) & 1]);   // that buffer was last read one step ago
+    __syncthreads();
   }
   // D layout: lane l holds G[4 (l / 16) + v][l % 16]
 #pragma unroll
@@ -319,17 +358,21 @@ __global__ __launch_bounds__(256) void allpairs4_mfma_kernel(InputPtrs in, int64
 #pragma unroll
     for (int v = 0; v < 4; ++v) S[wv][j][(4 * g + v) * 16 + r] = acc[j][v];
   __syncthreads();
-  if (wok) {
-    for (int it = lane; it < NP * 4; it += 64) {
-      const int j = it % 4, p = it / 4;
-      int a = 0, rem = p;
-      while (rem >= NIN - a) { rem -= NIN - a; ++a; }
-      const int b = a + rem;
-      const int* G = S[wv][j];
-      const long long re = (long long)G[a * 16 + b] + G[(8 + a) * 16 + 8 + b];
-      const long long im = (long long)G[(8 + a) * 16 + b] - G[a * 16 + 8 + b];
-      const int64_t ch = 4 * w + j;
-      long long* o = out + ((blk * NP + p) * ncol + ch) * 2;
+  const int64_t w = gx * kDW4 + wv;
+  for (int it = lane; it < NP * 4; it += 64) {
+    const int j = it % 4, p = it / 4;
+    int a = 0, rem = p;
+    while (rem >= NIN - a) { rem -= NIN - a; ++a; }
+    const int bb = a + rem;
+    const int* G = S[wv][j];
+    const long long re = (long long)G[a * 16 + bb] + G[(8 + a) * 16 + 8 + bb];
+    const long long im = (long long)G[(8 + a) * 16 + bb] - G[a * 16 + 8 + bb];
+    const int64_t ch = 4 * w + j;
+    long long* o = out + ((blk * NP + p) * ncol + ch) * 2;
+    if (nsplit > 1) {
+      if (re) atomic_add_i64(o, re);
+      if (im) atomic_add_i64(o + 1, im);
+    } else {
       o[0] = re;
       o[1] = im;
     }
@@ -339,42 +382,72 @@ __global__ __launch_bounds__(256) void allpairs4_mfma_kernel(InputPtrs in, int64
 // 1-bit: bit b -> x = 2b - 1.  The products run on the bits themselves
 // (0/1 int8): with G' = B B^T,  sum x_a x_b = 4 G'_ab - 2 G'_aa - 2 G'_bb + K
 // (b^2 = b, so the diagonal holds the bit counts).  Row u = input u; three
-// products per channel: re re^T, im im^T, re im^T.  A wave takes BPW bytes
-// (4 BPW channels) of a dword column: the two waves of a dword share its loads
-// through L1 and each keeps 12 BPW accumulator tiles.
-constexpr int kBPW1 = 2;
-constexpr int kDwordsPerWG1 = 4 * kBPW1 / 4;
+// products per channel: re re^T, im im^T, re im^T.
+//
+// Workgroup: 8 waves, 4 dword columns (16 bytes = 64 channels of every row)
+// of one block.  Each 64-spectrum step the workgroup loads the 16-byte row
+// pieces of all inputs (one 16-byte load per (input, spectrum)) into LDS,
+// double-buffered, laid out [spectrum][dword][input] with a 65-dword spectrum
+// stride so the matrix-operand reads (16 inputs x 4 spectrum groups per
+// instruction) hit 64 different banks.  Wave w takes bytes 2 (w % 2), +1 of
+// dword w / 2: 8 channels, 24 accumulator tiles.
+constexpr int kDW1 = 4;          // dword columns per workgroup
+constexpr int kTS1 = 65;         // LDS dwords per spectrum
+constexpr int kT1 = 512;         // threads per workgroup
 
 template <int NIN>
-__global__ __launch_bounds__(256) void allpairs1_mfma_kernel(InputPtrs in, int64_t ncol, int64_t nchan,
+__global__ __launch_bounds__(kT1) void allpairs1_mfma_kernel(InputPtrs in, int64_t ncol, int64_t nchan,
                                                              int64_t blocklen, int64_t nblock,
                                                              long long* __restrict__ out) {
   constexpr int NP = NIN * (NIN + 1) / 2;
-  constexpr int BPW = kBPW1;
-  __shared__ int S[4][4][3][256];
-  const int wv = threadIdx.x / 64, lane = threadIdx.x % 64;
+  __shared__ uint32_t buf[2][64 * kTS1];
+  const int tid = threadIdx.x;
+  const int wv = tid / 64, lane = tid % 64;
   int64_t blk, gx;
-  mfma_item((ncol / 4 + kDwordsPerWG1 - 1) / kDwordsPerWG1, nblock, &blk, &gx);
-  const int64_t w = gx * kDwordsPerWG1 + wv / (4 / BPW);
-  const int j0 = (wv % (4 / BPW)) * BPW;
-  const bool wok = w < ncol / 4;
+  mfma_item((ncol / 4) / kDW1, nblock, &blk, &gx);
+  const int dw = wv / 2, j0 = (wv % 2) * 2;
   const int u = lane % 16, g = lane / 16;
-  const bool uok = wok && u < NIN;
-  const uint8_t* src = in.p[uok ? u : 0] + 4 * (wok ? w : 0);
-  intx4 acc[BPW][4][3];
+  // loader: chunks c = tid + 512 e (e < 2): input lu = c % 16, spectrum lt = c / 16
+  const int lu = tid % 16, lt = tid / 16;           // lt in [0, 32); second chunk lt + 32
+  const bool lok = lu < NIN;
+  const uint8_t* lsrc = in.p[lok ? lu : 0] + 16 * gx;
+  const int64_t t_begin = blk * blocklen, nsteps = blocklen / 64;
+  uint4 ld0 = make_uint4(0, 0, 0, 0), ld1 = ld0;
+  if (lok) {
+    ld0 = *reinterpret_cast<const uint4*>(lsrc + (t_begin + lt) * ncol);
+    ld1 = *reinterpret_cast<const uint4*>(lsrc + (t_begin + lt + 32) * ncol);
+  }
+  auto stash = [&](uint32_t* b) {
+    b[lt * kTS1 + 0 * 16 + lu] = ld0.x;
+    b[lt * kTS1 + 1 * 16 + lu] = ld0.y;
+    b[lt * kTS1 + 2 * 16 + lu] = ld0.z;
+    b[lt * kTS1 + 3 * 16 + lu] = ld0.w;
+    b[(lt + 32) * kTS1 + 0 * 16 + lu] = ld1.x;
+    b[(lt + 32) * kTS1 + 1 * 16 + lu] = ld1.y;
+    b[(lt + 32) * kTS1 + 2 * 16 + lu] = ld1.z;
+    b[(lt + 32) * kTS1 + 3 * 16 + lu] = ld1.w;
+  };
+  stash(buf[0]);
+  __syncthreads();
+  intx4 acc[2][4][3];
 #pragma unroll
-  for (int j = 0; j < BPW; ++j)
+  for (int j = 0; j < 2; ++j)
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
       for (int m = 0; m < 3; ++m) acc[j][c][m] = intx4{0, 0, 0, 0};
-  for (int64_t t0 = blk * blocklen; t0 < (blk + 1) * blocklen; t0 += 64) {
+  for (int64_t st = 0; st < nsteps; ++st) {
+    if (st + 1 < nsteps && lok) {
+      const int64_t t1 = t_begin + (st + 1) * 64;
+      ld0 = *reinterpret_cast<const uint4*>(lsrc + (t1 + lt) * ncol);
+      ld1 = *reinterpret_cast<const uint4*>(lsrc + (t1 + lt + 32) * ncol);
+    }
+    const uint32_t* b = buf[st & 1];
     uint32_t x[16];
 #pragma unroll
-    for (int i = 0; i < 16; ++i)
-      x[i] = uok ? *reinterpret_cast<const uint32_t*>(src + (t0 + 16 * g + i) * ncol) : 0u;
+    for (int i = 0; i < 16; ++i) x[i] = b[(16 * g + i) * kTS1 + dw * 16 + u];
 #pragma unroll
-    for (int j = 0; j < BPW; ++j) {
+    for (int j = 0; j < 2; ++j) {
       uint32_t pd[4];
 #pragma unroll
       for (int d = 0; d < 4; ++d)
@@ -392,40 +465,43 @@ __global__ __launch_bounds__(256) void allpairs1_mfma_kernel(InputPtrs in, int64
         acc[j][c][2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(re, im, acc[j][c][2], 0, 0, 0);
       }
     }
+    if (st + 1 < nsteps) stash(buf[(st + 1) & 1]);   // that buffer was last read one step ago
+    __syncthreads();
   }
+  // outputs: per channel, the wave's three 16 x 16 products go through its LDS
+  // slot (768 dwords of the staging buffers) and lanes form the pairs
+  int* S = reinterpret_cast<int*>(&buf[0][0]) + wv * 768;
   const long long K = blocklen;
 #pragma unroll
-  for (int j = 0; j < BPW; ++j) {
+  for (int j = 0; j < 2; ++j) {
 #pragma unroll
-    for (int c = 0; c < 4; ++c)
+    for (int c = 0; c < 4; ++c) {
 #pragma unroll
       for (int m = 0; m < 3; ++m)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) S[wv][c][m][(4 * g + v) * 16 + u] = acc[j][c][m][v];
-    __syncthreads();
-    if (wok) {
-      for (int it = lane; it < NP * 4; it += 64) {
-        const int c = it % 4, p = it / 4;
-        int a = 0, rem = p;
-        while (rem >= NIN - a) { rem -= NIN - a; ++a; }
-        const int b = a + rem;
-        const int* RR = S[wv][c][0];
-        const int* II = S[wv][c][1];
-        const int* RI = S[wv][c][2];
-        const long long ra = RR[a * 17], rb = RR[b * 17], ia = II[a * 17], ib = II[b * 17];
-        const long long gre = 4LL * RR[a * 16 + b] - 2 * ra - 2 * rb + K;     // sum re_a re_b
-        const long long gim = 4LL * II[a * 16 + b] - 2 * ia - 2 * ib + K;     // sum im_a im_b
-        const long long xab = 4LL * RI[a * 16 + b] - 2 * ra - 2 * ib + K;     // sum re_a im_b
-        const long long xba = 4LL * RI[b * 16 + a] - 2 * rb - 2 * ia + K;     // sum re_b im_a
-        const int64_t ch = 4 * (4 * w + j0 + j) + c;
-        if (ch < nchan) {
+        for (int v = 0; v < 4; ++v) S[m * 256 + (4 * g + v) * 16 + u] = acc[j][c][m][v];
+      __syncthreads();
+      const int64_t ch = 4 * (4 * (gx * kDW1 + dw) + j0 + j) + c;
+      if (ch < nchan) {
+        for (int p = lane; p < NP; p += 64) {
+          int a = 0, rem = p;
+          while (rem >= NIN - a) { rem -= NIN - a; ++a; }
+          const int bb = a + rem;
+          const int* RR = S;
+          const int* II = S + 256;
+          const int* RI = S + 512;
+          const long long ra = RR[a * 17], rb = RR[bb * 17], ia = II[a * 17], ib = II[bb * 17];
+          const long long gre = 4LL * RR[a * 16 + bb] - 2 * ra - 2 * rb + K;     // sum re_a re_b
+          const long long gim = 4LL * II[a * 16 + bb] - 2 * ia - 2 * ib + K;     // sum im_a im_b
+          const long long xab = 4LL * RI[a * 16 + bb] - 2 * ra - 2 * ib + K;     // sum re_a im_b
+          const long long xba = 4LL * RI[bb * 16 + a] - 2 * rb - 2 * ia + K;     // sum re_b im_a
           long long* o = out + ((blk * NP + p) * nchan + ch) * 2;
           o[0] = gre + gim;
           o[1] = xba - xab;
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
@@ -505,13 +581,24 @@ int alb_xcorr_4bit_allpairs(const uint8_t* const* inputs, int ninput, int64_t nr
   const int64_t npair = int64_t(ninput) * (ninput + 1) / 2;
   hipStream_t s = as_stream(stream);
   const int64_t method = tune_get("xcorr_int_method");
-  const bool mfma_ok = blocklen % 64 == 0 && blocklen <= (int64_t(1) << 24);
-  ALB_REQUIRE(method != 2 || mfma_ok, "alb_xcorr_4bit_allpairs: the matrix-core form needs blocklen %% 64 == 0");
+  bool aligned16 = true;
+  for (int u = 0; u < ninput; ++u) aligned16 = aligned16 && (reinterpret_cast<uintptr_t>(inputs[u]) % 16 == 0);
+  const bool mfma_ok = blocklen % 64 == 0 && ncol % 16 == 0 && aligned16 && blocklen <= (int64_t(1) << 24);
+  ALB_REQUIRE(method != 2 || mfma_ok,
+              "alb_xcorr_4bit_allpairs: the matrix-core form needs blocklen %% 64 == 0, ncol %% 16 == 0 and "
+              "16-byte aligned inputs");
   if (method != 1 && mfma_ok) {
-    const int64_t total = ((ncol / 4 + 3) / 4) * nblock;
+    // enough workgroups to fill the chip: split blocks (exact partial sums,
+    // combined by atomics into the zero-filled output)
+    int nsplit = 1;
+    const int64_t ngx = (ncol / 4) / kDW4;
+    while (ngx * nblock * nsplit < 2048 && (blocklen / 64) % (2 * nsplit) == 0) nsplit *= 2;
+    const int64_t total = ngx * nblock * nsplit;
     ALB_REQUIRE(total < (int64_t(1) << 31), "alb_xcorr_4bit_allpairs: too many blocks");
+    if (nsplit > 1)
+      hipLaunchKernelGGL(fill_i64, grid1(nblock * npair * ncol * 2), dim3(256), 0, s, out, nblock * npair * ncol * 2);
     switch (ninput) {
-#define CASE(n) case n: hipLaunchKernelGGL(allpairs4_mfma_kernel<n>, dim3(unsigned(total)), dim3(256), 0, s, ip, ncol, blocklen, nblock, (long long*)out); break;
+#define CASE(n) case n: hipLaunchKernelGGL(allpairs4_mfma_kernel<n>, dim3(unsigned(total)), dim3(256), 0, s, ip, ncol, blocklen, nblock, nsplit, (long long*)out); break;
       CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
 #undef CASE
     }
@@ -548,14 +635,17 @@ int alb_xcorr_1bit_allpairs(const uint8_t* const* inputs, int ninput, int64_t nr
   const int npair = ninput * (ninput + 1) / 2;
   hipStream_t s = as_stream(stream);
   const int64_t method = tune_get("xcorr_int_method");
-  const bool mfma_ok = blocklen % 64 == 0 && ncol % 4 == 0 && blocklen <= (int64_t(1) << 28);
+  bool aligned16 = true;
+  for (int u = 0; u < ninput; ++u) aligned16 = aligned16 && (reinterpret_cast<uintptr_t>(inputs[u]) % 16 == 0);
+  const bool mfma_ok = blocklen % 64 == 0 && ncol % 16 == 0 && aligned16 && blocklen <= (int64_t(1) << 28);
   ALB_REQUIRE(method != 2 || mfma_ok,
-              "alb_xcorr_1bit_allpairs: the matrix-core form needs blocklen %% 64 == 0 and ncol %% 4 == 0");
+              "alb_xcorr_1bit_allpairs: the matrix-core form needs blocklen %% 64 == 0, ncol %% 16 == 0 and "
+              "16-byte aligned inputs");
   if (method != 1 && mfma_ok) {
-    const int64_t total = ((ncol / 4 + kDwordsPerWG1 - 1) / kDwordsPerWG1) * nblock;
+    const int64_t total = ((ncol / 4) / kDW1) * nblock;
     ALB_REQUIRE(total < (int64_t(1) << 31), "alb_xcorr_1bit_allpairs: too many blocks");
     switch (ninput) {
-#define CASE(n) case n: hipLaunchKernelGGL(allpairs1_mfma_kernel<n>, dim3(unsigned(total)), dim3(256), 0, s, ip, ncol, nchan, blocklen, nblock, (long long*)out); break;
+#define CASE(n) case n: hipLaunchKernelGGL(allpairs1_mfma_kernel<n>, dim3(unsigned(total)), dim3(kT1), 0, s, ip, ncol, nchan, blocklen, nblock, (long long*)out); break;
       CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
       CASE(14) CASE(15) CASE(16)
 #undef CASE

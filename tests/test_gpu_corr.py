@@ -85,13 +85,22 @@ def test_allpairs_4bit(cr, rng, method):
     exp = ocorr.allpairs_4bit_block_sums(inp, 4096)
     assert got.shape == (2, 10, 128, 2) and np.array_equal(got, exp)
     # edge: nibble 8 (+8) everywhere, the largest products
-    eights = np.full((3, 4096, 8), 0x88, dtype=np.uint8)
+    eights = np.full((3, 4096, 16), 0x88, dtype=np.uint8)
     got = cr.xcorr_4bit_allpairs(eights, 1024, method=method)
     assert np.all(got[..., 0] == 128 * 1024) and np.all(got[..., 1] == 0)
-    # every byte value, 8 inputs (a full 16-row product), 12 channels
-    raw = rng.integers(0, 256, size=(8, 640, 12), dtype=np.uint8)
+    # every byte value, 8 inputs (a full 16-row product), 32 channels
+    raw = rng.integers(0, 256, size=(8, 640, 32), dtype=np.uint8)
     got = cr.xcorr_4bit_allpairs(raw, 320, method=method)
     assert np.array_equal(got, ocorr.allpairs_4bit_block_sums(raw, 320))
+
+
+def test_allpairs_4bit_ragged_columns(cr, rng):
+    # 12 channels: the VALU form serves "auto", the matrix-core form refuses
+    raw = rng.integers(0, 256, size=(5, 640, 12), dtype=np.uint8)
+    got = cr.xcorr_4bit_allpairs(raw, 320)
+    assert np.array_equal(got, ocorr.allpairs_4bit_block_sums(raw, 320))
+    with pytest.raises(ValueError):
+        cr.xcorr_4bit_allpairs(raw, 320, method="mfma")
 
 
 @pytest.mark.parametrize("method", ["valu", "mfma"])
