@@ -263,12 +263,31 @@ __device__ __forceinline__ void cols_core(const float* __restrict__ yt, int64_t 
   constexpr int NP = CW / 2;
   constexpr int PADN = Padded<NSL>::value;
   const float* ys = yt + (k0 / ldy) * ldy * NSL + (k0 % ldy);
-  for (int idx = tid; idx < NSL * NP; idx += T) {
-    const int i = idx / NP, p = idx - i * NP;
-    U[p * PADN + pad16(i)] = reinterpret_cast<const float2*>(ys + i * ldy)[p];
+  // First radix-16 pass of the forward transform straight from memory:
+  // butterfly (j, p) takes rows j + NB r (r < 16) of sequence p (threads run p
+  // fastest, so a wave reads whole row pieces) and stores the Stockham pass
+  // output U[p][16 j + r]; the other passes start at sub-transform size 16.
+  {
+    constexpr int NB = NSL / 16;
+    constexpr int TOT = NB * NP;
+    constexpr int PER = (TOT + T - 1) / T;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int b = tid + e * T;
+      if (TOT % T == 0 || b < TOT) {
+        const int p = b % NP, j = b / NP;
+        float2 v[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) v[r] = reinterpret_cast<const float2*>(ys + int64_t(j + NB * r) * ldy)[p];
+        DFT<16>::run(v);
+        float2* Up = U + p * PADN;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) Up[pad16(16 * j + r)] = v[r];
+      }
+    }
   }
   __syncthreads();
-  fft_lds<NSL, NP, T>(U, TA, TB, lb, 0, tid);
+  fft_lds<NSL, NP, T, 16>(U, TA, TB, lb, 0, tid);
   // filter on the real-pair spectra, stored conjugated for the inverse
   const float sc = 1.0f / float(NSL);
   for (int idx = tid; idx < NP * (NSL / 2 + 1); idx += T) {
@@ -716,10 +735,24 @@ __global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __rest
     wc = make_float2(float(cs), float(sn));
   }
   __syncthreads();
-  for (int k = tid; k < N; k += kTPB) {
-    float2 v = src[k];
-    if (q1 != 0) v = cmul(v, cmul(wc, tw_lookup(TA, TB, tw4.lb, (q1 * (k / R1)) & (N - 1))));
-    Z[pad16(k)] = v;
+  // First radix-16 pass straight from memory: thread j takes k = j + 256 r
+  // (r < 16), twiddles, transforms and stores the Stockham pass output
+  // Z[16 j + r]; the remaining passes start at sub-transform size 16.
+  static_assert(N / 16 == kTPB, "one first-pass butterfly per thread");
+  {
+    float2 v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = src[tid + 256 * r];
+    if (q1 != 0) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int k = tid + 256 * r;
+        v[r] = cmul(v[r], cmul(wc, tw_lookup(TA, TB, tw4.lb, (q1 * (k / R1)) & (N - 1))));
+      }
+    }
+    DFT<16>::run(v);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) Z[pad16(16 * tid + r)] = v[r];
   }
   __syncthreads();
   int64_t alo, ahi, mlo = 0, mhi = -1;
@@ -732,7 +765,7 @@ __global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __rest
   if (na + nm <= 512) {
     // passes NS = 1, 16; the last radix-16 pass only for the wanted bins:
     // U[j] = sum_m Z[(j % 256) + 256 m] W_4096^(m j)
-    fft_lds<N, 1, kTPB, 1, 256>(Z, TA, TB, tw4.lb, 0, tid);
+    fft_lds<N, 1, kTPB, 16, 256>(Z, TA, TB, tw4.lb, 0, tid);
     for (int64_t t = tid; t < na + nm; t += kTPB) {
       const bool dir = t < na;
       const int64_t q2 = dir ? alo + t : mlo + (t - na);
@@ -750,7 +783,7 @@ __global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __rest
       ob[(q - c_lo) * sc] = dir ? make_float2(ax, ay) : make_float2(ax, -ay);
     }
   } else {
-    fft_lds<N, 1, kTPB>(Z, TA, TB, tw4.lb, 0, tid);
+    fft_lds<N, 1, kTPB, 16>(Z, TA, TB, tw4.lb, 0, tid);
     for (int64_t t = tid; t < na + nm; t += kTPB) {
       const bool dir = t < na;
       const int64_t q2 = dir ? alo + t : mlo + (t - na);
