@@ -51,6 +51,16 @@ def make_continuous_gpu(spec, chan_specnum, start_specnum, channels, nspec, ncha
         o = t.empty((nspec, nchans), dtype=t.complex64, device=sp.device)
     else:
         o = out
+    # a repeated channel number keeps its last column (numpy fancy assignment,
+    # baseband_data_classes.py:72); earlier copies are marked -1 and skipped
+    if len(chans) and len(np.unique(chans)) != len(chans):
+        chans = chans.copy()
+        seen = set()
+        for k in range(len(chans) - 1, -1, -1):
+            if chans[k] in seen:
+                chans[k] = -1
+            else:
+                seen.add(chans[k])
     sn = _lib.to_device(rows if len(rows) else np.zeros(1, dtype=np.int64), t.int64)
     ch = _lib.to_device(chans if len(chans) else np.zeros(1, dtype=np.int64), t.int64)
     _lib.call("alb_make_continuous", _lib.ptr(sp), sp.shape[1] if sp.ndim == 2 else 1, _lib.ptr(sn), len(rows),

@@ -112,16 +112,17 @@ __global__ __launch_bounds__(kTPB) void irfft_rows_kernel(DenseRows dense, Packe
   } else {
     for (int idx = tid; idx < F * MP; idx += kTPB) Z[idx] = make_float2(0.f, 0.f);
     __syncthreads();
-    // channel ch contributes x (1 + i conj W^ch) to Z[ch] and conj(x) (1 + i W^ch) to Z[M-ch]
+    // channel ch contributes x (1 + i conj W^ch) to Z[ch] and conj(x) (1 + i W^ch)
+    // to Z[M-ch]; columns with a negative channel number are skipped
     for (int64_t idx = tid; idx < F * pk.ncols; idx += kTPB) {
       const int f = int(idx / pk.ncols);
       const int64_t c = idx - int64_t(f) * pk.ncols;
       const int64_t row = row0 + f;
-      if (row >= nrows) continue;
+      const int ch = pk.chan[c];
+      if (row >= nrows || ch < 0) continue;
       const int64_t p = packed_row(pk, row, nslice);
       if (p < 0) continue;
       const int byte = pk.bytes[int64_t(p) * pk.ncols + c];
-      const int ch = pk.chan[c];
       float2 x = make_float2(float(nib4f(byte >> 4)), float(nib4f(byte & 15)));
       if (ch == 0 || ch == M) x.y = 0.f;
       const float2 w = tw_lookup(TA, TB, tw.lb, ch);
@@ -879,19 +880,26 @@ int launch_irfft(DenseRows d, Packed4Rows pk, int64_t nrows, int64_t nslice, flo
 template <int MODE>
 int irfft_dispatch(int64_t n, DenseRows d, Packed4Rows pk, int64_t nrows, int64_t nslice, float* out, int64_t ldo,
                    hipStream_t s) {
-  switch (n) {
-    case 64: return launch_irfft<32, 64, MODE>(d, pk, nrows, nslice, out, ldo, s);
-    case 128: return launch_irfft<64, 32, MODE>(d, pk, nrows, nslice, out, ldo, s);
-    case 256: return launch_irfft<128, 16, MODE>(d, pk, nrows, nslice, out, ldo, s);
-    case 512: return launch_irfft<256, 8, MODE>(d, pk, nrows, nslice, out, ldo, s);
-    case 1024: return launch_irfft<512, 4, MODE>(d, pk, nrows, nslice, out, ldo, s);
-    case 2048: return launch_irfft<1024, 2, MODE>(d, pk, nrows, nslice, out, ldo, s);
-    case 4096: return launch_irfft<2048, 2, MODE>(d, pk, nrows, nslice, out, ldo, s);
-    case 8192: return launch_irfft<4096, 1, MODE>(d, pk, nrows, nslice, out, ldo, s);
-    case 16384: return launch_irfft<8192, 1, MODE>(d, pk, nrows, nslice, out, ldo, s);
-    default:
-      set_error("irfft: length %lld not supported (power of two 64..16384)", (long long)n);
-      return ALB_ENOTSUP;
+  if constexpr (MODE == 1) {
+    // packed 4-bit rows: 2049-channel spectra only
+    if (n == 4096) return launch_irfft<2048, 2, 1>(d, pk, nrows, nslice, out, ldo, s);
+    set_error("irfft: packed rows need n = 4096 (got %lld)", (long long)n);
+    return ALB_ENOTSUP;
+  } else {
+    switch (n) {
+      case 64: return launch_irfft<32, 64, MODE>(d, pk, nrows, nslice, out, ldo, s);
+      case 128: return launch_irfft<64, 32, MODE>(d, pk, nrows, nslice, out, ldo, s);
+      case 256: return launch_irfft<128, 16, MODE>(d, pk, nrows, nslice, out, ldo, s);
+      case 512: return launch_irfft<256, 8, MODE>(d, pk, nrows, nslice, out, ldo, s);
+      case 1024: return launch_irfft<512, 4, MODE>(d, pk, nrows, nslice, out, ldo, s);
+      case 2048: return launch_irfft<1024, 2, MODE>(d, pk, nrows, nslice, out, ldo, s);
+      case 4096: return launch_irfft<2048, 2, MODE>(d, pk, nrows, nslice, out, ldo, s);
+      case 8192: return launch_irfft<4096, 1, MODE>(d, pk, nrows, nslice, out, ldo, s);
+      case 16384: return launch_irfft<8192, 1, MODE>(d, pk, nrows, nslice, out, ldo, s);
+      default:
+        set_error("irfft: length %lld not supported (power of two 64..16384)", (long long)n);
+        return ALB_ENOTSUP;
+    }
   }
 }
 

@@ -121,7 +121,17 @@ class RePFBPipeline:
         chan_numbers = np.asarray(chan_numbers)
         if chan_numbers.size == 0 or chan_numbers.min() < 0 or chan_numbers.max() > 2048:
             raise ValueError("coarse channel numbers must lie in [0, 2048]")
-        self.chan = _lib.to_device(chan_numbers.astype(np.int32), t.int32)
+        # a repeated channel number keeps its last column (numpy's fancy
+        # assignment, baseband_data_classes.py:72): earlier copies are -1, which
+        # the row passes skip
+        chan_use = chan_numbers.astype(np.int32).copy()
+        seen = set()
+        for k in range(len(chan_use) - 1, -1, -1):
+            if int(chan_use[k]) in seen:
+                chan_use[k] = -1
+            else:
+                seen.add(int(chan_use[k]))
+        self.chan = _lib.to_device(chan_use, t.int32)
         self.ncols = int(self.chan.numel())
         self.filt = _lib.to_device(filt, t.complex64) if filt is not None else _pfb.get_filter(self.nslice, filt_thresh,
                                                                                                   ntap=ntap)

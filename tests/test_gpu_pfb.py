@@ -330,12 +330,14 @@ def test_batched_stream_graph_and_per_chunk_forms():
 
 @pytest.mark.parametrize("nslice,chans", [(256, [0, 5, 2048, 1000, 1001, 1700, 7, 8, 9, 10, 11, 12, 13, 14, 15,
                                                   16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31,
-                                                  32, 33, 34, 35, 36, 2047]),
+                                                  32, 33, 34, 35, 36, 2047, 5]),
                                            (4096, list(range(64, 164)))])
 def test_row_pass_mfma_matches_fft(rng, nslice, chans):
     # matrix-core synthesis of the iPFB row pass (blocked tile layout) against
     # the FFT row pass; DC (0) and Nyquist (2048) channels drop their imaginary
-    # parts in both; ncols not a multiple of 16 exercises the padded k-steps
+    # parts in both; ncols not a multiple of 16 exercises the padded k-steps; a
+    # repeated channel number (5) keeps its last column, as numpy's fancy
+    # assignment of make_continuous does
     import torch
     from albatros_axion_detection_amd import _lib
     acclen, cut, ntiles = nslice - 8, 4, 3
@@ -347,7 +349,13 @@ def test_row_pass_mfma_matches_fft(rng, nslice, chans):
     dev = _lib.device()
     pk = _lib.to_device(packed)
     rm = _lib.to_device(rowmap)
-    ch = _lib.to_device(np.asarray(chans, dtype=np.int32))
+    # the entry points take -1 for the dropped copies of a repeated channel
+    # (RePFBPipeline marks them so); the reference assigns every column in order
+    marked = np.asarray(chans, dtype=np.int32).copy()
+    for k in range(len(marked)):
+        if chans[k] in chans[k + 1:]:
+            marked[k] = -1
+    ch = _lib.to_device(marked)
     y_fft = torch.empty((ntiles, nslice, 4096), dtype=torch.float32, device=dev)
     y_mma = torch.empty((ntiles, 4096 // 8, nslice, 8), dtype=torch.float32, device=dev)
     coeff = torch.empty(int(_lib.load().alb_ipfb_rows_coeff_bytes(ncols)), dtype=torch.uint8, device=dev)
@@ -374,7 +382,7 @@ def test_row_pass_mfma_matches_fft(rng, nslice, chans):
             chunk, crow = (t - 1, acclen - 2 * cut + row) if row < 2 * cut else (t, row - 2 * cut)
             if chunk < 0 or rowmap[chunk, crow] < 0:
                 continue
-            np.add.at(spec[t, row], np.asarray(chans), vals[rowmap[chunk, crow]])
+            spec[t, row, np.asarray(chans)] = vals[rowmap[chunk, crow]]
     ref = np.fft.irfft(spec, n=4096, axis=-1)
     # both are FP32 paths (max |err| / rms over ~3M samples): the FFT row pass
     # measures ~1.2e-6 here; the synthesis carries FP16 hi/lo coefficients

@@ -127,6 +127,12 @@ def test_make_continuous(rng):
     g = bdc.make_continuous_gpu(sp, sn, 1000 - 3 * acclen, ch, acclen, 2049)
     e = ostream.make_continuous(sp, sn, 1000 - 3 * acclen, ch, acclen, 2049)
     assert np.array_equal(g, e.astype(np.complex64))
+    # a repeated channel number keeps its last column
+    ch2 = ch.copy()
+    ch2[3] = ch2[20]
+    g = bdc.make_continuous_gpu(sp, sn, 1000 - 3 * acclen, ch2, acclen, 2049)
+    e = ostream.make_continuous(sp, sn, 1000 - 3 * acclen, ch2, acclen, 2049)
+    assert np.array_equal(g, e.astype(np.complex64))
 
 
 def test_dropped_packets(rng):

@@ -379,3 +379,13 @@ def test_stream_is_continuous_away_from_cut_edges():
     tile = np.concatenate([specs[acclen - 2 * cut: acclen], specs[acclen: 2 * acclen]])
     ref = opfb.ipfb(tile, filt)[cut:-cut].ravel()
     assert np.allclose(st[acclen * 4096: 2 * acclen * 4096], ref)
+
+
+def test_make_continuous_repeated_channel_keeps_last_column():
+    # numpy's fancy assignment (baseband_data_classes.py:72) keeps the last
+    # column of a repeated channel number; the device paths follow it
+    from oracle import stream as ostream
+    spec = np.array([[1 + 1j, 2 + 2j, 3 + 3j], [4 + 4j, 5 + 5j, 6 + 6j]])
+    out = ostream.make_continuous(spec, [10, 11], 10, [5, 7, 5], 4, nchans=9)
+    assert out[0, 5] == 3 + 3j and out[1, 5] == 6 + 6j and out[0, 7] == 2 + 2j
+    assert np.count_nonzero(out) == 4
