@@ -65,9 +65,9 @@ struct DenseRows {
 // Z[m] = (X[m] + conj X[M-m]) + i conj(W_N^m) (X[m] - conj X[M-m]) is formed
 // straight into LDS (stored conjugated, so a forward FFT gives the inverse),
 // then y[2n] + i y[2n+1] = IFFT_M(Z)[n] / N.
-template <int M, int F, int MODE>
-__global__ __launch_bounds__(kTPB) void irfft_rows_kernel(DenseRows dense, Packed4Rows pk, int64_t nrows, int64_t nslice,
-                                                          float* __restrict__ out, int64_t ldo, Twiddle tw) {
+template <int M, int F, int MODE, int TT = kTPB>
+__global__ __launch_bounds__(TT) void irfft_rows_kernel(DenseRows dense, Packed4Rows pk, int64_t nrows, int64_t nslice,
+                                                        float* __restrict__ out, int64_t ldo, Twiddle tw) {
   constexpr int N = 2 * M;
   constexpr int MP = Padded<M>::value;
   extern __shared__ float2 smem[];
@@ -82,11 +82,11 @@ __global__ __launch_bounds__(kTPB) void irfft_rows_kernel(DenseRows dense, Packe
   const int64_t per = (ngroups + 7) / 8;
   const int64_t item = int64_t(blockIdx.x % 8) * per + blockIdx.x / 8;
   if (item >= ngroups) return;   // whole workgroup: padding of the 8-way split
-  load_twiddle(tw, TA, TB, tid, kTPB);
+  load_twiddle(tw, TA, TB, tid, TT);
   const int64_t row0 = item * F;
   if constexpr (MODE == 0) {
     __syncthreads();
-    for (int idx = tid; idx < F * (M / 2 + 1); idx += kTPB) {
+    for (int idx = tid; idx < F * (M / 2 + 1); idx += TT) {
       const int f = idx / (M / 2 + 1), m = idx - f * (M / 2 + 1);
       const int64_t row = row0 + f;
       float2* Zf = Z + f * MP;
@@ -110,11 +110,11 @@ __global__ __launch_bounds__(kTPB) void irfft_rows_kernel(DenseRows dense, Packe
       }
     }
   } else {
-    for (int idx = tid; idx < F * MP; idx += kTPB) Z[idx] = make_float2(0.f, 0.f);
+    for (int idx = tid; idx < F * MP; idx += TT) Z[idx] = make_float2(0.f, 0.f);
     __syncthreads();
     // channel ch contributes x (1 + i conj W^ch) to Z[ch] and conj(x) (1 + i W^ch)
     // to Z[M-ch]; columns with a negative channel number are skipped
-    for (int64_t idx = tid; idx < F * pk.ncols; idx += kTPB) {
+    for (int64_t idx = tid; idx < F * pk.ncols; idx += TT) {
       const int f = int(idx / pk.ncols);
       const int64_t c = idx - int64_t(f) * pk.ncols;
       const int64_t row = row0 + f;
@@ -142,14 +142,14 @@ __global__ __launch_bounds__(kTPB) void irfft_rows_kernel(DenseRows dense, Packe
     }
   }
   __syncthreads();
-  fft_lds<M, F, kTPB>(Z, TA, TB, tw.lb, 1, tid);
+  fft_lds<M, F, TT>(Z, TA, TB, tw.lb, 1, tid);
   // ---- y[2n] + i y[2n+1] = conj(FFT) / N
   const float sc = 1.0f / float(N);
   if (ldo == 8) {
     // blocked tile layout: element (row i of tile t, column k) at
     // out[t][k / 8][i][k % 8] (rows of a tile nslice apart per block)
     const int64_t t0 = row0 / nslice, i0 = row0 - t0 * nslice;
-    for (int idx = tid; idx < F * M; idx += kTPB) {
+    for (int idx = tid; idx < F * M; idx += TT) {
       const int f = idx / M, n = idx - f * M;
       const int64_t row = row0 + f;
       if (row >= nrows) continue;
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(kTPB) void irfft_rows_kernel(DenseRows dense, Packe
     }
     return;
   }
-  for (int idx = tid; idx < F * M; idx += kTPB) {
+  for (int idx = tid; idx < F * M; idx += TT) {
     const int f = idx / M, n = idx - f * M;
     const int64_t row = row0 + f;
     if (row >= nrows) continue;
@@ -863,7 +863,7 @@ int cols_group() {
   return g > 0 ? int(g) : 32;
 }
 
-template <int M, int F, int MODE>
+template <int M, int F, int MODE, int TT = kTPB>
 int launch_irfft(DenseRows d, Packed4Rows pk, int64_t nrows, int64_t nslice, float* out, int64_t ldo, hipStream_t s) {
   Twiddle tw;
   int rc = get_twiddle(2 * M, &tw);
@@ -872,7 +872,8 @@ int launch_irfft(DenseRows d, Packed4Rows pk, int64_t nrows, int64_t nslice, flo
   const int64_t ngroups = (nrows + F - 1) / F;
   ALB_REQUIRE(ngroups < (int64_t(1) << 31) - 8, "irfft: too many rows");
   const unsigned grid = unsigned((ngroups + 7) / 8 * 8);
-  hipLaunchKernelGGL((irfft_rows_kernel<M, F, MODE>), dim3(grid), dim3(kTPB), lds, s, d, pk, nrows, nslice, out, ldo, tw);
+  hipLaunchKernelGGL((irfft_rows_kernel<M, F, MODE, TT>), dim3(grid), dim3(TT), lds, s, d, pk, nrows, nslice, out, ldo,
+                     tw);
   ALB_LAUNCH_CHECK("irfft_rows_kernel");
   return ALB_OK;
 }
@@ -882,7 +883,12 @@ int irfft_dispatch(int64_t n, DenseRows d, Packed4Rows pk, int64_t nrows, int64_
                    hipStream_t s) {
   if constexpr (MODE == 1) {
     // packed 4-bit rows: 2049-channel spectra only
-    if (n == 4096) return launch_irfft<2048, 2, 1>(d, pk, nrows, nslice, out, ldo, s);
+    if (n == 4096) {
+      // two rows per 256-thread workgroup (measured faster), or four per 512
+      // threads (each workgroup writing whole 128-byte lines of the blocked layout)
+      if (tune_get("ipfb_rows_group") == 4) return launch_irfft<2048, 4, 1, 512>(d, pk, nrows, nslice, out, ldo, s);
+      return launch_irfft<2048, 2, 1>(d, pk, nrows, nslice, out, ldo, s);
+    }
     set_error("irfft: packed rows need n = 4096 (got %lld)", (long long)n);
     return ALB_ENOTSUP;
   } else {

@@ -148,31 +148,35 @@ def rechannelizer(args, torch):
     sweep = {}
     widths = [args.cols_width] if args.cols_width else [8, 4, 2]
     groups = [args.cols_group] if args.cols_group else [32, 16, 8]
+    rgroups = [args.rows_group] if args.rows_group else [4, 2]
     for w in widths:
         for gsz in groups:
-            lib.alb_tune_set(b"ipfb_cols_width", w)
-            lib.alb_tune_set(b"ipfb_cols_group", gsz)
-            step()
-            torch.cuda.synchronize()
-            a = torch.cuda.Event(enable_timing=True)
-            b = torch.cuda.Event(enable_timing=True)
-            a.record()
-            for _ in range(3):
+            for rg in rgroups:
+                lib.alb_tune_set(b"ipfb_cols_width", w)
+                lib.alb_tune_set(b"ipfb_cols_group", gsz)
+                lib.alb_tune_set(b"ipfb_rows_group", rg)
                 step()
-            b.record()
-            torch.cuda.synchronize()
-            sweep["w%d_g%d" % (w, gsz)] = a.elapsed_time(b) / 3
+                torch.cuda.synchronize()
+                a = torch.cuda.Event(enable_timing=True)
+                b = torch.cuda.Event(enable_timing=True)
+                a.record()
+                for _ in range(3):
+                    step()
+                b.record()
+                torch.cuda.synchronize()
+                sweep["w%d_g%d_r%d" % (w, gsz, rg)] = a.elapsed_time(b) / 3
     best = min(sweep, key=sweep.get)
-    bw, bg = (int(x[1:]) for x in best.split("_"))
+    bw, bg, br = (int(x[1:]) for x in best.split("_"))
     lib.alb_tune_set(b"ipfb_cols_width", bw)
     lib.alb_tune_set(b"ipfb_cols_group", bg)
+    lib.alb_tune_set(b"ipfb_rows_group", br)
     log("[bench] column-pass launch-shape sweep (ms/step): %s -> %s" % (
         json.dumps({k: round(v, 4) for k, v in sweep.items()}), best))
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    timer.on = True
+    # timed region 1: K steps back to back (the throughput figure)
     t0e = torch.cuda.Event(enable_timing=True)
     t1e = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -184,6 +188,12 @@ def rechannelizer(args, torch):
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
     ms = t0e.elapsed_time(t1e) / args.steps
+    # timed region 2: the same K steps with a HIP event pair around every
+    # launch on its stream (per-kernel durations; the events add gaps between
+    # launches, so the step time above is taken without them)
+    timer.on = True
+    for _ in range(args.steps):
+        step()
     kern = timer.summary()
     timer.on = False
     log("[bench] rechannelizer: %.3f ms/step (wall %.3f ms/step); kernels %s" % (
@@ -225,7 +235,7 @@ def rechannelizer(args, torch):
                    "n_coarse": n_coarse, "acclen": acclen, "cut": cut, "osamp": osamp, "nchan_fine": c_hi - c_lo,
                    "nblock": nblk},
         "ref_unit_msps": n_coarse * 2049 * 8 / (ms * 1e-3) / 1e6,
-        "tuning": {"ipfb_cols_width": bw, "ipfb_cols_group": bg,
+        "tuning": {"ipfb_cols_width": bw, "ipfb_cols_group": bg, "ipfb_rows_group": br,
                    "sweep_ms_per_step": {k: round(v, 4) for k, v in sweep.items()}},
     }
     ctx = dict(packed_np=packed_np, acclen=acclen, cut=cut, osamp=osamp, chan0=chan0, nkeep=nkeep,
@@ -464,6 +474,8 @@ def main():
                     help="column-pass strip width (0: pick the fastest of 8, 4, 2 before timing)")
     ap.add_argument("--cols-group", type=int, default=0, choices=[0, 4, 8, 16, 32, 64],
                     help="column-pass strips per XCD group (0: pick the fastest of 32, 16, 8 before timing)")
+    ap.add_argument("--rows-group", type=int, default=0, choices=[0, 2, 4],
+                    help="spectra per workgroup of the FFT row pass (0: pick the faster of 4, 2 before timing)")
     ap.add_argument("--row-pass", default="auto", choices=["auto", "fft", "mfma"],
                     help="iPFB row pass: FFT per spectrum or matrix-core synthesis from the kept channels")
     ap.add_argument("--unfused", action="store_true",
