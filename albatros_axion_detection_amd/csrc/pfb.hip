@@ -287,28 +287,49 @@ __device__ __forceinline__ void cols_core(const float* __restrict__ yt, int64_t 
       }
     }
   }
+  // Filter taps: thread handles sequence fp = tid / TPS, bins
+  // f = tid % TPS + m TPS (m < KF) up to NSL / 2.  The first KP bins' taps are
+  // fetched now so that their loads are in flight during the forward passes
+  // (the rest are issued at the start of the filter loop, behind them).
+  constexpr int TPS = T / NP;
+  constexpr int NF = NSL / 2 + 1;
+  constexpr int KF = (NF + TPS - 1) / TPS;
+  constexpr int KP = KF < 2 ? KF : 2;
+  const int fp = tid / TPS, f0 = tid - fp * TPS;
+  const float2* h0p = filt + (k0 + 2 * fp) * ldf;
+  const float2* h1p = h0p + ldf;
+  float2 h0[KP], h1[KP];
+#pragma unroll
+  for (int m = 0; m < KP; ++m) {
+    const int f = f0 + m * TPS;
+    if ((m + 1) * TPS <= NF || f < NF) {
+      h0[m] = h0p[f];
+      h1[m] = h1p[f];
+    }
+  }
   __syncthreads();
   fft_lds<NSL, NP, T, 16>(U, TA, TB, lb, 0, tid);
   // filter on the real-pair spectra, stored conjugated for the inverse
   const float sc = 1.0f / float(NSL);
-  for (int idx = tid; idx < NP * (NSL / 2 + 1); idx += T) {
-    const int p = idx / (NSL / 2 + 1), f = idx - p * (NSL / 2 + 1);
-    float2* Up = U + p * PADN;
-    const float2 a = Up[pad16(f)];
-    const float2 b = Up[pad16((NSL - f) % NSL)];
-    const float2 y0 = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));   // (a + conj b)/2
-    const float2 dd = make_float2(0.5f * (a.x - b.x), 0.5f * (a.y + b.y));   // (a - conj b)/2
-    const float2 y1 = make_float2(dd.y, -dd.x);                              // -i dd
-    const float2 h0 = filt[(k0 + 2 * p) * ldf + f];
-    const float2 h1 = filt[(k0 + 2 * p + 1) * ldf + f];
-    float2 g0 = cscale(cmul(h0, y0), sc);
-    float2 g1 = cscale(cmul(h1, y1), sc);
-    if (f == 0 || 2 * f == NSL) { g0.y = 0.f; g1.y = 0.f; }
-    const float2 uf = make_float2(g0.x - g1.y, g0.y + g1.x);     // g0 + i g1
-    Up[pad16(f)] = cconj(uf);
-    if (f != 0 && 2 * f != NSL) {
-      const float2 ub = make_float2(g0.x + g1.y, -g0.y + g1.x);  // conj g0 + i conj g1
-      Up[pad16(NSL - f)] = cconj(ub);
+  float2* Uf = U + fp * PADN;
+#pragma unroll
+  for (int m = 0; m < KF; ++m) {
+    const int f = f0 + m * TPS;
+    if ((m + 1) * TPS <= NF || f < NF) {
+      const float2 a = Uf[pad16(f)];
+      const float2 b = Uf[pad16((NSL - f) & (NSL - 1))];
+      const float2 y0 = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));   // (a + conj b)/2
+      const float2 dd = make_float2(0.5f * (a.x - b.x), 0.5f * (a.y + b.y));   // (a - conj b)/2
+      const float2 y1 = make_float2(dd.y, -dd.x);                              // -i dd
+      float2 g0 = cscale(cmul(m < KP ? h0[m] : h0p[f], y0), sc);
+      float2 g1 = cscale(cmul(m < KP ? h1[m] : h1p[f], y1), sc);
+      if (f == 0 || 2 * f == NSL) { g0.y = 0.f; g1.y = 0.f; }
+      const float2 uf = make_float2(g0.x - g1.y, g0.y + g1.x);     // g0 + i g1
+      Uf[pad16(f)] = cconj(uf);
+      if (f != 0 && 2 * f != NSL) {
+        const float2 ub = make_float2(g0.x + g1.y, -g0.y + g1.x);  // conj g0 + i conj g1
+        Uf[pad16(NSL - f)] = cconj(ub);
+      }
     }
   }
   __syncthreads();
@@ -341,11 +362,28 @@ __global__ __launch_bounds__(T) void ipfb_cols_kernel(const float* __restrict__ 
   }
 }
 
+// ------------------------------------------- re-PFB stage-1 workspace layout
+// A[b] holds the bins q = 0..H (H = R1/2) of the R1-point real DFT down each of
+// the 4096 columns, grouped by 4 columns: bin slot s of columns 4g..4g+3 is
+// A[((b * 1024 + g) * QP + s) * 4 + k % 4].  Bins 0 and H of a real sequence
+// are real, so both share slot 0 (.x = bin 0, .y = bin H) and slots 1..H-1
+// hold the other bins: QP = H slots, and the bins of 4 columns of one block
+// are one contiguous run of 32 QP bytes (whole 128-byte lines for R1 >= 8).
+template <int R1>
+struct StageA {
+  static constexpr int H = R1 / 2;
+  static constexpr int QP = H > 0 ? H : 1;
+  static constexpr int64_t kBlock = int64_t(1024) * QP * 4;   // float2 per block
+  __host__ __device__ static int64_t at(int64_t b, int s, int64_t k) {
+    return (b * 1024 + (k >> 2)) * (QP * 4) + s * 4 + (k & 3);
+  }
+};
+
 // ------------------------------------ iPFB column pass fused with re-PFB stage 1
 // Stream row g = row0 + tile * rows_per_tile + (i - cut) holds kept row i of a
 // tile.  Re-PFB block b (R1 rows per block row, ntap taps) reads stream rows
 // [R1 b, R1 (b + ntap)).  The blocks whose rows all lie in this tile,
-// [bi0, bi1) below, get their stage-1 result A[b][q][k] (q <= R1/2) straight
+// [bi0, bi1) below, get their stage-1 result A[b][q][k] (q <= R1/2, StageA layout) straight
 // from LDS; only rows that other blocks read go to the stream buffer.
 __host__ __device__ inline void repfb_interior(int64_t g0, int64_t nkeep, int64_t R1, int64_t ntap, int64_t nblocks,
                                                int64_t* bi0, int64_t* bi1) {
@@ -402,22 +440,34 @@ __global__ __launch_bounds__(T) void ipfb_cols_repfb_kernel(const float* __restr
   int64_t bi0, bi1;
   repfb_interior(g0, nkeep, R1, ntap, nblocks, &bi0, &bi1);
   const int nint = int(bi1 - bi0);
-  // rows read by blocks outside [bi0, bi1) go to the stream buffer
-  const int64_t wlo = nint > 0 ? R1 * (bi0 + ntap - 1) : g0 + nkeep;   // rows [g0, wlo) are written
-  const int64_t whi = nint > 0 ? R1 * bi1 : g0 + nkeep;                // and rows [whi, g0 + nkeep)
-  for (int idx = tid; idx < nkeep * NP; idx += T) {
-    const int i = idx / NP, p = idx - i * NP;
-    const int64_t g = g0 + i;
-    if (g < wlo || g >= whi) {
+  // Rows read by blocks outside [bi0, bi1) go to the stream buffer: local
+  // rows [0, n1) and [n2, nkeep) (all rows when the two ranges meet).
+  {
+    int n1 = nkeep, n2 = nkeep;
+    if (nint > 0) {
+      n1 = int(R1 * (bi0 + ntap - 1) - g0);
+      n2 = int(R1 * bi1 - g0);
+      if (n1 > nkeep) n1 = nkeep;
+      if (n2 <= n1) n1 = n2 = nkeep;
+    }
+    const int nw = (n1 + nkeep - n2) * NP;
+    for (int idx = tid; idx < nw; idx += T) {
+      const int w = idx / NP, p = idx - w * NP;
+      const int i = w < n1 ? w : w - n1 + n2;
       const float2 v = U[p * PADN + pad16(i + cut)];
-      reinterpret_cast<float2*>(ts + g * C + k0)[p] = make_float2(v.x, -v.y);
+      reinterpret_cast<float2*>(ts + (g0 + i) * C + k0)[p] = make_float2(v.x, -v.y);
     }
   }
   if (nint == 0) return;
 
-  // polyphase sums and S-point DFTs: lane u = task * 4 + t, task = (b - bi0) + nint * p
+  // Polyphase sums and S-point DFTs: lane u = task * 4 + t, task = (b - bi0) + nint * p,
+  // sums rows r = 4 s + t (s < S) of block b over the taps.  The lane's rows
+  // R1 j + 4 (4 a + c) + t sit at padded offsets base_c + 17 a + (R1 + R1 / 16) j
+  // (R1 a multiple of 16), so one base per c serves every tap and every a.
   const int nlanes = nint * NP * 4;
-  constexpr int PER = 2;   // lane slots per thread (nint * NP * 4 <= PER * T is checked by the host)
+  constexpr int PER = 2;                     // lane slots per thread (nint * NP * 4 <= PER * T is checked by the host)
+  constexpr int JMAX = 512 / R1;             // ntap * R1 <= 512 is checked by the host
+  constexpr int RSTEP = R1 + R1 / 16;        // padded rows per tap
   float2 z[PER][S];
 #pragma unroll
   for (int e = 0; e < PER; ++e) {
@@ -425,19 +475,25 @@ __global__ __launch_bounds__(T) void ipfb_cols_repfb_kernel(const float* __restr
     if (u < nlanes) {
       const int t = u & 3, task = u >> 2;
       const int p = task / nint, bl = task - p * nint;
-      const int64_t b = bi0 + bl;
       const float2* Up = U + p * PADN;
+      const int x0 = int(R1 * (bi0 + bl) - g0) + cut + t;   // local row of s = 0, tap 0
+      const int c4 = (x0 & 15) >> 2;
+      const float2* vb[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) vb[c] = Up + pad16(x0) + 4 * c + (c + c4 >= 4 ? 1 : 0);
+      const float2* wb = reinterpret_cast<const float2*>(WIN + t * CW) + p;
 #pragma unroll
       for (int sI = 0; sI < S; ++sI) z[e][sI] = make_float2(0.f, 0.f);
-      for (int j = 0; j < ntap; ++j) {
-        const int ib = int(R1 * (b + j) - g0) + cut + t;   // local row of s = 0
-        const float2* wj = reinterpret_cast<const float2*>(WIN + (j * R1 + t) * CW) + p;
 #pragma unroll
-        for (int sI = 0; sI < S; ++sI) {
-          const float2 v = Up[pad16(ib + 4 * sI)];
-          const float2 w = wj[sI * 4 * NP];
-          z[e][sI].x = fmaf(v.x, w.x, z[e][sI].x);
-          z[e][sI].y = fmaf(-v.y, w.y, z[e][sI].y);
+      for (int j = 0; j < JMAX; ++j) {
+        if (j < ntap) {
+#pragma unroll
+          for (int sI = 0; sI < S; ++sI) {
+            const float2 v = vb[sI & 3][17 * (sI >> 2) + RSTEP * j];
+            const float2 w = wb[(j * R1 + 4 * sI) * NP];
+            z[e][sI].x = fmaf(v.x, w.x, z[e][sI].x);
+            z[e][sI].y = fmaf(-v.y, w.y, z[e][sI].y);
+          }
         }
       }
       DFT<S>::run(z[e]);
@@ -454,36 +510,62 @@ __global__ __launch_bounds__(T) void ipfb_cols_repfb_kernel(const float* __restr
     }
   }
   __syncthreads();
-  // radix-4 combine: lane (task, t) owns q0 = t, t+4, ...; Y[q0 + S pp] goes to slot (pp, q0)
-  for (int u = tid; u < nlanes; u += T) {
-    const int t = u & 3, task = u >> 2;
-    float2* Xt = X + task * 4 * SP;
-    for (int q0 = t; q0 < S; q0 += 4) {
-      float2 v[4];
+  // radix-4 combine: lane (task, t) owns q0 = t + 4 m; Y[q0 + S pp] goes to slot (pp, q0)
 #pragma unroll
-      for (int tt = 0; tt < 4; ++tt) v[tt] = cmul(Xt[tt * SP + q0], WR[(q0 * tt) % R1]);
-      const float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-      const float2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
-      Xt[0 * SP + q0] = cadd(s02, s13);                                    // pp = 0
-      Xt[1 * SP + q0] = make_float2(d02.x + d13.y, d02.y - d13.x);         // pp = 1: d02 - i d13
-      Xt[2 * SP + q0] = csub(s02, s13);                                    // pp = 2
-      Xt[3 * SP + q0] = make_float2(d02.x - d13.y, d02.y + d13.x);         // pp = 3: d02 + i d13
+  for (int e = 0; e < PER; ++e) {
+    const int u = tid + e * T;
+    if (u < nlanes) {
+      const int t = u & 3, task = u >> 2;
+      float2* Xt = X + task * 4 * SP;
+#pragma unroll
+      for (int m = 0; m < S / 4; ++m) {
+        const int q0 = t + 4 * m;
+        float2 v[4];
+        v[0] = Xt[q0];
+#pragma unroll
+        for (int tt = 1; tt < 4; ++tt) v[tt] = cmul(Xt[tt * SP + q0], WR[(q0 * tt) & (R1 - 1)]);
+        const float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+        const float2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+        Xt[0 * SP + q0] = cadd(s02, s13);                                    // pp = 0
+        Xt[1 * SP + q0] = make_float2(d02.x + d13.y, d02.y - d13.x);         // pp = 1: d02 - i d13
+        Xt[2 * SP + q0] = csub(s02, s13);                                    // pp = 2
+        Xt[3 * SP + q0] = make_float2(d02.x - d13.y, d02.y + d13.x);         // pp = 3: d02 + i d13
+      }
     }
   }
   __syncthreads();
-  // separate the two real columns and store A[b][q][k0 + 2p .. + 1], q <= H
-  const int nout = nint * (H + 1) * NP;
-  for (int it = tid; it < nout; it += T) {
-    const int p = it % NP, r = it / NP;
-    const int q = r % (H + 1), bl = r / (H + 1);
-    const float2* Xt = X + (bl + nint * p) * 4 * SP;
-    const int qm = (R1 - q) % R1;
-    const float2 a = Xt[(q / S) * SP + (q % S)];
-    const float2 c = Xt[(qm / S) * SP + (qm % S)];
-    const float2 y0 = make_float2(0.5f * (a.x + c.x), 0.5f * (a.y - c.y));     // (a + conj c) / 2
-    const float2 y1 = make_float2(0.5f * (a.y + c.y), -0.5f * (a.x - c.x));    // (a - conj c) / 2i
-    float4* dst = reinterpret_cast<float4*>(A + ((bi0 + bl) * (H + 1) + q) * C + k0) + p;
-    *dst = make_float4(y0.x, y0.y, y1.x, y1.y);
+  // Separate the two real columns k0 + 2p, k0 + 2p + 1 and store their bins
+  // in the StageA layout.  Thread: fixed p = tid % NP, slots r = bl H + q
+  // stepping by T / NP; slot 0 carries bins 0 and H (both real:
+  // Y0[0] = Z[0].x, Y1[0] = Z[0].y and likewise for Z[H]).
+  {
+    static_assert(H >= 1 && CW % 2 == 0, "fused stage 1 needs R1 >= 2 and column pairs");
+    constexpr int RS = T / NP;
+    constexpr int QA = RS % H, BA = RS / H;
+    const int p = tid % NP;
+    int r = tid / NP;
+    int q = r % H, bl = r / H;
+    const int nr = nint * H;
+    const float2* Xp = X + nint * p * 4 * SP;
+    for (; r < nr; r += RS) {
+      const float2* Xt = Xp + bl * 4 * SP;
+      float4 o;
+      if (q == 0) {
+        const float2 a = Xt[0];
+        const float2 c = Xt[(H / S) * SP + (H & (S - 1))];
+        o = make_float4(a.x, c.x, a.y, c.y);
+      } else {
+        const int qm = R1 - q;
+        const float2 a = Xt[(q / S) * SP + (q & (S - 1))];
+        const float2 c = Xt[(qm / S) * SP + (qm & (S - 1))];
+        o = make_float4(0.5f * (a.x + c.x), 0.5f * (a.y - c.y),      // (a + conj c) / 2
+                        0.5f * (a.y + c.y), -0.5f * (a.x - c.x));    // (a - conj c) / 2i
+      }
+      *reinterpret_cast<float4*>(A + StageA<R1>::at(bi0 + bl, q, k0 + 2 * p)) = o;
+      q += QA;
+      bl += BA;
+      if (q >= H) { q -= H; ++bl; }
+    }
   }
 }
 
@@ -558,7 +640,7 @@ __global__ __launch_bounds__(kTPB) void ipfb_cols_bluestein_kernel(const float* 
 
 // ------------------------------------------------------- re-PFB stage 1
 // For block b and column k (row width 4096): y[r] = sum_j ts[(b+j)R1 + r][k] w[(j R1 + r) 4096 + k],
-// then the R1-point real DFT down r -> A[b][q][k], q = 0..R1/2.
+// then the R1-point real DFT down r -> A[b][q][k], q = 0..R1/2 (StageA layout).
 template <int R1>
 __global__ __launch_bounds__(kTPB) void repfb_stage1_kernel(const float* __restrict__ ts, const float* __restrict__ win,
                                                             int ntap, int64_t nblock, int64_t blocks_per_wg,
@@ -589,14 +671,15 @@ __global__ __launch_bounds__(kTPB) void repfb_stage1_kernel(const float* __restr
 #pragma unroll
       for (int r = 0; r < R1; ++r) y[r] = fmaf(src[int64_t(r) * C], wj[r * KW], y[r]);
     }
+    float2* dst = A + StageA<R1>::at(b, 0, k);
     if constexpr (R1 == 1) {
-      A[(b * (H + 1)) * C + k] = make_float2(y[0], 0.f);
+      dst[0] = make_float2(y[0], 0.f);
     } else {
       float2 z[H];
 #pragma unroll
       for (int n = 0; n < H; ++n) z[n] = make_float2(y[2 * n], y[2 * n + 1]);
       DFT<H>::run(z);
-      float2* dst = A + (b * (H + 1)) * C + k;
+      float bin0 = 0.f;
       static_for<0, H + 1>([&](auto Q) {
         constexpr int q = decltype(Q)::value;
         const float2 a = z[q % H];
@@ -605,7 +688,10 @@ __global__ __launch_bounds__(kTPB) void repfb_stage1_kernel(const float* __restr
         const float2 d = make_float2(0.5f * (a.x - bb.x), 0.5f * (a.y + bb.y));
         const float2 o = make_float2(d.y, -d.x);
         const float2 wo = twc<R1, q>(o);
-        dst[int64_t(q) * C] = make_float2(e.x + wo.x, e.y + wo.y);
+        // bins 0 and H are real and share slot 0
+        if constexpr (q == 0) bin0 = e.x + wo.x;
+        else if constexpr (q == H) dst[0] = make_float2(bin0, e.x + wo.x);
+        else dst[q * 4] = make_float2(e.x + wo.x, e.y + wo.y);
       });
     }
   }
@@ -672,7 +758,8 @@ __global__ __launch_bounds__(kTPB) void repfb_stage1_split_kernel(const float* _
   }
   __syncthreads();
   // lane u combines q0 = u, u+4, ...: outputs q = q0 (p=0), q0+S (p=1), and q = 2S = H for q0 = 0
-  float2* dst = A + (b * (H + 1)) * C + k;
+  // (bins 0 and H are real and share slot 0 of the StageA layout)
+  float2* dst = A + StageA<R1>::at(b, 0, k);
   for (int q0 = t; q0 < S; q0 += 4) {
     float2 v[4];
 #pragma unroll
@@ -683,14 +770,14 @@ __global__ __launch_bounds__(kTPB) void repfb_stage1_split_kernel(const float* _
     // 4-point DFT over tt: p = 0, 1, 2
     const float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
     const float2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
-    dst[int64_t(q0) * C] = cadd(s02, s13);                                     // p = 0
-    dst[int64_t(q0 + S) * C] = make_float2(d02.x + d13.y, d02.y - d13.x);      // p = 1: d02 - i d13
-    if (q0 == 0) dst[int64_t(H) * C] = csub(s02, s13);                         // p = 2
+    if (q0 == 0) dst[0] = make_float2(s02.x + s13.x, s02.x - s13.x);        // p = 0 and p = 2 (bins 0, H)
+    else dst[q0 * 4] = cadd(s02, s13);                                        // p = 0
+    dst[(q0 + S) * 4] = make_float2(d02.x + d13.y, d02.y - d13.x);            // p = 1: d02 - i d13
   }
 }
 
 // ------------------------------------------------------- re-PFB stage 2
-// Workgroup (q1 <= R1/2, b): u[k] = A[b][q1][k] W_L^(q1 k), U = FFT_4096(u).
+// Workgroup (q1 <= R1/2, b): u[k] = A[b][q1][k] (StageA layout) W_L^(q1 k), U = FFT_4096(u).
 // Fine channel q = q1 + R1 q2 is U[q2].  Because the block is real, channel
 // q' = (R1 - q1) + R1 q2' is conj(X[L - q']) = conj(U[4095 - q2']), so one
 // transform serves both q1 and its mirror R1 - q1 and A is read once.
@@ -725,7 +812,9 @@ __global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __rest
   if (item >= total) return;   // the whole workgroup: padding of the 8-way split
   const int q1 = int(item % (H + 1));
   const int64_t b = item / (H + 1);
-  const float2* src = A + (b * (H + 1) + q1) * N;
+  // column k = tid + 256 r of bin q1 (StageA layout; bins 0 and H from slot 0)
+  const int slot = (q1 == 0 || q1 == H) ? 0 : q1;
+  const float2* src = A + StageA<R1>::at(b, slot, tid);
   // W_L^(q1 k) = W_L^(q1 c) W_4096^(q1 a) for k = R1 a + c; c = tid mod R1 is
   // the same for all of this thread's k (kTPB is a multiple of R1)
   static_assert(kTPB % R1 == 0, "thread stride must keep k mod R1 fixed");
@@ -743,7 +832,14 @@ __global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __rest
   {
     float2 v[16];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) v[r] = src[tid + 256 * r];
+    for (int r = 0; r < 16; ++r) v[r] = src[r * (64 * StageA<R1>::QP * 4)];
+    if (q1 == 0) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) v[r].y = 0.f;
+    } else if (q1 == H) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) v[r] = make_float2(v[r].y, 0.f);
+    }
     if (q1 != 0) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
@@ -1315,7 +1411,8 @@ int alb_ipfb_columns_repfb(const float* y, int64_t ldy, int64_t tile_stride, int
 }
 
 int64_t alb_repfb_workspace_bytes(int64_t nblock, int64_t osamp) {
-  return nblock * (osamp / 2 + 1) * 4096 * int64_t(sizeof(float2));
+  // StageA layout: max(osamp / 2, 1) bin slots per column
+  return nblock * (osamp / 2 > 0 ? osamp / 2 : 1) * 4096 * int64_t(sizeof(float2));
 }
 
 int alb_fft_rows(const void* in, int64_t ldi, void* out, int64_t ldo, int64_t nrows, int64_t n, int direction,

@@ -147,7 +147,7 @@ def rechannelizer(args, torch):
     lib = _lib.load()
     sweep = {}
     widths = [args.cols_width] if args.cols_width else [8, 4, 2]
-    groups = [args.cols_group] if args.cols_group else [32, 16, 8]
+    groups = [args.cols_group] if args.cols_group else [64, 32, 16, 8, 4, 2]
     rgroups = [args.rows_group] if args.rows_group else [4, 2]
     for w in widths:
         for gsz in groups:
@@ -472,8 +472,8 @@ def main():
     ap.add_argument("--onebit-T", type=int, default=2 ** 20)
     ap.add_argument("--cols-width", type=int, default=0, choices=[0, 2, 4, 8],
                     help="column-pass strip width (0: pick the fastest of 8, 4, 2 before timing)")
-    ap.add_argument("--cols-group", type=int, default=0, choices=[0, 4, 8, 16, 32, 64],
-                    help="column-pass strips per XCD group (0: pick the fastest of 32, 16, 8 before timing)")
+    ap.add_argument("--cols-group", type=int, default=0, choices=[0, 2, 4, 8, 16, 32, 64],
+                    help="column-pass strips per XCD group (0: pick the fastest of 64, 32, 16, 8, 4, 2 before timing)")
     ap.add_argument("--rows-group", type=int, default=0, choices=[0, 2, 4],
                     help="spectra per workgroup of the FFT row pass (0: pick the faster of 4, 2 before timing)")
     ap.add_argument("--row-pass", default="auto", choices=["auto", "fft", "mfma"],

@@ -47,7 +47,7 @@ def test_library_loads_and_reports_version():
         pytest.skip("library not built")
     lib = _lib.load()
     assert lib.alb_version() >= 100
-    assert lib.alb_repfb_workspace_bytes(2, 64) == 2 * 33 * 4096 * 8
+    assert lib.alb_repfb_workspace_bytes(2, 64) == 2 * 32 * 4096 * 8
 
 
 def test_invalid_arguments_fail_loudly_without_gpu_work():
