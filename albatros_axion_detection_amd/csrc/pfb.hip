@@ -18,6 +18,7 @@
 //                 DFT down the R1 rows of each block, per column.
 //  repfb_stage2   re-PFB second half: twiddle, 4096-point FFT along columns,
 //                 only the requested fine channels written.
+#include <atomic>
 #include "fft.hpp"
 #include "packed.hpp"
 
@@ -65,101 +66,33 @@ struct DenseRows {
 // Z[m] = (X[m] + conj X[M-m]) + i conj(W_N^m) (X[m] - conj X[M-m]) is formed
 // straight into LDS (stored conjugated, so a forward FFT gives the inverse),
 // then y[2n] + i y[2n+1] = IFFT_M(Z)[n] / N.
-template <int M, int F, int MODE, int TT = kTPB>
-__global__ __launch_bounds__(TT) void irfft_rows_kernel(DenseRows dense, Packed4Rows pk, int64_t nrows, int64_t nslice,
-                                                        float* __restrict__ out, int64_t ldo, Twiddle tw) {
+
+// y[2n] + i y[2n+1] = conj(Z[f][n]) / N for rows row0 + f < nrows: row-major
+// with row stride ldo, or (blocked) in the tile layout out[t][k / 8][i][k % 8]
+// (tile t = row / nslice, i = row % nslice).
+template <int M, int F, int TT>
+__device__ __forceinline__ void irfft_store(const float2* Z, int64_t row0, int64_t nrows, int64_t nslice,
+                                            float* __restrict__ out, int64_t ldo, bool blocked, int tid) {
   constexpr int N = 2 * M;
   constexpr int MP = Padded<M>::value;
-  extern __shared__ float2 smem[];
-  float2* Z = smem;                       // F x MP
-  float2* TA = Z + F * MP;
-  float2* TB = TA + tw.nA;
-  const int tid = threadIdx.x;
-  // Workgroup i runs on XCD i % 8: consecutive row groups go to one XCD back
-  // to back, so the blocked layout's 32-byte row pieces of neighbouring
-  // workgroups fill whole cache lines in that XCD's L2.
-  const int64_t ngroups = (nrows + F - 1) / F;
-  const int64_t per = (ngroups + 7) / 8;
-  const int64_t item = int64_t(blockIdx.x % 8) * per + blockIdx.x / 8;
-  if (item >= ngroups) return;   // whole workgroup: padding of the 8-way split
-  load_twiddle(tw, TA, TB, tid, TT);
-  const int64_t row0 = item * F;
-  if constexpr (MODE == 0) {
-    __syncthreads();
-    for (int idx = tid; idx < F * (M / 2 + 1); idx += TT) {
-      const int f = idx / (M / 2 + 1), m = idx - f * (M / 2 + 1);
-      const int64_t row = row0 + f;
-      float2* Zf = Z + f * MP;
-      float2 a = make_float2(0.f, 0.f), b = a;
-      if (row < nrows) {
-        const float2* X = dense.x + row * dense.ld;
-        a = X[m];
-        b = X[M - m];
-      }
-      if (m == 0) { a.y = 0.f; b.y = 0.f; }
-      const float2 w = tw_lookup(TA, TB, tw.lb, m);           // W_N^m
-      const float2 s = make_float2(a.x + b.x, a.y - b.y);     // a + conj b
-      const float2 d = make_float2(a.x - b.x, a.y + b.y);     // a - conj b
-      const float2 cd = cmulc(d, w);                          // d * conj(W^m)
-      Zf[pad16(m)] = make_float2(s.x - cd.y, -(s.y + cd.x));  // conj(s + i cd)
-      if (m != 0 && 2 * m != M) {
-        const float2 s2 = make_float2(b.x + a.x, b.y - a.y);  // b + conj a
-        const float2 d2 = make_float2(b.x - a.x, b.y + a.y);  // b - conj a
-        const float2 wd = cmul(d2, w);                        // W^m d2
-        Zf[pad16(M - m)] = make_float2(s2.x + wd.y, -(s2.y - wd.x));  // conj(s2 - i W^m d2)
-      }
-    }
-  } else {
-    for (int idx = tid; idx < F * MP; idx += TT) Z[idx] = make_float2(0.f, 0.f);
-    __syncthreads();
-    // channel ch contributes x (1 + i conj W^ch) to Z[ch] and conj(x) (1 + i W^ch)
-    // to Z[M-ch]; columns with a negative channel number are skipped
-    for (int64_t idx = tid; idx < F * pk.ncols; idx += TT) {
-      const int f = int(idx / pk.ncols);
-      const int64_t c = idx - int64_t(f) * pk.ncols;
-      const int64_t row = row0 + f;
-      const int ch = pk.chan[c];
-      if (row >= nrows || ch < 0) continue;
-      const int64_t p = packed_row(pk, row, nslice);
-      if (p < 0) continue;
-      const int byte = pk.bytes[int64_t(p) * pk.ncols + c];
-      float2 x = make_float2(float(nib4f(byte >> 4)), float(nib4f(byte & 15)));
-      if (ch == 0 || ch == M) x.y = 0.f;
-      const float2 w = tw_lookup(TA, TB, tw.lb, ch);
-      float* Zf = reinterpret_cast<float*>(Z + f * MP);
-      if (ch < M) {
-        const float2 g = make_float2(1.f + w.y, w.x);             // 1 + i conj(w)
-        const float2 v = cmul(x, g);
-        atomicAdd(&Zf[2 * pad16(ch)], v.x);
-        atomicAdd(&Zf[2 * pad16(ch) + 1], -v.y);
-      }
-      if (ch > 0) {
-        const float2 g = make_float2(1.f - w.y, w.x);             // 1 + i w
-        const float2 v = cmul(cconj(x), g);
-        atomicAdd(&Zf[2 * pad16(M - ch)], v.x);
-        atomicAdd(&Zf[2 * pad16(M - ch) + 1], -v.y);
-      }
-    }
-  }
-  __syncthreads();
-  fft_lds<M, F, TT>(Z, TA, TB, tw.lb, 1, tid);
-  // ---- y[2n] + i y[2n+1] = conj(FFT) / N
   const float sc = 1.0f / float(N);
-  if (ldo == 8) {
-    // blocked tile layout: element (row i of tile t, column k) at
-    // out[t][k / 8][i][k % 8] (rows of a tile nslice apart per block)
-    const int64_t t0 = row0 / nslice, i0 = row0 - t0 * nslice;
-    for (int idx = tid; idx < F * M; idx += TT) {
-      const int f = idx / M, n = idx - f * M;
-      const int64_t row = row0 + f;
-      if (row >= nrows) continue;
-      int64_t t = t0, i = i0 + f;
-      while (i >= nslice) { i -= nslice; ++t; }
-      const float2 v = Z[f * MP + pad16(n)];
-      reinterpret_cast<float2*>(out + t * int64_t(N) * nslice + int64_t(n / 4) * 8 * nslice + i * 8)[n % 4] =
-          make_float2(v.x * sc, -v.y * sc);
+  if constexpr (M % TT == 0) {
+    if (blocked) {
+      // thread tid stores n = tid + TT j: block n / 4 = tid / 4 + (TT / 4) j, slot n % 4 = tid % 4
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        const int64_t row = row0 + f;
+        if (row >= nrows) break;
+        const int64_t t = row / nslice, i = row - t * nslice;
+        float* base = out + t * int64_t(N) * nslice + i * 8 + int64_t(tid >> 2) * 8 * nslice + (tid & 3) * 2;
+#pragma unroll
+        for (int j = 0; j < M / TT; ++j) {
+          const float2 v = Z[f * MP + pad16(tid + TT * j)];
+          *reinterpret_cast<float2*>(base + int64_t(j) * (TT / 4) * 8 * nslice) = make_float2(v.x * sc, -v.y * sc);
+        }
+      }
+      return;
     }
-    return;
   }
   for (int idx = tid; idx < F * M; idx += TT) {
     const int f = idx / M, n = idx - f * M;
@@ -167,6 +100,219 @@ __global__ __launch_bounds__(TT) void irfft_rows_kernel(DenseRows dense, Packed4
     if (row >= nrows) continue;
     const float2 v = Z[f * MP + pad16(n)];
     reinterpret_cast<float2*>(out + row * ldo)[n] = make_float2(v.x * sc, -v.y * sc);
+  }
+}
+
+template <int M, int F, int TT = kTPB>
+__global__ __launch_bounds__(TT) void irfft_rows_kernel(DenseRows dense, int64_t nrows, float* __restrict__ out,
+                                                        int64_t ldo, Twiddle tw) {
+  constexpr int MP = Padded<M>::value;
+  extern __shared__ float2 smem[];
+  float2* Z = smem;                       // F x MP
+  float2* TA = Z + F * MP;
+  float2* TB = TA + tw.nA;
+  const int tid = threadIdx.x;
+  const int64_t row0 = int64_t(blockIdx.x) * F;
+  load_twiddle(tw, TA, TB, tid, TT);
+  __syncthreads();
+  for (int idx = tid; idx < F * (M / 2 + 1); idx += TT) {
+    const int f = idx / (M / 2 + 1), m = idx - f * (M / 2 + 1);
+    const int64_t row = row0 + f;
+    float2* Zf = Z + f * MP;
+    float2 a = make_float2(0.f, 0.f), b = a;
+    if (row < nrows) {
+      const float2* X = dense.x + row * dense.ld;
+      a = X[m];
+      b = X[M - m];
+    }
+    if (m == 0) { a.y = 0.f; b.y = 0.f; }
+    const float2 w = tw_lookup(TA, TB, tw.lb, m);           // W_N^m
+    const float2 s = make_float2(a.x + b.x, a.y - b.y);     // a + conj b
+    const float2 d = make_float2(a.x - b.x, a.y + b.y);     // a - conj b
+    const float2 cd = cmulc(d, w);                          // d * conj(W^m)
+    Zf[pad16(m)] = make_float2(s.x - cd.y, -(s.y + cd.x));  // conj(s + i cd)
+    if (m != 0 && 2 * m != M) {
+      const float2 s2 = make_float2(b.x + a.x, b.y - a.y);  // b + conj a
+      const float2 d2 = make_float2(b.x - a.x, b.y + a.y);  // b - conj a
+      const float2 wd = cmul(d2, w);                        // W^m d2
+      Zf[pad16(M - m)] = make_float2(s2.x + wd.y, -(s2.y - wd.x));  // conj(s2 - i W^m d2)
+    }
+  }
+  __syncthreads();
+  fft_lds<M, F, TT>(Z, TA, TB, tw.lb, 1, tid);
+  irfft_store<M, F, TT>(Z, row0, nrows, 1, out, ldo, false, tid);
+}
+
+// Packed 4-bit rows (2049-bin spectra, N = 4096): channel ch of a row
+// contributes x (1 + i conj W^ch) to Z[ch] and conj(x) (1 + i W^ch) to
+// Z[M-ch]; columns with a negative channel number are skipped.
+//
+// Persistent form: workgroup w runs on XCD w % 8 and walks the row groups
+// x per + it S + s of its XCD's range (S workgroups per XCD, slot s), so the
+// workgroups of one XCD transform neighbouring rows together and the blocked
+// layout's 32-byte row pieces meet in that XCD's L2.  Each thread's first
+// kRowItems (row, column) items keep their channel weights in registers; the
+// packed bytes of the next group and the row-map entries of the group after
+// it are fetched while the current group is transformed.
+constexpr int kRowItems = 2;
+
+template <int F, int TT>
+__global__ __launch_bounds__(TT, 1024 / TT) void irfft_packed4_kernel(Packed4Rows pk, int64_t nrows, int64_t nslice,
+                                                           float* __restrict__ out, int64_t ldo, Twiddle tw) {
+  constexpr int M = 2048;
+  constexpr int MP = Padded<M>::value;
+  extern __shared__ float2 smem[];
+  float2* Z = smem;                       // F x MP
+  float2* TA = Z + F * MP;
+  float2* TB = TA + tw.nA;
+  float* Zr = reinterpret_cast<float*>(Z);
+  const int tid = threadIdx.x;
+  load_twiddle(tw, TA, TB, tid, TT);
+  const int64_t ngroups = (nrows + F - 1) / F;
+  const int64_t per = (ngroups + 7) / 8;
+  const int64_t S = gridDim.x / 8;
+  const int64_t xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+  auto group = [&](int64_t it) -> int64_t {   // row group of iteration it, -1 past the end
+    const int64_t j = it * S + slot;
+    const int64_t g = xcd * per + j;
+    return (j < per && g < ngroups) ? g : -1;
+  };
+  const int ncols = int(pk.ncols), nit = F * ncols;
+  int fi[kRowItems], ci[kRowItems], ch[kRowItems];
+#pragma unroll
+  for (int k = 0; k < kRowItems; ++k) {
+    const int idx = tid + k * TT;
+    fi[k] = 0;
+    ci[k] = 0;
+    ch[k] = -1;
+    if (idx < nit) {
+      fi[k] = idx / ncols;
+      ci[k] = idx - fi[k] * ncols;
+      ch[k] = pk.chan[ci[k]];
+    }
+  }
+  // Row-map entry of item k in group g and the packed byte at packed row p.
+  // Both loads are issued unconditionally (from a readable dummy address when
+  // the item has no data) with the validity kept apart, so nothing waits for
+  // them before their use one group later.  Rows < 2^31 (checked by the host).
+  const int ns = int(nslice), nr = int(nrows), cut2 = int(pk.cut2), acc = int(pk.acclen), tile0 = int(pk.tile0);
+  auto prow = [&](int k, int64_t g, bool* ok) -> int {
+    const int row = int(g) * F + fi[k];
+    bool valid = g >= 0 && ch[k] >= 0 && row < nr;
+    const int rr = valid ? row : 0;
+    const int tile = rr / ns, r = rr - tile * ns;
+    const int chunk = tile0 + tile - (r < cut2 ? 1 : 0);
+    const int crow = r < cut2 ? acc - cut2 + r : r - cut2;
+    valid = valid && chunk >= 0;
+    *ok = valid;
+    return pk.rowmap[valid ? int64_t(chunk) * acc + crow : 0];
+  };
+  auto pbyte = [&](int k, int p, bool ok, bool* okb) -> int {
+    const bool valid = ok && p >= 0;
+    *okb = valid;
+    const uint8_t* a = valid ? pk.bytes + int64_t(p) * ncols + ci[k] : reinterpret_cast<const uint8_t*>(pk.chan);
+    return int(*a);
+  };
+  __syncthreads();  // twiddles
+  // item constants: weights 1 + i conj(w), 1 + i w and the float offsets of Z[ch], Z[M-ch]
+  float2 ga[kRowItems], gb[kRowItems];
+  int oa[kRowItems], ob[kRowItems];
+#pragma unroll
+  for (int k = 0; k < kRowItems; ++k) {
+    const int c = ch[k] < 0 ? 0 : ch[k];
+    const float2 w = tw_lookup(TA, TB, tw.lb, c);
+    ga[k] = make_float2(1.f + w.y, w.x);
+    gb[k] = make_float2(1.f - w.y, w.x);
+    oa[k] = fi[k] * 2 * MP + 2 * pad16(c);
+    ob[k] = fi[k] * 2 * MP + 2 * pad16(M - c);
+  }
+  // one row group: zero fill, channel scatter, inverse transform, store
+  auto transform = [&](int64_t g, const int* bc, const bool* vc) {
+    // an opaque copy of the thread index: the transform's LDS addresses are
+    // recomputed per group instead of being held in registers across the loop
+    int tl = tid;
+    asm volatile("" : "+v"(tl));
+    __syncthreads();  // the previous group is read out of Z
+    for (int idx = tl; idx < F * MP; idx += TT) Z[idx] = make_float2(0.f, 0.f);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kRowItems; ++k) {
+      if (vc[k]) {
+        const int c = ch[k];
+        float2 x = make_float2(float(nib4f(bc[k] >> 4)), float(nib4f(bc[k] & 15)));
+        if (c == 0 || c == M) x.y = 0.f;
+        if (c < M) {
+          const float2 v = cmul(x, ga[k]);
+          atomicAdd(&Zr[oa[k]], v.x);
+          atomicAdd(&Zr[oa[k] + 1], -v.y);
+        }
+        if (c > 0) {
+          const float2 v = cmul(cconj(x), gb[k]);
+          atomicAdd(&Zr[ob[k]], v.x);
+          atomicAdd(&Zr[ob[k] + 1], -v.y);
+        }
+      }
+    }
+    // items beyond the prefetched ones (more than kRowItems * TT / F channels)
+    for (int idx = tl + kRowItems * TT; idx < nit; idx += TT) {
+      const int f = idx / ncols;
+      const int c = idx - f * ncols;
+      const int64_t row = g * F + f;
+      const int cc = pk.chan[c];
+      if (row >= nrows || cc < 0) continue;
+      const int64_t p = packed_row(pk, row, nslice);
+      if (p < 0) continue;
+      const int byte = pk.bytes[p * ncols + c];
+      float2 x = make_float2(float(nib4f(byte >> 4)), float(nib4f(byte & 15)));
+      if (cc == 0 || cc == M) x.y = 0.f;
+      const float2 w = tw_lookup(TA, TB, tw.lb, cc);
+      float* Zf = Zr + f * 2 * MP;
+      if (cc < M) {
+        const float2 v = cmul(x, make_float2(1.f + w.y, w.x));
+        atomicAdd(&Zf[2 * pad16(cc)], v.x);
+        atomicAdd(&Zf[2 * pad16(cc) + 1], -v.y);
+      }
+      if (cc > 0) {
+        const float2 v = cmul(cconj(x), make_float2(1.f - w.y, w.x));
+        atomicAdd(&Zf[2 * pad16(M - cc)], v.x);
+        atomicAdd(&Zf[2 * pad16(M - cc) + 1], -v.y);
+      }
+    }
+    __syncthreads();
+    fft_lds<M, F, TT>(Z, TA, TB, tw.lb, 1, tl);
+    irfft_store<M, F, TT>(Z, g * F, nrows, nslice, out, ldo, ldo == 8, tl);
+  };
+  // Two sets of prefetch registers used in turn (no copies of values still in
+  // flight): while group it is transformed, the bytes of group it + 1 and the
+  // row map of group it + 2 arrive.
+  int pA[kRowItems], bA[kRowItems], pB[kRowItems], bB[kRowItems];
+  bool qA[kRowItems], vA[kRowItems], qB[kRowItems], vB[kRowItems];
+#pragma unroll
+  for (int k = 0; k < kRowItems; ++k) {
+    bool ok;
+    const int p = prow(k, group(0), &ok);
+    bA[k] = pbyte(k, p, ok, &vA[k]);
+    pA[k] = prow(k, group(1), &qA[k]);
+  }
+  for (int64_t it = 0;; it += 2) {
+    int64_t g = group(it);
+    if (g < 0) break;
+    const int64_t g2 = group(it + 2);
+#pragma unroll
+    for (int k = 0; k < kRowItems; ++k) {
+      bB[k] = pbyte(k, pA[k], qA[k], &vB[k]);
+      pB[k] = prow(k, g2, &qB[k]);
+    }
+    transform(g, bA, vA);
+    g = group(it + 1);
+    if (g < 0) break;
+    const int64_t g3 = group(it + 3);
+#pragma unroll
+    for (int k = 0; k < kRowItems; ++k) {
+      bA[k] = pbyte(k, pB[k], qB[k], &vA[k]);
+      pA[k] = prow(k, g3, &qA[k]);
+    }
+    transform(g, bB, vB);
   }
 }
 
@@ -379,6 +525,46 @@ struct StageA {
   }
 };
 
+// Row walk of the fused polyphase: sums of rows xrow + R1 (bl + j), j < ntap
+// (taps j >= ntap have weight 0), for the nb <= kWalkSeg blocks bl of one
+// lane, with the window weights of the lane's row at wl[j R1 CW] (.x and .y
+// for the two columns of the pair; U holds conj for the second).  Rows past
+// the lane's last block feed only block sums that are never stored, so every
+// read is unconditional (clamped into the sequence).  Ends with the block
+// sums written to U[pad16((bl0 + b) R1 + rb)] after a workgroup barrier.
+constexpr int kWalkSeg = 16;
+template <int NSL, int R1, int CW, int NT>
+__device__ __forceinline__ void polyphase_walk(float2* Up, int xrow, const float* wl, int ntap, int nb, int bl0,
+                                               int rb) {
+  float w0[NT], w1[NT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const float2 w = *reinterpret_cast<const float2*>(wl + j * R1 * CW);
+    w0[j] = j < ntap ? w.x : 0.f;
+    w1[j] = j < ntap ? w.y : 0.f;
+  }
+  float2 acc[kWalkSeg];
+#pragma unroll
+  for (int b = 0; b < kWalkSeg; ++b) acc[b] = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int m = 0; m < kWalkSeg + NT - 1; ++m) {
+    int x = xrow + R1 * m;
+    x = x < NSL - 1 ? x : NSL - 1;
+    const float2 v = Up[pad16(x)];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      if (m - j >= 0 && m - j < kWalkSeg) {
+        acc[m - j].x = fmaf(v.x, w0[j], acc[m - j].x);
+        acc[m - j].y = fmaf(-v.y, w1[j], acc[m - j].y);
+      }
+    }
+  }
+  __syncthreads();  // every lane has read its rows: U becomes Y
+#pragma unroll
+  for (int b = 0; b < kWalkSeg; ++b)
+    if (b < nb) Up[pad16((bl0 + b) * R1 + rb)] = acc[b];
+}
+
 // ------------------------------------ iPFB column pass fused with re-PFB stage 1
 // Stream row g = row0 + tile * rows_per_tile + (i - cut) holds kept row i of a
 // tile.  Re-PFB block b (R1 rows per block row, ntap taps) reads stream rows
@@ -460,14 +646,32 @@ __global__ __launch_bounds__(T) void ipfb_cols_repfb_kernel(const float* __restr
   }
   if (nint == 0) return;
 
-  // Polyphase sums and S-point DFTs: lane u = task * 4 + t, task = (b - bi0) + nint * p,
-  // sums rows r = 4 s + t (s < S) of block b over the taps.  The lane's rows
-  // R1 j + 4 (4 a + c) + t sit at padded offsets base_c + 17 a + (R1 + R1 / 16) j
-  // (R1 a multiple of 16), so one base per c serves every tap and every a.
+  // Polyphase sums, row-walk form: lane (rb, p, seg) owns row rb of the
+  // blocks bl in [seg SL, seg SL + SL) and walks the block rows m once
+  // (row rb + R1 m of the interior run is tap m - bl of block bl), so each
+  // stream row is read once and the lane's window weights stay in registers.
+  // Block sums are kept until every lane has read U, then written back as
+  // Y[p][bl R1 + rb] for the DFT step.  Taps are summed in order
+  // j = 0..ntap-1, as in the stage-1 kernels.
+  constexpr int NSEG = T / (R1 * NP);        // segments per (rb, p)
+  static_assert(NSEG * R1 * NP == T, "one walk lane per thread");
+  {
+    const int SL = (nint + NSEG - 1) / NSEG;  // <= kWalkSeg (checked by the host)
+    const int rb = tid % R1, pw = (tid / R1) % NP, seg = tid / (R1 * NP);
+    const int bl0 = seg * SL;
+    int nb = nint - bl0;
+    nb = nb < 0 ? 0 : (nb > SL ? SL : nb);
+    const int xrow = int(R1 * (bi0 + bl0) - g0) + cut + rb;   // local row of block row m = 0
+    const float* wl = WIN + rb * CW + 2 * pw;
+    if (ntap <= 4) polyphase_walk<NSL, R1, CW, 4>(U + pw * PADN, xrow, wl, ntap, nb, bl0, rb);
+    else polyphase_walk<NSL, R1, CW, 8>(U + pw * PADN, xrow, wl, ntap, nb, bl0, rb);
+  }
+  __syncthreads();
+
+  // S-point DFTs: lane u = task * 4 + t, task = bl + nint * p, takes the
+  // block sums of rows r = 4 s + t (s < S).
   const int nlanes = nint * NP * 4;
   constexpr int PER = 2;                     // lane slots per thread (nint * NP * 4 <= PER * T is checked by the host)
-  constexpr int JMAX = 512 / R1;             // ntap * R1 <= 512 is checked by the host
-  constexpr int RSTEP = R1 + R1 / 16;        // padded rows per tap
   float2 z[PER][S];
 #pragma unroll
   for (int e = 0; e < PER; ++e) {
@@ -475,27 +679,9 @@ __global__ __launch_bounds__(T) void ipfb_cols_repfb_kernel(const float* __restr
     if (u < nlanes) {
       const int t = u & 3, task = u >> 2;
       const int p = task / nint, bl = task - p * nint;
-      const float2* Up = U + p * PADN;
-      const int x0 = int(R1 * (bi0 + bl) - g0) + cut + t;   // local row of s = 0, tap 0
-      const int c4 = (x0 & 15) >> 2;
-      const float2* vb[4];
+      const float2* Yp = U + p * PADN + bl * (R1 + R1 / 16) + t;   // pad16(bl R1 + 4 s + t), R1 a multiple of 16
 #pragma unroll
-      for (int c = 0; c < 4; ++c) vb[c] = Up + pad16(x0) + 4 * c + (c + c4 >= 4 ? 1 : 0);
-      const float2* wb = reinterpret_cast<const float2*>(WIN + t * CW) + p;
-#pragma unroll
-      for (int sI = 0; sI < S; ++sI) z[e][sI] = make_float2(0.f, 0.f);
-#pragma unroll
-      for (int j = 0; j < JMAX; ++j) {
-        if (j < ntap) {
-#pragma unroll
-          for (int sI = 0; sI < S; ++sI) {
-            const float2 v = vb[sI & 3][17 * (sI >> 2) + RSTEP * j];
-            const float2 w = wb[(j * R1 + 4 * sI) * NP];
-            z[e][sI].x = fmaf(v.x, w.x, z[e][sI].x);
-            z[e][sI].y = fmaf(-v.y, w.y, z[e][sI].y);
-          }
-        }
-      }
+      for (int sI = 0; sI < S; ++sI) z[e][sI] = Yp[4 * sI + sI / 4];
       DFT<S>::run(z[e]);
     }
   }
@@ -959,49 +1145,75 @@ int cols_group() {
   return g > 0 ? int(g) : 32;
 }
 
-template <int M, int F, int MODE, int TT = kTPB>
-int launch_irfft(DenseRows d, Packed4Rows pk, int64_t nrows, int64_t nslice, float* out, int64_t ldo, hipStream_t s) {
+template <int M, int F, int TT = kTPB>
+int launch_irfft(DenseRows d, int64_t nrows, float* out, int64_t ldo, hipStream_t s) {
   Twiddle tw;
   int rc = get_twiddle(2 * M, &tw);
   if (rc) return rc;
   const size_t lds = sizeof(float2) * (size_t(F) * Padded<M>::value + tw.nA + tw.nB);
   const int64_t ngroups = (nrows + F - 1) / F;
-  ALB_REQUIRE(ngroups < (int64_t(1) << 31) - 8, "irfft: too many rows");
-  const unsigned grid = unsigned((ngroups + 7) / 8 * 8);
-  hipLaunchKernelGGL((irfft_rows_kernel<M, F, MODE, TT>), dim3(grid), dim3(TT), lds, s, d, pk, nrows, nslice, out, ldo,
-                     tw);
+  ALB_REQUIRE(ngroups < (int64_t(1) << 31), "irfft: too many rows");
+  hipLaunchKernelGGL((irfft_rows_kernel<M, F, TT>), dim3(unsigned(ngroups)), dim3(TT), lds, s, d, nrows, out, ldo, tw);
   ALB_LAUNCH_CHECK("irfft_rows_kernel");
   return ALB_OK;
 }
 
-template <int MODE>
-int irfft_dispatch(int64_t n, DenseRows d, Packed4Rows pk, int64_t nrows, int64_t nslice, float* out, int64_t ldo,
-                   hipStream_t s) {
-  if constexpr (MODE == 1) {
-    // packed 4-bit rows: 2049-channel spectra only
-    if (n == 4096) {
-      // two rows per 256-thread workgroup (measured faster), or four per 512
-      // threads (each workgroup writing whole 128-byte lines of the blocked layout)
-      if (tune_get("ipfb_rows_group") == 4) return launch_irfft<2048, 4, 1, 512>(d, pk, nrows, nslice, out, ldo, s);
-      return launch_irfft<2048, 2, 1>(d, pk, nrows, nslice, out, ldo, s);
-    }
-    set_error("irfft: packed rows need n = 4096 (got %lld)", (long long)n);
-    return ALB_ENOTSUP;
-  } else {
-    switch (n) {
-      case 64: return launch_irfft<32, 64, MODE>(d, pk, nrows, nslice, out, ldo, s);
-      case 128: return launch_irfft<64, 32, MODE>(d, pk, nrows, nslice, out, ldo, s);
-      case 256: return launch_irfft<128, 16, MODE>(d, pk, nrows, nslice, out, ldo, s);
-      case 512: return launch_irfft<256, 8, MODE>(d, pk, nrows, nslice, out, ldo, s);
-      case 1024: return launch_irfft<512, 4, MODE>(d, pk, nrows, nslice, out, ldo, s);
-      case 2048: return launch_irfft<1024, 2, MODE>(d, pk, nrows, nslice, out, ldo, s);
-      case 4096: return launch_irfft<2048, 2, MODE>(d, pk, nrows, nslice, out, ldo, s);
-      case 8192: return launch_irfft<4096, 1, MODE>(d, pk, nrows, nslice, out, ldo, s);
-      case 16384: return launch_irfft<8192, 1, MODE>(d, pk, nrows, nslice, out, ldo, s);
-      default:
-        set_error("irfft: length %lld not supported (power of two 64..16384)", (long long)n);
-        return ALB_ENOTSUP;
-    }
+// Persistent packed-row launch: as many workgroups as fit on the device at
+// once (a multiple of 8, so every XCD gets the same count), at most one per
+// row group.
+template <int F, int TT>
+int launch_irfft_packed4(const Packed4Rows& pk, int64_t nrows, int64_t nslice, float* out, int64_t ldo,
+                         hipStream_t s) {
+  Twiddle tw;
+  int rc = get_twiddle(4096, &tw);
+  if (rc) return rc;
+  const size_t lds = sizeof(float2) * (size_t(F) * Padded<2048>::value + tw.nA + tw.nB);
+  const int64_t ngroups = (nrows + F - 1) / F;
+  ALB_REQUIRE(nrows < (int64_t(1) << 31) - F && pk.tile0 >= 0 && pk.tile0 < (int64_t(1) << 30),
+              "irfft: too many rows");
+  static std::atomic<int> resident[64];
+  int dev = 0;
+  ALB_HIP(hipGetDevice(&dev));
+  int nres = dev < 64 ? resident[dev].load() : 0;
+  if (nres == 0) {
+    int ncu = 0, occ = 0;
+    ALB_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    ALB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, irfft_packed4_kernel<F, TT>, TT, lds));
+    nres = ncu * (occ > 0 ? occ : 1);
+    if (dev < 64) resident[dev].store(nres);
+  }
+  const int64_t per = (ngroups + 7) / 8;
+  int64_t S = nres / 8 > 0 ? nres / 8 : 1;
+  if (S > per) S = per;
+  hipLaunchKernelGGL((irfft_packed4_kernel<F, TT>), dim3(unsigned(8 * S)), dim3(TT), lds, s, pk, nrows, nslice, out,
+                     ldo, tw);
+  ALB_LAUNCH_CHECK("irfft_packed4_kernel");
+  return ALB_OK;
+}
+
+// packed 4-bit rows (2049-channel spectra, n = 4096): two rows per 256-thread
+// workgroup (measured faster), or four per 512 threads (each workgroup writing
+// whole 128-byte lines of the blocked layout)
+int irfft_packed4_dispatch(const Packed4Rows& pk, int64_t nrows, int64_t nslice, float* out, int64_t ldo,
+                           hipStream_t s) {
+  if (tune_get("ipfb_rows_group") == 4) return launch_irfft_packed4<4, 512>(pk, nrows, nslice, out, ldo, s);
+  return launch_irfft_packed4<2, kTPB>(pk, nrows, nslice, out, ldo, s);
+}
+
+int irfft_dispatch(int64_t n, DenseRows d, int64_t nrows, float* out, int64_t ldo, hipStream_t s) {
+  switch (n) {
+    case 64: return launch_irfft<32, 64>(d, nrows, out, ldo, s);
+    case 128: return launch_irfft<64, 32>(d, nrows, out, ldo, s);
+    case 256: return launch_irfft<128, 16>(d, nrows, out, ldo, s);
+    case 512: return launch_irfft<256, 8>(d, nrows, out, ldo, s);
+    case 1024: return launch_irfft<512, 4>(d, nrows, out, ldo, s);
+    case 2048: return launch_irfft<1024, 2>(d, nrows, out, ldo, s);
+    case 4096: return launch_irfft<2048, 2>(d, nrows, out, ldo, s);
+    case 8192: return launch_irfft<4096, 1>(d, nrows, out, ldo, s);
+    case 16384: return launch_irfft<8192, 1>(d, nrows, out, ldo, s);
+    default:
+      set_error("irfft: length %lld not supported (power of two 64..16384)", (long long)n);
+      return ALB_ENOTSUP;
   }
 }
 
@@ -1121,11 +1333,13 @@ int check_cols_repfb(int cut, int ntap) {
   constexpr int SP = S >= 16 ? S + 1 : S;
   const int64_t nkeep = NSL - 2 * int64_t(cut);
   const int64_t nint_max = nkeep / R1 - ntap + 1;
-  if (ntap < 1 || ntap * R1 > 512) {
-    set_error("fused iPFB/re-PFB: ntap * osamp must be <= 512");
+  if (ntap < 1 || ntap > 8) {
+    set_error("fused iPFB/re-PFB: ntap must be 1..8");
     return ALB_ENOTSUP;
   }
-  if (nint_max * NP * 4 > 2 * T || nint_max * NP * 4 * SP > NP * Padded<NSL>::value) {
+  // walk segments (T / (R1 NP) per row, at most 16 blocks each) and DFT lanes
+  if (nint_max > int64_t(T / (R1 * NP)) * kWalkSeg || nint_max * NP * 4 > 2 * T ||
+      nint_max * NP * 4 * SP > NP * Padded<NSL>::value) {
     set_error("fused iPFB/re-PFB: tile too long for osamp %d", R1);
     return ALB_ENOTSUP;
   }
@@ -1240,8 +1454,7 @@ int alb_irfft_rows(const void* in, int64_t ldi, float* out, int64_t ldo, int64_t
   ALB_REQUIRE(ldi >= n / 2 + 1 && ldo >= n && nrows >= 0, "alb_irfft_rows: bad strides");
   if (nrows == 0) return ALB_OK;
   DenseRows d{(const float2*)in, ldi};
-  Packed4Rows pk{};
-  return irfft_dispatch<0>(n, d, pk, nrows, 1, out, ldo, as_stream(stream));
+  return irfft_dispatch(n, d, nrows, out, ldo, as_stream(stream));
 }
 
 // pycufft.py:120-137 rfft along rows (unnormalised), bins [c0, c1) written.
@@ -1288,8 +1501,8 @@ int alb_ipfb_rows_packed4(const uint8_t* bytes, const int* rowmap, const int* ch
   const int64_t nslice = acclen + 2 * cut;
   ALB_REQUIRE(acclen >= 2 * cut && ncols > 0 && ncols <= 2049, "alb_ipfb_rows_packed4: bad sizes");
   Packed4Rows pk{bytes, rowmap, chan, ncols, acclen, 2 * int64_t(cut), tile0};
-  DenseRows d{};
-  return irfft_dispatch<1>(4096, d, pk, nslice * ntiles, nslice, out, 4096, as_stream(stream));
+  if (ntiles <= 0) return ALB_OK;
+  return irfft_packed4_dispatch(pk, nslice * ntiles, nslice, out, 4096, as_stream(stream));
 }
 
 // The same row pass written in the blocked tile layout out[ntiles][4096/8][nslice][8]
@@ -1300,8 +1513,8 @@ int alb_ipfb_rows_packed4_blocked(const uint8_t* bytes, const int* rowmap, const
   const int64_t nslice = acclen + 2 * cut;
   ALB_REQUIRE(acclen >= 2 * cut && ncols > 0 && ncols <= 2049, "alb_ipfb_rows_packed4_blocked: bad sizes");
   Packed4Rows pk{bytes, rowmap, chan, ncols, acclen, 2 * int64_t(cut), tile0};
-  DenseRows d{};
-  return irfft_dispatch<1>(4096, d, pk, nslice * ntiles, nslice, out, 8, as_stream(stream));
+  if (ntiles <= 0) return ALB_OK;
+  return irfft_packed4_dispatch(pk, nslice * ntiles, nslice, out, 8, as_stream(stream));
 }
 
 // Re-PFB (pfb_gpu_utils.py:105-121 with lblock = 4096*osamp) of nblock output
