@@ -109,6 +109,9 @@ struct DFT<2> {
 __device__ __forceinline__ int pad16(int i) { return i + (i >> 4); }
 template <int N>
 struct Padded { static constexpr int value = N + N / 16; };
+// W_256^(m r) table of the radix-16 pass at sub-transform size 16 (fill_w16)
+constexpr int kW16Row = 17;
+constexpr int kW16Size = 16 * kW16Row;
 
 // Twiddle W_N^m from the two-level LDS tables.
 __device__ __forceinline__ float2 tw_lookup(const float2* A, const float2* B, int lb, int m) {
@@ -121,35 +124,59 @@ __device__ __forceinline__ void load_twiddle(const Twiddle& tw, float2* A, float
   for (int i = tid; i < tw.nB; i += nthreads) B[i] = tw.B[i];
 }
 
+// Padded offset of element i + k (k a compile-time step) from the padded
+// offset of i, when the pad cannot change between them: K a multiple of 16
+// (pad16(i + K) = pad16(i) + 17 K / 16), or i a multiple of 16 and K < 16.
+template <int K>
+__device__ __forceinline__ constexpr int pad_step() { return K % 16 == 0 ? K + K / 16 : K; }
+
 // One radix-R Stockham pass of F FFTs of length N (sub-transform size NS).
-template <int N, int R, int NS, int F, int T>
-__device__ __forceinline__ void fft_pass(float2* lds, const float2* A, const float2* B, int lb, int tsh, int tid) {
+// Butterfly (f, j) reads j + r NB and writes d + r NS, d = (j / NS) NS R + j % NS.
+// With NB (and NS) multiples of 16, or NS = 1 (d a multiple of R, R <= 16),
+// every padded address is one base plus compile-time offsets.
+template <int N, int R, int NS, int F, int T, bool TAB = false>
+__device__ __forceinline__ void fft_pass(float2* lds, const float2* A, const float2* B, int lb, int tsh, int tid,
+                                         const float2* W16 = nullptr) {
   constexpr int NB = N / R;
   constexpr int TOT = NB * F;
   constexpr int PER = (TOT + T - 1) / T;
   constexpr int NP = Padded<N>::value;
+  constexpr bool RD_FAST = NB % 16 == 0;
+  constexpr bool WR_FAST = NS % 16 == 0 || (NS == 1 && 16 % R == 0);
   float2 v[PER][R];
 #pragma unroll
   for (int t = 0; t < PER; ++t) {
-    const int b = tid + t * T;
-    if (TOT % T == 0 || b < TOT) {
-      const int f = b / NB, j = b % NB;
+    const unsigned b = unsigned(tid) + unsigned(t * T);
+    if (TOT % T == 0 || b < unsigned(TOT)) {
+      const unsigned f = b / NB, j = b % NB;
       const float2* base = lds + f * NP;
+      if constexpr (RD_FAST) {
+        const float2* bj = base + pad16(int(j));
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[t][r] = base[pad16(j + r * NB)];
+        for (int r = 0; r < R; ++r) v[t][r] = bj[r * pad_step<NB>()];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[t][r] = base[pad16(int(j) + r * NB)];
+      }
     }
   }
   __syncthreads();
 #pragma unroll
   for (int t = 0; t < PER; ++t) {
-    const int b = tid + t * T;
-    if (TOT % T == 0 || b < TOT) {
-      const int f = b / NB, j = b % NB;
-      if constexpr (NS > 1) {
+    const unsigned b = unsigned(tid) + unsigned(t * T);
+    if (TOT % T == 0 || b < unsigned(TOT)) {
+      const unsigned f = b / NB, j = b % NB;
+      if constexpr (TAB && NS == 16 && R == 16) {
+        // sub-transform size 16: W_N^(r (j%16) N/256) = W_256^(r m), m = j % 16,
+        // from the table W16[17 m + r]
+        const float2* wt = W16 + (j % 16) * kW16Row;
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[t][r] = cmul(v[t][r], wt[r]);
+      } else if constexpr (NS > 1) {
         // W^(r*base) for r = 1..R-1 from the looked-up powers W^base, W^2base,
         // W^4base, W^8base (products of at most four factors)
         constexpr int STEP = N / (NS * R);
-        const int base = ((j % NS) * STEP) << tsh;
+        const int base = int(((j % NS) * STEP) << tsh);
         float2 wp[4];
 #pragma unroll
         for (int e = 0; (1 << e) < R; ++e) wp[e] = tw_lookup(A, B, lb, base << e);
@@ -164,10 +191,16 @@ __device__ __forceinline__ void fft_pass(float2* lds, const float2* A, const flo
         for (int r = 1; r < R; ++r) v[t][r] = cmul(v[t][r], w[r]);
       }
       DFT<R>::run(v[t]);
-      const int d = (j / NS) * NS * R + (j % NS);
+      const int d = int((j / NS) * NS * R + (j % NS));
       float2* base = lds + f * NP;
+      if constexpr (WR_FAST) {
+        float2* bd = base + pad16(d);
 #pragma unroll
-      for (int r = 0; r < R; ++r) base[pad16(d + r * NS)] = v[t][r];
+        for (int r = 0; r < R; ++r) bd[r * pad_step<NS>()] = v[t][r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < R; ++r) base[pad16(d + r * NS)] = v[t][r];
+      }
     }
   }
   __syncthreads();
@@ -183,12 +216,27 @@ struct PickRadix {
 // Passes run while the sub-transform size is below NSE (NSE < N stops early:
 // the caller finishes the last pass itself, e.g. for only some output bins).
 // The caller synchronises before the call; the call ends with a barrier.
-template <int N, int F, int T, int NS = 1, int NSE = N>
-__device__ __forceinline__ void fft_lds(float2* lds, const float2* A, const float2* B, int lb, int tsh, int tid) {
+// With TAB the radix-16 pass at sub-transform size 16 takes its twiddles from
+// W16 (fill_w16).
+template <int N, int F, int T, int NS = 1, int NSE = N, bool TAB = false>
+__device__ __forceinline__ void fft_lds(float2* lds, const float2* A, const float2* B, int lb, int tsh, int tid,
+                                        const float2* W16 = nullptr) {
   if constexpr (NS < NSE) {
     constexpr int R = PickRadix<N / NS>::value;
-    fft_pass<N, R, NS, F, T>(lds, A, B, lb, tsh, tid);
-    fft_lds<N, F, T, NS * R, NSE>(lds, A, B, lb, tsh, tid);
+    fft_pass<N, R, NS, F, T, TAB>(lds, A, B, lb, tsh, tid, W16);
+    fft_lds<N, F, T, NS * R, NSE, TAB>(lds, A, B, lb, tsh, tid, W16);
+  }
+}
+
+// W16[17 m + r] = W_256^(m r) (m, r < 16; rows of 17 so that the 16 rows
+// start in different LDS banks) from the two-level tables of a power-of-two
+// size NT >= 256 (W_256^e = W_NT^(e NT / 256)).  kW16Size entries; the caller
+// synchronises before the first use.
+__device__ __forceinline__ void fill_w16(float2* W16, const float2* A, const float2* B, int lb, int NT, int tid,
+                                         int nthreads) {
+  for (int i = tid; i < 256; i += nthreads) {
+    const int m = i >> 4, r = i & 15;
+    W16[m * kW16Row + r] = tw_lookup(A, B, lb, ((m * r) & 255) * (NT / 256));
   }
 }
 

@@ -165,9 +165,11 @@ __global__ __launch_bounds__(TT, 1024 / TT) void irfft_packed4_kernel(Packed4Row
   float2* Z = smem;                       // F x MP
   float2* TA = Z + F * MP;
   float2* TB = TA + tw.nA;
+  float2* W16 = TB + tw.nB;               // W_256^(m r), fill_w16
   float* Zr = reinterpret_cast<float*>(Z);
   const int tid = threadIdx.x;
   load_twiddle(tw, TA, TB, tid, TT);
+  fill_w16(W16, tw.A, tw.B, tw.lb, tw.nA * tw.nB, tid, TT);
   const int64_t ngroups = (nrows + F - 1) / F;
   const int64_t per = (ngroups + 7) / 8;
   const int64_t S = gridDim.x / 8;
@@ -279,7 +281,7 @@ __global__ __launch_bounds__(TT, 1024 / TT) void irfft_packed4_kernel(Packed4Row
       }
     }
     __syncthreads();
-    fft_lds<M, F, TT>(Z, TA, TB, tw.lb, 1, tl);
+    fft_lds<M, F, TT, 1, M, true>(Z, TA, TB, tw.lb, 1, tl, W16);
     irfft_store<M, F, TT>(Z, g * F, nrows, nslice, out, ldo, ldo == 8, tl);
   };
   // Two sets of prefetch registers used in turn (no copies of values still in
@@ -406,7 +408,7 @@ __device__ __forceinline__ void cols_item(int64_t nstrips, int group, int64_t* s
 template <int NSL, int CW, int T>
 __device__ __forceinline__ void cols_core(const float* __restrict__ yt, int64_t ldy, const float2* __restrict__ filt,
                                           int64_t ldf, int64_t k0, float2* U, const float2* TA, const float2* TB,
-                                          int lb, int tid) {
+                                          int lb, int tid, const float2* W16) {
   constexpr int NP = CW / 2;
   constexpr int PADN = Padded<NSL>::value;
   const float* ys = yt + (k0 / ldy) * ldy * NSL + (k0 % ldy);
@@ -454,7 +456,7 @@ __device__ __forceinline__ void cols_core(const float* __restrict__ yt, int64_t 
     }
   }
   __syncthreads();
-  fft_lds<NSL, NP, T, 16>(U, TA, TB, lb, 0, tid);
+  fft_lds<NSL, NP, T, 16, NSL, true>(U, TA, TB, lb, 0, tid, W16);
   // filter on the real-pair spectra, stored conjugated for the inverse
   const float sc = 1.0f / float(NSL);
   float2* Uf = U + fp * PADN;
@@ -479,7 +481,7 @@ __device__ __forceinline__ void cols_core(const float* __restrict__ yt, int64_t 
     }
   }
   __syncthreads();
-  fft_lds<NSL, NP, T>(U, TA, TB, lb, 0, tid);
+  fft_lds<NSL, NP, T, 1, NSL, true>(U, TA, TB, lb, 0, tid, W16);
 }
 
 template <int NSL, int CW, int T>
@@ -493,12 +495,14 @@ __global__ __launch_bounds__(T) void ipfb_cols_kernel(const float* __restrict__ 
   float2* U = smem;
   float2* TA = U + NP * PADN;
   float2* TB = TA + tw.nA;
+  float2* W16 = TB + tw.nB;
   const int tid = threadIdx.x;
   load_twiddle(tw, TA, TB, tid, T);
+  fill_w16(W16, tw.A, tw.B, tw.lb, tw.nA * tw.nB, tid, T);
   int64_t sidx, tile;
   cols_item(ncols / CW, group, &sidx, &tile);
   const int64_t k0 = sidx * CW;
-  cols_core<NSL, CW, T>(y + tile * tile_stride, ldy, filt, ldf, k0, U, TA, TB, tw.lb, tid);
+  cols_core<NSL, CW, T>(y + tile * tile_stride, ldy, filt, ldf, k0, U, TA, TB, tw.lb, tid, W16);
   float* ot = out + tile * out_tile_stride;
   const int nkeep = NSL - 2 * cut;
   for (int idx = tid; idx < nkeep * NP; idx += T) {
@@ -603,10 +607,12 @@ __global__ __launch_bounds__(T) void ipfb_cols_repfb_kernel(const float* __restr
   float2* U = smem;
   float2* TA = U + NP * PADN;
   float2* TB = TA + tw.nA;
-  float2* WR = TB + tw.nB;                                  // W_R1^m, m < R1
+  float2* W16 = TB + tw.nB;                                 // W_256^(m r), fill_w16
+  float2* WR = W16 + kW16Size;                                   // W_R1^m, m < R1
   float* WIN = reinterpret_cast<float*>(WR + R1);           // window strip [ntap*R1][CW]
   const int tid = threadIdx.x;
   load_twiddle(tw, TA, TB, tid, T);
+  fill_w16(W16, tw.A, tw.B, tw.lb, tw.nA * tw.nB, tid, T);
   int64_t sidx, tile;
   cols_item(C / CW, group, &sidx, &tile);
   const int64_t k0 = sidx * CW;
@@ -619,7 +625,7 @@ __global__ __launch_bounds__(T) void ipfb_cols_repfb_kernel(const float* __restr
     const int r = idx / CW, c = idx - r * CW;
     WIN[idx] = win[int64_t(r) * C + k0 + c];
   }
-  cols_core<NSL, CW, T>(y + tile * tile_stride, ldy, filt, ldf, k0, U, TA, TB, tw.lb, tid);
+  cols_core<NSL, CW, T>(y + tile * tile_stride, ldy, filt, ldf, k0, U, TA, TB, tw.lb, tid, W16);
 
   const int nkeep = NSL - 2 * cut;
   const int64_t g0 = row0 + tile * rows_per_tile;
@@ -988,8 +994,10 @@ __global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __rest
   float2* Z = smem;
   float2* TA = Z + NPAD;
   float2* TB = TA + tw4.nA;
+  float2* W16 = TB + tw4.nB;              // W_256^(m r), fill_w16
   const int tid = threadIdx.x;
   load_twiddle(tw4, TA, TB, tid, kTPB);
+  fill_w16(W16, tw4.A, tw4.B, tw4.lb, tw4.nA * tw4.nB, tid, kTPB);
   // Workgroup i runs on XCD i % 8: the H+1 workgroups of one block go to one
   // XCD back to back, so their interleaved output channels merge in its L2.
   const int64_t total = int64_t(H + 1) * nblock;
@@ -1048,7 +1056,7 @@ __global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __rest
   if (na + nm <= 512) {
     // passes NS = 1, 16; the last radix-16 pass only for the wanted bins:
     // U[j] = sum_m Z[(j % 256) + 256 m] W_4096^(m j)
-    fft_lds<N, 1, kTPB, 16, 256>(Z, TA, TB, tw4.lb, 0, tid);
+    fft_lds<N, 1, kTPB, 16, 256, true>(Z, TA, TB, tw4.lb, 0, tid, W16);
     for (int64_t t = tid; t < na + nm; t += kTPB) {
       const bool dir = t < na;
       const int64_t q2 = dir ? alo + t : mlo + (t - na);
@@ -1066,7 +1074,7 @@ __global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __rest
       ob[(q - c_lo) * sc] = dir ? make_float2(ax, ay) : make_float2(ax, -ay);
     }
   } else {
-    fft_lds<N, 1, kTPB, 16>(Z, TA, TB, tw4.lb, 0, tid);
+    fft_lds<N, 1, kTPB, 16, N, true>(Z, TA, TB, tw4.lb, 0, tid, W16);
     for (int64_t t = tid; t < na + nm; t += kTPB) {
       const bool dir = t < na;
       const int64_t q2 = dir ? alo + t : mlo + (t - na);
@@ -1167,7 +1175,7 @@ int launch_irfft_packed4(const Packed4Rows& pk, int64_t nrows, int64_t nslice, f
   Twiddle tw;
   int rc = get_twiddle(4096, &tw);
   if (rc) return rc;
-  const size_t lds = sizeof(float2) * (size_t(F) * Padded<2048>::value + tw.nA + tw.nB);
+  const size_t lds = sizeof(float2) * (size_t(F) * Padded<2048>::value + tw.nA + tw.nB + kW16Size);
   const int64_t ngroups = (nrows + F - 1) / F;
   ALB_REQUIRE(nrows < (int64_t(1) << 31) - F && pk.tile0 >= 0 && pk.tile0 < (int64_t(1) << 30),
               "irfft: too many rows");
@@ -1255,7 +1263,7 @@ int launch_cols(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles
   Twiddle tw;
   int rc = get_twiddle(NSL, &tw);
   if (rc) return rc;
-  const size_t lds = sizeof(float2) * (size_t(CW / 2) * Padded<NSL>::value + tw.nA + tw.nB);
+  const size_t lds = sizeof(float2) * (size_t(CW / 2) * Padded<NSL>::value + tw.nA + tw.nB + kW16Size);
   ALB_REQUIRE(ncols % CW == 0, "ipfb column pass: width %lld not a multiple of %d", (long long)ncols, CW);
   ALB_REQUIRE((ncols / CW) * ntiles < (int64_t(1) << 31), "ipfb column pass: too many tiles");
   hipLaunchKernelGGL((ipfb_cols_kernel<NSL, CW, T>), dim3(unsigned((ncols / CW) * ntiles)), dim3(T), lds, s, y,
@@ -1358,7 +1366,7 @@ int launch_cols_repfb(const float* y, int64_t ldy, int64_t tile_stride, int64_t 
   Twiddle tw;
   rc = get_twiddle(NSL, &tw);
   if (rc) return rc;
-  const size_t lds = sizeof(float2) * (size_t(NP) * Padded<NSL>::value + tw.nA + tw.nB + R1) +
+  const size_t lds = sizeof(float2) * (size_t(NP) * Padded<NSL>::value + tw.nA + tw.nB + kW16Size + R1) +
                      sizeof(float) * size_t(ntap) * R1 * CW;
   ALB_REQUIRE(lds <= 160 * 1024, "fused iPFB/re-PFB: LDS budget exceeded");
   hipLaunchKernelGGL((ipfb_cols_repfb_kernel<NSL, CW, T, R1>), dim3(unsigned((4096 / CW) * ntiles)), dim3(T), lds, s, y,
@@ -1374,7 +1382,7 @@ int launch_repfb_stage2(int64_t nblock, const float2* A, int64_t c_lo, int64_t c
   Twiddle tw4;
   int rc = get_twiddle(4096, &tw4);
   if (rc) return rc;
-  const size_t lds2 = sizeof(float2) * (size_t(Padded<4096>::value) + tw4.nA + tw4.nB);
+  const size_t lds2 = sizeof(float2) * (size_t(Padded<4096>::value) + tw4.nA + tw4.nB + kW16Size);
   const int64_t total = int64_t(R1 / 2 + 1) * nblock;
   ALB_REQUIRE(total < (int64_t(1) << 31) - 8, "alb_repfb: too many blocks");
   hipLaunchKernelGGL((repfb_stage2_kernel<R1>), dim3(unsigned((total + 7) / 8 * 8)), dim3(kTPB), lds2, s, A, nblock,
