@@ -68,13 +68,21 @@ _lib = None
 
 
 def load():
-    """Load the library (once).  Raises ImportError if it has not been built."""
+    """Load the library (once).  Raises ImportError if it has not been built.
+
+    torch carries its own HIP runtime; the library uses the system one.  With
+    a GPU present, torch's device state is initialised before the library is
+    loaded: loaded the other way round, the library's launches later found no
+    device."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(
                 "libalbatros_hip.so is not built; run `python -m albatros_axion_detection_amd.build` "
                 "(there is no CPU fallback)")
+        t = torch()
+        if t.cuda.is_available():
+            t.cuda.init()
         lib = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)

@@ -469,8 +469,11 @@ __device__ __forceinline__ void cols_core(const float* __restrict__ yt, int64_t 
       const float2 y0 = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));   // (a + conj b)/2
       const float2 dd = make_float2(0.5f * (a.x - b.x), 0.5f * (a.y + b.y));   // (a - conj b)/2
       const float2 y1 = make_float2(dd.y, -dd.x);                              // -i dd
-      float2 g0 = cscale(cmul(m < KP ? h0[m] : h0p[f], y0), sc);
-      float2 g1 = cscale(cmul(m < KP ? h1[m] : h1p[f], y1), sc);
+      const float2 hv0 = m < KP ? h0[m] : h0p[f];
+      const float2 hv1 = m < KP ? h1[m] : h1p[f];
+      // explicit fused forms, so every build of this step rounds alike
+      float2 g0 = make_float2(fmaf(hv0.x, y0.x, -(hv0.y * y0.y)) * sc, fmaf(hv0.x, y0.y, hv0.y * y0.x) * sc);
+      float2 g1 = make_float2(fmaf(hv1.x, y1.x, -(hv1.y * y1.y)) * sc, fmaf(hv1.x, y1.y, hv1.y * y1.x) * sc);
       if (f == 0 || 2 * f == NSL) { g0.y = 0.f; g1.y = 0.f; }
       const float2 uf = make_float2(g0.x - g1.y, g0.y + g1.x);     // g0 + i g1
       Uf[pad16(f)] = cconj(uf);
@@ -589,43 +592,23 @@ __host__ __device__ inline void repfb_interior(int64_t g0, int64_t nkeep, int64_
 // lanes as in the split kernel below (lane t: rows 4s + t, S-point DFT, radix-4
 // combine through LDS), then Y0[q] = (Z[q] + conj Z[-q]) / 2 and
 // Y1[q] = (Z[q] - conj Z[-q]) / 2i.
+// One tile of the fused column pass: column pass of strip k0 of tile `tile`,
+// stream rows for the tile-boundary blocks, stage 1 of the interior blocks.
 template <int NSL, int CW, int T, int R1>
-__global__ __launch_bounds__(T) void ipfb_cols_repfb_kernel(const float* __restrict__ y, int64_t ldy,
-                                                               int64_t tile_stride, const float2* __restrict__ filt,
-                                                               int64_t ldf, int cut, float* __restrict__ ts,
-                                                               int64_t row0, int64_t rows_per_tile,
-                                                               const float* __restrict__ win, int ntap,
-                                                               int64_t nblocks, float2* __restrict__ A, int group,
-                                                               Twiddle tw) {
+__device__ __forceinline__ void cols_repfb_tile(const float* __restrict__ y, int64_t ldy, int64_t tile_stride,
+                                                const float2* __restrict__ filt, int64_t ldf, int cut,
+                                                float* __restrict__ ts, int64_t row0, int64_t rows_per_tile,
+                                                int ntap, int64_t nblocks, float2* __restrict__ A, int64_t tile,
+                                                int64_t k0, float2* U, const float2* TA, const float2* TB,
+                                                const float2* W16, const float2* WR, const float* WIN, int lb,
+                                                int tid) {
   constexpr int NP = CW / 2;
   constexpr int PADN = Padded<NSL>::value;
   constexpr int S = R1 / 4;
   constexpr int H = R1 / 2;
   constexpr int SP = S >= 16 ? S + 1 : S;   // row of the exchange buffer (padded when it fits)
   constexpr int64_t C = 4096;
-  extern __shared__ float2 smem[];
-  float2* U = smem;
-  float2* TA = U + NP * PADN;
-  float2* TB = TA + tw.nA;
-  float2* W16 = TB + tw.nB;                                 // W_256^(m r), fill_w16
-  float2* WR = W16 + kW16Size;                                   // W_R1^m, m < R1
-  float* WIN = reinterpret_cast<float*>(WR + R1);           // window strip [ntap*R1][CW]
-  const int tid = threadIdx.x;
-  load_twiddle(tw, TA, TB, tid, T);
-  fill_w16(W16, tw.A, tw.B, tw.lb, tw.nA * tw.nB, tid, T);
-  int64_t sidx, tile;
-  cols_item(C / CW, group, &sidx, &tile);
-  const int64_t k0 = sidx * CW;
-  if (tid < R1) {
-    double sn, cs;
-    sincospi(-2.0 * double(tid) / double(R1), &sn, &cs);
-    WR[tid] = make_float2(float(cs), float(sn));
-  }
-  for (int idx = tid; idx < ntap * R1 * CW; idx += T) {
-    const int r = idx / CW, c = idx - r * CW;
-    WIN[idx] = win[int64_t(r) * C + k0 + c];
-  }
-  cols_core<NSL, CW, T>(y + tile * tile_stride, ldy, filt, ldf, k0, U, TA, TB, tw.lb, tid, W16);
+  cols_core<NSL, CW, T>(y + tile * tile_stride, ldy, filt, ldf, k0, U, TA, TB, lb, tid, W16);
 
   const int nkeep = NSL - 2 * cut;
   const int64_t g0 = row0 + tile * rows_per_tile;
@@ -759,6 +742,46 @@ __global__ __launch_bounds__(T) void ipfb_cols_repfb_kernel(const float* __restr
       if (q >= H) { q -= H; ++bl; }
     }
   }
+}
+
+template <int NSL, int CW, int T, int R1>
+__global__ __launch_bounds__(T) void ipfb_cols_repfb_kernel(const float* __restrict__ y, int64_t ldy,
+                                                               int64_t tile_stride, const float2* __restrict__ filt,
+                                                               int64_t ldf, int cut, float* __restrict__ ts,
+                                                               int64_t row0, int64_t rows_per_tile,
+                                                               const float* __restrict__ win, int ntap,
+                                                               int64_t nblocks, float2* __restrict__ A, int group,
+                                                               Twiddle tw) {
+  constexpr int NP = CW / 2;
+  constexpr int PADN = Padded<NSL>::value;
+  constexpr int S = R1 / 4;
+  constexpr int H = R1 / 2;
+  constexpr int SP = S >= 16 ? S + 1 : S;   // row of the exchange buffer (padded when it fits)
+  constexpr int64_t C = 4096;
+  extern __shared__ float2 smem[];
+  float2* U = smem;
+  float2* TA = U + NP * PADN;
+  float2* TB = TA + tw.nA;
+  float2* W16 = TB + tw.nB;                                 // W_256^(m r), fill_w16
+  float2* WR = W16 + kW16Size;                                   // W_R1^m, m < R1
+  float* WIN = reinterpret_cast<float*>(WR + R1);           // window strip [ntap*R1][CW]
+  const int tid = threadIdx.x;
+  load_twiddle(tw, TA, TB, tid, T);
+  fill_w16(W16, tw.A, tw.B, tw.lb, tw.nA * tw.nB, tid, T);
+  int64_t sidx, tile;
+  cols_item(C / CW, group, &sidx, &tile);
+  const int64_t k0 = sidx * CW;
+  if (tid < R1) {
+    double sn, cs;
+    sincospi(-2.0 * double(tid) / double(R1), &sn, &cs);
+    WR[tid] = make_float2(float(cs), float(sn));
+  }
+  for (int idx = tid; idx < ntap * R1 * CW; idx += T) {
+    const int r = idx / CW, c = idx - r * CW;
+    WIN[idx] = win[int64_t(r) * C + k0 + c];
+  }
+  cols_repfb_tile<NSL, CW, T, R1>(y, ldy, tile_stride, filt, ldf, cut, ts, row0, rows_per_tile, ntap, nblocks, A, tile,
+                                  k0, U, TA, TB, W16, WR, WIN, tw.lb, tid);
 }
 
 // ---------------------------------------- iPFB column pass, any even nslice

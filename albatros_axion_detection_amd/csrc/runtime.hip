@@ -84,8 +84,8 @@ int alb_tune_set(const char* key, int64_t value) {
     ALB_REQUIRE(value == 0 || value == 2 || value == 4 || value == 8,
                 "alb_tune_set: ipfb_cols_width must be 0, 2, 4 or 8");
   } else if (k == "ipfb_cols_group") {
-    ALB_REQUIRE(value == 0 || value == 4 || value == 8 || value == 16 || value == 32 || value == 64,
-                "alb_tune_set: ipfb_cols_group must be 0, 4, 8, 16, 32 or 64");
+    ALB_REQUIRE(value == 0 || value == 2 || value == 4 || value == 8 || value == 16 || value == 32 || value == 64,
+                "alb_tune_set: ipfb_cols_group must be 0, 2, 4, 8, 16, 32 or 64");
   } else if (k == "ipfb_rows_group") {
     ALB_REQUIRE(value == 0 || value == 2 || value == 4, "alb_tune_set: ipfb_rows_group must be 0, 2 or 4");
   } else if (k == "xcorr_int_method") {

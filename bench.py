@@ -17,6 +17,7 @@ precision correlator).
 """
 import argparse
 import ctypes
+import itertools
 import json
 import os
 import sys
@@ -143,33 +144,31 @@ def rechannelizer(args, torch):
         timer("repfb_stage2", lambda: _lib.call(
             "alb_repfb_stage2", _lib.ptr(ws), nblk, osamp, c_lo, c_hi, _lib.ptr(out), c_hi - c_lo, 1, nblk, 0, s()))
 
-    # launch-shape autotuning (untimed): strip width of the column pass
-    lib = _lib.load()
-    sweep = {}
-    widths = [args.cols_width] if args.cols_width else [8, 4, 2]
-    groups = [args.cols_group] if args.cols_group else [64, 32, 16, 8, 4, 2]
-    rgroups = [args.rows_group] if args.rows_group else [4, 2]
-    for w in widths:
-        for gsz in groups:
-            for rg in rgroups:
-                lib.alb_tune_set(b"ipfb_cols_width", w)
-                lib.alb_tune_set(b"ipfb_cols_group", gsz)
-                lib.alb_tune_set(b"ipfb_rows_group", rg)
-                step()
-                torch.cuda.synchronize()
-                a = torch.cuda.Event(enable_timing=True)
-                b = torch.cuda.Event(enable_timing=True)
-                a.record()
-                for _ in range(3):
-                    step()
-                b.record()
-                torch.cuda.synchronize()
-                sweep["w%d_g%d_r%d" % (w, gsz, rg)] = a.elapsed_time(b) / 3
+    # launch-shape autotuning (untimed): column-pass strip width and XCD group
+    # size, row-pass group size
+    knobs = (("ipfb_cols_width", "w", [args.cols_width] if args.cols_width else [8, 4, 2]),
+             ("ipfb_cols_group", "g", [args.cols_group] if args.cols_group else [64, 32, 16, 8, 4, 2]),
+             ("ipfb_rows_group", "r", [args.rows_group] if args.rows_group else [4, 2]))
+    sweep, combos = {}, {}
+    for vals in itertools.product(*(k[2] for k in knobs)):
+        for (key, _, _), v in zip(knobs, vals):
+            _lib.call("alb_tune_set", key.encode(), v)
+        step()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(3):
+            step()
+        b.record()
+        torch.cuda.synchronize()
+        name = "_".join("%s%d" % (k[1], v) for k, v in zip(knobs, vals))
+        sweep[name] = a.elapsed_time(b) / 3
+        combos[name] = vals
     best = min(sweep, key=sweep.get)
-    bw, bg, br = (int(x[1:]) for x in best.split("_"))
-    lib.alb_tune_set(b"ipfb_cols_width", bw)
-    lib.alb_tune_set(b"ipfb_cols_group", bg)
-    lib.alb_tune_set(b"ipfb_rows_group", br)
+    for (key, _, _), v in zip(knobs, combos[best]):
+        _lib.call("alb_tune_set", key.encode(), v)
+    tuned = {key: v for (key, _, _), v in zip(knobs, combos[best])}
     log("[bench] column-pass launch-shape sweep (ms/step): %s -> %s" % (
         json.dumps({k: round(v, 4) for k, v in sweep.items()}), best))
 
@@ -235,8 +234,7 @@ def rechannelizer(args, torch):
                    "n_coarse": n_coarse, "acclen": acclen, "cut": cut, "osamp": osamp, "nchan_fine": c_hi - c_lo,
                    "nblock": nblk},
         "ref_unit_msps": n_coarse * 2049 * 8 / (ms * 1e-3) / 1e6,
-        "tuning": {"ipfb_cols_width": bw, "ipfb_cols_group": bg, "ipfb_rows_group": br,
-                   "sweep_ms_per_step": {k: round(v, 4) for k, v in sweep.items()}},
+        "tuning": dict(tuned, sweep_ms_per_step={k: round(v, 4) for k, v in sweep.items()}),
     }
     ctx = dict(packed_np=packed_np, acclen=acclen, cut=cut, osamp=osamp, chan0=chan0, nkeep=nkeep,
                filt=pipe.filt, win=pipe.win)

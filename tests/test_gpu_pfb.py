@@ -183,8 +183,9 @@ def test_stream_spectra_fused(rng, osamp, nblock):
 
 
 def test_column_pass_width_invariance(rng):
-    # the strip width only changes how many columns share a workgroup: every
-    # column's arithmetic is the same, so outputs must be bit-identical
+    # the strip width and the workgroup order only change how columns are
+    # shared out: every column's arithmetic is the same, so outputs must be
+    # bit-identical
     from albatros_axion_detection_amd import _lib, repfb
     lib = _lib.load()
     acclen, cut, nchunks, osamp, nblock = 4088, 4, 2, 64, 10
@@ -195,19 +196,23 @@ def test_column_pass_width_invariance(rng):
     packed = _lib.to_device(repfb.stack_packed(chunks, "pol0", acclen, len(chan)))
     outs = {}
     try:
-        for w, g in ((8, 32), (4, 32), (2, 32), (8, 8), (8, 64), (4, 16)):
-            lib.alb_tune_set(b"ipfb_cols_width", w)
-            lib.alb_tune_set(b"ipfb_cols_group", g)
+        for w, g in ((8, 32), (4, 32), (2, 32), (8, 8), (8, 64), (4, 16), (2, 2)):
+            _lib.call("alb_tune_set", b"ipfb_cols_width", w)
+            _lib.call("alb_tune_set", b"ipfb_cols_group", g)
             print("[gpu-case] width", w, "group", g, flush=True)
             outs[w, g] = (pipe.spectra(packed, rm, nchunks).cpu().numpy(),
                           pipe.spectra(packed, rm, nchunks, fused=False).cpu().numpy())
     finally:
-        lib.alb_tune_set(b"ipfb_cols_width", 0)
-        lib.alb_tune_set(b"ipfb_cols_group", 0)
+        for key in (b"ipfb_cols_width", b"ipfb_cols_group"):
+            lib.alb_tune_set(key, 0)
+    ref = outs[8, 32]
     for key in outs:
-        assert np.array_equal(outs[key][0], outs[8, 32][0]), key
-        assert np.array_equal(outs[key][1], outs[8, 32][1]), key
-    assert rel_err(outs[8, 32][0], outs[8, 32][1]) < 2e-6
+        d = np.abs(outs[key][0] - ref[0]).max() / np.abs(ref[0]).max()
+        print("[gpu-case]", key, "fused max diff / max", d, flush=True)
+    for key in outs:
+        assert np.array_equal(outs[key][0], ref[0]), key
+        assert np.array_equal(outs[key][1], ref[1]), key
+    assert rel_err(ref[0], ref[1]) < 2e-6
 
 
 def test_stream_xcorr_two_antennas(rng):

@@ -1,4 +1,5 @@
 """GPU parity of the unpack family against the oracle (bit exact)."""
+import os
 import numpy as np
 import pytest
 
@@ -149,3 +150,25 @@ def test_dropped_packets(rng):
     exp = data.copy()
     exp[2] = 0
     assert np.array_equal(out, exp) and int(nd.item()) == 1
+
+
+@pytest.mark.gpu
+def test_library_loaded_before_torch_in_a_fresh_process():
+    # torch and the library each bring a HIP runtime: loading the library
+    # first and touching the GPU through torch afterwards must still leave the
+    # library's launches working
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = ("import sys\n"
+            "sys.path.insert(0, %r)\n"
+            "from albatros_axion_detection_amd import _lib\n"
+            "_lib.load()\n"
+            "import torch\n"
+            "x = torch.ones(4096, device='cuda')\n"
+            "_lib.call('alb_fill_zero', _lib.ptr(x), x.numel() * 4, _lib.stream())\n"
+            "torch.cuda.synchronize()\n"
+            "assert float(x.abs().sum()) == 0.0\n"
+            "print('launch ok')\n") % root
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "launch ok" in r.stdout, r.stdout + r.stderr
