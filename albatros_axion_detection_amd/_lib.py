@@ -37,6 +37,7 @@ SIGNATURES = {
     "alb_xcorr_1bit_allpairs": (_i32, [_vp, _i32, _i64, _i64, _i64, _i64, _vp, _vp]),
     "alb_xcorr_all_ant": (_i32, [_vp, _i64, _i64, _i64, _vp, _f32, _vp]),
     "alb_xcorr_all_ant_prec": (_i32, [_vp, _i64, _i64, _i64, _vp, _f32, _i32, _vp]),
+    "alb_xcorr_all_ant_strided": (_i32, [_vp, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _vp, _f32, _i32, _vp]),
     "alb_matft_filter": (_i32, [_vp, _i32, _i64, _i64, _f64, _vp, _vp, _vp]),
     "alb_filter_from_matft": (_i32, [_vp, _i64, _f64, _vp, _vp]),
     "alb_irfft_rows": (_i32, [_vp, _i64, _vp, _i64, _i64, _i64, _vp]),
@@ -55,6 +56,8 @@ SIGNATURES = {
     "alb_repfb_workspace_bytes": (_i64, [_i64, _i64]),
     "alb_repfb_stage1": (_i32, [_vp, _vp, _i32, _i64, _i64, _vp, _vp]),
     "alb_repfb_stage2": (_i32, [_vp, _i64, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _vp]),
+    "alb_repfb_stage2_inputs": (_i32, [_vp, _i64, _i64, _i64, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _i64,
+                                       _vp]),
     "alb_ipfb_columns_repfb": (_i32, [_vp, _i64, _i64, _i64, _i64, _vp, _i64, _i32, _vp, _i64, _i64, _vp, _i32, _i64,
                                       _i64, _vp, _vp]),
     "alb_repfb_interior": (_i32, [_i64, _i64, _i64, _i32, _i64, _vp, _vp]),

@@ -153,10 +153,9 @@ __global__ __launch_bounds__(TT) void irfft_rows_kernel(DenseRows dense, int64_t
 // layout's 32-byte row pieces meet in that XCD's L2.  Each thread's first
 // kRowItems (row, column) items keep their channel weights in registers; the
 // packed bytes of the next group and the row-map entries of the group after
-// it are fetched while the current group is transformed.
-constexpr int kRowItems = 2;
-
-template <int F, int TT>
+// it are fetched while the current group is transformed.  KI = kRowItems
+// covers F * ncols <= KI * TT (100-channel spectra: 2; 300 channels: 4).
+template <int F, int TT, int kRowItems>
 __global__ __launch_bounds__(TT, 1024 / TT) void irfft_packed4_kernel(Packed4Rows pk, int64_t nrows, int64_t nslice,
                                                            float* __restrict__ out, int64_t ldo, Twiddle tw) {
   constexpr int M = 2048;
@@ -1007,8 +1006,9 @@ __device__ __forceinline__ void q2_range(int64_t q1, int64_t R1, int64_t c_lo, i
 }
 
 template <int R1>
-__global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __restrict__ A, int64_t nblock, int64_t c_lo,
-                                                            int64_t c_hi, float2* __restrict__ out, int64_t sb, int64_t sc,
+__global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __restrict__ A, int64_t a_stride, int ninput,
+                                                            int64_t nblock, int64_t c_lo, int64_t c_hi,
+                                                            float2* __restrict__ out, int64_t sm, int64_t sb, int64_t sc,
                                                             int64_t bpj, int64_t sj, Twiddle tw4) {
   constexpr int N = 4096;
   constexpr int H = R1 / 2;
@@ -1021,17 +1021,20 @@ __global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __rest
   const int tid = threadIdx.x;
   load_twiddle(tw4, TA, TB, tid, kTPB);
   fill_w16(W16, tw4.A, tw4.B, tw4.lb, tw4.nA * tw4.nB, tid, kTPB);
-  // Workgroup i runs on XCD i % 8: the H+1 workgroups of one block go to one
-  // XCD back to back, so their interleaved output channels merge in its L2.
-  const int64_t total = int64_t(H + 1) * nblock;
+  // Workgroup i runs on XCD i % 8: the (H+1) * ninput workgroups of one block
+  // go to one XCD back to back, inputs fastest, so the neighbouring output
+  // elements they write (input m of block b at out + m sm) merge in its L2.
+  const int64_t total = int64_t(H + 1) * nblock * ninput;
   const int64_t per = (total + 7) / 8;
   const int64_t item = int64_t(blockIdx.x % 8) * per + blockIdx.x / 8;
   if (item >= total) return;   // the whole workgroup: padding of the 8-way split
-  const int q1 = int(item % (H + 1));
-  const int64_t b = item / (H + 1);
+  const int m_in = int(item % ninput);
+  const int64_t qb = item / ninput;
+  const int q1 = int(qb % (H + 1));
+  const int64_t b = qb / (H + 1);
   // column k = tid + 256 r of bin q1 (StageA layout; bins 0 and H from slot 0)
   const int slot = (q1 == 0 || q1 == H) ? 0 : q1;
-  const float2* src = A + StageA<R1>::at(b, slot, tid);
+  const float2* src = A + m_in * a_stride + StageA<R1>::at(b, slot, tid);
   // W_L^(q1 k) = W_L^(q1 c) W_4096^(q1 a) for k = R1 a + c; c = tid mod R1 is
   // the same for all of this thread's k (kTPB is a multiple of R1)
   static_assert(kTPB % R1 == 0, "thread stride must keep k mod R1 fixed");
@@ -1075,7 +1078,7 @@ __global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __rest
   if (mirror) q2_range(R1 - q1, R1, c_lo, c_hi, &mlo, &mhi);
   const int64_t na = ahi >= alo ? ahi - alo + 1 : 0;
   const int64_t nm = mhi >= mlo ? mhi - mlo + 1 : 0;
-  float2* ob = out + (b / bpj) * sj + (b % bpj) * sb;
+  float2* ob = out + m_in * sm + (b / bpj) * sj + (b % bpj) * sb;
   if (na + nm <= 512) {
     // passes NS = 1, 16; the last radix-16 pass only for the wanted bins:
     // U[j] = sum_m Z[(j % 256) + 256 m] W_4096^(m j)
@@ -1170,6 +1173,9 @@ using namespace alb;
 // ---------------------------------------------------------------- host side
 namespace {
 
+// columns per workgroup of the nslice = 4096 column passes when the knob is unset
+constexpr int64_t kColsWidth = 4;
+
 // strips per group of the column-pass workgroup order (tuning knob, default 32)
 int cols_group() {
   const int64_t g = tune_get("ipfb_cols_group");
@@ -1192,7 +1198,7 @@ int launch_irfft(DenseRows d, int64_t nrows, float* out, int64_t ldo, hipStream_
 // Persistent packed-row launch: as many workgroups as fit on the device at
 // once (a multiple of 8, so every XCD gets the same count), at most one per
 // row group.
-template <int F, int TT>
+template <int F, int TT, int KI>
 int launch_irfft_packed4(const Packed4Rows& pk, int64_t nrows, int64_t nslice, float* out, int64_t ldo,
                          hipStream_t s) {
   Twiddle tw;
@@ -1202,22 +1208,22 @@ int launch_irfft_packed4(const Packed4Rows& pk, int64_t nrows, int64_t nslice, f
   const int64_t ngroups = (nrows + F - 1) / F;
   ALB_REQUIRE(nrows < (int64_t(1) << 31) - F && pk.tile0 >= 0 && pk.tile0 < (int64_t(1) << 30),
               "irfft: too many rows");
-  static std::atomic<int> resident[64];
+  static std::atomic<int> resident[64];   // one cache per instantiation
   int dev = 0;
   ALB_HIP(hipGetDevice(&dev));
   int nres = dev < 64 ? resident[dev].load() : 0;
   if (nres == 0) {
     int ncu = 0, occ = 0;
     ALB_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    ALB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, irfft_packed4_kernel<F, TT>, TT, lds));
+    ALB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, irfft_packed4_kernel<F, TT, KI>, TT, lds));
     nres = ncu * (occ > 0 ? occ : 1);
     if (dev < 64) resident[dev].store(nres);
   }
   const int64_t per = (ngroups + 7) / 8;
   int64_t S = nres / 8 > 0 ? nres / 8 : 1;
   if (S > per) S = per;
-  hipLaunchKernelGGL((irfft_packed4_kernel<F, TT>), dim3(unsigned(8 * S)), dim3(TT), lds, s, pk, nrows, nslice, out,
-                     ldo, tw);
+  hipLaunchKernelGGL((irfft_packed4_kernel<F, TT, KI>), dim3(unsigned(8 * S)), dim3(TT), lds, s, pk, nrows, nslice,
+                     out, ldo, tw);
   ALB_LAUNCH_CHECK("irfft_packed4_kernel");
   return ALB_OK;
 }
@@ -1227,8 +1233,13 @@ int launch_irfft_packed4(const Packed4Rows& pk, int64_t nrows, int64_t nslice, f
 // whole 128-byte lines of the blocked layout)
 int irfft_packed4_dispatch(const Packed4Rows& pk, int64_t nrows, int64_t nslice, float* out, int64_t ldo,
                            hipStream_t s) {
-  if (tune_get("ipfb_rows_group") == 4) return launch_irfft_packed4<4, 512>(pk, nrows, nslice, out, ldo, s);
-  return launch_irfft_packed4<2, kTPB>(pk, nrows, nslice, out, ldo, s);
+  // prefetched (row, column) items per thread: enough for every item when it fits
+  if (tune_get("ipfb_rows_group") == 4) {
+    if (4 * pk.ncols <= 2 * 512) return launch_irfft_packed4<4, 512, 2>(pk, nrows, nslice, out, ldo, s);
+    return launch_irfft_packed4<4, 512, 4>(pk, nrows, nslice, out, ldo, s);
+  }
+  if (2 * pk.ncols <= 2 * kTPB) return launch_irfft_packed4<2, kTPB, 2>(pk, nrows, nslice, out, ldo, s);
+  return launch_irfft_packed4<2, kTPB, 4>(pk, nrows, nslice, out, ldo, s);
 }
 
 int irfft_dispatch(int64_t n, DenseRows d, int64_t nrows, float* out, int64_t ldo, hipStream_t s) {
@@ -1301,7 +1312,8 @@ int cols_dispatch(int64_t nslice, const float* y, int64_t ldy, int64_t tile_stri
   if (nslice == 4096) {
     // strip width (columns per workgroup): 8 -> one 1024-thread workgroup per
     // CU, 4 -> two of 512, 2 -> four of 256
-    switch (tune_get("ipfb_cols_width")) {
+    const int64_t kw = tune_get("ipfb_cols_width");
+    switch (kw > 0 ? kw : kColsWidth) {
       case 2: return launch_cols<4096, 2, 256>(y, ldy, tile_stride, ntiles, filt, ldf, ncols, cut, out, ldo, out_tile_stride, s);
       case 4: return launch_cols<4096, 4, 512>(y, ldy, tile_stride, ntiles, filt, ldf, ncols, cut, out, ldo, out_tile_stride, s);
       default: break;
@@ -1400,16 +1412,16 @@ int launch_cols_repfb(const float* y, int64_t ldy, int64_t tile_stride, int64_t 
 }
 
 template <int R1>
-int launch_repfb_stage2(int64_t nblock, const float2* A, int64_t c_lo, int64_t c_hi, float2* out, int64_t sb, int64_t sc,
-                        int64_t bpj, int64_t sj, hipStream_t s) {
+int launch_repfb_stage2(const float2* A, int64_t a_stride, int64_t ninput, int64_t nblock, int64_t c_lo, int64_t c_hi,
+                        float2* out, int64_t sm, int64_t sb, int64_t sc, int64_t bpj, int64_t sj, hipStream_t s) {
   Twiddle tw4;
   int rc = get_twiddle(4096, &tw4);
   if (rc) return rc;
   const size_t lds2 = sizeof(float2) * (size_t(Padded<4096>::value) + tw4.nA + tw4.nB + kW16Size);
-  const int64_t total = int64_t(R1 / 2 + 1) * nblock;
+  const int64_t total = int64_t(R1 / 2 + 1) * nblock * ninput;
   ALB_REQUIRE(total < (int64_t(1) << 31) - 8, "alb_repfb: too many blocks");
-  hipLaunchKernelGGL((repfb_stage2_kernel<R1>), dim3(unsigned((total + 7) / 8 * 8)), dim3(kTPB), lds2, s, A, nblock,
-                     c_lo, c_hi, out, sb, sc, bpj, sj, tw4);
+  hipLaunchKernelGGL((repfb_stage2_kernel<R1>), dim3(unsigned((total + 7) / 8 * 8)), dim3(kTPB), lds2, s, A, a_stride,
+                     int(ninput), nblock, c_lo, c_hi, out, sm, sb, sc, bpj, sj, tw4);
   ALB_LAUNCH_CHECK("repfb_stage2_kernel");
   return ALB_OK;
 }
@@ -1569,16 +1581,33 @@ int alb_repfb_stage1(const float* ts, const float* win, int ntap, int64_t nblock
   }
 }
 
+int alb_repfb_stage2_inputs(const void* A, int64_t a_stride, int64_t ninput, int64_t nblock, int64_t osamp,
+                            int64_t c_lo, int64_t c_hi, void* out, int64_t sm, int64_t sb, int64_t sc, int64_t bpj,
+                            int64_t sj, void* stream);
+int64_t alb_repfb_workspace_bytes(int64_t nblock, int64_t osamp);
+
 int alb_repfb_stage2(const void* A, int64_t nblock, int64_t osamp, int64_t c_lo, int64_t c_hi, void* out, int64_t sb,
                      int64_t sc, int64_t bpj, int64_t sj, void* stream) {
+  return alb_repfb_stage2_inputs(A, 0, 1, nblock, osamp, c_lo, c_hi, out, 0, sb, sc, bpj, sj, stream);
+}
+
+// Stage 2 of ninput inputs at once: input m's workspace at A + m * a_stride
+// bytes, its channels at out + m * sm (complex elements) + the alb_repfb_stage2
+// addressing.
+int alb_repfb_stage2_inputs(const void* A, int64_t a_stride, int64_t ninput, int64_t nblock, int64_t osamp,
+                            int64_t c_lo, int64_t c_hi, void* out, int64_t sm, int64_t sb, int64_t sc, int64_t bpj,
+                            int64_t sj, void* stream) {
   ALB_REQUIRE(A && out, "alb_repfb_stage2: null pointer");
-  ALB_REQUIRE(nblock >= 0 && c_lo >= 0 && c_hi > c_lo && c_hi <= 2048 * osamp + 1 && bpj >= 1,
+  ALB_REQUIRE(nblock >= 0 && ninput >= 1 && ninput <= 1024 && c_lo >= 0 && c_hi > c_lo &&
+                  c_hi <= 2048 * osamp + 1 && bpj >= 1,
               "alb_repfb_stage2: bad sizes");
+  ALB_REQUIRE(a_stride % int64_t(sizeof(float2)) == 0 && (ninput == 1 || a_stride >= alb_repfb_workspace_bytes(nblock, osamp)),
+              "alb_repfb_stage2: workspace stride must be a multiple of 8 bytes and hold nblock blocks");
   if (nblock == 0) return ALB_OK;
-  ALB_REQUIRE(nblock <= 65535, "alb_repfb: at most 65535 blocks per call");
   hipStream_t s = as_stream(stream);
+  const int64_t as = a_stride / int64_t(sizeof(float2));
   switch (osamp) {
-#define CASE(r) case r: return launch_repfb_stage2<r>(nblock, (const float2*)A, c_lo, c_hi, (float2*)out, sb, sc, bpj, sj, s);
+#define CASE(r) case r: return launch_repfb_stage2<r>((const float2*)A, as, ninput, nblock, c_lo, c_hi, (float2*)out, sm, sb, sc, bpj, sj, s);
     CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) CASE(64)
 #undef CASE
     default:
@@ -1639,7 +1668,9 @@ int alb_ipfb_columns_repfb(const float* y, int64_t ldy, int64_t tile_stride, int
               "alb_ipfb_columns_repfb: bad strides (ldy >= 4096 row-major, or 8 for the blocked tile layout)");
   hipStream_t s = as_stream(stream);
   if (nslice == 4096 && (osamp == 32 || osamp == 64)) {
-    const int64_t w = tune_get("ipfb_cols_width");
+    // strip width: the knob, else 4 columns (two 512-thread workgroups per CU)
+    const int64_t kw = tune_get("ipfb_cols_width");
+    const int64_t w = kw > 0 ? kw : kColsWidth;
 #define CASE(W, T, r) if (w == W && osamp == r) return launch_cols_repfb<4096, W, T, r>(y, ldy, tile_stride, ntiles, (const float2*)filt, ldf, cut, ts, row0, rows_per_tile, win, ntap, nblocks, (float2*)A, s);
     CASE(2, 256, 32) CASE(2, 256, 64) CASE(4, 512, 32) CASE(4, 512, 64)
 #undef CASE

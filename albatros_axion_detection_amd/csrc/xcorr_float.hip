@@ -7,15 +7,28 @@
 // feeds Y[l % 16][t0 + l / 16] to both operands (A = Y, B = Y^T share registers).
 // Then Re = G[i][j] + G[i+8][j+8], Im = G[i+8][j] - G[i][j+8].
 // Other M: a VALU kernel with one thread per (channel, baseline).
+//
+// The matrix-core kernels also take a strided input: channel c = s F + f of
+// S jobs of F channels, element (m, t) of it at x[s ss + f sf + t sk + m]
+// (complex elements; the F-order layout is sk = M, sf = M K, ss = 0 with S = 1).
 #include "common.hpp"
 
 namespace alb {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
+// Input addressing of the matrix-core correlators (complex-element strides).
+struct XLayout {
+  int64_t sk, sf, ss, F;   // time, channel and job strides; channels per job
+  __device__ __forceinline__ int64_t channel(int64_t c) const {
+    const int64_t s = c / F;
+    return s * ss + (c - s * F) * sf;
+  }
+};
+
 template <int M>
 __global__ __launch_bounds__(256) void xcorr_mfma16_kernel(const float* __restrict__ x, int64_t K, int64_t F,
-                                                           float2* __restrict__ out, float scale) {
+                                                           XLayout L, float2* __restrict__ out, float scale) {
   __shared__ float G[4][16 * 16];
   const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int64_t f = int64_t(blockIdx.x) * 4 + w;
@@ -23,14 +36,15 @@ __global__ __launch_bounds__(256) void xcorr_mfma16_kernel(const float* __restri
   const int r = lane % 16, slot = lane / 16;
   const int m = r % 8, part = r / 8;
   const bool row_ok = valid && m < M;
-  const float* base = x + (valid ? f : 0) * int64_t(2 * M) * K + 2 * m + part;
+  const float* base = x + 2 * L.channel(valid ? f : 0) + 2 * m + part;
+  const int64_t st = 2 * L.sk;
   floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0, acc2 = acc0, acc3 = acc0;
   int64_t t0 = 0;
   for (; t0 + 16 <= K; t0 += 16) {
-    const float a0 = row_ok ? base[int64_t(2 * M) * (t0 + slot)] : 0.f;
-    const float a1 = row_ok ? base[int64_t(2 * M) * (t0 + 4 + slot)] : 0.f;
-    const float a2 = row_ok ? base[int64_t(2 * M) * (t0 + 8 + slot)] : 0.f;
-    const float a3 = row_ok ? base[int64_t(2 * M) * (t0 + 12 + slot)] : 0.f;
+    const float a0 = row_ok ? base[st * (t0 + slot)] : 0.f;
+    const float a1 = row_ok ? base[st * (t0 + 4 + slot)] : 0.f;
+    const float a2 = row_ok ? base[st * (t0 + 8 + slot)] : 0.f;
+    const float a3 = row_ok ? base[st * (t0 + 12 + slot)] : 0.f;
     acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, a0, acc0, 0, 0, 0);
     acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, a1, acc1, 0, 0, 0);
     acc2 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, a2, acc2, 0, 0, 0);
@@ -38,7 +52,7 @@ __global__ __launch_bounds__(256) void xcorr_mfma16_kernel(const float* __restri
   }
   for (; t0 < K; t0 += 4) {
     const int64_t t = t0 + slot;
-    const float a0 = (row_ok && t < K) ? base[int64_t(2 * M) * t] : 0.f;
+    const float a0 = (row_ok && t < K) ? base[st * t] : 0.f;
     acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, a0, acc0, 0, 0, 0);
   }
   // D layout: lane l holds column l % 16, rows 4*(l/16) + v.
@@ -71,7 +85,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 template <int M, int MODE>
 __global__ __launch_bounds__(256) void xcorr_mfma16_lp_kernel(const float* __restrict__ x, int64_t K, int64_t F,
-                                                              float2* __restrict__ out, float scale) {
+                                                              XLayout L, float2* __restrict__ out, float scale) {
   __shared__ float G[4][16 * 16];
   const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int64_t f = int64_t(blockIdx.x) * 4 + w;
@@ -79,14 +93,15 @@ __global__ __launch_bounds__(256) void xcorr_mfma16_lp_kernel(const float* __res
   const int r = lane % 16, grp = lane / 16;
   const int m = r % 8, part = r / 8;
   const bool row_ok = valid && m < M;
-  const float* base = x + (valid ? f : 0) * int64_t(2 * M) * K + 2 * m + part;
+  const float* base = x + 2 * L.channel(valid ? f : 0) + 2 * m + part;
+  const int64_t st = 2 * L.sk;
   floatx4 accH = {0.f, 0.f, 0.f, 0.f}, accX = accH;
   for (int64_t t0 = 0; t0 < K; t0 += 32) {
     float v[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const int64_t t = t0 + 8 * grp + i;
-      v[i] = (row_ok && t < K) ? base[int64_t(2 * M) * t] : 0.f;
+      v[i] = (row_ok && t < K) ? base[st * t] : 0.f;
     }
     if constexpr (MODE == PREC_BF16X3) {
       bf16x8 hi, lo;
@@ -160,23 +175,21 @@ using namespace alb;
 
 extern "C" {
 
+int alb_xcorr_all_ant_strided(const void* x, int64_t M, int64_t K, int64_t F, int64_t S, int64_t sk, int64_t sf,
+                              int64_t ss, void* out, float scale, int mode, void* stream);
+
 // correlations_gpu.py:95-121 avg_xcorr_all_ant_gpu (split = 1): out = x x^H * scale
 // per channel; scale is normally fl32(1/K).
 int alb_xcorr_all_ant(const void* x, int64_t M, int64_t K, int64_t F, void* out, float scale, void* stream) {
   ALB_REQUIRE(x && out, "alb_xcorr_all_ant: null pointer");
   ALB_REQUIRE(M >= 1 && K >= 0 && F >= 0, "alb_xcorr_all_ant: bad sizes");
   if (F == 0) return ALB_OK;
+  if (M <= 8) return alb_xcorr_all_ant_strided(x, M, K, F, 1, M, M * K, 0, out, scale, PREC_FP32, stream);
   hipStream_t s = as_stream(stream);
-  switch (M) {
-#define CASE(m) case m: hipLaunchKernelGGL(xcorr_mfma16_kernel<m>, dim3(unsigned((F + 3) / 4)), dim3(256), 0, s, (const float*)x, K, F, (float2*)out, scale); break;
-    CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
-#undef CASE
-    default:
-      ALB_REQUIRE(M * (M + 1) / 2 <= 4096, "alb_xcorr_all_ant: M too large");
-      ALB_REQUIRE(F <= 0x7fffffff, "alb_xcorr_all_ant: too many channels");
-      hipLaunchKernelGGL(xcorr_valu_kernel, dim3(unsigned(F)), dim3(256), 0, s, (const float2*)x, int(M), K, F,
-                         (float2*)out, scale);
-  }
+  ALB_REQUIRE(M * (M + 1) / 2 <= 4096, "alb_xcorr_all_ant: M too large");
+  ALB_REQUIRE(F <= 0x7fffffff, "alb_xcorr_all_ant: too many channels");
+  hipLaunchKernelGGL(xcorr_valu_kernel, dim3(unsigned(F)), dim3(256), 0, s, (const float2*)x, int(M), K, F,
+                     (float2*)out, scale);
   ALB_LAUNCH_CHECK("xcorr_all_ant");
   return ALB_OK;
 }
@@ -188,23 +201,44 @@ int alb_xcorr_all_ant_prec(const void* x, int64_t M, int64_t K, int64_t F, void*
                            void* stream) {
   ALB_REQUIRE(mode >= 0 && mode <= 3, "alb_xcorr_all_ant_prec: mode must be 0..3");
   if (mode == PREC_FP32) return alb_xcorr_all_ant(x, M, K, F, out, scale, stream);
-  ALB_REQUIRE(x && out, "alb_xcorr_all_ant_prec: null pointer");
-  ALB_REQUIRE(M >= 1 && K >= 0 && F >= 0, "alb_xcorr_all_ant_prec: bad sizes");
+  return alb_xcorr_all_ant_strided(x, M, K, F, 1, M, M * K, 0, out, scale, mode, stream);
+}
+
+// Strided input: job s, channel f, time k, input m at x[s*ss + f*sf + k*sk + m]; out as
+// alb_xcorr_all_ant over F*S channels (job-major), M <= 8.
+int alb_xcorr_all_ant_strided(const void* x, int64_t M, int64_t K, int64_t F, int64_t S, int64_t sk, int64_t sf,
+                              int64_t ss, void* out, float scale, int mode, void* stream) {
+  ALB_REQUIRE(mode >= 0 && mode <= 3, "alb_xcorr_all_ant_strided: mode must be 0..3");
+  ALB_REQUIRE(x && out, "alb_xcorr_all_ant_strided: null pointer");
+  ALB_REQUIRE(M >= 1 && K >= 0 && F >= 0 && S >= 1 && sk >= M && sf >= 0 && ss >= 0,
+              "alb_xcorr_all_ant_strided: bad sizes");
   if (M > 8) {
-    set_error("alb_xcorr_all_ant_prec: reduced-precision forms take M <= 8 (got %lld)", (long long)M);
+    set_error("alb_xcorr_all_ant_strided: the matrix-core forms take M <= 8 (got %lld)", (long long)M);
     return ALB_ENOTSUP;
   }
   if (F == 0) return ALB_OK;
+  const int64_t nch = F * S;
+  ALB_REQUIRE((nch + 3) / 4 <= 0x7fffffff, "alb_xcorr_all_ant_strided: too many channels");
   hipStream_t s = as_stream(stream);
-  const dim3 grid(unsigned((F + 3) / 4));
-#define CASE(m, md) case m: hipLaunchKernelGGL((xcorr_mfma16_lp_kernel<m, md>), grid, dim3(256), 0, s, (const float*)x, K, F, (float2*)out, scale); break;
+  const dim3 grid(unsigned((nch + 3) / 4));
+  const XLayout L{sk, sf, ss, F};
+  if (mode == PREC_FP32) {
+    switch (M) {
+#define CASE(m) case m: hipLaunchKernelGGL(xcorr_mfma16_kernel<m>, grid, dim3(256), 0, s, (const float*)x, K, nch, L, (float2*)out, scale); break;
+      CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+      default: break;
+    }
+  } else {
+#define CASE(m, md) case m: hipLaunchKernelGGL((xcorr_mfma16_lp_kernel<m, md>), grid, dim3(256), 0, s, (const float*)x, K, nch, L, (float2*)out, scale); break;
 #define MODES(md) switch (M) { CASE(1, md) CASE(2, md) CASE(3, md) CASE(4, md) CASE(5, md) CASE(6, md) CASE(7, md) CASE(8, md) default: break; }
-  if (mode == PREC_BF16X3) { MODES(PREC_BF16X3) }
-  else if (mode == PREC_FP16) { MODES(PREC_FP16) }
-  else { MODES(PREC_FP16X3) }
+    if (mode == PREC_BF16X3) { MODES(PREC_BF16X3) }
+    else if (mode == PREC_FP16) { MODES(PREC_FP16) }
+    else { MODES(PREC_FP16X3) }
 #undef MODES
 #undef CASE
-  ALB_LAUNCH_CHECK("xcorr_all_ant_prec");
+  }
+  ALB_LAUNCH_CHECK("xcorr_all_ant_strided");
   return ALB_OK;
 }
 

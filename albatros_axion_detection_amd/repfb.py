@@ -29,6 +29,8 @@ from . import pfb as _pfb
 
 # largest re-PFB workspace (all blocks of one input at once) the fused path allocates
 FUSED_WORKSPACE_LIMIT = 16 << 30
+# stage-1 workspaces of every input kept at once for one multi-input stage 2
+INPUTS_WORKSPACE_LIMIT = 64 << 30
 
 
 def get_chunkier(nchunks, lchunk, nblock, lblock, ntap):
@@ -138,6 +140,7 @@ class RePFBPipeline:
         self.win = _lib.to_device(win, t.float32) if win is not None else _pfb.window(ntap, self.lblock)
         self._tiles = None
         self._ws = None
+        self._xin_f = None
         # row pass: "fft" (2048-point FFT per spectrum) or "mfma" (matrix-core
         # synthesis from the ncols non-zero channels); with a power-of-two
         # nslice both write the blocked tile layout, which the column pass reads
@@ -189,12 +192,21 @@ class RePFBPipeline:
             self._tiles = t.empty(need, dtype=t.float32, device=_lib.device())
         return self._tiles
 
-    def _workspace(self, nblk):
+    def _workspace(self, nblk, ninput=1):
+        """Stage-1 workspace of ``ninput`` inputs of ``nblk`` blocks each
+        (input m at element m * per); returns (tensor, per)."""
         t = _lib.torch()
-        need = int(_lib.load().alb_repfb_workspace_bytes(nblk, self.osamp)) // 8
+        per = int(_lib.load().alb_repfb_workspace_bytes(nblk, self.osamp)) // 8
+        need = per * ninput
         if self._ws is None or self._ws.numel() < need:
             self._ws = t.empty(need, dtype=t.complex64, device=_lib.device())
-        return self._ws
+        return self._ws, per
+
+    def inputs_stage2_supported(self, nblocks_total):
+        """True when every input's stage-1 workspace fits in
+        ``INPUTS_WORKSPACE_LIMIT`` at once (one stage-2 launch for all inputs)."""
+        ws = int(_lib.load().alb_repfb_workspace_bytes(nblocks_total, self.osamp))
+        return self.fused_supported(nblocks_total) and ws * self.ninput <= INPUTS_WORKSPACE_LIMIT
 
     # --------------------------------------------------------------- stages
     def ipfb_stream(self, packed, rowmap, nchunks, stream_out):
@@ -222,7 +234,7 @@ class RePFBPipeline:
         s = _lib.stream()
         for b0 in range(0, nblocks_total, block_batch):
             nb = min(block_batch, nblocks_total - b0)
-            ws = self._workspace(nb)
+            ws, _ = self._workspace(nb)
             ts = stream_rows[b0 * self.osamp:]
             # the batch's first spectrum is global block b0: shift the output base
             j0, r0 = divmod(b0, bpj)
@@ -270,6 +282,15 @@ class RePFBPipeline:
             self.ipfb_stream(packed, rowmap, nchunks, stream_out)
             self.repfb(stream_out, nblocks_total, out, sb, sc, bpj, sj, block_batch=max(1, 512 // bpj) * bpj)
             return
+        ws, _ = self._workspace(nblocks_total)
+        self.stream_stage1(packed, rowmap, nchunks, stream_out, nblocks_total, ws)
+        _lib.call("alb_repfb_stage2", _lib.ptr(ws), nblocks_total, self.osamp, self.c_lo, self.c_hi, _lib.ptr(out),
+                  sb, sc, bpj, sj, _lib.stream())
+
+    def stream_stage1(self, packed, rowmap, nchunks, stream_out, nblocks_total, ws):
+        """Fused form up to re-PFB stage 1: every block's stage-1 result of one
+        input in the workspace ``ws`` (row pass, column pass with stage 1 of
+        the interior blocks, then the edge blocks from the stream buffer)."""
         if stream_out.shape[0] < max((nblocks_total + self.ntap - 1) * self.osamp, nchunks * self.acclen):
             raise ValueError("stream buffer too short for the re-PFB blocks")
         s = _lib.stream()
@@ -278,7 +299,6 @@ class RePFBPipeline:
         if total > rows_valid:
             tail = stream_out[rows_valid:]
             _lib.call("alb_fill_zero", _lib.ptr(tail), tail.numel() * 4, s)
-        ws = self._workspace(nblocks_total)
         tb = self.tile_batch
         y = self._tile_buffer(min(tb, nchunks))
         for t0 in range(0, nchunks, tb):
@@ -290,8 +310,6 @@ class RePFBPipeline:
         edges = self.edge_blocks(nchunks, nblocks_total)
         _lib.call("alb_repfb_stage1_list", _lib.ptr(stream_out), _lib.ptr(self.win), self.ntap,
                   edges.ctypes.data_as(ctypes.c_void_p), len(edges), self.osamp, _lib.ptr(ws), s)
-        _lib.call("alb_repfb_stage2", _lib.ptr(ws), nblocks_total, self.osamp, self.c_lo, self.c_hi, _lib.ptr(out),
-                  sb, sc, bpj, sj, s)
 
     # ------------------------------------------------------------ pipelines
     def spectra(self, packed, rowmap, nchunks, fused=True):
@@ -317,12 +335,17 @@ class RePFBPipeline:
         M, K, F = self.ninput, self.nblock, self.nfine
         dev = _lib.device()
         bufs = {"jobs": jobs, "nbt": nbt, "rows": rows,
-                # F-order (M, K, F, S) == C-order [S][F][K][M]
-                "xin": t.empty((len(jobs), F, K, M), dtype=t.complex64, device=dev),
+                # correlator input [S][K][F][M]: a re-PFB block's channels of all
+                # inputs are one contiguous run (whole-line stage-2 stores); the
+                # correlator takes it through its strided form
+                "xin": t.empty((len(jobs), K, F, M), dtype=t.complex64, device=dev),
                 "stream": t.empty((rows, 4096), dtype=t.float32, device=dev),
                 "vis": t.empty((len(jobs), F, M, M), dtype=t.complex64, device=dev)}
         self._tile_buffer(min(self.tile_batch, nchunks))
-        self._workspace(nbt if self.fused_supported(nbt) else max(1, 512 // K) * K)
+        if self.inputs_stage2_supported(nbt):
+            self._workspace(nbt, M)
+        else:
+            self._workspace(nbt if self.fused_supported(nbt) else max(1, 512 // K) * K)
         return bufs
 
     def correlate_into(self, packed_list, rowmaps, nchunks, bufs, fused=True, precision="fp32"):
@@ -334,16 +357,25 @@ class RePFBPipeline:
         xin, stream, vis, nbt = bufs["xin"], bufs["stream"], bufs["vis"], bufs["nbt"]
         njobs = len(bufs["jobs"])
         bb = max(1, 512 // K) * K
-        for m in range(M):
-            if fused:
-                self.stream_repfb(packed_list[m], rowmaps[m], nchunks, stream, nbt, xin.view(-1)[m:], M, M * K, K,
-                                  M * K * F)
-            else:
-                self.ipfb_stream(packed_list[m], rowmaps[m], nchunks, stream)
-                self.repfb(stream, nbt, xin.view(-1)[m:], M, M * K, K, M * K * F, block_batch=bb)
+        if fused and self.inputs_stage2_supported(nbt):
+            # stage 1 of every input into its own workspace, then one stage-2
+            # launch writes all inputs' channels into the correlator layout
+            ws, per = self._workspace(nbt, M)
+            for m in range(M):
+                self.stream_stage1(packed_list[m], rowmaps[m], nchunks, stream, nbt, ws[m * per:])
+            _lib.call("alb_repfb_stage2_inputs", _lib.ptr(ws), per * 8, M, nbt, self.osamp, self.c_lo, self.c_hi,
+                      _lib.ptr(xin), 1, F * M, M, K, K * F * M, _lib.stream())
+        else:
+            for m in range(M):
+                if fused:
+                    self.stream_repfb(packed_list[m], rowmaps[m], nchunks, stream, nbt, xin.view(-1)[m:], F * M, M,
+                                      K, K * F * M)
+                else:
+                    self.ipfb_stream(packed_list[m], rowmaps[m], nchunks, stream)
+                    self.repfb(stream, nbt, xin.view(-1)[m:], F * M, M, K, K * F * M, block_batch=bb)
         scale = np.float32(1.0) / np.float32(K)
-        _lib.call("alb_xcorr_all_ant_prec", _lib.ptr(xin), M, K, F * njobs, _lib.ptr(vis), float(scale),
-                  PRECISION_MODES[precision], _lib.stream())
+        _lib.call("alb_xcorr_all_ant_strided", _lib.ptr(xin), M, K, F, njobs, F * M, M, K * F * M, _lib.ptr(vis),
+                  float(scale), PRECISION_MODES[precision], _lib.stream())
         return vis.permute(3, 2, 1, 0)
 
     def correlate_per_chunk(self, packed_list, rowmaps, nchunks, bufs):
@@ -352,11 +384,16 @@ class RePFBPipeline:
         row pass and one column pass, per job and input one re-PFB launch, per
         job one correlator launch.  Same result as :meth:`correlate_into`."""
         M, K, F = self.ninput, self.nblock, self.nfine
-        xin, stream, vis = bufs["xin"], bufs["stream"], bufs["vis"]
+        stream, vis = bufs["stream"], bufs["vis"]
         njobs = len(bufs["jobs"])
+        # the reference's per-job correlator input: F-order (M, K, F) per job
+        t = _lib.torch()
+        if self._xin_f is None or tuple(self._xin_f.shape) != (njobs, F, K, M):
+            self._xin_f = t.empty((njobs, F, K, M), dtype=t.complex64, device=_lib.device())
+        xin = self._xin_f
         s = _lib.stream()
         y = self._tile_buffer(1)
-        ws = self._workspace(K)
+        ws, _ = self._workspace(K)
         rows_valid = nchunks * self.acclen
         scale = np.float32(1.0) / np.float32(K)
         for m in range(M):
@@ -383,7 +420,7 @@ class RePFBPipeline:
         bufs = self.buffers(nchunks)
         res = self.correlate_into(packed_list, rowmaps, nchunks, bufs, fused=fused)
         if return_xin:
-            return res, bufs["xin"].permute(3, 2, 1, 0)
+            return res, bufs["xin"].permute(3, 1, 2, 0)
         return res
 
 

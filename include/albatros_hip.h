@@ -89,6 +89,13 @@ int alb_xcorr_all_ant(const void* x, int64_t M, int64_t K, int64_t F, void* out,
  * 2 FP16, 3 FP16 hi/lo split; modes 1-3 take M <= 8 (ENOTSUP otherwise). */
 int alb_xcorr_all_ant_prec(const void* x, int64_t M, int64_t K, int64_t F, void* out, float scale, int mode,
                            void* stream);
+/* The matrix-core correlator on a strided input (no counterpart in the reference, whose
+ * cgemm_strided_batched takes the F-order (M, K, F) block per job): S jobs of F channels,
+ * element (input m, time k, channel f, job s) at x[s*ss + f*sf + k*sk + m] (complex64),
+ * out as alb_xcorr_all_ant over the F*S channels (job-major).  M <= 8; mode as
+ * alb_xcorr_all_ant_prec.  The batched re-PFB writes [S][K][F][M] (sk = F*M, sf = M). */
+int alb_xcorr_all_ant_strided(const void* x, int64_t M, int64_t K, int64_t F, int64_t S, int64_t sk, int64_t sf,
+                              int64_t ss, void* out, float scale, int mode, void* stream);
 
 /* ---- PFB / inverse PFB (src/utils/pfb_gpu_utils.py, pycufft.py/.cu) ---- */
 /* pfb_gpu_utils.py:123 get_matft + :29 calculate_filter in float64: win float32 [ntap*lblock];
@@ -156,6 +163,14 @@ int alb_repfb_stage1(const float* ts, const float* win, int ntap, int64_t nblock
                      void* stream);
 int alb_repfb_stage2(const void* A, int64_t nblock, int64_t osamp, int64_t c_lo, int64_t c_hi, void* out, int64_t sb,
                      int64_t sc, int64_t bpj, int64_t sj, void* stream);
+/* alb_repfb_stage2 for ninput inputs in one launch (no counterpart in the reference, which runs
+ * cupy_pfb per input, helper_gpu_stream.py:222): input m's workspace at A + m*a_stride bytes
+ * (a_stride >= alb_repfb_workspace_bytes(nblock, osamp)), its channels at out + m*sm complex
+ * elements plus the alb_repfb_stage2 addressing.  With the correlator layout (sm = 1) the inputs'
+ * neighbouring elements are written by neighbouring workgroups. */
+int alb_repfb_stage2_inputs(const void* A, int64_t a_stride, int64_t ninput, int64_t nblock, int64_t osamp,
+                            int64_t c_lo, int64_t c_hi, void* out, int64_t sm, int64_t sb, int64_t sc, int64_t bpj,
+                            int64_t sj, void* stream);
 /* The iPFB column pass (alb_ipfb_columns, ncols = 4096, row stride 4096) fused with re-PFB stage 1:
  * the time stream is the input of pfb_gpu_utils.py:105 cupy_pfb at lblock = 4096*osamp
  * (helper_gpu_stream.py:199-222).  Kept row i of tile c is stream row
@@ -185,7 +200,7 @@ int alb_build_rowmap(const int64_t* specnum, int64_t npresent, int64_t start, in
 
 /* ---- tuning (no counterpart in the reference) ---------------------------- */
 /* Set a launch-shape knob; 0 restores the built-in choice.  Keys:
- *   "ipfb_cols_width"  columns per workgroup of the nslice = 4096 column pass (2, 4 or 8);
+ *   "ipfb_cols_width"  columns per workgroup of the nslice = 4096 column pass (2, 4 or 8, default 4);
  *   "ipfb_cols_group"  column strips per XCD group of the column-pass order (2..64, default 32);
  *   "ipfb_rows_group"  spectra per workgroup of the packed 4-bit row pass (2 or 4, default 2);
  *   "xcorr_int_method" all-pairs integer correlators: 1 VALU (XOR form for 1-bit),

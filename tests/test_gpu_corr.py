@@ -194,3 +194,29 @@ def test_xcorr_precision_rejects_large_m(cr):
     x = synth.precision_xcorr(M=10, T=16, F=2, seed=0)
     with pytest.raises(NotImplementedError):
         cr.avg_xcorr_all_ant_gpu(x, 10, 1, 16, 2, precision="fp16")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 1, 3])
+def test_strided_correlator_matches_f_order(rng, mode):
+    # [S][K][F][M] input through the strided entry == the same data in the
+    # reference's F-order (M, K, F) per job through alb_xcorr_all_ant(_prec)
+    import torch
+    from albatros_axion_detection_amd import _lib
+    M, K, F, S = 4, 100, 37, 3
+    x = (rng.standard_normal((S, K, F, M)) + 1j * rng.standard_normal((S, K, F, M))).astype(np.complex64)
+    xs = _lib.to_device(x)
+    xf = _lib.to_device(np.ascontiguousarray(x.transpose(0, 2, 1, 3)))       # [S][F][K][M]
+    a = torch.empty((S, F, M, M), dtype=torch.complex64, device=xs.device)
+    b = torch.empty_like(a)
+    scale = float(np.float32(1.0) / np.float32(K))
+    print("[gpu-case] strided correlator mode", mode, flush=True)
+    _lib.call("alb_xcorr_all_ant_strided", _lib.ptr(xs), M, K, F, S, F * M, M, K * F * M, _lib.ptr(a), scale, mode,
+              _lib.stream())
+    _lib.call("alb_xcorr_all_ant_prec", _lib.ptr(xf), M, K, F * S, _lib.ptr(b), scale, mode, _lib.stream())
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    ref = np.einsum("skfi,skfj->sfji", x.astype(np.complex128), np.conj(x.astype(np.complex128))) / K
+    got = a.cpu().numpy()
+    tol = {0: 2e-6, 1: 1e-5, 3: 2e-6}[mode]      # FP32, BF16 hi/lo, FP16 hi/lo
+    assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < tol

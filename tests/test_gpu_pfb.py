@@ -398,3 +398,38 @@ def test_row_pass_mfma_matches_fft(rng, nslice, chans):
     assert ea < 5e-6 and eb < 5e-6
     # tile 0's carried rows and the missing spectra are exact zeros
     assert np.all(b[0, :2 * cut] == 0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("osamp", [64, 8])
+def test_repfb_stage2_inputs_matches_per_input(rng, osamp):
+    # one launch over several inputs' stage-1 workspaces writes the same
+    # channels as a stage-2 launch per input (same arithmetic, bit-identical)
+    import torch
+    from albatros_axion_detection_amd import _lib
+    M, nblock, bpj, ntap = 3, 10, 5, 4
+    c_lo, c_hi = 100 * osamp, 140 * osamp + 3
+    F = c_hi - c_lo
+    win = _lib.to_device(np.asarray(rng.standard_normal(ntap * 4096 * osamp), dtype=np.float32))
+    per = int(_lib.load().alb_repfb_workspace_bytes(nblock, osamp)) // 8
+    ws = torch.empty(M * per, dtype=torch.complex64, device=_lib.device())
+    s = _lib.stream()
+    for m in range(M):
+        ts = _lib.to_device(np.asarray(rng.standard_normal(((nblock + ntap - 1) * osamp, 4096)), dtype=np.float32))
+        _lib.call("alb_repfb_stage1", _lib.ptr(ts), _lib.ptr(win), ntap, nblock, osamp, _lib.ptr(ws[m * per:]), s)
+    # correlator layout [job][F][K][M] (sm = 1, sb = M, sc = M K, sj = M K F)
+    njobs = nblock // bpj
+    one = torch.zeros((njobs, F, bpj, M), dtype=torch.complex64, device=ws.device)
+    each = torch.zeros_like(one)
+    print("[gpu-case] stage2 inputs osamp", osamp, flush=True)
+    _lib.call("alb_repfb_stage2_inputs", _lib.ptr(ws), per * 8, M, nblock, osamp, c_lo, c_hi, _lib.ptr(one), 1, M,
+              M * bpj, bpj, M * bpj * F, s)
+    for m in range(M):
+        _lib.call("alb_repfb_stage2", _lib.ptr(ws[m * per:]), nblock, osamp, c_lo, c_hi, _lib.ptr(each.view(-1)[m:]),
+                  M, M * bpj, bpj, M * bpj * F, s)
+    torch.cuda.synchronize()
+    assert torch.count_nonzero(one).item() > 0.9 * one.numel()
+    assert torch.equal(one, each)
+    with pytest.raises(ValueError):
+        _lib.call("alb_repfb_stage2_inputs", _lib.ptr(ws), 8, M, nblock, osamp, c_lo, c_hi, _lib.ptr(one), 1, M,
+                  M * bpj, bpj, M * bpj * F, s)
