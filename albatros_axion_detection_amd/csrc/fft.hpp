@@ -134,9 +134,12 @@ __device__ __forceinline__ constexpr int pad_step() { return K % 16 == 0 ? K + K
 // Butterfly (f, j) reads j + r NB and writes d + r NS, d = (j / NS) NS R + j % NS.
 // With NB (and NS) multiples of 16, or NS = 1 (d a multiple of R, R <= 16),
 // every padded address is one base plus compile-time offsets.
-template <int N, int R, int NS, int F, int T, bool TAB = false>
+// TWC 1: also keep this pass's twiddles w[1..R-1] in twc; TWC 2: take them from
+// twc (a later transform of the same size on the same thread mapping, one
+// butterfly per thread).
+template <int N, int R, int NS, int F, int T, bool TAB = false, int TWC = 0>
 __device__ __forceinline__ void fft_pass(float2* lds, const float2* A, const float2* B, int lb, int tsh, int tid,
-                                         const float2* W16 = nullptr) {
+                                         const float2* W16 = nullptr, float2* twc = nullptr) {
   constexpr int NB = N / R;
   constexpr int TOT = NB * F;
   constexpr int PER = (TOT + T - 1) / T;
@@ -172,6 +175,10 @@ __device__ __forceinline__ void fft_pass(float2* lds, const float2* A, const flo
         const float2* wt = W16 + (j % 16) * kW16Row;
 #pragma unroll
         for (int r = 1; r < R; ++r) v[t][r] = cmul(v[t][r], wt[r]);
+      } else if constexpr (NS > 1 && TWC == 2) {
+        static_assert(PER == 1, "cached twiddles need one butterfly per thread");
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[t][r] = cmul(v[t][r], twc[r - 1]);
       } else if constexpr (NS > 1) {
         // W^(r*base) for r = 1..R-1 from the looked-up powers W^base, W^2base,
         // W^4base, W^8base (products of at most four factors)
@@ -186,6 +193,11 @@ __device__ __forceinline__ void fft_pass(float2* lds, const float2* A, const flo
         for (int r = 1; r < R; ++r) {
           const int hi = 31 - __builtin_clz(r);  // highest set bit (compile time after unrolling)
           w[r] = (r == (1 << hi)) ? wp[hi] : cmul(w[r - (1 << hi)], wp[hi]);
+        }
+        if constexpr (TWC == 1) {
+          static_assert(PER == 1, "cached twiddles need one butterfly per thread");
+#pragma unroll
+          for (int r = 1; r < R; ++r) twc[r - 1] = w[r];
         }
 #pragma unroll
         for (int r = 1; r < R; ++r) v[t][r] = cmul(v[t][r], w[r]);
@@ -217,14 +229,15 @@ struct PickRadix {
 // the caller finishes the last pass itself, e.g. for only some output bins).
 // The caller synchronises before the call; the call ends with a barrier.
 // With TAB the radix-16 pass at sub-transform size 16 takes its twiddles from
-// W16 (fill_w16).
-template <int N, int F, int T, int NS = 1, int NSE = N, bool TAB = false>
+// W16 (fill_w16).  TWC (see fft_pass) applies to the pass at sub-transform
+// size 256.
+template <int N, int F, int T, int NS = 1, int NSE = N, bool TAB = false, int TWC = 0>
 __device__ __forceinline__ void fft_lds(float2* lds, const float2* A, const float2* B, int lb, int tsh, int tid,
-                                        const float2* W16 = nullptr) {
+                                        const float2* W16 = nullptr, float2* twc = nullptr) {
   if constexpr (NS < NSE) {
     constexpr int R = PickRadix<N / NS>::value;
-    fft_pass<N, R, NS, F, T, TAB>(lds, A, B, lb, tsh, tid, W16);
-    fft_lds<N, F, T, NS * R, NSE, TAB>(lds, A, B, lb, tsh, tid, W16);
+    fft_pass<N, R, NS, F, T, TAB, (NS == 256 && R == 16) ? TWC : 0>(lds, A, B, lb, tsh, tid, W16, twc);
+    fft_lds<N, F, T, NS * R, NSE, TAB, TWC>(lds, A, B, lb, tsh, tid, W16, twc);
   }
 }
 

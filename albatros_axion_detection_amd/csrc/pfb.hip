@@ -455,7 +455,11 @@ __device__ __forceinline__ void cols_core(const float* __restrict__ yt, int64_t 
     }
   }
   __syncthreads();
-  fft_lds<NSL, NP, T, 16, NSL, true>(U, TA, TB, lb, 0, tid, W16);
+  // the sub-transform-256 twiddles of this thread are the same in the inverse
+  // transform: kept in registers when a thread owns one butterfly per pass
+  constexpr bool KEEP = (NSL / 16) * NP == T && NSL >= 4096;
+  float2 tw256[15];
+  fft_lds<NSL, NP, T, 16, NSL, true, KEEP ? 1 : 0>(U, TA, TB, lb, 0, tid, W16, tw256);
   // filter on the real-pair spectra, stored conjugated for the inverse
   const float sc = 1.0f / float(NSL);
   float2* Uf = U + fp * PADN;
@@ -483,7 +487,7 @@ __device__ __forceinline__ void cols_core(const float* __restrict__ yt, int64_t 
     }
   }
   __syncthreads();
-  fft_lds<NSL, NP, T, 1, NSL, true>(U, TA, TB, lb, 0, tid, W16);
+  fft_lds<NSL, NP, T, 1, NSL, true, KEEP ? 2 : 0>(U, TA, TB, lb, 0, tid, W16, tw256);
 }
 
 template <int NSL, int CW, int T>
