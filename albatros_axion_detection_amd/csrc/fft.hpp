@@ -176,7 +176,7 @@ __device__ __forceinline__ void fft_pass(float2* lds, const float2* A, const flo
 #pragma unroll
         for (int r = 1; r < R; ++r) v[t][r] = cmul(v[t][r], wt[r]);
       } else if constexpr (NS > 1 && TWC == 2) {
-        static_assert(PER == 1, "cached twiddles need one butterfly per thread");
+        static_assert(T % NB == 0, "cached twiddles need every butterfly of a thread at one j");
 #pragma unroll
         for (int r = 1; r < R; ++r) v[t][r] = cmul(v[t][r], twc[r - 1]);
       } else if constexpr (NS > 1) {
@@ -195,7 +195,7 @@ __device__ __forceinline__ void fft_pass(float2* lds, const float2* A, const flo
           w[r] = (r == (1 << hi)) ? wp[hi] : cmul(w[r - (1 << hi)], wp[hi]);
         }
         if constexpr (TWC == 1) {
-          static_assert(PER == 1, "cached twiddles need one butterfly per thread");
+          static_assert(PER == 1, "storing cached twiddles needs one butterfly per thread");
 #pragma unroll
           for (int r = 1; r < R; ++r) twc[r - 1] = w[r];
         }
@@ -236,9 +236,31 @@ __device__ __forceinline__ void fft_lds(float2* lds, const float2* A, const floa
                                         const float2* W16 = nullptr, float2* twc = nullptr) {
   if constexpr (NS < NSE) {
     constexpr int R = PickRadix<N / NS>::value;
-    fft_pass<N, R, NS, F, T, TAB, (NS == 256 && R == 16) ? TWC : 0>(lds, A, B, lb, tsh, tid, W16, twc);
+    fft_pass<N, R, NS, F, T, TAB, NS == 256 ? TWC : 0>(lds, A, B, lb, tsh, tid, W16, twc);
     fft_lds<N, F, T, NS * R, NSE, TAB, TWC>(lds, A, B, lb, tsh, tid, W16, twc);
   }
+}
+
+// Twiddles w[r - 1] = W^(r base), r = 1..R-1, of the radix-R pass at
+// sub-transform size NS for butterfly index j = tid % (N / R) (the values that
+// fft_pass computes), for use with TWC = 2.
+template <int N, int R, int NS>
+__device__ __forceinline__ void pass_twiddles(const float2* A, const float2* B, int lb, int tsh, int tid, float2* twc) {
+  constexpr int NB = N / R;
+  constexpr int STEP = N / (NS * R);
+  const int base = int(((unsigned(tid) % NB % NS) * STEP) << tsh);
+  float2 wp[4];
+#pragma unroll
+  for (int e = 0; (1 << e) < R; ++e) wp[e] = tw_lookup(A, B, lb, base << e);
+  float2 w[R];
+  w[0] = make_float2(1.f, 0.f);
+#pragma unroll
+  for (int r = 1; r < R; ++r) {
+    const int hi = 31 - __builtin_clz(r);
+    w[r] = (r == (1 << hi)) ? wp[hi] : cmul(w[r - (1 << hi)], wp[hi]);
+  }
+#pragma unroll
+  for (int r = 1; r < R; ++r) twc[r - 1] = w[r];
 }
 
 // W16[17 m + r] = W_256^(m r) (m, r < 16; rows of 17 so that the 16 rows

@@ -215,6 +215,10 @@ __global__ __launch_bounds__(TT, 1024 / TT) void irfft_packed4_kernel(Packed4Row
     return int(*a);
   };
   __syncthreads();  // twiddles
+  // this thread's twiddles of the last (radix-8, sub-transform 256) pass: the
+  // same in every group
+  float2 tw8[7];
+  pass_twiddles<M, 8, 256>(TA, TB, tw.lb, 1, tid, tw8);
   // item constants: weights 1 + i conj(w), 1 + i w and the float offsets of Z[ch], Z[M-ch]
   float2 ga[kRowItems], gb[kRowItems];
   int oa[kRowItems], ob[kRowItems];
@@ -280,7 +284,7 @@ __global__ __launch_bounds__(TT, 1024 / TT) void irfft_packed4_kernel(Packed4Row
       }
     }
     __syncthreads();
-    fft_lds<M, F, TT, 1, M, true>(Z, TA, TB, tw.lb, 1, tl, W16);
+    fft_lds<M, F, TT, 1, M, true, 2>(Z, TA, TB, tw.lb, 1, tl, W16, tw8);
     irfft_store<M, F, TT>(Z, g * F, nrows, nslice, out, ldo, ldo == 8, tl);
   };
   // Two sets of prefetch registers used in turn (no copies of values still in
