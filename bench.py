@@ -426,7 +426,17 @@ def batched_stream_leg(args, torch):
         return a.elapsed_time(b) / n
 
     ms_graph = timed(g.replay, 3)
-    ms_chunk = timed(run_p, 2)
+    # the per-chunk form at its own best column-pass width (the sweep above
+    # tuned the fused pass; the per-chunk form runs the plain column pass)
+    lib = _lib.load()
+    tuned_w = int(lib.alb_tune_get(b"ipfb_cols_width"))
+    chunk_ms = {}
+    for w in (2, 4, 8):
+        _lib.call("alb_tune_set", b"ipfb_cols_width", w)
+        run_p()
+        chunk_ms[w] = timed(run_p, 2)
+    _lib.call("alb_tune_set", b"ipfb_cols_width", tuned_w)
+    ms_chunk = min(chunk_ms.values())
     nsamp = 4 * nfixed * acclen * 4096
     per_spec = 300 + 2 * 16384 + 2 * 19200 * 8 // 64       # raw + time stream + kept fine channels, per input
     hbm = 4 * nfixed * acclen * per_spec
@@ -440,6 +450,7 @@ def batched_stream_leg(args, torch):
             "per_chunk_Msamples_s": round(nsamp / (ms_chunk * 1e-3) / 1e6, 1),
             "hbm_frac": round(hbm / (ms_graph * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "batched_correlator": "fp16x3", "per_chunk_correlator": "fp32",
+            "per_chunk_ms_by_cols_width": {str(w): round(v, 3) for w, v in chunk_ms.items()},
             "max_abs_diff_over_rms_vs_per_chunk": err}
 
 
