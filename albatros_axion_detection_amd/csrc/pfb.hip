@@ -76,19 +76,25 @@ __device__ __forceinline__ void irfft_store(const float2* Z, int64_t row0, int64
   constexpr int N = 2 * M;
   constexpr int MP = Padded<M>::value;
   const float sc = 1.0f / float(N);
-  if constexpr (M % TT == 0) {
+  if constexpr (M % TT == 0 && TT % (4 * F) == 0) {
     if (blocked) {
-      // thread tid stores n = tid + TT j: block n / 4 = tid / 4 + (TT / 4) j, slot n % 4 = tid % 4
-#pragma unroll
-      for (int f = 0; f < F; ++f) {
-        const int64_t row = row0 + f;
-        if (row >= nrows) break;
+      // Lanes 4F apart cover one block (8 columns) of all F rows, which are
+      // adjacent in the layout, so a store instruction writes runs of 32 F
+      // contiguous bytes: thread (b0, f, c) = (tid / 4F, (tid / 4) % F, tid % 4)
+      // stores column pair n = 4 (b0 + (TT / 4F) j) + c of row row0 + f.
+      constexpr int LB = 4 * F;               // lanes per block
+      constexpr int BPS = TT / LB;            // blocks per store instruction
+      static_assert((M / 4) % BPS == 0, "blocks must split evenly over the store instructions");
+      const int c = tid & 3, f = (tid >> 2) % F, b0 = tid / LB;
+      const int64_t row = row0 + f;
+      if (row < nrows) {
         const int64_t t = row / nslice, i = row - t * nslice;
-        float* base = out + t * int64_t(N) * nslice + i * 8 + int64_t(tid >> 2) * 8 * nslice + (tid & 3) * 2;
+        float* base = out + t * int64_t(N) * nslice + i * 8 + int64_t(b0) * 8 * nslice + c * 2;
+        const float2* zf = Z + f * MP;
 #pragma unroll
-        for (int j = 0; j < M / TT; ++j) {
-          const float2 v = Z[f * MP + pad16(tid + TT * j)];
-          *reinterpret_cast<float2*>(base + int64_t(j) * (TT / 4) * 8 * nslice) = make_float2(v.x * sc, -v.y * sc);
+        for (int j = 0; j < (M / 4) / BPS; ++j) {
+          const float2 v = zf[pad16(4 * (b0 + BPS * j) + c)];
+          *reinterpret_cast<float2*>(base + int64_t(j) * BPS * 8 * nslice) = make_float2(v.x * sc, -v.y * sc);
         }
       }
       return;
@@ -219,9 +225,22 @@ __global__ __launch_bounds__(TT, 1024 / TT) void irfft_packed4_kernel(Packed4Row
   // same in every group
   float2 tw8[7];
   pass_twiddles<M, 8, 256>(TA, TB, tw.lb, 1, tid, tw8);
+  // Channel c contributes to Z[c] and Z[M - c]; a bin with one contributor
+  // (no repeat of c, no channel M - c) is written with a plain store, the rest
+  // are summed with LDS atomics, which cost about ten times a store.  The
+  // channel counts live in Z until the first group clears it.
+  int* cnt = reinterpret_cast<int*>(Z);
+  for (int idx = tid; idx <= M; idx += TT) cnt[idx] = 0;
+  __syncthreads();
+  for (int c = tid; c < ncols; c += TT) {
+    const int h = pk.chan[c];
+    if (h >= 0 && h <= M) atomicAdd(&cnt[h], 1);
+  }
+  __syncthreads();
   // item constants: weights 1 + i conj(w), 1 + i w and the float offsets of Z[ch], Z[M-ch]
   float2 ga[kRowItems], gb[kRowItems];
   int oa[kRowItems], ob[kRowItems];
+  bool own[kRowItems];
 #pragma unroll
   for (int k = 0; k < kRowItems; ++k) {
     const int c = ch[k] < 0 ? 0 : ch[k];
@@ -230,6 +249,7 @@ __global__ __launch_bounds__(TT, 1024 / TT) void irfft_packed4_kernel(Packed4Row
     gb[k] = make_float2(1.f - w.y, w.x);
     oa[k] = fi[k] * 2 * MP + 2 * pad16(c);
     ob[k] = fi[k] * 2 * MP + 2 * pad16(M - c);
+    own[k] = ch[k] >= 0 && ch[k] <= M && cnt[c] == 1 && (2 * c == M || cnt[M - c] == 0);
   }
   // one row group: zero fill, channel scatter, inverse transform, store
   auto transform = [&](int64_t g, const int* bc, const bool* vc) {
@@ -246,15 +266,24 @@ __global__ __launch_bounds__(TT, 1024 / TT) void irfft_packed4_kernel(Packed4Row
         const int c = ch[k];
         float2 x = make_float2(float(nib4f(bc[k] >> 4)), float(nib4f(bc[k] & 15)));
         if (c == 0 || c == M) x.y = 0.f;
-        if (c < M) {
-          const float2 v = cmul(x, ga[k]);
-          atomicAdd(&Zr[oa[k]], v.x);
-          atomicAdd(&Zr[oa[k] + 1], -v.y);
-        }
-        if (c > 0) {
-          const float2 v = cmul(cconj(x), gb[k]);
-          atomicAdd(&Zr[ob[k]], v.x);
-          atomicAdd(&Zr[ob[k] + 1], -v.y);
+        const float2 va = cmul(x, ga[k]);
+        const float2 vb = cmul(cconj(x), gb[k]);
+        if (own[k]) {
+          if (2 * c == M) {
+            *reinterpret_cast<float2*>(&Zr[oa[k]]) = make_float2(va.x + vb.x, -va.y + -vb.y);
+          } else {
+            if (c < M) *reinterpret_cast<float2*>(&Zr[oa[k]]) = make_float2(va.x, -va.y);
+            if (c > 0) *reinterpret_cast<float2*>(&Zr[ob[k]]) = make_float2(vb.x, -vb.y);
+          }
+        } else {
+          if (c < M) {
+            atomicAdd(&Zr[oa[k]], va.x);
+            atomicAdd(&Zr[oa[k] + 1], -va.y);
+          }
+          if (c > 0) {
+            atomicAdd(&Zr[ob[k]], vb.x);
+            atomicAdd(&Zr[ob[k] + 1], -vb.y);
+          }
         }
       }
     }

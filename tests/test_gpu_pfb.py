@@ -336,13 +336,16 @@ def test_batched_stream_graph_and_per_chunk_forms():
 @pytest.mark.parametrize("nslice,chans", [(256, [0, 5, 2048, 1000, 1001, 1700, 7, 8, 9, 10, 11, 12, 13, 14, 15,
                                                   16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31,
                                                   32, 33, 34, 35, 36, 2047, 5]),
-                                           (4096, list(range(64, 164)))])
+                                           (4096, list(range(64, 164))),
+                                           (4096, [1024, 1000, 1048, 0, 2048, 5] + list(range(100, 400)) + [2043])])
 def test_row_pass_mfma_matches_fft(rng, nslice, chans):
     # matrix-core synthesis of the iPFB row pass (blocked tile layout) against
     # the FFT row pass; DC (0) and Nyquist (2048) channels drop their imaginary
     # parts in both; ncols not a multiple of 16 exercises the padded k-steps; a
     # repeated channel number (5) keeps its last column, as numpy's fancy
-    # assignment of make_continuous does
+    # assignment of make_continuous does; channel pairs c, 2048 - c (0/2048,
+    # 1000/1048, 5/2043) and 1024 share bins of the packed transform, which
+    # the FFT row pass sums with atomics while lone channels are stored
     import torch
     from albatros_axion_detection_amd import _lib
     acclen, cut, ntiles = nslice - 8, 4, 3
