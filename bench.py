@@ -144,11 +144,14 @@ def rechannelizer(args, torch):
         timer("repfb_stage2", lambda: _lib.call(
             "alb_repfb_stage2", _lib.ptr(ws), nblk, osamp, c_lo, c_hi, _lib.ptr(out), c_hi - c_lo, 1, nblk, 0, s()))
 
-    # launch-shape autotuning (untimed): column-pass strip width and XCD group
-    # size, row-pass group size
+    # launch-shape autotuning (untimed): column-pass strip width and row-pass
+    # group size.  The column-pass XCD group size stays at its default (2 at
+    # this tile count: the filter taps are fetched once per launch, and larger
+    # groups, which fetch them once per tile, are no faster).
     knobs = (("ipfb_cols_width", "w", [args.cols_width] if args.cols_width else [8, 4, 2]),
-             ("ipfb_cols_group", "g", [args.cols_group] if args.cols_group else [64, 32, 16, 8, 4, 2]),
              ("ipfb_rows_group", "r", [args.rows_group] if args.rows_group else [4, 2]))
+    if args.cols_group:
+        knobs += (("ipfb_cols_group", "g", [args.cols_group]),)
     sweep, combos = {}, {}
     for vals in itertools.product(*(k[2] for k in knobs)):
         for (key, _, _), v in zip(knobs, vals):

@@ -6,6 +6,8 @@ with the float32 FFT error budget ~ log2(n) * 6e-8; 1e-5 is the bar of the
 path (the reference's own CPU-vs-GPU float32 envelope is 1e-7..2e-4 absolute,
 src/utils/test_results/out_comparison_test.out:78-186).
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -436,3 +438,41 @@ def test_repfb_stage2_inputs_matches_per_input(rng, osamp):
     with pytest.raises(ValueError):
         _lib.call("alb_repfb_stage2_inputs", _lib.ptr(ws), 8, M, nblock, osamp, c_lo, c_hi, _lib.ptr(one), 1, M,
                   M * bpj, bpj, M * bpj * F, s)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("osamp,ntap", [(64, 4), (32, 6), (16, 9)])
+def test_repfb_stage1_list_matches_all_blocks(rng, osamp, ntap):
+    # the edge-block entry: any order of the listed blocks gives the same
+    # bits, unlisted blocks are untouched, and the listed blocks match the
+    # all-blocks stage 1
+    import torch
+    from albatros_axion_detection_amd import _lib
+    nblock = 14
+    ts = _lib.to_device(np.asarray(rng.standard_normal(((nblock + ntap - 1) * osamp, 4096)), dtype=np.float32))
+    win = _lib.to_device(np.asarray(rng.standard_normal(ntap * 4096 * osamp), dtype=np.float32))
+    per = int(_lib.load().alb_repfb_workspace_bytes(nblock, osamp)) // 8
+    s = _lib.stream()
+    full = torch.zeros(per, dtype=torch.complex64, device=_lib.device())
+    _lib.call("alb_repfb_stage1", _lib.ptr(ts), _lib.ptr(win), ntap, nblock, osamp, _lib.ptr(full), s)
+    lists = {
+        "ascending": [0, 1, 2, 3, 4, 5, 7, 8, 9, 10, 11, 12, 13],     # block 6 left out
+        "shuffled": [13, 11, 9, 12, 10, 8, 7, 5, 3, 1, 4, 2, 0],
+    }
+    got = {}
+    for name, blocks in lists.items():
+        ws = torch.zeros(per, dtype=torch.complex64, device=_lib.device())
+        arr = np.asarray(blocks, dtype=np.int32)
+        _lib.call("alb_repfb_stage1_list", _lib.ptr(ts), _lib.ptr(win), ntap, arr.ctypes.data_as(ctypes.c_void_p),
+                  len(arr), osamp, _lib.ptr(ws), s)
+        got[name] = ws.view(nblock, -1)
+        print("[gpu-case] stage-1 list", name, osamp, ntap, flush=True)
+    assert torch.equal(got["ascending"], got["shuffled"])
+    ref = full.view(nblock, -1)
+    keep = [b for b in range(nblock) if b != 6]
+    assert torch.count_nonzero(got["ascending"][6]).item() == 0
+    assert rel_err(got["ascending"][keep].cpu().numpy(), ref[keep].cpu().numpy()) < 2e-6
+    with pytest.raises(ValueError):
+        bad = np.asarray([-1], dtype=np.int32)
+        _lib.call("alb_repfb_stage1_list", _lib.ptr(ts), _lib.ptr(win), ntap, bad.ctypes.data_as(ctypes.c_void_p),
+                  1, osamp, _lib.ptr(full), s)
