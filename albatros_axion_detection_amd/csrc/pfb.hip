@@ -1200,7 +1200,10 @@ __global__ void rowmap_scatter_kernel(const int64_t* __restrict__ specnum, int64
   if (p >= npresent) return;
   int64_t r = (specnum[p] - start) % acclen;
   if (r < 0) r += acclen;
-  rowmap[r] = int(row_base + p);
+  // spectra that land on one row (a repeated spec_num, or spec_nums acclen
+  // apart): the last one in packet order is kept, as numpy's fancy assignment
+  // of make_continuous keeps it
+  atomicMax(&rowmap[r], int(row_base + p));
 }
 
 }  // namespace alb

@@ -194,7 +194,7 @@ int alb_repfb_stage1_list(const float* ts, const float* win, int ntap, const int
 /* ---- streaming helpers (scripts/xcorr/helper_gpu_stream.py) ------------ */
 /* Row map of one chunk for the fused iPFB: rowmap[(specnum[p]-start) % acclen] = row_base + p
  * (the packed row holding spectrum p), -1 elsewhere (baseband_data_classes.py:53 modular row of
- * make_continuous_gpu). */
+ * make_continuous_gpu); of several spectra on one row the last (largest p) is kept. */
 int alb_build_rowmap(const int64_t* specnum, int64_t npresent, int64_t start, int64_t acclen, int64_t row_base,
                      int* rowmap, void* stream);
 

@@ -476,3 +476,23 @@ def test_repfb_stage1_list_matches_all_blocks(rng, osamp, ntap):
         bad = np.asarray([-1], dtype=np.int32)
         _lib.call("alb_repfb_stage1_list", _lib.ptr(ts), _lib.ptr(win), ntap, bad.ctypes.data_as(ctypes.c_void_p),
                   1, osamp, _lib.ptr(full), s)
+
+
+@pytest.mark.gpu
+def test_rowmap_keeps_last_spectrum_of_a_row(rng):
+    # spectra on one row (a repeated spec_num, spec_nums acclen apart, one
+    # before start): the last in packet order is kept, as numpy's fancy
+    # assignment does in make_continuous (baseband_data_classes.py:53)
+    from albatros_axion_detection_amd import _lib, repfb
+    acclen, start = 50, 1000
+    sn = np.concatenate([rng.choice(np.arange(start, start + acclen), 30, replace=False),
+                         [start + 7, start + 7, start + 7 + acclen, start + 12 - acclen, start + 3]]).astype(np.int64)
+    rng.shuffle(sn)
+    exp = np.full(acclen, -1, dtype=np.int64)
+    base = 17
+    for p, s in enumerate(sn):
+        exp[(s - start) % acclen] = base + p
+    for rep in range(3):
+        got = repfb.rowmaps_from_specnums([sn], start, acclen, row_bases=[base]).cpu().numpy()[0]
+        print("[gpu-case] rowmap duplicates", rep, flush=True)
+        assert np.array_equal(got, exp)
