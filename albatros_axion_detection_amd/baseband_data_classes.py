@@ -61,6 +61,15 @@ def make_continuous_gpu(spec, chan_specnum, start_specnum, channels, nspec, ncha
                 chans[k] = -1
             else:
                 seen.add(chans[k])
+    # spectra landing on one row (a repeated spec_num, or spec_nums nspec
+    # apart) keep the last one, as numpy's fancy assignment does: only the
+    # last spectrum of each row is scattered
+    slot = (rows - int(start_specnum)) % int(nspec)
+    if len(slot) != len(np.unique(slot)):
+        _, first_rev = np.unique(slot[::-1], return_index=True)
+        keep = np.sort(len(slot) - 1 - first_rev)
+        sp = sp[_lib.to_device(keep, t.int64)]
+        rows = rows[keep]
     sn = _lib.to_device(rows if len(rows) else np.zeros(1, dtype=np.int64), t.int64)
     ch = _lib.to_device(chans if len(chans) else np.zeros(1, dtype=np.int64), t.int64)
     _lib.call("alb_make_continuous", _lib.ptr(sp), sp.shape[1] if sp.ndim == 2 else 1, _lib.ptr(sn), len(rows),

@@ -134,6 +134,15 @@ def test_make_continuous(rng):
     g = bdc.make_continuous_gpu(sp, sn, 1000 - 3 * acclen, ch2, acclen, 2049)
     e = ostream.make_continuous(sp, sn, 1000 - 3 * acclen, ch2, acclen, 2049)
     assert np.array_equal(g, e.astype(np.complex64))
+    # spectra on one row (a repeated spec_num, spec_nums acclen apart) keep
+    # the last one, as numpy's fancy assignment does
+    sn2 = sn.copy()
+    sn2[5] = sn2[2]
+    sn2[9] = sn2[30] + acclen
+    g = bdc.make_continuous_gpu(sp, sn2, 1000 - 3 * acclen, ch, acclen, 2049)
+    e = ostream.make_continuous(sp, sn2, 1000 - 3 * acclen, ch, acclen, 2049)
+    assert np.array_equal(g, e.astype(np.complex64))
+    assert np.array_equal(g[(sn2[30] - 1000 + 3 * acclen) % acclen, ch], sp[30])
 
 
 def test_dropped_packets(rng):
