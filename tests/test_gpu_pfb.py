@@ -496,3 +496,48 @@ def test_rowmap_keeps_last_spectrum_of_a_row(rng):
         got = repfb.rowmaps_from_specnums([sn], start, acclen, row_bases=[base]).cpu().numpy()[0]
         print("[gpu-case] rowmap duplicates", rep, flush=True)
         assert np.array_equal(got, exp)
+
+
+@pytest.mark.gpu
+def test_row_pass_every_channel(rng):
+    # 2049 columns over the channels in a shuffled order: more items than the
+    # prefetched ones (the overflow loop), bins shared by c and 2048 - c
+    # (atomic sums) next to lone ones, a repeated channel marked -1; dense and
+    # blocked FFT row passes against the float64 irfft of the zero-filled rows
+    import torch
+    from albatros_axion_detection_amd import _lib
+    nslice, cut, ntiles = 64, 4, 2
+    acclen = nslice - 2 * cut
+    chans = rng.permutation(2049).astype(np.int32)
+    chans[100] = chans[7]            # one channel twice (the last column wins), one channel absent
+    marked = chans.copy()
+    marked[7] = -1
+    ncols = len(chans)
+    packed = rng.integers(0, 256, size=(ntiles * acclen, ncols), dtype=np.uint8)
+    rowmap = np.arange(ntiles * acclen, dtype=np.int32).reshape(ntiles, acclen)
+    rowmap[rng.random(rowmap.shape) < 0.1] = -1
+    pk, rm, ch = _lib.to_device(packed), _lib.to_device(rowmap), _lib.to_device(marked)
+    dev = _lib.device()
+    y = torch.empty((ntiles, nslice, 4096), dtype=torch.float32, device=dev)
+    yb = torch.empty((ntiles, 4096 // 8, nslice, 8), dtype=torch.float32, device=dev)
+    s = _lib.stream()
+    _lib.call("alb_ipfb_rows_packed4", _lib.ptr(pk), _lib.ptr(rm), _lib.ptr(ch), ncols, acclen, cut, 0, ntiles,
+              _lib.ptr(y), s)
+    _lib.call("alb_ipfb_rows_packed4_blocked", _lib.ptr(pk), _lib.ptr(rm), _lib.ptr(ch), ncols, acclen, cut, 0,
+              ntiles, _lib.ptr(yb), s)
+    a = y.cpu().numpy()
+    assert np.array_equal(yb.cpu().numpy().transpose(0, 2, 1, 3).reshape(ntiles, nslice, 4096), a)
+    from oracle import unpack as ounpack
+    r, i = ounpack.decode_4bit_bytes(packed)
+    vals = (r + 1j * i).astype(np.complex128)
+    spec = np.zeros((ntiles, nslice, 2049), dtype=np.complex128)
+    for t in range(ntiles):
+        for row in range(nslice):
+            chunk, crow = (t - 1, acclen - 2 * cut + row) if row < 2 * cut else (t, row - 2 * cut)
+            if chunk < 0 or rowmap[chunk, crow] < 0:
+                continue
+            spec[t, row, chans] = vals[rowmap[chunk, crow]]          # numpy: the last of a repeated channel wins
+    ref = np.fft.irfft(spec, n=4096, axis=-1)
+    e = rel_err(a, ref)
+    print("[gpu-case] row pass, every channel: error vs float64 %.3g" % e, flush=True)
+    assert e < 5e-6
