@@ -961,12 +961,58 @@ struct BlockList {
   int b[kBlockListCap];
 };
 
+// S-point real DFT of the polyphase sums y[s] (rows 4 s + t of column k of
+// block b), radix-4 combine through the exchange buffer zx and the StageA store
+// of the block's bins (when store is set); one lane per (t, column), four
+// lanes per column.  Every lane of the workgroup calls it (one barrier).
+template <int R1>
+__device__ __forceinline__ void stage1_split_finish(const float* y, int t, int cl, int64_t b, int64_t k,
+                                                    float2 (*zx)[R1 / 8 + 1][64], const float2* wr,
+                                                    float2* __restrict__ A, bool store = true) {
+  constexpr int S = R1 / 4;
+  // S-point real DFT of y -> Z_t[q0], q0 = 0..S/2
+  {
+    float2 z[S / 2];
+#pragma unroll
+    for (int n = 0; n < S / 2; ++n) z[n] = make_float2(y[2 * n], y[2 * n + 1]);
+    DFT<S / 2>::run(z);
+    static_for<0, S / 2 + 1>([&](auto Q) {
+      constexpr int q = decltype(Q)::value;
+      const float2 a = z[q % (S / 2)];
+      const float2 bb = z[(S / 2 - q) % (S / 2)];
+      const float2 e = make_float2(0.5f * (a.x + bb.x), 0.5f * (a.y - bb.y));
+      const float2 d = make_float2(0.5f * (a.x - bb.x), 0.5f * (a.y + bb.y));
+      const float2 o = make_float2(d.y, -d.x);
+      const float2 wo = twc<S, q>(o);
+      zx[t][q][cl] = make_float2(e.x + wo.x, e.y + wo.y);
+    });
+  }
+  __syncthreads();
+  // lane u combines q0 = u, u+4, ...: outputs q = q0 (p=0), q0+S (p=1), and q = 2S = H for q0 = 0
+  // (bins 0 and H are real and share slot 0 of the StageA layout)
+  float2* dst = A + StageA<R1>::at(b, 0, k);
+  for (int q0 = t; store && q0 < S; q0 += 4) {
+    float2 v[4];
+#pragma unroll
+    for (int tt = 0; tt < 4; ++tt) {
+      float2 zz = (2 * q0 <= S) ? zx[tt][q0][cl] : cconj(zx[tt][S - q0][cl]);
+      v[tt] = cmul(zz, wr[(q0 * tt) % R1]);
+    }
+    // 4-point DFT over tt: p = 0, 1, 2
+    const float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
+    const float2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
+    if (q0 == 0) dst[0] = make_float2(s02.x + s13.x, s02.x - s13.x);        // p = 0 and p = 2 (bins 0, H)
+    else dst[q0 * 4] = cadd(s02, s13);                                        // p = 0
+    dst[(q0 + S) * 4] = make_float2(d02.x + d13.y, d02.y - d13.x);            // p = 1: d02 - i d13
+  }
+}
+
+
 template <int R1>
 __global__ __launch_bounds__(kTPB) void repfb_stage1_split_kernel(const float* __restrict__ ts,
                                                                   const float* __restrict__ win, int ntap,
                                                                   float2* __restrict__ A, BlockList list) {
   constexpr int S = R1 / 4;
-  constexpr int H = R1 / 2;
   constexpr int ZH = S / 2 + 1;      // stored half spectrum of Z_t
   constexpr int64_t C = 4096;
   __shared__ float2 zx[4][ZH][64];
@@ -990,41 +1036,96 @@ __global__ __launch_bounds__(kTPB) void repfb_stage1_split_kernel(const float* _
 #pragma unroll
     for (int sI = 0; sI < S; ++sI) y[sI] = fmaf(src[int64_t(4 * sI) * C], wj[int64_t(4 * sI) * C], y[sI]);
   }
-  // S-point real DFT of y -> Z_t[q0], q0 = 0..S/2
+  stage1_split_finish<R1>(y, t, cl, b, k, zx, wr, A);
+}
+
+// Edge blocks in runs of up to kRunMax consecutive blocks b0 .. b0 + nb - 1
+// (list entry b0 * 8 + nb).  Block b0 + bl reads rows R1 (b0 + bl + j) + r as
+// tap j, so the run's nb + ntap - 1 block rows are walked once: 1024 lanes
+// per 64 columns, lane (u0, column) owns rows r = u0 + 16 u (u < R1 / 16) of
+// every block row and adds row m to block m - j as tap j, its window weights
+// in registers.  The taps of each block are summed in the order
+// j = 0 .. ntap - 1 with the same fmaf as the split kernel, so both give the
+// same bits.  The block sums go through LDS to the split kernel's DFT lanes,
+// 256 per block, all blocks of the run at once.
+constexpr int kRunMax = 4;
+struct RunList {
+  int n;
+  int e[kBlockListCap];
+};
+
+template <int R1>
+constexpr size_t run_kernel_lds() {
+  constexpr size_t y = size_t(kRunMax) * R1 * 64 * sizeof(float);
+  constexpr size_t z = size_t(kRunMax) * 4 * (R1 / 8 + 1) * 64 * sizeof(float2);
+  return y > z ? y : z;
+}
+
+template <int R1, int NT>
+__global__ __launch_bounds__(1024) void repfb_stage1_run_kernel(const float* __restrict__ ts,
+                                                                const float* __restrict__ win, int ntap,
+                                                                float2* __restrict__ A, RunList list) {
+  constexpr int S = R1 / 4;
+  constexpr int ZH = S / 2 + 1;
+  constexpr int RPT = R1 / 16;              // rows per lane of one block row
+  constexpr int64_t C = 4096;
+  static_assert(R1 % 16 == 0, "16 row lanes per column");
+  extern __shared__ float run_smem[];
+  float* Y = run_smem;                      // [kRunMax][R1][64] polyphase sums
+  auto zx = reinterpret_cast<float2 (*)[ZH][64]>(run_smem);   // [kRunMax * 4][ZH][64] once Y is read
+  __shared__ float2 wr[R1];
+  const int tid = threadIdx.x;
+  const int cl = tid & 63;
+  const int64_t k = int64_t(blockIdx.x) * 64 + cl;
+  const int e = list.e[blockIdx.y];
+  const int64_t b0 = e >> 3;
+  const int nb = e & 7;
+  if (tid < R1) {
+    double sn, cs;
+    sincospi(-2.0 * double(tid) / double(R1), &sn, &cs);
+    wr[tid] = make_float2(float(cs), float(sn));
+  }
   {
-    float2 z[S / 2];
+    const int u0 = tid >> 6;
+    float y[kRunMax][RPT], w[NT][RPT];
 #pragma unroll
-    for (int n = 0; n < S / 2; ++n) z[n] = make_float2(y[2 * n], y[2 * n + 1]);
-    DFT<S / 2>::run(z);
-    static_for<0, S / 2 + 1>([&](auto Q) {
-      constexpr int q = decltype(Q)::value;
-      const float2 a = z[q % (S / 2)];
-      const float2 bb = z[(S / 2 - q) % (S / 2)];
-      const float2 e = make_float2(0.5f * (a.x + bb.x), 0.5f * (a.y - bb.y));
-      const float2 d = make_float2(0.5f * (a.x - bb.x), 0.5f * (a.y + bb.y));
-      const float2 o = make_float2(d.y, -d.x);
-      const float2 wo = twc<S, q>(o);
-      zx[t][q][cl] = make_float2(e.x + wo.x, e.y + wo.y);
-    });
+    for (int u = 0; u < RPT; ++u) {
+      const int64_t r = u0 + 16 * u;
+#pragma unroll
+      for (int bl = 0; bl < kRunMax; ++bl) y[bl][u] = 0.f;
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const float v = win[(int64_t(j < ntap ? j : 0) * R1 + r) * C + k];   // in bounds for every j
+        w[j][u] = j < ntap ? v : 0.f;
+      }
+    }
+    const int nrow = nb + ntap - 1;
+#pragma unroll
+    for (int m = 0; m < kRunMax + NT - 1; ++m) {
+      if (m < nrow) {
+#pragma unroll
+        for (int u = 0; u < RPT; ++u) {
+          const float x = ts[((b0 + m) * R1 + u0 + 16 * u) * C + k];
+#pragma unroll
+          for (int bl = 0; bl < kRunMax; ++bl) {
+            const int j = m - bl;
+            if (j >= 0 && j < NT && bl < nb && j < ntap) y[bl][u] = fmaf(x, w[j][u], y[bl][u]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int bl = 0; bl < kRunMax; ++bl)
+#pragma unroll
+      for (int u = 0; u < RPT; ++u) Y[(bl * R1 + u0 + 16 * u) * 64 + cl] = y[bl][u];
   }
   __syncthreads();
-  // lane u combines q0 = u, u+4, ...: outputs q = q0 (p=0), q0+S (p=1), and q = 2S = H for q0 = 0
-  // (bins 0 and H are real and share slot 0 of the StageA layout)
-  float2* dst = A + StageA<R1>::at(b, 0, k);
-  for (int q0 = t; q0 < S; q0 += 4) {
-    float2 v[4];
+  const int bl = tid >> 8, t = (tid >> 6) & 3;
+  float ys[S];
 #pragma unroll
-    for (int tt = 0; tt < 4; ++tt) {
-      float2 zz = (2 * q0 <= S) ? zx[tt][q0][cl] : cconj(zx[tt][S - q0][cl]);
-      v[tt] = cmul(zz, wr[(q0 * tt) % R1]);
-    }
-    // 4-point DFT over tt: p = 0, 1, 2
-    const float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-    const float2 s13 = cadd(v[1], v[3]), d13 = csub(v[1], v[3]);
-    if (q0 == 0) dst[0] = make_float2(s02.x + s13.x, s02.x - s13.x);        // p = 0 and p = 2 (bins 0, H)
-    else dst[q0 * 4] = cadd(s02, s13);                                        // p = 0
-    dst[(q0 + S) * 4] = make_float2(d02.x + d13.y, d02.y - d13.x);            // p = 1: d02 - i d13
-  }
+  for (int sI = 0; sI < S; ++sI) ys[sI] = Y[(bl * R1 + 4 * sI + t) * 64 + cl];
+  __syncthreads();   // Y is read: its space becomes the blocks' exchange buffers
+  stage1_split_finish<R1>(ys, t, cl, b0 + bl, k, zx + 4 * bl, wr, A, bl < nb);
 }
 
 // ------------------------------------------------------- re-PFB stage 2
@@ -1397,6 +1498,34 @@ template <int R1>
 int launch_repfb_stage1_list(const float* ts, const float* win, int ntap, const int* blocks, int64_t n, float2* A,
                              hipStream_t s) {
   if constexpr (R1 >= 16) {
+    if (ntap <= 8) {
+      // consecutive entries b, b + 1, ... form runs of up to kRunMax blocks
+      RunList rl{};
+      auto flush = [&]() -> int {
+        if (rl.n == 0) return ALB_OK;
+        constexpr size_t lds = run_kernel_lds<R1>();
+        if (ntap <= 4)
+          hipLaunchKernelGGL((repfb_stage1_run_kernel<R1, 4>), dim3(4096 / 64, unsigned(rl.n)), dim3(1024), lds, s,
+                             ts, win, ntap, A, rl);
+        else
+          hipLaunchKernelGGL((repfb_stage1_run_kernel<R1, 8>), dim3(4096 / 64, unsigned(rl.n)), dim3(1024), lds, s,
+                             ts, win, ntap, A, rl);
+        ALB_LAUNCH_CHECK("repfb_stage1_run_kernel");
+        rl.n = 0;
+        return ALB_OK;
+      };
+      for (int64_t i = 0; i < n;) {
+        int len = 1;
+        while (len < kRunMax && i + len < n && blocks[i + len] == blocks[i] + len) ++len;
+        if (rl.n == kBlockListCap) {
+          const int rc = flush();
+          if (rc != ALB_OK) return rc;
+        }
+        rl.e[rl.n++] = blocks[i] * 8 + len;
+        i += len;
+      }
+      return flush();
+    }
     for (int64_t i0 = 0; i0 < n; i0 += kBlockListCap) {
       BlockList bl{};
       bl.n = int(n - i0 < kBlockListCap ? n - i0 : kBlockListCap);
@@ -1681,7 +1810,7 @@ int alb_repfb_stage1_list(const float* ts, const float* win, int ntap, const int
   ALB_REQUIRE(ts && win && A && blocks, "alb_repfb_stage1_list: null pointer");
   ALB_REQUIRE(ntap >= 1, "alb_repfb_stage1_list: bad ntap");
   for (int64_t i = 0; i < n; ++i)
-    ALB_REQUIRE(blocks[i] >= 0, "alb_repfb_stage1_list: negative block index");
+    ALB_REQUIRE(blocks[i] >= 0 && blocks[i] < (1 << 27), "alb_repfb_stage1_list: block index out of range");
   hipStream_t s = as_stream(stream);
   switch (osamp) {
 #define CASE(r) case r: return launch_repfb_stage1_list<r>(ts, win, ntap, blocks, n, (float2*)A, s);
