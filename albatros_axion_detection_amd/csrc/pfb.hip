@@ -1374,12 +1374,16 @@ int launch_irfft_packed4(const Packed4Rows& pk, int64_t nrows, int64_t nslice, f
 }
 
 // packed 4-bit rows (2049-channel spectra, n = 4096): two rows per 256-thread
-// workgroup (measured faster), or four per 512 threads (each workgroup writing
-// whole 128-byte lines of the blocked layout)
+// workgroup, or four per 512 threads (each store instruction writing whole
+// 128-byte lines of the blocked layout).  Unset knob: two up to 4 tiles per
+// launch, whose output stays in the Infinity Cache (within 3% either way),
+// four above (10% faster per tile at 16 tiles, 100 and 300 channels).
 int irfft_packed4_dispatch(const Packed4Rows& pk, int64_t nrows, int64_t nslice, float* out, int64_t ldo,
                            hipStream_t s) {
+  const int64_t knob = tune_get("ipfb_rows_group");
+  const int group = knob > 0 ? int(knob) : (nrows > 4 * nslice ? 4 : 2);
   // prefetched (row, column) items per thread: enough for every item when it fits
-  if (tune_get("ipfb_rows_group") == 4) {
+  if (group == 4) {
     if (4 * pk.ncols <= 2 * 512) return launch_irfft_packed4<4, 512, 2>(pk, nrows, nslice, out, ldo, s);
     return launch_irfft_packed4<4, 512, 4>(pk, nrows, nslice, out, ldo, s);
   }

@@ -145,7 +145,7 @@ def rechannelizer(args, torch):
             "alb_repfb_stage2", _lib.ptr(ws), nblk, osamp, c_lo, c_hi, _lib.ptr(out), c_hi - c_lo, 1, nblk, 0, s()))
 
     # Launch shapes: the library defaults (column-pass strip width 4, XCD
-    # group 2 at this tile count, row-pass group 2), the shapes the committed
+    # group 2 and row-pass group 2 at this tile count), the shapes the committed
     # rocprofv3 summaries were taken with, unless a flag sets one.  --sweep
     # times every strip width x row group first (untimed) and keeps the
     # fastest; they lie within a few per cent of each other.
@@ -509,7 +509,7 @@ def main():
     ap.add_argument("--cols-group", type=int, default=0, choices=[0, 2, 4, 8, 16, 32, 64],
                     help="column-pass strips per XCD group (0: library default, 2 up to 4 tiles, 64 above)")
     ap.add_argument("--rows-group", type=int, default=0, choices=[0, 2, 4],
-                    help="spectra per workgroup of the FFT row pass (0: library default, 2)")
+                    help="spectra per workgroup of the FFT row pass (0: library default, 2 up to 4 tiles, 4 above)")
     ap.add_argument("--sweep", action="store_true",
                     help="time every column-pass strip width x row-pass group before timing and keep the fastest")
     ap.add_argument("--row-pass", default="auto", choices=["auto", "fft", "mfma"],

@@ -202,7 +202,7 @@ int alb_build_rowmap(const int64_t* specnum, int64_t npresent, int64_t start, in
 /* Set a launch-shape knob; 0 restores the built-in choice.  Keys:
  *   "ipfb_cols_width"  columns per workgroup of the nslice = 4096 column pass (2, 4 or 8, default 4);
  *   "ipfb_cols_group"  column strips per XCD group of the column-pass order (2..64; unset: 2 for launches of up to 4 tiles, 64 above);
- *   "ipfb_rows_group"  spectra per workgroup of the packed 4-bit row pass (2 or 4, default 2);
+ *   "ipfb_rows_group"  spectra per workgroup of the packed 4-bit row pass (2 or 4; unset: 2 for launches of up to 4 tiles, 4 above);
  *   "xcorr_int_method" all-pairs integer correlators: 1 VALU (XOR form for 1-bit),
  *                      2 int8 matrix cores (blocklen % 64 == 0; 1-bit also ncol % 4 == 0),
  *                      0 matrix cores whenever the shape allows. */
