@@ -3,9 +3,14 @@
 // (F-order): x[(m, t, f)] at m + M*(t + K*f), out[(i, j, f)] at i + M*(j + M*f).
 //
 // M <= 8: the realified Gram G = Y Y^T of Y = [Re x; Im x] (16 x K) is built on the
-// FP32 matrix cores with v_mfma_f32_16x16x4f32, one wavefront per channel.  Lane l
-// feeds Y[l % 16][t0 + l / 16] to both operands (A = Y, B = Y^T share registers).
-// Then Re = G[i][j] + G[i+8][j+8], Im = G[i+8][j] - G[i][j+8].
+// FP32 matrix cores with v_mfma_f32_16x16x4f32.  Lane l feeds Y[l % 16][t0 + l / 16]
+// to both operands (A = Y, B = Y^T share registers).  Row m < 8 of Y is input
+// m % SLOT (real part; row m + 8 the imaginary part) of channel m / SLOT of the
+// wavefront: SLOT = 8 for M > 4 (one channel per wavefront), 4 for M = 3, 4 and
+// 2 for M <= 2, so 2 or 4 channels share one product and every lane loads.  A
+// channel's block of G is its own Gram (each element sums its two rows only, in
+// the same order whatever the other rows hold).  Then, with i, j offset by the
+// channel's slot, Re = G[i][j] + G[i+8][j+8], Im = G[i+8][j] - G[i][j+8].
 // Other M: a VALU kernel with one thread per (channel, baseline).
 //
 // The matrix-core kernels also take a strided input: channel c = s F + f of
@@ -26,17 +31,58 @@ struct XLayout {
   }
 };
 
+// Channels per wavefront and input slots per channel of the matrix-core forms.
+template <int M>
+struct XPack {
+  static constexpr int SLOT = M <= 2 ? 2 : (M <= 4 ? 4 : 8);
+  static constexpr int CPW = 8 / SLOT;
+};
+
+// Row of Y fed by lane r = l % 16: its channel (f0 + sub) and input (mi); the
+// rows of absent inputs and channels are zero.
+template <int M>
+struct XRow {
+  int64_t f0;
+  int sub, mi, part;
+  bool ok;
+  __device__ __forceinline__ XRow(int64_t wave, int r, int64_t F) {
+    const int m = r % 8;
+    f0 = wave * XPack<M>::CPW;
+    sub = m / XPack<M>::SLOT;
+    mi = m % XPack<M>::SLOT;
+    part = r / 8;
+    ok = f0 + sub < F && mi < M;
+  }
+};
+
+// Re/Im of the channels of one wavefront from its Gram G (16 x 16, row-major):
+// lane e < CPW M M writes baseline (i, j) of channel f0 + e / (M M).
+template <int M>
+__device__ __forceinline__ void xgram_store(const float* g, int lane, int64_t f0, int64_t F,
+                                            float2* __restrict__ out, float scale) {
+  constexpr int SLOT = XPack<M>::SLOT;
+  if (lane < XPack<M>::CPW * M * M) {
+    const int sub = lane / (M * M), e = lane % (M * M);
+    const int i = e % M, j = e / M;
+    const int64_t f = f0 + sub;
+    if (f < F) {
+      const int io = sub * SLOT + i, jo = sub * SLOT + j;
+      const float re = g[io * 16 + jo] + g[(io + 8) * 16 + (jo + 8)];
+      const float im = g[(io + 8) * 16 + jo] - g[io * 16 + (jo + 8)];
+      out[f * M * M + int64_t(j) * M + i] = make_float2(re * scale, im * scale);
+    }
+  }
+}
+
 template <int M>
 __global__ __launch_bounds__(256) void xcorr_mfma16_kernel(const float* __restrict__ x, int64_t K, int64_t F,
                                                            XLayout L, float2* __restrict__ out, float scale) {
   __shared__ float G[4][16 * 16];
   const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const int64_t f = int64_t(blockIdx.x) * 4 + w;
-  const bool valid = f < F;
   const int r = lane % 16, slot = lane / 16;
-  const int m = r % 8, part = r / 8;
-  const bool row_ok = valid && m < M;
-  const float* base = x + 2 * L.channel(valid ? f : 0) + 2 * m + part;
+  const XRow<M> row(int64_t(blockIdx.x) * 4 + w, r, F);
+  const bool row_ok = row.ok;
+  const float* base = x + 2 * L.channel(row_ok ? row.f0 + row.sub : 0) + 2 * row.mi + row.part;
   const int64_t st = 2 * L.sk;
   floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = acc0, acc2 = acc0, acc3 = acc0;
   int64_t t0 = 0;
@@ -59,13 +105,7 @@ __global__ __launch_bounds__(256) void xcorr_mfma16_kernel(const float* __restri
 #pragma unroll
   for (int v = 0; v < 4; ++v) G[w][(4 * slot + v) * 16 + r] = (acc0[v] + acc1[v]) + (acc2[v] + acc3[v]);
   __syncthreads();
-  if (valid && lane < M * M) {
-    const int i = lane % M, j = lane / M;
-    const float* g = G[w];
-    const float re = g[i * 16 + j] + g[(i + 8) * 16 + (j + 8)];
-    const float im = g[(i + 8) * 16 + j] - g[i * 16 + (j + 8)];
-    out[f * M * M + int64_t(j) * M + i] = make_float2(re * scale, im * scale);
-  }
+  xgram_store<M>(G[w], lane, row.f0, F, out, scale);
 }
 
 // Reduced-precision forms of the same Gram product on the 16x16x32 matrix cores
@@ -88,12 +128,10 @@ __global__ __launch_bounds__(256) void xcorr_mfma16_lp_kernel(const float* __res
                                                               XLayout L, float2* __restrict__ out, float scale) {
   __shared__ float G[4][16 * 16];
   const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const int64_t f = int64_t(blockIdx.x) * 4 + w;
-  const bool valid = f < F;
   const int r = lane % 16, grp = lane / 16;
-  const int m = r % 8, part = r / 8;
-  const bool row_ok = valid && m < M;
-  const float* base = x + 2 * L.channel(valid ? f : 0) + 2 * m + part;
+  const XRow<M> row(int64_t(blockIdx.x) * 4 + w, r, F);
+  const bool row_ok = row.ok;
+  const float* base = x + 2 * L.channel(row_ok ? row.f0 + row.sub : 0) + 2 * row.mi + row.part;
   const int64_t st = 2 * L.sk;
   floatx4 accH = {0.f, 0.f, 0.f, 0.f}, accX = accH;
   for (int64_t t0 = 0; t0 < K; t0 += 32) {
@@ -134,13 +172,7 @@ __global__ __launch_bounds__(256) void xcorr_mfma16_lp_kernel(const float* __res
 #pragma unroll
   for (int v = 0; v < 4; ++v) G[w][(4 * grp + v) * 16 + r] = accH[v] + accX[v] * xs;
   __syncthreads();
-  if (valid && lane < M * M) {
-    const int i = lane % M, j = lane / M;
-    const float* g = G[w];
-    const float re = g[i * 16 + j] + g[(i + 8) * 16 + (j + 8)];
-    const float im = g[(i + 8) * 16 + j] - g[i * 16 + (j + 8)];
-    out[f * M * M + int64_t(j) * M + i] = make_float2(re * scale, im * scale);
-  }
+  xgram_store<M>(G[w], lane, row.f0, F, out, scale);
 }
 
 // Generic: workgroup per channel, thread per baseline i <= j.
@@ -218,9 +250,11 @@ int alb_xcorr_all_ant_strided(const void* x, int64_t M, int64_t K, int64_t F, in
   }
   if (F == 0) return ALB_OK;
   const int64_t nch = F * S;
-  ALB_REQUIRE((nch + 3) / 4 <= 0x7fffffff, "alb_xcorr_all_ant_strided: too many channels");
+  const int64_t cpw = M <= 2 ? 4 : (M <= 4 ? 2 : 1);       // channels per wavefront (XPack)
+  const int64_t nwave = (nch + cpw - 1) / cpw;
+  ALB_REQUIRE((nwave + 3) / 4 <= 0x7fffffff, "alb_xcorr_all_ant_strided: too many channels");
   hipStream_t s = as_stream(stream);
-  const dim3 grid(unsigned((nch + 3) / 4));
+  const dim3 grid(unsigned((nwave + 3) / 4));
   const XLayout L{sk, sf, ss, F};
   if (mode == PREC_FP32) {
     switch (M) {

@@ -166,7 +166,7 @@ def test_xcorr_all_ant_torch_fortran_view(cr):
     assert np.max(np.abs(out.cpu().numpy() - ref) / (d[:, None] * d[None])) < 1e-5
 
 
-@pytest.mark.parametrize("M,K,F", [(8, 1024, 64), (5, 1000, 33), (2, 77, 9)])
+@pytest.mark.parametrize("M,K,F", [(8, 1024, 64), (5, 1000, 33), (2, 77, 9), (4, 300, 7), (3, 64, 5), (1, 40, 3)])
 def test_xcorr_all_ant_precision_forms(cr, M, K, F):
     # diagonal-normalised error |dC_ij| / sqrt(C_ii C_jj) against the float64
     # oracle for each arithmetic; the FP16 form must show its 11-bit rounding
@@ -220,3 +220,26 @@ def test_strided_correlator_matches_f_order(rng, mode):
     got = a.cpu().numpy()
     tol = {0: 2e-6, 1: 1e-5, 3: 2e-6}[mode]      # FP32, BF16 hi/lo, FP16 hi/lo
     assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < tol
+
+
+@pytest.mark.parametrize("M", [1, 2, 3, 4, 8])
+@pytest.mark.parametrize("mode", [0, 3])
+def test_correlator_channel_independent_of_wave_partners(M, mode):
+    # for M <= 4 two or four channels share one matrix product; every channel
+    # gives the same bits alone (no partner) as next to its partners
+    import torch
+    from albatros_axion_detection_amd import _lib
+    rng = np.random.default_rng(M)
+    K, F = 96, 7
+    x = (rng.standard_normal((F, K, M)) + 1j * rng.standard_normal((F, K, M))).astype(np.complex64)
+    xd = _lib.to_device(x)
+    allc = torch.empty((F, M, M), dtype=torch.complex64, device=xd.device)
+    s = _lib.stream()
+    _lib.call("alb_xcorr_all_ant_strided", _lib.ptr(xd), M, K, F, 1, M, M * K, 0, _lib.ptr(allc), 1.0 / K, mode, s)
+    for f in range(F):
+        one = torch.empty((1, M, M), dtype=torch.complex64, device=xd.device)
+        _lib.call("alb_xcorr_all_ant_strided", _lib.ptr(xd[f:]), M, K, 1, 1, M, M * K, 0, _lib.ptr(one), 1.0 / K,
+                  mode, s)
+        assert torch.equal(one[0], allc[f]), (M, mode, f)
+    ref = np.einsum("fki,fkj->fji", x.astype(np.complex128), np.conj(x.astype(np.complex128))) / K
+    assert np.max(np.abs(allc.cpu().numpy() - ref)) < 1e-5 * np.max(np.abs(ref))
