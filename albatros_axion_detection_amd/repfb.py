@@ -100,11 +100,14 @@ class RePFBPipeline:
     ``cut`` carried/dropped rows, ``osamp`` upchannelization, ``nblock`` re-PFB
     spectra per job, fine channels ``[repfb_chanstart, repfb_chanend)``,
     ``chan_numbers`` = coarse channel number of each packed column
-    (helper_gpu_stream.py:196).
+    (helper_gpu_stream.py:196).  ``tile_batch`` iPFB tiles go through one row
+    and column launch; ``lanes`` inputs run their stage 1 concurrently in
+    :meth:`correlate_into` (one HIP stream and stream buffer each).
     """
 
     def __init__(self, ninput, chan_numbers, acclen, cut, osamp, nblock, repfb_chanstart, repfb_chanend,
-                 filt_thresh=0.45, ntap=4, filt=None, win=None, tile_batch=4, nchan=2049, row_pass="auto"):
+                 filt_thresh=0.45, ntap=4, filt=None, win=None, tile_batch=4, nchan=2049, row_pass="auto",
+                 lanes=2):
         t = _lib.torch()
         self.ninput = ninput
         self.acclen = int(acclen)
@@ -118,6 +121,9 @@ class RePFBPipeline:
         self.c_hi = int(repfb_chanend)
         self.nfine = self.c_hi - self.c_lo
         self.tile_batch = int(tile_batch)
+        # inputs whose stage 1 runs concurrently in correlate_into (one HIP
+        # stream, tile buffer and stream buffer each)
+        self.lanes = max(1, int(lanes))
         if nchan != 2049:
             raise NotImplementedError("the fused iPFB handles 2049-channel spectra (lblock 4096)")
         chan_numbers = np.asarray(chan_numbers)
@@ -138,9 +144,10 @@ class RePFBPipeline:
         self.filt = _lib.to_device(filt, t.complex64) if filt is not None else _pfb.get_filter(self.nslice, filt_thresh,
                                                                                                   ntap=ntap)
         self.win = _lib.to_device(win, t.float32) if win is not None else _pfb.window(ntap, self.lblock)
-        self._tiles = None
+        self._tiles = {}
         self._ws = None
         self._xin_f = None
+        self._lane_streams = None
         # row pass: "fft" (2048-point FFT per spectrum) or "mfma" (matrix-core
         # synthesis from the ncols non-zero channels); with a power-of-two
         # nslice both write the blocked tile layout, which the column pass reads
@@ -185,12 +192,12 @@ class RePFBPipeline:
         rows = max(rows_needed, nchunks * self.acclen)
         return jobs, nblocks_total, rows
 
-    def _tile_buffer(self, ntiles):
+    def _tile_buffer(self, ntiles, lane=0):
         t = _lib.torch()
         need = ntiles * self.nslice * 4096
-        if self._tiles is None or self._tiles.numel() < need:
-            self._tiles = t.empty(need, dtype=t.float32, device=_lib.device())
-        return self._tiles
+        if lane not in self._tiles or self._tiles[lane].numel() < need:
+            self._tiles[lane] = t.empty(need, dtype=t.float32, device=_lib.device())
+        return self._tiles[lane]
 
     def _workspace(self, nblk, ninput=1):
         """Stage-1 workspace of ``ninput`` inputs of ``nblk`` blocks each
@@ -287,10 +294,11 @@ class RePFBPipeline:
         _lib.call("alb_repfb_stage2", _lib.ptr(ws), nblocks_total, self.osamp, self.c_lo, self.c_hi, _lib.ptr(out),
                   sb, sc, bpj, sj, _lib.stream())
 
-    def stream_stage1(self, packed, rowmap, nchunks, stream_out, nblocks_total, ws):
+    def stream_stage1(self, packed, rowmap, nchunks, stream_out, nblocks_total, ws, lane=0):
         """Fused form up to re-PFB stage 1: every block's stage-1 result of one
         input in the workspace ``ws`` (row pass, column pass with stage 1 of
-        the interior blocks, then the edge blocks from the stream buffer)."""
+        the interior blocks, then the edge blocks from the stream buffer), on
+        the current stream with tile buffer ``lane``."""
         if stream_out.shape[0] < max((nblocks_total + self.ntap - 1) * self.osamp, nchunks * self.acclen):
             raise ValueError("stream buffer too short for the re-PFB blocks")
         s = _lib.stream()
@@ -300,7 +308,7 @@ class RePFBPipeline:
             tail = stream_out[rows_valid:]
             _lib.call("alb_fill_zero", _lib.ptr(tail), tail.numel() * 4, s)
         tb = self.tile_batch
-        y = self._tile_buffer(min(tb, nchunks))
+        y = self._tile_buffer(min(tb, nchunks), lane)
         for t0 in range(0, nchunks, tb):
             nt = min(tb, nchunks - t0)
             ldy = self._rows(packed, rowmap, t0, nt, y)
@@ -344,6 +352,15 @@ class RePFBPipeline:
         self._tile_buffer(min(self.tile_batch, nchunks))
         if self.inputs_stage2_supported(nbt):
             self._workspace(nbt, M)
+            # concurrent stage 1 of up to ``lanes`` inputs: a stream buffer and
+            # a tile buffer per lane
+            nl = min(self.lanes, M)
+            bufs["lane_stream_bufs"] = [bufs["stream"]] + [t.empty((rows, 4096), dtype=t.float32, device=dev)
+                                                           for _ in range(nl - 1)]
+            for lane in range(1, nl):
+                self._tile_buffer(min(self.tile_batch, nchunks), lane)
+            if nl > 1 and (self._lane_streams is None or len(self._lane_streams) < nl):
+                self._lane_streams = [t.cuda.Stream() for _ in range(nl)]
         else:
             self._workspace(nbt if self.fused_supported(nbt) else max(1, 512 // K) * K)
         return bufs
@@ -359,10 +376,29 @@ class RePFBPipeline:
         bb = max(1, 512 // K) * K
         if fused and self.inputs_stage2_supported(nbt):
             # stage 1 of every input into its own workspace, then one stage-2
-            # launch writes all inputs' channels into the correlator layout
+            # launch writes all inputs' channels into the correlator layout.
+            # Inputs go round-robin to ``lanes`` HIP streams (default
+            # priority), each with its own stream and tile buffer, so one
+            # input's small edge-block launches overlap the next input's row
+            # and column passes; the current stream waits for all of them.
             ws, per = self._workspace(nbt, M)
-            for m in range(M):
-                self.stream_stage1(packed_list[m], rowmaps[m], nchunks, stream, nbt, ws[m * per:])
+            lane_bufs = bufs.get("lane_stream_bufs", [stream])
+            nl = len(lane_bufs)
+            if nl == 1:
+                for m in range(M):
+                    self.stream_stage1(packed_list[m], rowmaps[m], nchunks, stream, nbt, ws[m * per:])
+            else:
+                t = _lib.torch()
+                main = t.cuda.current_stream()
+                for st in self._lane_streams[:nl]:
+                    st.wait_stream(main)
+                for m in range(M):
+                    lane = m % nl
+                    with t.cuda.stream(self._lane_streams[lane]):
+                        self.stream_stage1(packed_list[m], rowmaps[m], nchunks, lane_bufs[lane], nbt,
+                                           ws[m * per:], lane=lane)
+                for st in self._lane_streams[:nl]:
+                    main.wait_stream(st)
             _lib.call("alb_repfb_stage2_inputs", _lib.ptr(ws), per * 8, M, nbt, self.osamp, self.c_lo, self.c_hi,
                       _lib.ptr(xin), 1, F * M, M, K, K * F * M, _lib.stream())
         else:
