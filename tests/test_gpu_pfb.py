@@ -541,3 +541,37 @@ def test_row_pass_every_channel(rng):
     e = rel_err(a, ref)
     print("[gpu-case] row pass, every channel: error vs float64 %.3g" % e, flush=True)
     assert e < 5e-6
+
+
+def test_rechannelizer_full_size_against_fp64_oracle():
+    # the rechannelizer workload at full size: 4 chunks x 4088 coarse 4-bit
+    # spectra of channels 64..163 (seeded noise + tone, coarse-PFB'd in
+    # float64), iPFB nslice 4096 thresh 0.45, re-PFB 64x ntap 4, 6400 fine
+    # channels of every block against the float64 oracle at the path's 1e-5:
+    # the job planner (helper_gpu_stream.py:256-284) gives one full job of
+    # 252 blocks and a partial one over the zero-filled stream tail
+    from albatros_axion_detection_amd import repfb, synth
+    acclen, cut, osamp, nchunks, chan0, nkeep = 4088, 4, 64, 4, 64, 100
+    n_coarse = acclen * nchunks
+    packed, chans = synth.rechannelizer_input(n_coarse=n_coarse, chan0=chan0, nkeep=nkeep, seed=1)
+    nblk = n_coarse // osamp - 3
+    c_lo, c_hi = chan0 * osamp, (chan0 + nkeep) * osamp
+    pipe = repfb.RePFBPipeline(1, chans, acclen, cut, osamp, nblk, c_lo, c_hi, filt_thresh=0.45, tile_batch=nchunks)
+    chunks = [{"pol0": packed[c * acclen:(c + 1) * acclen], "specnums": c * acclen + np.arange(acclen)}
+              for c in range(nchunks)]
+    rm = repfb.rowmaps_from_specnums([c["specnums"] for c in chunks], 0, acclen)
+    from albatros_axion_detection_amd import _lib
+    got = pipe.spectra(_lib.to_device(packed), rm, nchunks).cpu().numpy()
+    print("[gpu-case] full-size rechannelizer: GPU done, oracle running", flush=True)
+    jobs, nbt, _ = pipe.plan(nchunks)
+    assert len(jobs) == 2 and nbt == 2 * nblk
+    filt = opfb.wiener_filter(opfb.matft_fp64(acclen + 2 * cut), 0.45)
+    st = ostream.ipfb_stream(_decode(chunks), 0, chans, acclen, cut, filt, 0)
+    need = (nbt + 3) * 4096 * osamp
+    buf = np.zeros(max(need, st.size))
+    buf[: st.size] = st
+    ref = opfb.pfb(buf[:need], opfb.window_f32(4, 4096 * osamp), 2048 * osamp + 1, 4)[:, c_lo:c_hi]
+    e_full, e_all = rel_err(got[:nblk], ref[:nblk]), rel_err(got, ref)
+    print("[gpu-case] full-size rechannelizer: error vs float64 %.3g (full job), %.3g (both jobs)" % (e_full, e_all),
+          flush=True)
+    assert got.shape == ref.shape == (nbt, 6400) and e_full < TOL and e_all < TOL
