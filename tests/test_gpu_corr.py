@@ -94,6 +94,19 @@ def test_allpairs_4bit(cr, rng, method):
     assert np.array_equal(got, ocorr.allpairs_4bit_block_sums(raw, 320))
 
 
+def test_allpairs_4bit_full_size(cr):
+    # the direct 4-bit workload at full size: 2 antennas x 2 pols x 2^17
+    # spectra x 128 channels, 4096-spectrum blocks, 10 pairs; both forms
+    # bit-exact against the oracle
+    from albatros_axion_detection_amd import synth
+    inp = synth.direct4(T=2 ** 17, nchan=128, seed=0)
+    exp = ocorr.allpairs_4bit_block_sums(inp, 4096)
+    for method in ("mfma", "valu"):
+        got = cr.xcorr_4bit_allpairs(inp, 4096, method=method)
+        print("[gpu-case] direct 4-bit full size", method, flush=True)
+        assert got.shape == (32, 10, 128, 2) and np.array_equal(got, exp), method
+
+
 def test_allpairs_4bit_ragged_columns(cr, rng):
     # 12 channels: the VALU form serves "auto", the matrix-core form refuses
     raw = rng.integers(0, 256, size=(5, 640, 12), dtype=np.uint8)
@@ -116,6 +129,26 @@ def test_allpairs_1bit(cr, method):
     inp = synth.direct1(ninput=16, T=1024, nchan=64, seed=7)
     got = cr.xcorr_1bit_allpairs(inp, 62, 512, method=method)
     assert np.array_equal(got, ocorr.allpairs_1bit_block_sums(inp, 62, 512))
+
+
+def test_allpairs_1bit_full_size(cr):
+    # the direct 1-bit workload at full size: 16 inputs x 2^20 spectra x 1024
+    # channels (4 GiB), 8192-spectrum blocks, 136 pairs: the matrix-core and
+    # XOR forms agree bit for bit everywhere, autocorrelations are exactly
+    # 2 per spectrum, and the first block's first 64 channels of every pair
+    # match the oracle
+    import torch
+    from albatros_axion_detection_amd import _lib, synth
+    inp = synth.direct1(ninput=16, T=2 ** 20, nchan=1024, seed=3, device=_lib.device())
+    a = cr.xcorr_1bit_allpairs(inp, 1024, 8192, method="mfma")
+    b = cr.xcorr_1bit_allpairs(inp, 1024, 8192, method="valu")
+    print("[gpu-case] direct 1-bit full size", flush=True)
+    assert a.shape == (128, 136, 1024, 2) and torch.equal(a, b)
+    autos = [p for p, (i, j) in enumerate(ocorr.pair_list(16)) if i == j]
+    assert torch.all(a[:, autos, :, 0] == 2 * 8192) and torch.all(a[:, autos, :, 1] == 0)
+    part = inp[:, :8192, :16].cpu().numpy()
+    exp = ocorr.allpairs_1bit_block_sums(part, 64, 8192)
+    assert np.array_equal(a[:1, :, :64].cpu().numpy(), exp)
 
 
 def test_allpairs_1bit_ragged_columns(cr):
