@@ -199,12 +199,14 @@ def test_xcorr_all_ant_torch_fortran_view(cr):
     assert np.max(np.abs(out.cpu().numpy() - ref) / (d[:, None] * d[None])) < 1e-5
 
 
-@pytest.mark.parametrize("M,K,F", [(8, 1024, 64), (5, 1000, 33), (2, 77, 9), (4, 300, 7), (3, 64, 5), (1, 40, 3)])
+@pytest.mark.parametrize("M,K,F", [(8, 1024, 64), (5, 1000, 33), (2, 77, 9), (4, 300, 7), (3, 64, 5), (1, 40, 3),
+                                   (8, 1024, 6400)])
 def test_xcorr_all_ant_precision_forms(cr, M, K, F):
     # diagonal-normalised error |dC_ij| / sqrt(C_ii C_jj) against the float64
     # oracle for each arithmetic; the FP16 form must show its 11-bit rounding
     # (so it is not silently an FP32 product) and the split forms must recover
-    # close to FP32 accuracy
+    # close to FP32 accuracy; (8, 1024, 6400) is the precision-correlator
+    # workload at full size
     from albatros_axion_detection_amd import synth
     x = synth.precision_xcorr(M=M, T=K, F=F, seed=3 * M + K)
     ref = ocorr.avg_xcorr_all_ant(x, K)
