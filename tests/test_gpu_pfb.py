@@ -575,3 +575,49 @@ def test_rechannelizer_full_size_against_fp64_oracle():
     print("[gpu-case] full-size rechannelizer: error vs float64 %.3g (full job), %.3g (both jobs)" % (e_full, e_all),
           flush=True)
     assert got.shape == ref.shape == (nbt, 6400) and e_full < TOL and e_all < TOL
+
+
+def test_batched_stream_full_size():
+    # the batched streaming workload at full size: 2 antennas x 2 pols, 300
+    # coarse channels, 64 arrival chunks of 2^10..2^14 spectra with runs of 5
+    # lost spectra (~0.5%), 10 jobs of 1024 re-PFB spectra over 19 200 fine
+    # channels.  The batched pass (two stage-1 lanes, one stage 2, FP32 and
+    # FP16 hi/lo correlators) against the per-chunk FP32 form, and its
+    # hipGraph replay against its eager run bit for bit (the per-chunk form is
+    # tied to the float64 oracle by the smaller cases above)
+    import torch
+    from albatros_axion_detection_amd import _lib, repfb, synth
+    dev = _lib.device()
+    acclen, cut, osamp, nblock, nchan, c0 = 4088, 4, 64, 1024, 300, 1000
+    packed, specnums, lengths = synth.batched_stream(nant=2, nchan=nchan, nchunks=64, lmin=2 ** 10, lmax=2 ** 14,
+                                                     loss=0.005, run=5, seed=4, device=dev)
+    nfixed = int(lengths.sum()) // acclen
+    pipe = repfb.RePFBPipeline(4, np.arange(c0, c0 + nchan), acclen, cut, osamp, nblock, c0 * osamp,
+                               (c0 + nchan) * osamp, tile_batch=16)
+    rm = repfb.rowmap_stream(specnums, 0, nfixed, acclen)
+    plist = [packed[m] for m in range(4)]
+    bufs = pipe.buffers(nfixed)
+    assert len(bufs["jobs"]) == 10 and len(bufs.get("lane_stream_bufs", [])) == 2
+    per_chunk = pipe.correlate_per_chunk(plist, [rm] * 4, nfixed, bufs).clone()
+    scale = torch.sqrt(torch.mean(torch.abs(per_chunk) ** 2))
+    for prec, tol in (("fp32", 5e-6), ("fp16x3", 5e-6)):
+        eager = pipe.correlate_into(plist, [rm] * 4, nfixed, bufs, precision=prec).clone()
+        e = float(torch.max(torch.abs(eager - per_chunk)) / scale)
+        print("[gpu-case] batched full size", prec, "max |d| / rms vs per-chunk %.3g" % e, flush=True)
+        assert e < tol, prec
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        pipe.correlate_into(plist, [rm] * 4, nfixed, bufs, precision="fp16x3")
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        vis = pipe.correlate_into(plist, [rm] * 4, nfixed, bufs, precision="fp16x3")
+    for rep in range(2):
+        bufs["vis"].zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        print("[gpu-case] batched full size graph replay", rep, flush=True)
+        assert torch.equal(vis, eager), rep
