@@ -195,7 +195,8 @@ __global__ __launch_bounds__(TT, 1024 / TT) void irfft_packed4_kernel(Packed4Row
     if (idx < nit) {
       fi[k] = idx / ncols;
       ci[k] = idx - fi[k] * ncols;
-      ch[k] = pk.chan[ci[k]];
+      const int cv = pk.chan[ci[k]];
+      ch[k] = cv <= M ? cv : -1;   // channel numbers outside 0..2048 are skipped like negative ones
     }
   }
   // Row-map entry of item k in group g and the packed byte at packed row p.
@@ -293,7 +294,7 @@ __global__ __launch_bounds__(TT, 1024 / TT) void irfft_packed4_kernel(Packed4Row
       const int c = idx - f * ncols;
       const int64_t row = g * F + f;
       const int cc = pk.chan[c];
-      if (row >= nrows || cc < 0) continue;
+      if (row >= nrows || cc < 0 || cc > M) continue;
       const int64_t p = packed_row(pk, row, nslice);
       if (p < 0) continue;
       const int byte = pk.bytes[p * ncols + c];

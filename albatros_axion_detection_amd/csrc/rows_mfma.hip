@@ -43,7 +43,7 @@ __global__ void rows_coeff_kernel(const int* __restrict__ chan, int64_t ncols, i
     double v = 0.0;
     // a repeated channel number keeps only its last column (numpy's fancy
     // assignment in baseband_data_classes.py:72)
-    bool last = c < ncols && chan[c] >= 0;
+    bool last = c < ncols && chan[c] >= 0 && chan[c] <= 2048;
     for (int64_t c2 = c + 1; last && c2 < ncols; ++c2) last = chan[c2] != chan[c];
     if (last) {
       const int64_t ch = chan[c];

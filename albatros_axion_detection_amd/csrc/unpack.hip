@@ -151,7 +151,7 @@ __global__ void scatter_kernel(const float2* __restrict__ spec, int64_t spec_ld,
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
     const int64_t p = i / ncols, k = i - p * ncols;
     const int64_t c = channels[k];
-    if (c < 0) continue;   // column dropped by the caller (an earlier copy of a repeated channel)
+    if (c < 0 || c >= nchans) continue;   // dropped by the caller (an earlier copy of a repeated channel) or out of range
     int64_t row = (specnum[p] - start) % nspec;
     if (row < 0) row += nspec;
     out[row * nchans + c] = spec[p * spec_ld + k];

@@ -50,7 +50,7 @@ int alb_hist(const uint8_t* data, uint64_t* hist, int64_t rowstart, int64_t rowe
              int mode, void* stream);
 /* baseband_data_classes.py:48 make_continuous_gpu: zero-filled [nspec][nchans] complex64 with
  * out[(specnum[p]-start) % nspec][channels[k]] = spec[p*spec_ld + k]; columns with channels[k] < 0
- * are skipped (callers mark all but the last copy of a repeated channel so). */
+ * (callers mark all but the last copy of a repeated channel so) or >= nchans are skipped. */
 int alb_make_continuous(const void* spec, int64_t spec_ld, const int64_t* specnum, int64_t npresent, int64_t start,
                         int64_t nspec, const int64_t* channels, int64_t ncols, void* out, int64_t nchans,
                         void* stream);
@@ -131,8 +131,8 @@ int alb_ipfb_columns_any(const float* y, int64_t ldy, int64_t tile_stride, int64
 /* Row half of the streaming iPFB with the 4-bit unpack and spec_num zero fill fused in
  * (unpacking.py:140 + baseband_data_classes.py:48 + pfb_gpu_utils.py:80, helper_gpu_stream.py:195-199):
  * bytes [rows][ncols] packed 4-bit spectra of one pol, rowmap [nchunk][acclen] from alb_build_rowmap,
- * chan[ncols] coarse channel numbers (< 2049; a negative number drops the column, which is how a
- * caller keeps only the last copy of a repeated channel); tiles tile0 .. tile0+ntiles-1 of
+ * chan[ncols] coarse channel numbers (a negative number drops the column, which is how a caller
+ * keeps only the last copy of a repeated channel; numbers above 2048 are dropped too); tiles tile0 .. tile0+ntiles-1 of
  * nslice = acclen+2*cut rows each -> out [ntiles][nslice][4096] float32. */
 int alb_ipfb_rows_packed4(const uint8_t* bytes, const int* rowmap, const int* chan, int64_t ncols, int64_t acclen,
                           int cut, int64_t tile0, int64_t ntiles, float* out, void* stream);

@@ -621,3 +621,33 @@ def test_batched_stream_full_size():
         torch.cuda.synchronize()
         print("[gpu-case] batched full size graph replay", rep, flush=True)
         assert torch.equal(vis, eager), rep
+
+
+def test_row_pass_skips_out_of_range_channels(rng):
+    # channel numbers above 2048 drop their column like negative ones, in the
+    # FFT and matrix-core row passes alike (no write outside the spectrum)
+    import torch
+    from albatros_axion_detection_amd import _lib
+    nslice, cut, ntiles = 64, 4, 1
+    acclen = nslice - 2 * cut
+    good = np.array([5, 700, 2048, 1024], dtype=np.int32)
+    chans = np.array([5, 3000, 700, -1, 2048, 4096, 1024, 2049], dtype=np.int32)
+    keep = np.array([0, 2, 4, 6])
+    packed = rng.integers(0, 256, size=(acclen, len(chans)), dtype=np.uint8)
+    rowmap = np.arange(acclen, dtype=np.int32).reshape(1, acclen)
+    dev = _lib.device()
+    s = _lib.stream()
+    outs = {}
+    for name, ch, pk in (("all", chans, packed), ("kept", good, np.ascontiguousarray(packed[:, keep]))):
+        y = torch.empty((ntiles, nslice, 4096), dtype=torch.float32, device=dev)
+        yb = torch.empty((ntiles, 4096 // 8, nslice, 8), dtype=torch.float32, device=dev)
+        coeff = torch.empty(int(_lib.load().alb_ipfb_rows_coeff_bytes(len(ch))), dtype=torch.uint8, device=dev)
+        pkd, rmd, chd = _lib.to_device(pk), _lib.to_device(rowmap), _lib.to_device(ch)
+        _lib.call("alb_ipfb_rows_packed4", _lib.ptr(pkd), _lib.ptr(rmd), _lib.ptr(chd), len(ch), acclen, cut, 0,
+                  ntiles, _lib.ptr(y), s)
+        _lib.call("alb_ipfb_rows_coeff", _lib.ptr(chd), len(ch), _lib.ptr(coeff), s)
+        _lib.call("alb_ipfb_rows_packed4_mfma", _lib.ptr(pkd), _lib.ptr(rmd), _lib.ptr(chd), len(ch), acclen, cut, 0,
+                  ntiles, _lib.ptr(coeff), _lib.ptr(yb), s)
+        outs[name] = (y.cpu().numpy(), yb.cpu().numpy())
+    assert np.array_equal(outs["all"][0], outs["kept"][0])
+    assert np.array_equal(outs["all"][1], outs["kept"][1])

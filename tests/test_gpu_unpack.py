@@ -143,6 +143,15 @@ def test_make_continuous(rng):
     e = ostream.make_continuous(sp, sn2, 1000 - 3 * acclen, ch, acclen, 2049)
     assert np.array_equal(g, e.astype(np.complex64))
     assert np.array_equal(g[(sn2[30] - 1000 + 3 * acclen) % acclen, ch], sp[30])
+    # channel numbers outside the output's columns are skipped (no write past a row)
+    ch3 = ch.copy()
+    ch3[4] = 2049
+    ch3[9] = 5000
+    g = bdc.make_continuous_gpu(sp, sn, 1000 - 3 * acclen, ch3, acclen, 2049)
+    ok = np.ones(len(ch), dtype=bool)
+    ok[[4, 9]] = False
+    e = ostream.make_continuous(sp[:, ok], sn, 1000 - 3 * acclen, ch[ok], acclen, 2049)
+    assert np.array_equal(g, e.astype(np.complex64))
 
 
 def test_dropped_packets(rng):
