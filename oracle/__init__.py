@@ -2,8 +2,9 @@
 
 TEST INFRASTRUCTURE ONLY.  Nothing in the product package imports this
 package.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
-``cpu_baseline`` leg of ``bench.py`` may use it, and only as the checker or
-as the timed CPU restatement of the reference algorithm -- never as the
+``cpu_baseline`` leg of ``bench.py`` may use it, and only as the reference
+results tests compare against or as the timed CPU restatement of the
+reference algorithm -- never as the
 thing measured on the GPU and never as a fallback.
 
 Modules
