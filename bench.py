@@ -13,7 +13,8 @@ Also reported: live per-kernel HIP-event times over the timed steps, the
 roofline of the dominant kernel, the pipeline-level roofline, a CPU baseline
 (the reference CPU algorithm restated, timed on a bounded sample of the same
 workload) and secondary workloads (direct 4-bit and 1-bit correlators, the
-precision correlator).
+precision correlator, the batched streaming pass against its per-chunk
+launch form).
 """
 import argparse
 import ctypes
