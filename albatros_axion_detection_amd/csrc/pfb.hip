@@ -70,8 +70,10 @@ struct DenseRows {
 // y[2n] + i y[2n+1] = conj(Z[f][n]) / N for rows row0 + f < nrows: row-major
 // with row stride ldo, or (blocked) in the tile layout out[t][k / 8][i][k % 8]
 // (tile t = row / nslice, i = row % nslice).
-template <int M, int F, int TT>
-__device__ __forceinline__ void irfft_store(const float2* Z, int64_t row0, int64_t nrows, int64_t nslice,
+// ZERO: also clear every entry of Z it reads (rows past nrows included), so Z
+// is all zeros again for the next group's scatter.
+template <int M, int F, int TT, bool ZERO = false>
+__device__ __forceinline__ void irfft_store(float2* Z, int64_t row0, int64_t nrows, int64_t nslice,
                                             float* __restrict__ out, int64_t ldo, bool blocked, int tid) {
   constexpr int N = 2 * M;
   constexpr int MP = Padded<M>::value;
@@ -87,15 +89,16 @@ __device__ __forceinline__ void irfft_store(const float2* Z, int64_t row0, int64
       static_assert((M / 4) % BPS == 0, "blocks must split evenly over the store instructions");
       const int c = tid & 3, f = (tid >> 2) % F, b0 = tid / LB;
       const int64_t row = row0 + f;
-      if (row < nrows) {
-        const int64_t t = row / nslice, i = row - t * nslice;
-        float* base = out + t * int64_t(N) * nslice + i * 8 + int64_t(b0) * 8 * nslice + c * 2;
-        const float2* zf = Z + f * MP;
+      const bool ok = row < nrows;
+      const int64_t t = row / nslice, i = row - t * nslice;
+      float* base = out + t * int64_t(N) * nslice + i * 8 + int64_t(b0) * 8 * nslice + c * 2;
+      float2* zf = Z + f * MP;
 #pragma unroll
-        for (int j = 0; j < (M / 4) / BPS; ++j) {
-          const float2 v = zf[pad16(4 * (b0 + BPS * j) + c)];
-          *reinterpret_cast<float2*>(base + int64_t(j) * BPS * 8 * nslice) = make_float2(v.x * sc, -v.y * sc);
-        }
+      for (int j = 0; j < (M / 4) / BPS; ++j) {
+        float2* zp = zf + pad16(4 * (b0 + BPS * j) + c);
+        const float2 v = *zp;
+        if constexpr (ZERO) *zp = make_float2(0.f, 0.f);
+        if (ok) *reinterpret_cast<float2*>(base + int64_t(j) * BPS * 8 * nslice) = make_float2(v.x * sc, -v.y * sc);
       }
       return;
     }
@@ -103,8 +106,10 @@ __device__ __forceinline__ void irfft_store(const float2* Z, int64_t row0, int64
   for (int idx = tid; idx < F * M; idx += TT) {
     const int f = idx / M, n = idx - f * M;
     const int64_t row = row0 + f;
+    float2* zp = Z + f * MP + pad16(n);
+    const float2 v = *zp;
+    if constexpr (ZERO) *zp = make_float2(0.f, 0.f);
     if (row >= nrows) continue;
-    const float2 v = Z[f * MP + pad16(n)];
     reinterpret_cast<float2*>(out + row * ldo)[n] = make_float2(v.x * sc, -v.y * sc);
   }
 }
@@ -229,7 +234,7 @@ __global__ __launch_bounds__(TT, 1024 / TT) void irfft_packed4_kernel(Packed4Row
   // Channel c contributes to Z[c] and Z[M - c]; a bin with one contributor
   // (no repeat of c, no channel M - c) is written with a plain store, the rest
   // are summed with LDS atomics, which cost about ten times a store.  The
-  // channel counts live in Z until the first group clears it.
+  // channel counts live in Z until it is cleared for the first group.
   int* cnt = reinterpret_cast<int*>(Z);
   for (int idx = tid; idx <= M; idx += TT) cnt[idx] = 0;
   __syncthreads();
@@ -252,15 +257,15 @@ __global__ __launch_bounds__(TT, 1024 / TT) void irfft_packed4_kernel(Packed4Row
     ob[k] = fi[k] * 2 * MP + 2 * pad16(M - c);
     own[k] = ch[k] >= 0 && ch[k] <= M && cnt[c] == 1 && (2 * c == M || cnt[M - c] == 0);
   }
+  __syncthreads();  // every thread has read the channel counts: Z starts all zeros
+  for (int idx = tid; idx < F * MP; idx += TT) Z[idx] = make_float2(0.f, 0.f);
   // one row group: zero fill, channel scatter, inverse transform, store
   auto transform = [&](int64_t g, const int* bc, const bool* vc) {
     // an opaque copy of the thread index: the transform's LDS addresses are
     // recomputed per group instead of being held in registers across the loop
     int tl = tid;
     asm volatile("" : "+v"(tl));
-    __syncthreads();  // the previous group is read out of Z
-    for (int idx = tl; idx < F * MP; idx += TT) Z[idx] = make_float2(0.f, 0.f);
-    __syncthreads();
+    __syncthreads();  // Z is all zeros: the initial fill, or the previous group's store cleared it
 #pragma unroll
     for (int k = 0; k < kRowItems; ++k) {
       if (vc[k]) {
@@ -315,7 +320,7 @@ __global__ __launch_bounds__(TT, 1024 / TT) void irfft_packed4_kernel(Packed4Row
     }
     __syncthreads();
     fft_lds<M, F, TT, 1, M, true, 2>(Z, TA, TB, tw.lb, 1, tl, W16, tw8);
-    irfft_store<M, F, TT>(Z, g * F, nrows, nslice, out, ldo, ldo == 8, tl);
+    irfft_store<M, F, TT, true>(Z, g * F, nrows, nslice, out, ldo, ldo == 8, tl);
   };
   // Two sets of prefetch registers used in turn (no copies of values still in
   // flight): while group it is transformed, the bytes of group it + 1 and the
