@@ -651,3 +651,43 @@ def test_row_pass_skips_out_of_range_channels(rng):
         outs[name] = (y.cpu().numpy(), yb.cpu().numpy())
     assert np.array_equal(outs["all"][0], outs["kept"][0])
     assert np.array_equal(outs["all"][1], outs["kept"][1])
+
+
+@pytest.mark.parametrize("group", [2, 4])
+def test_row_pass_partial_last_group(rng, group):
+    # odd nslice (101 rows) and 3 tiles: the last row group of the launch runs
+    # past the last spectrum (303 rows), whose entries the store still clears
+    # for the next group; dense row pass at 2 and 4 spectra per workgroup
+    # against the float64 irfft of the zero-filled rows
+    import torch
+    from albatros_axion_detection_amd import _lib
+    acclen, cut, ntiles = 93, 4, 3
+    nslice = acclen + 2 * cut
+    chans = np.concatenate([np.arange(10, 60), [0, 2048, 1024, 1990]]).astype(np.int32)
+    ncols = len(chans)
+    packed = rng.integers(0, 256, size=(ntiles * acclen, ncols), dtype=np.uint8)
+    rowmap = np.arange(ntiles * acclen, dtype=np.int32).reshape(ntiles, acclen)
+    rowmap[rng.random(rowmap.shape) < 0.1] = -1
+    pk, rm, ch = _lib.to_device(packed), _lib.to_device(rowmap), _lib.to_device(chans)
+    y = torch.full((ntiles, nslice, 4096), 7.0, dtype=torch.float32, device=_lib.device())
+    _lib.call("alb_tune_set", b"ipfb_rows_group", group)
+    try:
+        _lib.call("alb_ipfb_rows_packed4", _lib.ptr(pk), _lib.ptr(rm), _lib.ptr(ch), ncols, acclen, cut, 0, ntiles,
+                  _lib.ptr(y), _lib.stream())
+        a = y.cpu().numpy()
+    finally:
+        _lib.call("alb_tune_set", b"ipfb_rows_group", 0)
+    from oracle import unpack as ounpack
+    r, i = ounpack.decode_4bit_bytes(packed)
+    vals = (r + 1j * i).astype(np.complex128)
+    spec = np.zeros((ntiles, nslice, 2049), dtype=np.complex128)
+    for t in range(ntiles):
+        for row in range(nslice):
+            chunk, crow = (t - 1, acclen - 2 * cut + row) if row < 2 * cut else (t, row - 2 * cut)
+            if chunk < 0 or rowmap[chunk, crow] < 0:
+                continue
+            spec[t, row, chans] = vals[rowmap[chunk, crow]]
+    ref = np.fft.irfft(spec, n=4096, axis=-1)
+    e = rel_err(a, ref)
+    print("[gpu-case] partial last group, %d spectra per workgroup: %.3g" % (group, e), flush=True)
+    assert e < 5e-6
