@@ -443,7 +443,7 @@ __device__ __forceinline__ void cols_item(int64_t nstrips, int group, int64_t* s
 // Tile layout: element (i, k) at yt[(k / ldy) * ldy * NSL + i * ldy + k % ldy]:
 // row-major with row stride ldy when ldy >= the column count, or blocks of
 // ldy columns (each block's rows contiguous, ldy a multiple of CW) otherwise.
-template <int NSL, int CW, int T>
+template <int NSL, int CW, int T, int KPN = 2>
 __device__ __forceinline__ void cols_core(const float* __restrict__ yt, int64_t ldy, const float2* __restrict__ filt,
                                           int64_t ldf, int64_t k0, float2* U, const float2* TA, const float2* TB,
                                           int lb, int tid, const float2* W16) {
@@ -477,10 +477,14 @@ __device__ __forceinline__ void cols_core(const float* __restrict__ yt, int64_t 
   // f = tid % TPS + m TPS (m < KF) up to NSL / 2.  The first KP bins' taps are
   // fetched now so that their loads are in flight during the forward passes
   // (the rest are issued at the start of the filter loop, behind them).
+  // KPN: the fused column pass at nslice 4096 (KF = 9) takes 5, 3% faster
+  // than 2 at 111 VGPRs; all 9 take it past 128 VGPRs, below the two
+  // workgroups per CU that its LDS allows, and 50% slower.  The plain column
+  // pass keeps 2 (5 would take it to 127 VGPRs).
   constexpr int TPS = T / NP;
   constexpr int NF = NSL / 2 + 1;
   constexpr int KF = (NF + TPS - 1) / TPS;
-  constexpr int KP = KF < 2 ? KF : 2;
+  constexpr int KP = KF < KPN ? KF : KPN;
   const int fp = tid / TPS, f0 = tid - fp * TPS;
   const float2* h0p = filt + (k0 + 2 * fp) * ldf;
   const float2* h1p = h0p + ldf;
@@ -650,7 +654,7 @@ __device__ __forceinline__ void cols_repfb_tile(const float* __restrict__ y, int
   constexpr int H = R1 / 2;
   constexpr int SP = S >= 16 ? S + 1 : S;   // row of the exchange buffer (padded when it fits)
   constexpr int64_t C = 4096;
-  cols_core<NSL, CW, T>(y + tile * tile_stride, ldy, filt, ldf, k0, U, TA, TB, lb, tid, W16);
+  cols_core<NSL, CW, T, 5>(y + tile * tile_stride, ldy, filt, ldf, k0, U, TA, TB, lb, tid, W16);
 
   const int nkeep = NSL - 2 * cut;
   const int64_t g0 = row0 + tile * rows_per_tile;
