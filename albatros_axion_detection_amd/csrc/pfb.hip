@@ -1183,6 +1183,14 @@ __global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __rest
   // column k = tid + 256 r of bin q1 (StageA layout; bins 0 and H from slot 0)
   const int slot = (q1 == 0 || q1 == H) ? 0 : q1;
   const float2* src = A + m_in * a_stride + StageA<R1>::at(b, slot, tid);
+  // First radix-16 pass straight from memory: thread j takes k = j + 256 r
+  // (r < 16), twiddles, transforms and stores the Stockham pass output
+  // Z[16 j + r]; the remaining passes start at sub-transform size 16.  The
+  // loads are issued before the barrier that publishes the LDS tables.
+  static_assert(N / 16 == kTPB, "one first-pass butterfly per thread");
+  float2 v[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) v[r] = src[r * (64 * StageA<R1>::QP * 4)];
   // W_L^(q1 k) = W_L^(q1 c) W_4096^(q1 a) for k = R1 a + c; c = tid mod R1 is
   // the same for all of this thread's k (kTPB is a multiple of R1)
   static_assert(kTPB % R1 == 0, "thread stride must keep k mod R1 fixed");
@@ -1193,14 +1201,7 @@ __global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __rest
     wc = make_float2(float(cs), float(sn));
   }
   __syncthreads();
-  // First radix-16 pass straight from memory: thread j takes k = j + 256 r
-  // (r < 16), twiddles, transforms and stores the Stockham pass output
-  // Z[16 j + r]; the remaining passes start at sub-transform size 16.
-  static_assert(N / 16 == kTPB, "one first-pass butterfly per thread");
   {
-    float2 v[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) v[r] = src[r * (64 * StageA<R1>::QP * 4)];
     if (q1 == 0) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) v[r].y = 0.f;
