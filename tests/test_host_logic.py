@@ -57,3 +57,28 @@ def test_header_parse(tmp_path):
     assert list(h["channels"]) == [452, 453]
     assert h["gps_timestamp"] == 1234567 and h["gps_elevation"] == 3.0
     assert h["num_packets"] == 7
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # the benchmark measures one GPU; asked for more it exits with a message
+    # before importing torch
+    import subprocess
+    import sys
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(here, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 2 and "one GPU" in r.stderr and r.stdout == ""
+
+
+def test_bench_reads_committed_traffic_summaries():
+    # the roofline's traffic field comes from the rocprofv3 summaries under
+    # profiles/; each dominant-kernel candidate has one with a byte count
+    import importlib.util
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(here, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    for k in ("ipfb_columns_repfb", "ipfb_rows_packed4", "repfb_stage2", "repfb_stage1_edges"):
+        v = bench.read_traffic(k + "_rechannelizer")
+        assert v is not None and v > 1e6, k
+    assert bench.read_traffic("no_such_kernel") is None
