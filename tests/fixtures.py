@@ -140,3 +140,22 @@ def make_continuous_case():
     channels = [100, 200]
     specnum = np.asarray([10, 20, 30, 40, 50], dtype="int64")
     return nspec, nchans, channels, specnum
+
+
+def write_raw(path, chans, spp, specnums, bits=4, missing=()):
+    """A .raw file in the format of generate_test_baseband.py:52-152."""
+    bps = len(chans)
+    bpp = bps * spp + 4
+    hdr = np.asarray([8 * 10 + 8 * len(chans), bpp, len(chans), spp, bits, 0], dtype=">Q")
+    with open(path, "wb") as f:
+        hdr.tofile(f)
+        np.asarray(chans, dtype=">Q").tofile(f)
+        np.asarray([0, 1234567], dtype=">Q").tofile(f)
+        np.asarray([1.0, 2.0, 3.0], dtype=">d").tofile(f)
+        for p, s in enumerate(specnums):
+            if p in missing:
+                continue
+            spectrum = np.ones(bps, dtype="u1")
+            spectrum[2:] = spectrum[2:] << 4
+            np.asarray((int(s) % 2 ** 32, np.tile(spectrum, spp)),
+                       dtype=[("spec_num", ">I"), ("spectra", "%dB" % (bpp - 4))]).tofile(f)

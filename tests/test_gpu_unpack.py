@@ -190,3 +190,44 @@ def test_library_loaded_before_torch_in_a_fresh_process():
             "print('launch ok')\n") % root
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and "launch ok" in r.stdout, r.stdout + r.stderr
+
+
+def test_file_iterator_spec_num_wraps(tmp_path):
+    # test_specnum_wrap.py:30-69 on files built as generate_test_baseband.py:
+    # file 1 starts 10 spectra before the 2^32 wrap (5 spectra per packet,
+    # packet 2 lost), file 2 continues it, file 3 wraps again mid-file.  The
+    # iterator yields acclen = 10 spectra per step across files, gaps and
+    # wraps, present rows first in zero-filled (acclen, ncols) arrays.
+    from albatros_axion_detection_amd import baseband_data_classes as bdc
+    chans = np.ravel(np.column_stack((np.arange(452, 454), np.arange(452, 454))))
+    files = [str(tmp_path / ("specnum_wrap%d.raw" % k)) for k in (1, 2, 3)]
+    spec_start = 2 ** 32 - 10
+    fx.write_raw(files[0], chans, 5, [spec_start + 5 * i for i in range(7)], missing=(2,))
+    fx.write_raw(files[1], chans, 5, [25 + 5 * i for i in range(3)])
+    fx.write_raw(files[2], chans, 5, [40, 5, 10])
+    it = bdc.BasebandFileIterator(files, 0, 0, 10)
+    steps, overflowed = [], []
+    for _ in range(6):
+        steps.append(next(it))
+        overflowed.append(it.obj._overflowed)
+    lens = [len(d["specnums"]) for d in steps]
+    assert overflowed[0] and not overflowed[3]               # file 1 wraps, file 2 does not
+    print("[gpu-case] file iterator: present spectra per step", lens, flush=True)
+    # the reference's asserts
+    assert lens[0] == 10 and steps[0]["specnums"][-1] == spec_start + 9
+    assert lens[1] == 5 and steps[1]["specnums"][-1] == spec_start + 19
+    assert steps[3]["specnums"][-1] == spec_start + 39      # spans files 1 and 2
+    assert it._OVERFLOW_CTR == 2 and lens[5] == 5            # second wrap in file 3
+    # every step: present rows carry the written bytes (channel 452 -> 0x01,
+    # 453 -> 0x10 in both pols), the rest of the chunk is zero
+    for d, n in zip(steps, lens):
+        for pol in ("pol0", "pol1"):
+            a = _lib_host(d[pol])
+            assert a.shape == (10, 2)
+            assert np.all(a[:n] == np.array([1, 16], dtype=np.uint8)) and np.all(a[n:] == 0)
+        assert np.all(np.diff(np.asarray(d["specnums"])) > 0)
+
+
+def _lib_host(x):
+    from albatros_axion_detection_amd import _lib
+    return _lib.host(x) if not isinstance(x, np.ndarray) else x

@@ -5,6 +5,7 @@ import struct
 import numpy as np
 
 from oracle import stream as ostream
+from tests import fixtures as fx
 
 
 def test_chunk_planner_matches_oracle():
@@ -28,30 +29,11 @@ def test_missing_fraction_matches_oracle():
     assert np.allclose(got, ref, equal_nan=True)
 
 
-def _write_raw(path, chans, spp, specnums, bits=4, missing=()):
-    """A .raw file in the format of generate_test_baseband.py:52-152."""
-    bps = len(chans)
-    bpp = bps * spp + 4
-    hdr = np.asarray([8 * 10 + 8 * len(chans), bpp, len(chans), spp, bits, 0], dtype=">Q")
-    with open(path, "wb") as f:
-        hdr.tofile(f)
-        np.asarray(chans, dtype=">Q").tofile(f)
-        np.asarray([0, 1234567], dtype=">Q").tofile(f)
-        np.asarray([1.0, 2.0, 3.0], dtype=">d").tofile(f)
-        for p, s in enumerate(specnums):
-            if p in missing:
-                continue
-            spectrum = np.ones(bps, dtype="u1")
-            spectrum[2:] = spectrum[2:] << 4
-            np.asarray((int(s) % 2 ** 32, np.tile(spectrum, spp)),
-                       dtype=[("spec_num", ">I"), ("spectra", "%dB" % (bpp - 4))]).tofile(f)
-
-
 def test_header_parse(tmp_path):
     from albatros_axion_detection_amd import baseband_data_classes as bdc
     chans = np.ravel(np.column_stack((np.arange(452, 454), np.arange(452, 454))))
     p = str(tmp_path / "a.raw")
-    _write_raw(p, chans, 5, [2 ** 32 - 10 + 5 * i for i in range(7)])
+    fx.write_raw(p, chans, 5, [2 ** 32 - 10 + 5 * i for i in range(7)])
     h = bdc.get_header(p)
     assert h["length_channels"] == 2 and h["spectra_per_packet"] == 5 and h["bit_mode"] == 4
     assert list(h["channels"]) == [452, 453]
