@@ -151,6 +151,46 @@ def van_vleck_correction(R0, R1, I0, I1, rowcount):
 PRECISION_MODES = {"fp32": 0, "bf16x3": 1, "fp16": 2, "fp16x3": 3}
 
 
+def _same_array(a, b):
+    if a is b:
+        return True
+    try:
+        return (np.asarray(a).__array_interface__["data"][0] == np.asarray(b).__array_interface__["data"][0]
+                and isinstance(a, np.ndarray) and isinstance(b, np.ndarray))
+    except (TypeError, KeyError):
+        return False
+
+
+def avg_xcorr_4bit_float_gpu(data0, data1, specnum0, specnum1):
+    """correlations.py:180-204: one baseline of unpacked (float) spectra,
+    ``mean_t data0[t] * conj(data1[t])`` per channel over the
+    ``N = len(specnum0)`` present rows (packed at the top, as the float
+    BasebandFileIterator yields them), when ``specnum0`` and ``specnum1`` are
+    the same array (one antenna).  Returns ``(complex64 [nchan], N)``; the
+    reference leaves the two-antenna case unimplemented and returns None, as
+    does this.  The sum runs on the matrix-core correlator (two inputs); no
+    present rows gives NaN, as the reference's 0/0."""
+    if not _same_array(specnum0, specnum1):
+        return None
+    t = _lib.torch()
+    N = len(specnum0)
+    d0 = _lib.to_device(data0, t.complex64)
+    d1 = _lib.to_device(data1, t.complex64)
+    if d0.dim() != 2 or tuple(d0.shape[1:]) != tuple(d1.shape[1:]) or d0.shape[0] < N or d1.shape[0] < N:
+        raise ValueError("data0 and data1 must be (n_spectra >= len(specnum), n_channels)")
+    nchan = int(d0.shape[1])
+    if N == 0:
+        res = t.full((nchan,), complex("nan"), dtype=t.complex64, device=d0.device)
+    else:
+        # correlator input [channel][time][input] (F-order (2, N, nchan))
+        x = t.stack([d0[:N], d1[:N]], dim=-1).permute(1, 0, 2).contiguous()
+        o = t.empty((nchan, 2, 2), dtype=t.complex64, device=x.device)
+        scale = np.float32(1.0) / np.float32(N)
+        _lib.call("alb_xcorr_all_ant", _lib.ptr(x), 2, N, nchan, _lib.ptr(o), float(scale), _lib.stream())
+        res = o[:, 1, 0].contiguous()   # out[i + 2 (j + 2 f)] = C_ij: baseline (0, 1)
+    return (_lib.host(res) if _lib.is_numpy(data0) else res), N
+
+
 def avg_xcorr_all_ant_gpu(x, nant, npol, ntime, nfreq, split=1, scratch=None, out=None, precision="fp32"):
     """correlations_gpu.py:95-121: ``out[i,j,f] = sum_k x[i,k,f] conj(x[j,k,f]) / K``.
 

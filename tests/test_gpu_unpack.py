@@ -231,3 +231,40 @@ def test_file_iterator_spec_num_wraps(tmp_path):
 def _lib_host(x):
     from albatros_axion_detection_amd import _lib
     return _lib.host(x) if not isinstance(x, np.ndarray) else x
+
+
+def test_float_file_iterator_and_single_baseline_average(tmp_path):
+    # test_specnum_wrap_gpu.py:7-97: the float iterator over the same three
+    # files (channel 452 = 0+1j, 453 = 1+0j in both pols), and
+    # avg_xcorr_4bit_float_gpu over the present rows of one antenna
+    import torch
+    from albatros_axion_detection_amd import baseband_data_classes as bdc
+    from albatros_axion_detection_amd import correlations as cr
+    chans = np.ravel(np.column_stack((np.arange(452, 454), np.arange(452, 454))))
+    files = [str(tmp_path / ("specnum_wrap%d.raw" % k)) for k in (1, 2, 3)]
+    spec_start = 2 ** 32 - 10
+    fx.write_raw(files[0], chans, 5, [spec_start + 5 * i for i in range(7)], missing=(2,))
+    fx.write_raw(files[1], chans, 5, [25 + 5 * i for i in range(3)])
+    fx.write_raw(files[2], chans, 5, [40, 5, 10])
+    it = bdc.BasebandFileIterator(files, 0, 0, 10, type="float")
+    for step in range(6):
+        d = next(it)
+        n = len(d["specnums"])
+        p0, p1, sn = d["pol0"], d["pol1"], d["specnums"]
+        assert isinstance(p0, torch.Tensor) and p0.is_cuda and isinstance(sn, np.ndarray)
+        assert torch.allclose(p0[:n, 0] + p0[:n, 1], torch.tensor(1 + 1j, dtype=torch.complex64, device=p0.device))
+        assert torch.allclose(p0[:n, 0] + p1[:n, 1], torch.tensor(1 + 1j, dtype=torch.complex64, device=p0.device))
+        assert torch.count_nonzero(p0[n:]) == 0 and torch.count_nonzero(p1[n:]) == 0
+        for a, b, want in ((p0, p1, 1), (p0, p0, 1), (p0[:, 0:1], p0[:, 1:], 1j)):
+            xc, nr = cr.avg_xcorr_4bit_float_gpu(a, b, sn, sn)
+            assert nr == n and torch.allclose(xc, torch.full_like(xc, want), atol=1e-7), (step, want)
+        if step == 0:
+            assert it.obj._overflowed and n == 10 and sn[-1] == spec_start + 9
+        if step == 1:
+            assert n == 5 and sn[-1] == spec_start + 19
+        if step == 3:
+            assert not it.obj._overflowed and sn[-1] == spec_start + 39
+        if step == 5:
+            assert it._OVERFLOW_CTR == 2 and n == 5
+    # another antenna's spec_num array: the reference returns None (unimplemented)
+    assert cr.avg_xcorr_4bit_float_gpu(p0, p1, sn, sn.copy()) is None
