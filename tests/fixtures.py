@@ -142,8 +142,10 @@ def make_continuous_case():
     return nspec, nchans, channels, specnum
 
 
-def write_raw(path, chans, spp, specnums, bits=4, missing=()):
-    """A .raw file in the format of generate_test_baseband.py:52-152."""
+def write_raw(path, chans, spp, specnums, bits=4, missing=(), payload=None):
+    """A .raw file in the format of generate_test_baseband.py:52-152; the
+    spectra are that script's pattern, or ``payload[p]`` (uint8, one row of
+    bytes_per_packet - 4 per packet) when given."""
     bps = len(chans)
     bpp = bps * spp + 4
     hdr = np.asarray([8 * 10 + 8 * len(chans), bpp, len(chans), spp, bits, 0], dtype=">Q")
@@ -155,7 +157,11 @@ def write_raw(path, chans, spp, specnums, bits=4, missing=()):
         for p, s in enumerate(specnums):
             if p in missing:
                 continue
-            spectrum = np.ones(bps, dtype="u1")
-            spectrum[2:] = spectrum[2:] << 4
-            np.asarray((int(s) % 2 ** 32, np.tile(spectrum, spp)),
+            if payload is None:
+                spectrum = np.ones(bps, dtype="u1")
+                spectrum[2:] = spectrum[2:] << 4
+                body = np.tile(spectrum, spp)
+            else:
+                body = payload[p]
+            np.asarray((int(s) % 2 ** 32, body),
                        dtype=[("spec_num", ">I"), ("spectra", "%dB" % (bpp - 4))]).tofile(f)

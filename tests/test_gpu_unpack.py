@@ -268,3 +268,31 @@ def test_float_file_iterator_and_single_baseline_average(tmp_path):
             assert it._OVERFLOW_CTR == 2 and n == 5
     # another antenna's spec_num array: the reference returns None (unimplemented)
     assert cr.avg_xcorr_4bit_float_gpu(p0, p1, sn, sn.copy()) is None
+
+
+@pytest.mark.parametrize("bits", [4, 1])
+def test_partial_read_matches_full_read(tmp_path, bits):
+    # test_partial_read.py:47-56: reading the first packets of a file
+    # (readlen) gives the leading rows of a full read, for 4-bit and 1-bit
+    # files (random payload, a gap in the spec_nums)
+    import torch
+    from albatros_axion_detection_amd import baseband_data_classes as bdc
+    rng = np.random.default_rng(bits)
+    spp, npk = 5, 40
+    if bits == 4:
+        chans = np.ravel(np.column_stack((np.arange(452, 462), np.arange(452, 462))))   # 2 bytes per channel
+    else:
+        chans = np.arange(452, 472, 2)                                                 # 2 channels per byte
+    payload = rng.integers(0, 256, size=(npk, len(chans) * spp), dtype=np.uint8)
+    specnums = [1000 + spp * i + (spp * 3 if i >= 20 else 0) for i in range(npk)]
+    f = str(tmp_path / "part.raw")
+    fx.write_raw(f, chans, spp, specnums, bits=bits, payload=payload)
+    full = bdc.BasebandPacked(f)
+    part = bdc.BasebandPacked(f, readlen=25)
+    en = spp * 25
+    assert np.array_equal(np.asarray(part.spec_idx), np.asarray(full.spec_idx)[:en])
+    for pol in ("pol0", "pol1"):
+        a, b = getattr(full, pol), getattr(part, pol)
+        a = a if isinstance(a, torch.Tensor) else torch.as_tensor(a)
+        b = b if isinstance(b, torch.Tensor) else torch.as_tensor(b)
+        assert b.shape[0] == en and torch.equal(a[:en].cpu(), b.cpu()), pol
