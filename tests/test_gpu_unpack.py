@@ -296,3 +296,62 @@ def test_partial_read_matches_full_read(tmp_path, bits):
         a = a if isinstance(a, torch.Tensor) else torch.as_tensor(a)
         b = b if isinstance(b, torch.Tensor) else torch.as_tensor(b)
         assert b.shape[0] == en and torch.equal(a[:en].cpu(), b.cpu()), pol
+
+
+@pytest.mark.parametrize("bits", [4, 1])
+def test_file_readers_float_and_packed_agree(tmp_path, bits):
+    # test_4bit_unpack_with_file.py:22-33, test_4bit_xcorr_with_file.py:31-96,
+    # test_1bit_xcorr_with_file.py:36-96 on a generated file (the reference's
+    # .raw fixtures are not in its tree): the float reader matches an
+    # independent decode of the packet bytes, the packed reader matches the
+    # oracle's sortpols, and the packed correlators equal the float reader's
+    # data correlated in float64, for the whole file and for the reference's
+    # row / channel windows
+    import torch
+    from albatros_axion_detection_amd import baseband_data_classes as bdc
+    from albatros_axion_detection_amd import correlations as cr
+    rng = np.random.default_rng(40 + bits)
+    spp, npk = 7, 30
+    if bits == 4:
+        chans = np.ravel(np.column_stack((np.arange(452, 468), np.arange(452, 468))))   # 16 channels
+    else:
+        chans = np.arange(452, 468, 2)                                                 # 16 channels
+    payload = rng.integers(0, 256, size=(npk, len(chans) * spp), dtype=np.uint8)
+    f = str(tmp_path / "file.raw")
+    fx.write_raw(f, chans, spp, [5000 + spp * i for i in range(npk)], bits=bits, payload=payload)
+    fl = bdc.BasebandFloat(f)
+    pk = bdc.BasebandPacked(f)
+    nrows, nchan = npk * spp, fl.length_channels
+    assert nchan == 16 and fl.pol0.shape == (nrows, nchan) and pk.pol0.shape[0] == nrows
+    raw = payload.reshape(nrows, -1)
+    if bits == 4:
+        want = ounpack.unpack_4bit(raw, 0, nrows, np.arange(nchan), nchan)
+    else:
+        want = ounpack.unpack_1bit(raw, 0, nrows, 0, nchan, nchan)
+    f0, f1 = _lib_host(fl.pol0).astype(np.complex128), _lib_host(fl.pol1).astype(np.complex128)
+    assert np.array_equal(f0, want[0]) and np.array_equal(f1, want[1])
+    sp = ounpack.sortpols(raw, nchan, bits, 0, nrows, 0, nchan)
+    assert np.array_equal(_lib_host(pk.pol0), sp[0]) and np.array_equal(_lib_host(pk.pol1), sp[1])
+
+    def close(got, ref):
+        got = np.asarray(got)
+        assert np.all(np.abs(got - ref) <= 2e-7 * np.maximum(1.0, np.abs(ref))), (got, ref)
+
+    windows = [(0, nrows, 0, nchan)] + ([(10, 20, 4, 12)] if bits == 4 else [(10, 20, 6, 10), (10, 20, 6, 12)])
+    for r0, r1, c0, c1 in windows:
+        if (r0, c0) == (0, 0):
+            p0, p1 = pk.pol0, pk.pol1
+        else:
+            w = bdc.BasebandPacked(f, rowstart=r0, rowend=r1, chanstart=c0, chanend=c1)
+            p0, p1 = w.pol0, w.pol1
+            assert p0.shape[0] == r1 - r0
+        x0, x1 = f0[r0:r1, c0:c1], f1[r0:r1, c0:c1]
+        n = r1 - r0
+        specnums = np.arange(n)
+        ref = np.sum(x0 * np.conj(x1), axis=0) / n
+        if bits == 4:
+            close(cr.avg_xcorr_4bit(p0, p1, specnums), ref)
+            for p, x in ((p0, x0), (p1, x1)):
+                close(cr.avg_autocorr_4bit(p, specnums), np.sum(np.real(x * np.conj(x)).astype(int), axis=0) / n)
+        else:
+            close(cr.avg_xcorr_1bit(p0, p1, specnums, c1 - c0), ref)
