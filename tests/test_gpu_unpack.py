@@ -355,3 +355,78 @@ def test_file_readers_float_and_packed_agree(tmp_path, bits):
                 close(cr.avg_autocorr_4bit(p, specnums), np.sum(np.real(x * np.conj(x)).astype(int), axis=0) / n)
         else:
             close(cr.avg_xcorr_1bit(p0, p1, specnums, c1 - c0), ref)
+
+
+def test_hist_1bit_matches_packed_bits(tmp_path):
+    # test_histogram_1bit.py:8-66: counting the bits of the sortpols-packed
+    # 1-bit pols (4 channels a byte, (re, im) of channel 4j+k at bits
+    # 7-2k, 6-2k) gives the file's per-pol histogram, and bits are conserved
+    from albatros_axion_detection_amd import baseband_data_classes as bdc
+    rng = np.random.default_rng(17)
+    spp, npk = 9, 40
+    chans = np.arange(452, 468, 2)                                                     # 16 channels
+    payload = rng.integers(0, 256, size=(npk, len(chans) * spp), dtype=np.uint8)
+    f = str(tmp_path / "h1.raw")
+    fx.write_raw(f, chans, spp, [300 + spp * i for i in range(npk)], bits=1, payload=payload)
+    pk = bdc.BasebandPacked(f)
+    assert pk.pol0.shape[0] == len(pk.spec_idx)
+    for mode, pol in ((0, pk.pol0), (1, pk.pol1)):
+        b = _lib_host(pol).astype(np.int64)
+        nrow, ncol = b.shape
+        want = np.zeros((2, 4 * ncol), dtype=np.int64)
+        for k in range(4):
+            for s in (7 - 2 * k, 6 - 2 * k):
+                ones = ((b >> s) & 1).sum(axis=0)
+                want[1, k::4] += ones
+                want[0, k::4] += nrow - ones
+        got = np.asarray(_lib_host(pk.get_hist(mode=mode))).astype(np.int64)
+        assert want.sum() == nrow * ncol * 8 and got.sum() == nrow * ncol * 8
+        assert np.array_equal(got, want), mode
+
+
+@pytest.mark.parametrize("bits", [4, 1])
+@pytest.mark.parametrize("split", [120, 130])
+@pytest.mark.parametrize("acclen,nchunks", [(1000, 3), (3000, 1)])
+def test_file_iterator_across_files_matches_full_file(tmp_path, bits, split, acclen, nchunks):
+    # test_missing_block_bw_files.py:42-66 on generated files: a recording
+    # with a 2500-spectrum gap, read whole and as two files split before or
+    # at the gap; every iterator step holds the full file's rows whose
+    # spec_nums fall in the step, in order, and zeros after them (a step
+    # inside the gap is empty).  The start (idxstart spectra after the first
+    # file's first spectrum) lies inside the first file, as the reference's
+    # iterator requires (baseband_data_classes.py:634)
+    import torch
+    from albatros_axion_detection_amd import baseband_data_classes as bdc
+    rng = np.random.default_rng(bits * 1000 + split)
+    spp, npk, gap_at, gap = 5, 400, 130, 2500
+    if bits == 4:
+        chans = np.ravel(np.column_stack((np.arange(452, 468), np.arange(452, 468))))
+    else:
+        chans = np.arange(452, 468, 2)
+    payload = rng.integers(0, 256, size=(npk, len(chans) * spp), dtype=np.uint8)
+    specs = [1000 + spp * p + (gap if p >= gap_at else 0) for p in range(npk)]
+    full_path = str(tmp_path / "full.raw")
+    fx.write_raw(full_path, chans, spp, specs, bits=bits, payload=payload)
+    parts = [str(tmp_path / "part1.raw"), str(tmp_path / "part2.raw")]
+    fx.write_raw(parts[0], chans, spp, specs[:split], bits=bits, payload=payload[:split])
+    fx.write_raw(parts[1], chans, spp, specs[split:], bits=bits, payload=payload[split:])
+    full = bdc.BasebandPacked(full_path)
+    sidx = np.asarray(full.spec_idx)
+    f0, f1 = _lib_host(full.pol0), _lib_host(full.pol1)
+    idxstart = 550
+    it = bdc.BasebandFileIterator(parts, 0, idxstart, acclen, nchunks=nchunks)
+    counts = []
+    for k in range(nchunks):
+        d = next(it)
+        lo = sidx[0] + idxstart + k * acclen
+        sel = np.where((sidx >= lo) & (sidx < lo + acclen))[0]
+        n = len(sel)
+        counts.append(n)
+        assert np.array_equal(d["specnums"], sidx[sel]), k
+        p0, p1 = _lib_host(d["pol0"]), _lib_host(d["pol1"])
+        assert p0.shape == (acclen, f0.shape[1])
+        assert np.array_equal(p0[:n], f0[sel]) and np.array_equal(p1[:n], f1[sel]), k
+        assert not p0[n:].any() and not p1[n:].any(), k
+    assert counts == ([100, 0, 400] if acclen == 1000 else [500])
+    with pytest.raises(StopIteration):
+        next(it)
