@@ -18,6 +18,8 @@ def unpk():
 
 
 def test_unpack_4bit_reference_fixture(unpk):
+    # test_4bit_float_unpack.py:28-125 and test_4bit_unpack_gpu.py:6-89: the
+    # reference's packed fixture, all channels, one channel, and channels [2, 5]
     packed = fx.packed_4bit()
     t0, t1 = fx.expected_complex_4bit()
     p0, p1 = unpk.unpack_4bit(packed, 0, 2, np.arange(0, 10), 10)
