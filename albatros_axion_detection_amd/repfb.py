@@ -382,7 +382,7 @@ class RePFBPipeline:
             # input's small edge-block launches overlap the next input's row
             # and column passes; the current stream waits for all of them.
             ws, per = self._workspace(nbt, M)
-            lane_bufs = bufs.get("lane_stream_bufs", [stream])
+            lane_bufs = bufs.get("lane_stream_bufs", [stream])[:self.lanes]
             nl = len(lane_bufs)
             if nl == 1:
                 for m in range(M):
@@ -413,6 +413,76 @@ class RePFBPipeline:
         _lib.call("alb_xcorr_all_ant_strided", _lib.ptr(xin), M, K, F, njobs, F * M, M, K * F * M, _lib.ptr(vis),
                   float(scale), PRECISION_MODES[precision], _lib.stream())
         return vis.permute(3, 2, 1, 0)
+
+    # launch-shape knobs of the library tried by autotune (0 = built-in rule)
+    TUNE_KNOBS = (("ipfb_cols_width", (0, 2, 8)), ("ipfb_cols_group", (0, 2, 8, 64)), ("ipfb_rows_group", (0, 2, 4)))
+
+    def autotune(self, packed_list, rowmaps, nchunks, bufs, precision="fp32", reps=3, tile_batches=(4, 8, 16, 32)):
+        """Choose the chunk schedule and launch shapes of :meth:`correlate_into`
+        for this pipeline's shapes by timing whole passes on the given inputs
+        (no counterpart in the reference, whose launch shapes are cuFFT's and
+        cuBLAS's own).
+
+        Coordinate descent, one setting at a time with the others at their
+        best so far: iPFB tiles per row / column launch (``tile_batch``, up
+        to ``nchunks``), concurrent stage-1 lanes (up to the lane buffers
+        ``bufs`` holds), then the library's launch-shape knobs in
+        ``TUNE_KNOBS`` (``alb_tune_set``; process-wide, so they also apply
+        to other pipelines in this process).  Each candidate runs one untimed
+        pass and ``reps`` timed ones (HIP events on the current stream, which
+        joins every lane stream); its time is the median.  The best settings
+        stay in force.  Returns ``{"tile_batch", "lanes", <knob>: value ...,
+        "ms": best pass time, "tried": {candidate: ms}}``.  Larger tile
+        batches allocate larger tile buffers (kept for later passes).
+        """
+        t = _lib.torch()
+
+        def timed():
+            self.correlate_into(packed_list, rowmaps, nchunks, bufs, precision=precision)
+            ms = []
+            for _ in range(int(reps)):
+                a = t.cuda.Event(enable_timing=True)
+                b = t.cuda.Event(enable_timing=True)
+                a.record()
+                self.correlate_into(packed_list, rowmaps, nchunks, bufs, precision=precision)
+                b.record()
+                b.synchronize()
+                ms.append(a.elapsed_time(b))
+            return float(np.median(ms))
+
+        lib = _lib.load()
+        tried = {}
+        nl_max = len(bufs.get("lane_stream_bufs", [None]))
+
+        def set_tile_batch(v):
+            self.tile_batch = v
+            for lane in range(nl_max):
+                self._tile_buffer(min(v, nchunks), lane)
+
+        def set_lanes(v):
+            self.lanes = v
+
+        def knob_setter(key):
+            return lambda v: _lib.call("alb_tune_set", key.encode(), int(v))
+
+        space = [("tile_batch", sorted({min(int(v), nchunks) for v in tile_batches} | {min(self.tile_batch, nchunks)}),
+                  self.tile_batch, set_tile_batch),
+                 ("lanes", sorted({1, nl_max}), min(self.lanes, nl_max), set_lanes)]
+        space += [(key, vals, int(lib.alb_tune_get(key.encode())), knob_setter(key)) for key, vals in self.TUNE_KNOBS]
+        best = {}
+        for name, vals, cur, setter in space:
+            setter(cur)
+            times = {}
+            for v in vals if cur in vals else [cur] + list(vals):
+                setter(v)
+                times[v] = timed()
+                tried["%s=%s" % (name, v)] = round(times[v], 4)
+            pick = min(times, key=times.get)
+            setter(pick)
+            best[name] = pick
+        best["ms"] = timed()
+        best["tried"] = tried
+        return best
 
     def correlate_per_chunk(self, packed_list, rowmaps, nchunks, bufs):
         """The per-chunk launch form in plain FP32, following the reference's

@@ -396,7 +396,10 @@ def batched_stream_leg(args, torch):
     -> 4x4 visibilities per 1024-spectrum job.  The batched form (fused column
     pass, one stage 2 for every input, every job correlated at once, FP16 hi/lo
     split correlator) is one hipGraph replay; the per-chunk form issues
-    per-chunk / per-job launches in plain FP32."""
+    per-chunk / per-job launches in plain FP32.  Unless --no-autotune, the
+    pipeline's autotuner first picks the batched form's tile batch, lanes and
+    launch shapes on these inputs (reported under "autotune"); the library's
+    knobs are restored afterwards."""
     from albatros_axion_detection_amd import _lib, repfb, synth
     dev = _lib.device()
     acclen, cut, osamp, nblock, nchan, c0 = 4088, 4, 64, 1024, 300, 1000
@@ -411,6 +414,14 @@ def batched_stream_leg(args, torch):
     rms = [rm] * 4
     bufs = pipe.buffers(nfixed)
     njobs = len(bufs["jobs"])
+    lib = _lib.load()
+    knob_keys = [k for k, _ in repfb.RePFBPipeline.TUNE_KNOBS]
+    knobs_before = {k: int(lib.alb_tune_get(k.encode())) for k in knob_keys}
+    tune = None
+    if not args.no_autotune:
+        log("[bench] autotuning the batched pass ...")
+        tune = pipe.autotune(plist, rms, nfixed, bufs, precision="fp16x3", reps=2)
+        log("[bench] autotune: %s" % json.dumps({k: v for k, v in tune.items() if k != "tried"}))
     # batched form: the correlator in the FP16 hi/lo split (matrix cores, FP32
     # accumulation, ~6e-7 vs FP64); the per-chunk form stays plain FP32
     run_b = lambda: pipe.correlate_into(plist, rms, nfixed, bufs, precision="fp16x3")
@@ -449,7 +460,6 @@ def batched_stream_leg(args, torch):
     ms_eager = timed(run_b, 3)
     # the per-chunk form at its own best column-pass width (it runs the plain
     # column pass, whose fastest width differs from the fused pass's)
-    lib = _lib.load()
     tuned_w = int(lib.alb_tune_get(b"ipfb_cols_width"))
     chunk_ms = {}
     for w in (2, 4, 8):
@@ -458,6 +468,8 @@ def batched_stream_leg(args, torch):
         chunk_ms[w] = timed(run_p, 2)
     _lib.call("alb_tune_set", b"ipfb_cols_width", tuned_w)
     ms_chunk = min(chunk_ms.values())
+    for k, v in knobs_before.items():
+        _lib.call("alb_tune_set", k.encode(), v)
     nsamp = 4 * nfixed * acclen * 4096
     per_spec = 300 + 2 * 16384 + 2 * 19200 * 8 // 64       # raw + time stream + kept fine channels, per input
     hbm = 4 * nfixed * acclen * per_spec
@@ -473,7 +485,8 @@ def batched_stream_leg(args, torch):
             "hbm_frac": round(hbm / (ms_graph * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "batched_correlator": "fp16x3", "per_chunk_correlator": "fp32",
             "per_chunk_ms_by_cols_width": {str(w): round(v, 3) for w, v in chunk_ms.items()},
-            "max_abs_diff_over_rms_vs_per_chunk": err}
+            "max_abs_diff_over_rms_vs_per_chunk": err,
+            "autotune": tune}
 
 
 def copy_calibration(torch, nbytes=1 << 30, reps=5):
@@ -511,6 +524,8 @@ def main():
                     help="column-pass strips per XCD group (0: library default, 2 up to 4 tiles, 64 above)")
     ap.add_argument("--rows-group", type=int, default=0, choices=[0, 2, 4],
                     help="spectra per workgroup of the FFT row pass (0: library default, 2 up to 4 tiles, 4 above)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the pipeline autotuner before the batched streaming leg")
     ap.add_argument("--sweep", action="store_true",
                     help="time every column-pass strip width x row-pass group before timing and keep the fastest")
     ap.add_argument("--row-pass", default="auto", choices=["auto", "fft", "mfma"],

@@ -691,3 +691,39 @@ def test_row_pass_partial_last_group(rng, group):
     e = rel_err(a, ref)
     print("[gpu-case] partial last group, %d spectra per workgroup: %.3g" % (group, e), flush=True)
     assert e < 5e-6
+
+
+@pytest.mark.gpu
+def test_autotune_keeps_results_and_leaves_its_choice_in_force():
+    # the pipeline autotuner times whole batched passes over tile batches,
+    # lanes and the launch-shape knobs; the settings it leaves in force are
+    # the ones it reports, and the visibilities after tuning equal those
+    # before it to FP32 rounding
+    from albatros_axion_detection_amd import _lib, repfb, synth
+    lib = _lib.load()
+    nant, nchan, acclen, cut, osamp, nblock = 2, 24, 4088, 4, 64, 16
+    packed, specnums, lengths = synth.batched_stream(nant=nant, nchan=nchan, nchunks=6, lmin=1024, lmax=4096,
+                                                     loss=0.01, run=5, seed=12)
+    nfixed = int(lengths.sum()) // acclen
+    chan = np.arange(100, 100 + nchan)
+    M = 2 * nant
+    pipe = repfb.RePFBPipeline(M, chan, acclen, cut, osamp, nblock, 100 * osamp, (100 + nchan) * osamp, tile_batch=2)
+    rm = repfb.rowmap_stream(specnums, 0, nfixed, acclen)
+    plist = [_lib.to_device(packed[m]) for m in range(M)]
+    bufs = pipe.buffers(nfixed)
+    before = pipe.correlate_into(plist, [rm] * M, nfixed, bufs).cpu().numpy().copy()
+    keys = [k for k, _ in repfb.RePFBPipeline.TUNE_KNOBS]
+    try:
+        print("[gpu-case] autotune nfixed", nfixed, flush=True)
+        best = pipe.autotune(plist, [rm] * M, nfixed, bufs, reps=1, tile_batches=(1, 2, 4))
+        print("[gpu-case] autotune", {k: v for k, v in best.items() if k != "tried"}, flush=True)
+        assert best["tile_batch"] == pipe.tile_batch and best["tile_batch"] in (1, 2, min(4, nfixed))
+        assert best["lanes"] == pipe.lanes and best["lanes"] in (1, 2)
+        for k in keys:
+            assert lib.alb_tune_get(k.encode()) == best[k], k
+        assert best["ms"] > 0 and len(best["tried"]) >= 2 + 1 + sum(len(v) for _, v in repfb.RePFBPipeline.TUNE_KNOBS)
+        after = pipe.correlate_into(plist, [rm] * M, nfixed, bufs).cpu().numpy()
+    finally:
+        for k in keys:
+            lib.alb_tune_set(k.encode(), 0)
+    assert rel_err(after, before) < 1e-6
