@@ -60,6 +60,66 @@ __global__ void unpack1_kernel(const uint8_t* __restrict__ data, float2* __restr
   }
 }
 
+// Row-tiled forms of the three kernels above and below for aligned shapes:
+// x indexes a group of columns, rows run over blockIdx.y + gridDim.y * j, so no
+// 64-bit division per element, and each lane writes 16 bytes per pol (4-bit
+// and 1-bit unpack, two complex64 each) or 4 bytes per pol (sortpols, four
+// channels).
+__global__ void unpack4_pair_kernel(const uint8_t* __restrict__ data, float4* __restrict__ pol0,
+                                    float4* __restrict__ pol1, int64_t rowstart, int64_t nrows,
+                                    const int64_t* __restrict__ channels, int64_t npair, int64_t nchan) {
+  const int64_t kp = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (kp >= npair) return;
+  const int64_t o0 = 2 * channels[2 * kp], o1 = 2 * channels[2 * kp + 1];
+  for (int64_t r = blockIdx.y; r < nrows; r += gridDim.y) {
+    const uint8_t* row = data + (rowstart + r) * 2 * nchan;
+    const int a = *reinterpret_cast<const uint16_t*>(row + o0);   // byte 0 pol0, byte 1 pol1
+    const int b = *reinterpret_cast<const uint16_t*>(row + o1);
+    pol0[r * npair + kp] = make_float4(float(nib((a >> 4) & 15)), float(nib(a & 15)), float(nib((b >> 4) & 15)),
+                                       float(nib(b & 15)));
+    pol1[r * npair + kp] = make_float4(float(nib((a >> 12) & 15)), float(nib((a >> 8) & 15)),
+                                       float(nib((b >> 12) & 15)), float(nib((b >> 8) & 15)));
+  }
+}
+
+__global__ void unpack1_rows_kernel(const uint8_t* __restrict__ data, float4* __restrict__ pol0,
+                                    float4* __restrict__ pol1, int64_t rowstart, int64_t nrows, int64_t chanstart,
+                                    int64_t nb, int64_t nchan) {
+  const int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (j >= nb) return;
+  const int64_t bps = nchan / 2;
+  for (int64_t r = blockIdx.y; r < nrows; r += gridDim.y) {
+    const int b = data[(r + rowstart) * bps + chanstart / 2 + j];
+    auto pm = [&](int s) { return float(2 * ((b >> s) & 1) - 1); };
+    pol0[r * nb + j] = make_float4(pm(7), pm(6), pm(3), pm(2));
+    pol1[r * nb + j] = make_float4(pm(5), pm(4), pm(1), pm(0));
+  }
+}
+
+// four channels per lane: bytes c0p0 c0p1 c1p0 c1p1 c2p0 c2p1 c3p0 c3p1 -> one
+// 32-bit word per pol
+__global__ void sortpols4_quad_kernel(const uint8_t* __restrict__ data, uint32_t* __restrict__ pol0,
+                                      uint32_t* __restrict__ pol1, int64_t rowstart, int64_t nrows, int64_t nquad,
+                                      int64_t nchan, int64_t chanstart) {
+  const int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (q >= nquad) return;
+  for (int64_t r = blockIdx.y; r < nrows; r += gridDim.y) {
+    const uint2 v = *reinterpret_cast<const uint2*>(data + (r + rowstart) * 2 * nchan + 2 * chanstart + 8 * q);
+    pol0[r * nquad + q] = (v.x & 0xffu) | ((v.x >> 8) & 0xff00u) | ((v.y & 0xffu) << 16) | ((v.y << 8) & 0xff000000u);
+    pol1[r * nquad + q] = ((v.x >> 8) & 0xffu) | ((v.x >> 16) & 0xff00u) | ((v.y << 8) & 0xff0000u) |
+                          (v.y & 0xff000000u);
+  }
+}
+
+static inline dim3 grid_rows(int64_t nx, int64_t nrows, int bs = 256) {
+  const int64_t gx = (nx + bs - 1) / bs;
+  int64_t gy = nrows < 4096 ? nrows : 4096;
+  if (gy < 1) gy = 1;
+  return dim3(unsigned(gx), unsigned(gy));
+}
+
+static inline bool aligned(const void* p, uintptr_t a) { return (reinterpret_cast<uintptr_t>(p) & (a - 1)) == 0; }
+
 // unpacking.c:263-275: 4-bit per-pol byte split.
 __global__ void sortpols4_kernel(const uint8_t* __restrict__ data, uint8_t* __restrict__ pol0, uint8_t* __restrict__ pol1,
                                  int64_t rowstart, int64_t nrows, int64_t ncols, int64_t nchan, int64_t chanstart) {
@@ -181,6 +241,12 @@ int alb_unpack_4bit_float(const uint8_t* data, void* pol0, void* pol1, int64_t r
   const int64_t nrows = rowend - rowstart;
   if (nrows * ncols == 0) return ALB_OK;
   ALB_REQUIRE(data && pol0 && pol1 && channels, "alb_unpack_4bit_float: null pointer");
+  if (ncols % 2 == 0 && aligned(data, 2) && aligned(pol0, 16) && aligned(pol1, 16) && ncols / 2 < (int64_t(1) << 31)) {
+    hipLaunchKernelGGL(unpack4_pair_kernel, grid_rows(ncols / 2, nrows), dim3(256), 0, as_stream(stream), data,
+                       (float4*)pol0, (float4*)pol1, rowstart, nrows, channels, ncols / 2, nchan);
+    ALB_LAUNCH_CHECK("unpack4_pair_kernel");
+    return ALB_OK;
+  }
   hipLaunchKernelGGL(unpack4_kernel, grid_for(nrows * ncols), dim3(256), 0, as_stream(stream), data, (float2*)pol0,
                      (float2*)pol1, rowstart, nrows, channels, ncols, nchan);
   ALB_LAUNCH_CHECK("unpack4_kernel");
@@ -197,6 +263,12 @@ int alb_unpack_1bit_float(const uint8_t* data, void* pol0, void* pol1, int64_t r
               "alb_unpack_1bit_float: channel range must start even and hold an even count");
   const int64_t nrows = rowend - rowstart, nb = (chanend - chanstart) / 2;
   if (nrows * nb == 0) return ALB_OK;
+  if (aligned(pol0, 16) && aligned(pol1, 16) && nb < (int64_t(1) << 31)) {
+    hipLaunchKernelGGL(unpack1_rows_kernel, grid_rows(nb, nrows), dim3(256), 0, as_stream(stream), data, (float4*)pol0,
+                       (float4*)pol1, rowstart, nrows, chanstart, nb, nchan);
+    ALB_LAUNCH_CHECK("unpack1_rows_kernel");
+    return ALB_OK;
+  }
   hipLaunchKernelGGL(unpack1_kernel, grid_for(nrows * nb), dim3(256), 0, as_stream(stream), data, (float2*)pol0,
                      (float2*)pol1, rowstart, nrows, chanstart, nb, nchan);
   ALB_LAUNCH_CHECK("unpack1_kernel");
@@ -212,6 +284,13 @@ int alb_sortpols(const uint8_t* data, uint8_t* pol0, uint8_t* pol1, int64_t rows
   ALB_REQUIRE(data && pol0 && pol1, "alb_sortpols: null pointer");
   if (bit_depth == 4) {
     ALB_REQUIRE(ncols == chanend - chanstart, "alb_sortpols: 4-bit ncols must equal chanend-chanstart");
+    if (ncols % 4 == 0 && chanstart % 4 == 0 && nchan % 4 == 0 && aligned(data, 8) && aligned(pol0, 4) &&
+        aligned(pol1, 4) && ncols / 4 < (int64_t(1) << 31)) {
+      hipLaunchKernelGGL(sortpols4_quad_kernel, grid_rows(ncols / 4, nrows), dim3(256), 0, as_stream(stream), data,
+                         (uint32_t*)pol0, (uint32_t*)pol1, rowstart, nrows, ncols / 4, nchan, chanstart);
+      ALB_LAUNCH_CHECK("sortpols4_quad_kernel");
+      return ALB_OK;
+    }
     hipLaunchKernelGGL(sortpols4_kernel, grid_for(nrows * ncols), dim3(256), 0, as_stream(stream), data, pol0, pol1,
                        rowstart, nrows, ncols, nchan, chanstart);
   } else if (bit_depth == 1) {

@@ -300,8 +300,9 @@ def cpu_baseline(ctx, budget_s):
 
 
 def secondary(args, torch):
-    """Direct 4-bit (2 ant x 2 pol, 128 ch, 2^17 spectra, 4096-blocks), precision
-    correlator (8 x 1024 x 6400, FP32 on MFMA) and direct 1-bit correlator."""
+    """Unpack family (2^18 spectra x 1024 channels), direct 4-bit (2 ant x 2
+    pol, 128 ch, 2^17 spectra, 4096-blocks), precision correlator (8 x 1024 x
+    6400, FP32 on MFMA) and direct 1-bit correlator."""
     from albatros_axion_detection_amd import _lib, synth
     out = {}
     dev = _lib.device()
@@ -318,6 +319,36 @@ def secondary(args, torch):
         torch.cuda.synchronize()
         return a.elapsed_time(b) / n
 
+    # unpack family on 2^18 spectra x 1024 channels of random raw bytes:
+    # algorithmic bytes = raw bytes read + both pols written
+    T, nch = 2 ** 18, 1024
+    torch.cuda.manual_seed(5)
+    raw4 = torch.randint(0, 256, (T, 2 * nch), dtype=torch.uint8, device=dev)
+    raw1 = raw4[:, : nch // 2].contiguous()
+    chans = torch.arange(nch, dtype=torch.int64, device=dev)
+    c0 = torch.empty((T, nch), dtype=torch.complex64, device=dev)
+    c1 = torch.empty_like(c0)
+    s0 = torch.empty((T, nch), dtype=torch.uint8, device=dev)
+    s1 = torch.empty_like(s0)
+    legs = {
+        "unpack_4bit_float": (lambda: _lib.call("alb_unpack_4bit_float", _lib.ptr(raw4), _lib.ptr(c0), _lib.ptr(c1), 0,
+                                                T, _lib.ptr(chans), nch, nch, _lib.stream()),
+                              T * 2 * nch + 2 * T * nch * 8),
+        "unpack_1bit_float": (lambda: _lib.call("alb_unpack_1bit_float", _lib.ptr(raw1), _lib.ptr(c0), _lib.ptr(c1), 0,
+                                                T, 0, nch, nch, _lib.stream()),
+                              T * nch // 2 + 2 * T * nch * 8),
+        "sortpols_4bit": (lambda: _lib.call("alb_sortpols", _lib.ptr(raw4), _lib.ptr(s0), _lib.ptr(s1), 0, T, nch, nch,
+                                            4, 0, nch, _lib.stream()),
+                          T * 2 * nch + 2 * T * nch),
+    }
+    forms = {}
+    for name, (fn, nbytes) in legs.items():
+        ms = timeit(fn)
+        forms[name] = {"ms": round(ms, 5), "GB_s": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                       "hbm_frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    out["unpack"] = {"workload": "2^18 spectra x 1024 channels, raw bytes -> complex64 pols (unpack) or per-pol "
+                                 "bytes (sortpols); GB/s of raw bytes read + pols written", "forms": forms}
+    del raw4, raw1, chans, c0, c1, s0, s1
     # direct 4-bit: 4 inputs x 2^17 spectra x 128 ch, 10 pairs, 4096-blocks
     lib = _lib.load()
     inp = _lib.to_device(synth.direct4(T=2 ** 17, nchan=128, seed=0))

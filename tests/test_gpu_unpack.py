@@ -432,3 +432,24 @@ def test_file_iterator_across_files_matches_full_file(tmp_path, bits, split, acc
     assert counts == ([100, 0, 400] if acclen == 1000 else [500])
     with pytest.raises(StopIteration):
         next(it)
+
+
+def test_unpack_row_tiled_forms(unpk, rng):
+    # shapes that take the row-tiled kernels (even column count / 4-aligned
+    # channel span, 16-byte aligned pols) with more rows than the 4096-row
+    # grid, beside shapes that keep the per-element kernels
+    nrows, nchan = 5000, 256
+    raw4 = rng.integers(0, 256, size=(nrows + 3, 2 * nchan), dtype=np.uint8)
+    for ch in (np.arange(0, 256), np.sort(rng.choice(nchan, 90, replace=False)), np.arange(7, 100)):
+        p0, p1 = unpk.unpack_4bit(raw4, 3, nrows + 3, ch, nchan)
+        e0, e1 = ounpack.unpack_4bit(raw4, 3, nrows + 3, ch, nchan)
+        assert np.array_equal(p0, e0) and np.array_equal(p1, e1), len(ch)
+    for cs, ce in ((0, 256), (12, 252), (10, 250), (4, 8)):
+        g = unpk.sortpols(raw4, nchan, 4, 3, nrows + 3, cs, ce)
+        e = ounpack.sortpols(raw4, nchan, 4, 3, nrows + 3, cs, ce)
+        assert np.array_equal(g[0], e[0]) and np.array_equal(g[1], e[1]), (cs, ce)
+    raw1 = rng.integers(0, 256, size=(nrows + 2, nchan // 2), dtype=np.uint8)
+    for cs, ce in ((0, 256), (6, 200), (2, 4)):
+        p0, p1 = unpk.unpack_1bit(raw1, 2, nrows + 2, np.arange(cs, ce), nchan)
+        e0, e1 = ounpack.unpack_1bit(raw1, 2, nrows + 2, cs, ce, nchan)
+        assert np.array_equal(p0, e0) and np.array_equal(p1, e1), (cs, ce)
