@@ -65,6 +65,13 @@ __global__ void unpack1_kernel(const uint8_t* __restrict__ data, float2* __restr
 // 64-bit division per element, and each lane writes 16 bytes per pol (4-bit
 // and 1-bit unpack, two complex64 each) or 4 bytes per pol (sortpols, four
 // channels).
+// streaming 16-byte store: the unpacked pols are written once and not read back here
+typedef float nt_float4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void store_nt(float4* p, float a, float b, float c, float d) {
+  const nt_float4 v = {a, b, c, d};
+  __builtin_nontemporal_store(v, reinterpret_cast<nt_float4*>(p));
+}
+
 __global__ void unpack4_pair_kernel(const uint8_t* __restrict__ data, float4* __restrict__ pol0,
                                     float4* __restrict__ pol1, int64_t rowstart, int64_t nrows,
                                     const int64_t* __restrict__ channels, int64_t npair, int64_t nchan) {
@@ -75,10 +82,10 @@ __global__ void unpack4_pair_kernel(const uint8_t* __restrict__ data, float4* __
     const uint8_t* row = data + (rowstart + r) * 2 * nchan;
     const int a = *reinterpret_cast<const uint16_t*>(row + o0);   // byte 0 pol0, byte 1 pol1
     const int b = *reinterpret_cast<const uint16_t*>(row + o1);
-    pol0[r * npair + kp] = make_float4(float(nib((a >> 4) & 15)), float(nib(a & 15)), float(nib((b >> 4) & 15)),
-                                       float(nib(b & 15)));
-    pol1[r * npair + kp] = make_float4(float(nib((a >> 12) & 15)), float(nib((a >> 8) & 15)),
-                                       float(nib((b >> 12) & 15)), float(nib((b >> 8) & 15)));
+    store_nt(&pol0[r * npair + kp], float(nib((a >> 4) & 15)), float(nib(a & 15)), float(nib((b >> 4) & 15)),
+             float(nib(b & 15)));
+    store_nt(&pol1[r * npair + kp], float(nib((a >> 12) & 15)), float(nib((a >> 8) & 15)),
+             float(nib((b >> 12) & 15)), float(nib((b >> 8) & 15)));
   }
 }
 
@@ -91,8 +98,8 @@ __global__ void unpack1_rows_kernel(const uint8_t* __restrict__ data, float4* __
   for (int64_t r = blockIdx.y; r < nrows; r += gridDim.y) {
     const int b = data[(r + rowstart) * bps + chanstart / 2 + j];
     auto pm = [&](int s) { return float(2 * ((b >> s) & 1) - 1); };
-    pol0[r * nb + j] = make_float4(pm(7), pm(6), pm(3), pm(2));
-    pol1[r * nb + j] = make_float4(pm(5), pm(4), pm(1), pm(0));
+    store_nt(&pol0[r * nb + j], pm(7), pm(6), pm(3), pm(2));
+    store_nt(&pol1[r * nb + j], pm(5), pm(4), pm(1), pm(0));
   }
 }
 
