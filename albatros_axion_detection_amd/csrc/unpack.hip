@@ -74,14 +74,23 @@ __device__ __forceinline__ void store_nt(float4* p, float a, float b, float c, f
 
 __global__ void unpack4_pair_kernel(const uint8_t* __restrict__ data, float4* __restrict__ pol0,
                                     float4* __restrict__ pol1, int64_t rowstart, int64_t nrows,
-                                    const int64_t* __restrict__ channels, int64_t npair, int64_t nchan) {
+                                    const int64_t* __restrict__ channels, int64_t npair, int64_t nchan, bool word_rows) {
   const int64_t kp = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (kp >= npair) return;
   const int64_t o0 = 2 * channels[2 * kp], o1 = 2 * channels[2 * kp + 1];
+  // consecutive channels starting on a 4-byte boundary: one 32-bit load
+  const bool word = word_rows && o1 == o0 + 2 && (o0 & 3) == 0;
   for (int64_t r = blockIdx.y; r < nrows; r += gridDim.y) {
     const uint8_t* row = data + (rowstart + r) * 2 * nchan;
-    const int a = *reinterpret_cast<const uint16_t*>(row + o0);   // byte 0 pol0, byte 1 pol1
-    const int b = *reinterpret_cast<const uint16_t*>(row + o1);
+    int a, b;   // byte 0 pol0, byte 1 pol1 of each channel
+    if (word) {
+      const uint32_t w = *reinterpret_cast<const uint32_t*>(row + o0);
+      a = int(w & 0xffffu);
+      b = int(w >> 16);
+    } else {
+      a = *reinterpret_cast<const uint16_t*>(row + o0);
+      b = *reinterpret_cast<const uint16_t*>(row + o1);
+    }
     store_nt(&pol0[r * npair + kp], float(nib((a >> 4) & 15)), float(nib(a & 15)), float(nib((b >> 4) & 15)),
              float(nib(b & 15)));
     store_nt(&pol1[r * npair + kp], float(nib((a >> 12) & 15)), float(nib((a >> 8) & 15)),
@@ -250,7 +259,8 @@ int alb_unpack_4bit_float(const uint8_t* data, void* pol0, void* pol1, int64_t r
   ALB_REQUIRE(data && pol0 && pol1 && channels, "alb_unpack_4bit_float: null pointer");
   if (ncols % 2 == 0 && aligned(data, 2) && aligned(pol0, 16) && aligned(pol1, 16) && ncols / 2 < (int64_t(1) << 31)) {
     hipLaunchKernelGGL(unpack4_pair_kernel, grid_rows(ncols / 2, nrows), dim3(256), 0, as_stream(stream), data,
-                       (float4*)pol0, (float4*)pol1, rowstart, nrows, channels, ncols / 2, nchan);
+                       (float4*)pol0, (float4*)pol1, rowstart, nrows, channels, ncols / 2, nchan,
+                       aligned(data, 4) && nchan % 2 == 0);
     ALB_LAUNCH_CHECK("unpack4_pair_kernel");
     return ALB_OK;
   }
