@@ -152,13 +152,14 @@ __global__ void sortpols4_kernel(const uint8_t* __restrict__ data, uint8_t* __re
 __global__ void sortpols1_kernel(const uint8_t* __restrict__ data, uint8_t* __restrict__ pol0, uint8_t* __restrict__ pol1,
                                  int64_t rowstart, int64_t nrows, int64_t ncols, int64_t nchan, int64_t chanstart,
                                  int cndn) {
-  const int64_t n = nrows * ncols;
+  // lane = output byte j of every row r = blockIdx.y + gridDim.y * k
+  const int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (j >= ncols) return;
   const int64_t bps = nchan / 2;
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
-    const int64_t r = i / ncols, j = i - r * ncols;
+  const int use = j == ncols - 1 ? cndn : 0;   // 0 = two full raw bytes
+  for (int64_t r = blockIdx.y; r < nrows; r += gridDim.y) {
+    const int64_t i = r * ncols + j;
     const int64_t idx = bps * (r + rowstart) + 2 * j + chanstart / 2;
-    const bool last = (j == ncols - 1);
-    const int use = last ? cndn : 0;   // 0 = two full raw bytes
     const int b1 = data[idx];
     const int b2 = (use == 0 || use == 3) ? int(data[idx + 1]) : 0;
     const int p0c0 = b1 & 192, p1c0 = b1 & 48, p0c1 = b1 & 12, p1c1 = b1 & 3;
@@ -313,7 +314,8 @@ int alb_sortpols(const uint8_t* data, uint8_t* pol0, uint8_t* pol1, int64_t rows
   } else if (bit_depth == 1) {
     ALB_REQUIRE(chanstart % 2 == 0, "ERROR: Start channel index must be even.");
     ALB_REQUIRE(ncols == (chanend - chanstart + 3) / 4, "alb_sortpols: 1-bit ncols must be ceil((chanend-chanstart)/4)");
-    hipLaunchKernelGGL(sortpols1_kernel, grid_for(nrows * ncols), dim3(256), 0, as_stream(stream), data, pol0, pol1,
+    ALB_REQUIRE(ncols < (int64_t(1) << 31), "alb_sortpols: too many columns");
+    hipLaunchKernelGGL(sortpols1_kernel, grid_rows(ncols, nrows), dim3(256), 0, as_stream(stream), data, pol0, pol1,
                        rowstart, nrows, ncols, nchan, chanstart, int((chanend - chanstart) % 4));
   } else {
     ALB_REQUIRE(false, "alb_sortpols: bit_depth must be 1 or 4 (got %d)", bit_depth);

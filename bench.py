@@ -340,6 +340,9 @@ def secondary(args, torch):
         "sortpols_4bit": (lambda: _lib.call("alb_sortpols", _lib.ptr(raw4), _lib.ptr(s0), _lib.ptr(s1), 0, T, nch, nch,
                                             4, 0, nch, _lib.stream()),
                           T * 2 * nch + 2 * T * nch),
+        "sortpols_1bit": (lambda: _lib.call("alb_sortpols", _lib.ptr(raw1), _lib.ptr(s0), _lib.ptr(s1), 0, T, nch // 4,
+                                            nch, 1, 0, nch, _lib.stream()),
+                          T * nch // 2 + 2 * T * nch // 4),
     }
     forms = {}
     for name, (fn, nbytes) in legs.items():

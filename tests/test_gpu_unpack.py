@@ -453,3 +453,7 @@ def test_unpack_row_tiled_forms(unpk, rng):
         p0, p1 = unpk.unpack_1bit(raw1, 2, nrows + 2, np.arange(cs, ce), nchan)
         e0, e1 = ounpack.unpack_1bit(raw1, 2, nrows + 2, cs, ce, nchan)
         assert np.array_equal(p0, e0) and np.array_equal(p1, e1), (cs, ce)
+    for cs, ce in ((0, 256), (2, 255), (6, 201), (8, 10)):
+        g = unpk.sortpols(raw1, nchan, 1, 2, nrows + 2, cs, ce)
+        e = ounpack.sortpols(raw1, nchan, 1, 2, nrows + 2, cs, ce)
+        assert np.array_equal(g[0], e[0]) and np.array_equal(g[1], e[1]), (cs, ce)
