@@ -12,9 +12,9 @@ value = ADC-equivalent samples per second (coarse spectra x 4096 / s / 1e6).
 Also reported: live per-kernel HIP-event times over the timed steps, the
 roofline of the dominant kernel, the pipeline-level roofline, a CPU baseline
 (the reference CPU algorithm restated, timed on a bounded sample of the same
-workload) and secondary workloads (direct 4-bit and 1-bit correlators, the
-precision correlator, the batched streaming pass against its per-chunk
-launch form).
+workload) and secondary workloads (4-bit / 1-bit unpack and sortpols
+throughput, direct 4-bit and 1-bit correlators, the precision correlator,
+the batched streaming pass, autotuned, against its per-chunk launch form).
 """
 import argparse
 import ctypes
