@@ -1,7 +1,9 @@
 // Baseband unpack / repack kernels: 4-bit and 1-bit decode to complex64,
 // per-pol repack (sortpols), raw-value histograms, spec_num zero-fill scatter
-// and the dropped-packet zero fill.  All memory-bound; grid-stride loops with
-// 64-bit indexing, every thread owning whole output elements.
+// and the dropped-packet zero fill.  All memory-bound.  The unpack and sortpols
+// kernels are row-tiled (a lane owns a column or column group and walks rows
+// blockIdx.y + gridDim.y * m); the others are grid-stride loops with 64-bit
+// indexing, every thread owning whole output elements.
 #include "common.hpp"
 
 namespace alb {
@@ -32,10 +34,13 @@ static inline dim3 grid_for(int64_t n, int bs = 256, int64_t cap = 65536) {
 __global__ void unpack4_kernel(const uint8_t* __restrict__ data, float2* __restrict__ pol0, float2* __restrict__ pol1,
                                int64_t rowstart, int64_t nrows, const int64_t* __restrict__ channels, int64_t ncols,
                                int64_t nchan) {
-  const int64_t n = nrows * ncols;
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
-    const int64_t r = i / ncols, k = i - r * ncols;
-    const uint8_t* p = data + (rowstart + r) * 2 * nchan + 2 * channels[k];
+  // lane = column k of every row r = blockIdx.y + gridDim.y * m
+  const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (k >= ncols) return;
+  const int64_t off = 2 * channels[k];
+  for (int64_t r = blockIdx.y; r < nrows; r += gridDim.y) {
+    const int64_t i = r * ncols + k;
+    const uint8_t* p = data + (rowstart + r) * 2 * nchan + off;
     const int b0 = p[0], b1 = p[1];
     pol0[i] = make_float2(float(nib(b0 >> 4)), float(nib(b0 & 15)));
     pol1[i] = make_float2(float(nib(b1 >> 4)), float(nib(b1 & 15)));
@@ -139,9 +144,10 @@ static inline bool aligned(const void* p, uintptr_t a) { return (reinterpret_cas
 // unpacking.c:263-275: 4-bit per-pol byte split.
 __global__ void sortpols4_kernel(const uint8_t* __restrict__ data, uint8_t* __restrict__ pol0, uint8_t* __restrict__ pol1,
                                  int64_t rowstart, int64_t nrows, int64_t ncols, int64_t nchan, int64_t chanstart) {
-  const int64_t n = nrows * ncols;
-  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
-    const int64_t r = i / ncols, k = i - r * ncols;
+  const int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (k >= ncols) return;
+  for (int64_t r = blockIdx.y; r < nrows; r += gridDim.y) {
+    const int64_t i = r * ncols + k;
     const uint8_t* p = data + (r + rowstart) * 2 * nchan + 2 * (k + chanstart);
     pol0[i] = p[0];
     pol1[i] = p[1];
@@ -265,7 +271,8 @@ int alb_unpack_4bit_float(const uint8_t* data, void* pol0, void* pol1, int64_t r
     ALB_LAUNCH_CHECK("unpack4_pair_kernel");
     return ALB_OK;
   }
-  hipLaunchKernelGGL(unpack4_kernel, grid_for(nrows * ncols), dim3(256), 0, as_stream(stream), data, (float2*)pol0,
+  ALB_REQUIRE(ncols < (int64_t(1) << 31), "alb_unpack_4bit_float: too many columns");
+  hipLaunchKernelGGL(unpack4_kernel, grid_rows(ncols, nrows), dim3(256), 0, as_stream(stream), data, (float2*)pol0,
                      (float2*)pol1, rowstart, nrows, channels, ncols, nchan);
   ALB_LAUNCH_CHECK("unpack4_kernel");
   return ALB_OK;
@@ -309,7 +316,8 @@ int alb_sortpols(const uint8_t* data, uint8_t* pol0, uint8_t* pol1, int64_t rows
       ALB_LAUNCH_CHECK("sortpols4_quad_kernel");
       return ALB_OK;
     }
-    hipLaunchKernelGGL(sortpols4_kernel, grid_for(nrows * ncols), dim3(256), 0, as_stream(stream), data, pol0, pol1,
+    ALB_REQUIRE(ncols < (int64_t(1) << 31), "alb_sortpols: too many columns");
+    hipLaunchKernelGGL(sortpols4_kernel, grid_rows(ncols, nrows), dim3(256), 0, as_stream(stream), data, pol0, pol1,
                        rowstart, nrows, ncols, nchan, chanstart);
   } else if (bit_depth == 1) {
     ALB_REQUIRE(chanstart % 2 == 0, "ERROR: Start channel index must be even.");
