@@ -24,6 +24,7 @@ SIGNATURES = {
     "alb_version": (_i32, []),
     "alb_stream_synchronize": (_i32, [_vp]),
     "alb_fill_zero": (_i32, [_vp, _i64, _vp]),
+    "alb_prepare_transform": (_i32, [_i64]),
     "alb_unpack_4bit_float": (_i32, [_vp, _vp, _vp, _i64, _i64, _vp, _i64, _i64, _vp]),
     "alb_unpack_1bit_float": (_i32, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _vp]),
     "alb_sortpols": (_i32, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i32, _i64, _i64, _vp]),
@@ -38,6 +39,8 @@ SIGNATURES = {
     "alb_xcorr_all_ant": (_i32, [_vp, _i64, _i64, _i64, _vp, _f32, _vp]),
     "alb_xcorr_all_ant_prec": (_i32, [_vp, _i64, _i64, _i64, _vp, _f32, _i32, _vp]),
     "alb_xcorr_all_ant_strided": (_i32, [_vp, _i64, _i64, _i64, _i64, _i64, _i64, _i64, _vp, _f32, _i32, _vp]),
+    "alb_xcorr_all_ant_f64acc": (_i32, [_vp, _i64, _i64, _i64, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _vp]),
+    "alb_cgemm_strided_batched": (_i32, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _vp]),
     "alb_matft_filter": (_i32, [_vp, _i32, _i64, _i64, _f64, _vp, _vp, _vp]),
     "alb_filter_from_matft": (_i32, [_vp, _i64, _f64, _vp, _vp]),
     "alb_irfft_rows": (_i32, [_vp, _i64, _vp, _i64, _i64, _i64, _vp]),

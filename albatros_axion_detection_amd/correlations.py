@@ -239,18 +239,82 @@ def avg_xcorr_all_ant_gpu(x, nant, npol, ntime, nfreq, split=1, scratch=None, ou
 
 
 def avg_xcorr_all_ant_cpu(x, nant, npol, ntime, nfreq, split=1, out=None):
-    """correlations.py:115-151 call surface (computed on the GPU); returns a
-    C-order ``(M, M, nfreq)`` complex64 like the reference."""
-    if ntime % split != 0:
+    """correlations.py:115-151: ``out[i,j,f] = sum_k x[i,k,f] conj(x[j,k,f]) / K``
+    with the reference's arithmetic -- every product rounded to complex64, the
+    sum and the division by K in float64 (its Numba loop accumulates into
+    ``0.0+0.0j``), the lower triangle the conjugate -- computed on the GPU by
+    ``alb_xcorr_all_ant_f64acc``.  ``x``: complex64 ``(M, ntime, nfreq)`` in any
+    memory order (numpy or a device tensor).  Returns a C-order
+    ``(M, M, nfreq)`` complex64 like the reference (numpy for numpy input);
+    ``out``, when given, receives it."""
+    t = _lib.torch()
+    M = nant * npol
+    K = ntime
+    if K % split != 0:
         raise ValueError("split should be a divisor of ntime")
     if split > 1:
         raise NotImplementedError("Splitting not yet implemented for CPU version")
-    r = avg_xcorr_all_ant_gpu(x, nant, npol, ntime, nfreq)
-    r = np.ascontiguousarray(r) if _lib.is_numpy(r) else r.contiguous()
+    isnp = _lib.is_numpy(x)
+    if isnp:
+        xd = _lib.to_device(np.asarray(x, dtype=np.complex64))
+    else:
+        xd = x if x.dtype == t.complex64 else x.to(t.complex64)
+        if xd.device != _lib.device():
+            xd = xd.to(_lib.device())
+    if tuple(xd.shape) != (M, K, nfreq):
+        raise ValueError("x must have shape (nant*npol, ntime, nfreq) = (%d, %d, %d)" % (M, K, nfreq))
+    sm, sk, sf = (int(v) for v in xd.stride())
+    o = t.empty((M, M, nfreq), dtype=t.complex64, device=xd.device)
+    _lib.call("alb_xcorr_all_ant_f64acc", _lib.ptr(xd), M, K, nfreq, sm, sk, sf, _lib.ptr(o), M * nfreq, nfreq, 1,
+              _lib.stream())
+    r = _lib.host(o) if isnp else o
     if out is not None:
-        out[...] = r
+        if tuple(out.shape) != (M, M, nfreq):
+            raise ValueError("invalid out buffer")
+        out[...] = r if (_lib.is_numpy(out) or not isnp) else t.from_numpy(r)
         return out
     return r
+
+
+def _fortran_device(a, shape, name):
+    """numpy array (any order; its F-order element sequence is used) or a
+    device tensor of ``shape`` whose memory is column-major -> contiguous
+    device tensor holding the column-major element sequence."""
+    t = _lib.torch()
+    if _lib.is_numpy(a):
+        a = np.asarray(a, dtype=np.complex64)
+        if a.shape != tuple(shape):
+            raise ValueError("%s must have shape %s" % (name, tuple(shape)))
+        return _lib.to_device(a.reshape(-1, order="F"))
+    if tuple(a.shape) != tuple(shape) or a.dtype != t.complex64:
+        raise ValueError("%s must be complex64 of shape %s" % (name, tuple(shape)))
+    return a.permute(*reversed(range(a.dim()))).contiguous().to(_lib.device())
+
+
+def cgemm_strided_batched(A, B, M, N, K, batch_count, out=None):
+    """cgemm_batch.cu:8-54 ``C = A B^H`` per batch (column-major, alpha 1):
+    ``A`` (M, K, batch) and ``B`` (N, K, batch) complex64, numpy arrays or
+    device tensors with column-major memory.  Returns ``C`` (M, N, batch) in
+    F-order (numpy for numpy ``A``); ``out`` (a column-major device tensor),
+    when given, receives it."""
+    t = _lib.torch()
+    Ad = _fortran_device(A, (M, K, batch_count), "A")
+    Bd = _fortran_device(B, (N, K, batch_count), "B")
+    if out is not None:
+        if tuple(out.shape) != (M, N, batch_count) or out.dtype != t.complex64:
+            raise ValueError("invalid out buffer")
+        o = out.permute(2, 1, 0)
+        if not o.is_contiguous():
+            raise ValueError("invalid out buffer")
+    else:
+        o = t.empty((batch_count, N, M), dtype=t.complex64, device=Ad.device)
+    _lib.call("alb_cgemm_strided_batched", _lib.ptr(Ad), _lib.ptr(Bd), _lib.ptr(o), M, N, K, batch_count,
+              _lib.stream())
+    if out is not None:
+        return out
+    if _lib.is_numpy(A):
+        return np.asfortranarray(_lib.host(o).transpose(2, 1, 0))
+    return o.permute(2, 1, 0)
 
 
 def pair_index(ninput):

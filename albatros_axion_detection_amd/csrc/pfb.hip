@@ -1341,7 +1341,7 @@ int cols_group(int64_t ntiles) {
 template <int M, int F, int TT = kTPB>
 int launch_irfft(DenseRows d, int64_t nrows, float* out, int64_t ldo, hipStream_t s) {
   Twiddle tw;
-  int rc = get_twiddle(2 * M, &tw);
+  int rc = get_twiddle(2 * M, &tw, s);
   if (rc) return rc;
   const size_t lds = sizeof(float2) * (size_t(F) * Padded<M>::value + tw.nA + tw.nB);
   const int64_t ngroups = (nrows + F - 1) / F;
@@ -1358,7 +1358,7 @@ template <int F, int TT, int KI>
 int launch_irfft_packed4(const Packed4Rows& pk, int64_t nrows, int64_t nslice, float* out, int64_t ldo,
                          hipStream_t s) {
   Twiddle tw;
-  int rc = get_twiddle(4096, &tw);
+  int rc = get_twiddle(4096, &tw, s);
   if (rc) return rc;
   const size_t lds = sizeof(float2) * (size_t(F) * Padded<2048>::value + tw.nA + tw.nB + kW16Size);
   const int64_t ngroups = (nrows + F - 1) / F;
@@ -1423,7 +1423,7 @@ template <int M, int F>
 int launch_rfft(const float* x, int64_t ldx, int64_t nrows, const float* win, int ntap, int64_t c0, int64_t c1,
                 float2* out, int64_t ldo, hipStream_t s) {
   Twiddle tw;
-  int rc = get_twiddle(2 * M, &tw);
+  int rc = get_twiddle(2 * M, &tw, s);
   if (rc) return rc;
   const size_t lds = sizeof(float2) * (size_t(F) * Padded<M>::value + tw.nA + tw.nB);
   const unsigned grid = unsigned((nrows + F - 1) / F);
@@ -1455,7 +1455,7 @@ template <int NSL, int CW, int T>
 int launch_cols(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, const float2* filt, int64_t ldf,
                 int64_t ncols, int cut, float* out, int64_t ldo, int64_t out_tile_stride, hipStream_t s) {
   Twiddle tw;
-  int rc = get_twiddle(NSL, &tw);
+  int rc = get_twiddle(NSL, &tw, s);
   if (rc) return rc;
   const size_t lds = sizeof(float2) * (size_t(CW / 2) * Padded<NSL>::value + tw.nA + tw.nB + kW16Size);
   ALB_REQUIRE(ncols % CW == 0, "ipfb column pass: width %lld not a multiple of %d", (long long)ncols, CW);
@@ -1587,7 +1587,7 @@ int launch_cols_repfb(const float* y, int64_t ldy, int64_t tile_stride, int64_t 
   ALB_REQUIRE(y && filt && ts && win && A, "alb_ipfb_columns_repfb: null pointer");
   ALB_REQUIRE((4096 / CW) * ntiles < (int64_t(1) << 31), "fused iPFB/re-PFB: too many tiles");
   Twiddle tw;
-  rc = get_twiddle(NSL, &tw);
+  rc = get_twiddle(NSL, &tw, s);
   if (rc) return rc;
   const size_t lds = sizeof(float2) * (size_t(NP) * Padded<NSL>::value + tw.nA + tw.nB + kW16Size + R1) +
                      sizeof(float) * size_t(ntap) * R1 * CW;
@@ -1603,7 +1603,7 @@ template <int R1>
 int launch_repfb_stage2(const float2* A, int64_t a_stride, int64_t ninput, int64_t nblock, int64_t c_lo, int64_t c_hi,
                         float2* out, int64_t sm, int64_t sb, int64_t sc, int64_t bpj, int64_t sj, hipStream_t s) {
   Twiddle tw4;
-  int rc = get_twiddle(4096, &tw4);
+  int rc = get_twiddle(4096, &tw4, s);
   if (rc) return rc;
   const size_t lds2 = sizeof(float2) * (size_t(Padded<4096>::value) + tw4.nA + tw4.nB + kW16Size);
   const int64_t total = int64_t(R1 / 2 + 1) * nblock * ninput;
@@ -1617,7 +1617,7 @@ int launch_repfb_stage2(const float2* A, int64_t a_stride, int64_t ninput, int64
 template <int N, int F>
 int launch_fft(const float2* x, int64_t ldx, int64_t nrows, float2* out, int64_t ldo, int inverse, hipStream_t s) {
   Twiddle tw;
-  int rc = get_twiddle(N, &tw);
+  int rc = get_twiddle(N, &tw, s);
   if (rc) return rc;
   const size_t lds = sizeof(float2) * (size_t(F) * Padded<N>::value + tw.nA + tw.nB);
   hipLaunchKernelGGL((fft_rows_kernel<N, F>), dim3(unsigned((nrows + F - 1) / F)), dim3(kTPB), lds, s, x, ldx, nrows,
@@ -1650,7 +1650,7 @@ int launch_cols_bluestein(const float* y, int64_t ldy, int64_t tile_stride, int6
                           int64_t ldf, int64_t ncols, int cut, float* out, int64_t ldo, int64_t out_tile_stride,
                           const float2* chirp, const float2* bhat, hipStream_t s) {
   Twiddle tw;
-  int rc = get_twiddle(P, &tw);
+  int rc = get_twiddle(P, &tw, s);
   if (rc) return rc;
   const size_t lds = sizeof(float2) * (size_t(Padded<P>::value) + tw.nA + tw.nB);
   hipLaunchKernelGGL((ipfb_cols_bluestein_kernel<P>), dim3(unsigned(ncols / 2), unsigned(ntiles)), dim3(kTPB), lds, s, y,

@@ -38,12 +38,18 @@ struct TwKey {
 static std::mutex g_tw_mu;
 static std::map<TwKey, Twiddle> g_tw;
 
-int get_twiddle(int64_t N, Twiddle* out) {
+int get_twiddle(int64_t N, Twiddle* out, hipStream_t s) {
   int dev = 0;
   ALB_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(g_tw_mu);
   auto it = g_tw.find(TwKey{dev, N});
   if (it != g_tw.end()) { *out = it->second; return ALB_OK; }
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  ALB_HIP(hipStreamIsCapturing(s, &cap));
+  ALB_REQUIRE(cap == hipStreamCaptureStatusNone,
+              "the %lld-point twiddle table is not built yet and the stream is being captured (building it "
+              "allocates and synchronises): call alb_prepare_transform(%lld) before the capture",
+              (long long)N, (long long)N);
   int lg = ilog2(N);
   ALB_REQUIRE((int64_t(1) << lg) == N && lg >= 1 && lg <= 30, "twiddle size %lld is not a power of two",
               (long long)N);
@@ -102,7 +108,14 @@ int64_t alb_tune_get(const char* key) { return key ? alb::tune_get(key) : 0; }
 
 const char* alb_last_error(void) { return alb::last_error(); }
 
-int alb_version(void) { return 100; }
+int alb_version(void) { return 101; }
+
+// Build the twiddle table of an N-point transform on the current device now
+// (first uses otherwise build it lazily, which a graph capture cannot do).
+int alb_prepare_transform(int64_t N) {
+  alb::Twiddle tw;
+  return alb::get_twiddle(N, &tw, nullptr);
+}
 
 int alb_stream_synchronize(void* stream) {
   ALB_HIP(hipStreamSynchronize(alb::as_stream(stream)));

@@ -244,12 +244,14 @@ __global__ void scatter_kernel(const float2* __restrict__ spec, int64_t spec_ld,
 // unpacking_gpu.cu:6-21: zero packet i's bytes when spec_num[i+1]-spec_num[i] != spectra_per_packet.
 __global__ void dropped_kernel(uint8_t* data, const int64_t* spec_num, int64_t npackets, int64_t spp, int64_t bpp,
                                unsigned long long* ndropped) {
-  const int64_t i = blockIdx.y;  // packet
-  if (i >= npackets - 1) return;
-  if (spec_num[i + 1] - spec_num[i] == spp) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(ndropped, 1ull);
-  for (int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; j < bpp; j += int64_t(gridDim.x) * blockDim.x)
-    data[i * bpp + j] = 0;
+  // packets i = blockIdx.y + gridDim.y * m (any count), bytes of one packet
+  // over the x dimension
+  for (int64_t i = blockIdx.y; i < npackets - 1; i += gridDim.y) {
+    if (spec_num[i + 1] - spec_num[i] == spp) continue;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(ndropped, 1ull);
+    for (int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; j < bpp; j += int64_t(gridDim.x) * blockDim.x)
+      data[i * bpp + j] = 0;
+  }
 }
 
 }  // namespace alb
@@ -280,8 +282,9 @@ int alb_unpack_4bit_float(const uint8_t* data, void* pol0, void* pol1, int64_t r
 
 int alb_unpack_1bit_float(const uint8_t* data, void* pol0, void* pol1, int64_t rowstart, int64_t rowend,
                           int64_t chanstart, int64_t chanend, int64_t nchan, void* stream) {
-  ALB_REQUIRE(rowend >= rowstart && rowstart >= 0 && chanend >= chanstart && nchan > 0,
-              "alb_unpack_1bit_float: bad sizes");
+  ALB_REQUIRE(rowend >= rowstart && rowstart >= 0 && chanend >= chanstart && chanstart >= 0 && chanend <= nchan &&
+                  nchan > 0,
+              "alb_unpack_1bit_float: bad sizes (need 0 <= chanstart <= chanend <= nchan)");
   if ((rowend - rowstart) * (chanend - chanstart) == 0) return ALB_OK;
   ALB_REQUIRE(data && pol0 && pol1, "alb_unpack_1bit_float: null pointer");
   ALB_REQUIRE(chanstart % 2 == 0 && (chanend - chanstart) % 2 == 0,
@@ -383,9 +386,9 @@ int alb_dropped_packets(uint8_t* data, const int64_t* spec_num, int64_t num_pack
   hipStream_t s = as_stream(stream);
   hipLaunchKernelGGL(fill_u64, dim3(1), dim3(64), 0, s, num_dropped, int64_t(1));
   if (num_packets >= 2) {
-    ALB_REQUIRE(num_packets - 1 <= 65535, "alb_dropped_packets: at most 65536 packets per call");
-    hipLaunchKernelGGL(dropped_kernel, dim3(unsigned((bpp + 255) / 256 < 64 ? (bpp + 255) / 256 : 64),
-                                            unsigned(num_packets - 1)),
+    ALB_REQUIRE(bpp > 0, "alb_dropped_packets: empty packets");
+    const int64_t gy = num_packets - 1 < 65535 ? num_packets - 1 : 65535;
+    hipLaunchKernelGGL(dropped_kernel, dim3(unsigned((bpp + 255) / 256 < 64 ? (bpp + 255) / 256 : 64), unsigned(gy)),
                        dim3(256), 0, s, data, spec_num, num_packets, spectra_per_packet, bpp,
                        (unsigned long long*)num_dropped);
   }

@@ -11,9 +11,10 @@
 // channel's block of G is its own Gram (each element sums its two rows only, in
 // the same order whatever the other rows hold).  Then, with i, j offset by the
 // channel's slot, Re = G[i][j] + G[i+8][j+8], Im = G[i+8][j] - G[i][j+8].
-// Other M: a VALU kernel with one thread per (channel, baseline).
+// Other M: a VALU kernel, workgroup per channel with its inputs staged through LDS
+// and one thread per baseline (F-order and strided inputs alike).
 //
-// The matrix-core kernels also take a strided input: channel c = s F + f of
+// The correlator kernels also take a strided input: channel c = s F + f of
 // S jobs of F channels, element (m, t) of it at x[s ss + f sf + t sk + m]
 // (complex elements; the F-order layout is sk = M, sf = M K, ss = 0 with S = 1).
 #include "common.hpp"
@@ -175,30 +176,118 @@ __global__ __launch_bounds__(256) void xcorr_mfma16_lp_kernel(const float* __res
   xgram_store<M>(G[w], lane, row.f0, F, out, scale);
 }
 
-// Generic: workgroup per channel, thread per baseline i <= j.
-__global__ __launch_bounds__(256) void xcorr_valu_kernel(const float2* __restrict__ x, int M, int64_t K, int64_t F,
+// Generic VALU form (any M): workgroup per channel c of the XLayout, thread per
+// baseline i <= j (more than 256 baselines loop in batches).  The channel's M
+// inputs are staged through LDS kVT time samples at a time (one contiguous run
+// of M complex per sample in every layout used here), so the loads coalesce;
+// FP32 products into FP32 partial sums, flushed into float64 every 256 samples.
+constexpr int kVT = 32;
+
+__global__ __launch_bounds__(256) void xcorr_valu_kernel(const float2* __restrict__ x, int M, int64_t K, XLayout L,
                                                          float2* __restrict__ out, float scale) {
-  const int64_t f = blockIdx.x;
+  extern __shared__ float2 xs[];   // [kVT][M]
+  const int64_t c = blockIdx.x;
+  const float2* xc = x + L.channel(c);
   const int npair = M * (M + 1) / 2;
-  const float2* xf = x + f * int64_t(M) * K;
-  for (int p = threadIdx.x; p < npair; p += blockDim.x) {
-    int i = 0, rem = p;
-    while (rem >= M - i) { rem -= M - i; ++i; }
-    const int j = i + rem;
+  for (int p0 = 0; p0 < npair; p0 += blockDim.x) {
+    const int p = p0 + threadIdx.x;
+    int i = 0, j = 0;
+    if (p < npair) {
+      int rem = p;
+      while (rem >= M - i) { rem -= M - i; ++i; }
+      j = i + rem;
+    }
     double sr = 0.0, si = 0.0;
     float cr = 0.f, ci = 0.f;
-    for (int64_t t = 0; t < K; ++t) {
-      const float2 a = xf[t * M + i], b = xf[t * M + j];
-      cr = fmaf(a.x, b.x, fmaf(a.y, b.y, cr));
-      ci = fmaf(a.y, b.x, fmaf(-a.x, b.y, ci));
-      if ((t & 255) == 255) { sr += cr; si += ci; cr = 0.f; ci = 0.f; }
+    for (int64_t t0 = 0; t0 < K; t0 += kVT) {
+      const int nt = K - t0 < kVT ? int(K - t0) : kVT;
+      __syncthreads();
+      for (int e = threadIdx.x; e < nt * M; e += blockDim.x) {
+        const int tt = e / M, m = e - tt * M;
+        xs[e] = xc[(t0 + tt) * L.sk + m];
+      }
+      __syncthreads();
+      if (p < npair) {
+        for (int tt = 0; tt < nt; ++tt) {
+          const float2 a = xs[tt * M + i], b = xs[tt * M + j];
+          cr = fmaf(a.x, b.x, fmaf(a.y, b.y, cr));
+          ci = fmaf(a.y, b.x, fmaf(-a.x, b.y, ci));
+        }
+        if (((t0 / kVT) & 7) == 7) { sr += cr; si += ci; cr = 0.f; ci = 0.f; }
+      }
     }
-    sr += cr;
-    si += ci;
-    const float re = float(sr), im = float(si);
-    out[f * M * M + int64_t(j) * M + i] = make_float2(re * scale, im * scale);
-    if (i != j) out[f * M * M + int64_t(i) * M + j] = make_float2(re * scale, -im * scale);
+    if (p < npair) {
+      sr += cr;
+      si += ci;
+      const float re = float(sr), im = float(si);
+      out[c * M * M + int64_t(j) * M + i] = make_float2(re * scale, im * scale);
+      if (i != j) out[c * M * M + int64_t(i) * M + j] = make_float2(re * scale, -im * scale);
+    }
   }
+}
+
+// correlations.py:115-151 avg_xcorr_all_ant_cpu: every product x_i x_j^* rounded
+// to FP32 (two FP32 multiplies and one FP32 add per component, no fused
+// multiply-add, as the complex64 product of the reference's Numba loop), summed
+// in float64, divided by K in float64 and rounded to complex64; the lower
+// triangle is the conjugate.  Thread per channel f (consecutive f are
+// neighbouring lanes), grid y = baseline.  Element (m, k, f) at
+// x[m*sm + k*sk + f*sf], output (i, j, f) at out[i*oi + j*oj + f*of].
+__global__ __launch_bounds__(256) void xcorr_f64acc_kernel(const float2* __restrict__ x, int M, int64_t K, int64_t F,
+                                                           int64_t sm, int64_t sk, int64_t sf,
+                                                           float2* __restrict__ out, int64_t oi, int64_t oj,
+                                                           int64_t of) {
+#pragma clang fp contract(off)
+  const int64_t f = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (f >= F) return;
+  int i = 0, rem = int(blockIdx.y);
+  while (rem >= M - i) { rem -= M - i; ++i; }
+  const int j = i + rem;
+  const float2* xi = x + i * sm + f * sf;
+  const float2* xj = x + j * sm + f * sf;
+  double sr = 0.0, si = 0.0;
+  for (int64_t k = 0; k < K; ++k) {
+    const float2 a = xi[k * sk], b = xj[k * sk];
+    const float pr = a.x * b.x + a.y * b.y;
+    const float pi = a.y * b.x - a.x * b.y;
+    sr += double(pr);
+    si += double(pi);
+  }
+  const double dk = double(K);
+  const double re = sr / dk, im = si / dk;
+  out[i * oi + j * oj + f * of] = make_float2(float(re), float(im));
+  if (i != j) out[j * oi + i * oj + f * of] = make_float2(float(re), float(-im));
+}
+
+// cgemm_batch.cu:8-54 cgemm_strided_batched: C = A B^H per batch, column-major,
+// A (M, K), B (N, K), C (M, N), batch strides M K, N K, M N.  A 16 x 16 tile of
+// C per workgroup (thread per element), A and B staged through LDS 16 columns
+// of K at a time; FP32 products and sums (alpha 1, beta 0, as the reference).
+__global__ __launch_bounds__(256) void cgemm_abh_kernel(const float2* __restrict__ A, const float2* __restrict__ B,
+                                                        float2* __restrict__ C, int64_t M, int64_t N, int64_t K) {
+  __shared__ float2 As[16][17];   // [k][m]
+  __shared__ float2 Bs[16][17];   // [k][n]
+  const int64_t b = blockIdx.x;
+  const int64_t m0 = int64_t(blockIdx.y) * 16, n0 = int64_t(blockIdx.z) * 16;
+  const int tm = threadIdx.x % 16, tn = threadIdx.x / 16;
+  const float2* Ab = A + b * M * K;
+  const float2* Bb = B + b * N * K;
+  float cr = 0.f, ci = 0.f;
+  for (int64_t k0 = 0; k0 < K; k0 += 16) {
+    // lane (tm, tn) loads A[m0 + tm, k0 + tn] and B[n0 + tm, k0 + tn]
+    const int64_t k = k0 + tn;
+    As[tn][tm] = (m0 + tm < M && k < K) ? Ab[k * M + m0 + tm] : make_float2(0.f, 0.f);
+    Bs[tn][tm] = (n0 + tm < N && k < K) ? Bb[k * N + n0 + tm] : make_float2(0.f, 0.f);
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      const float2 a = As[kk][tm], bb = Bs[kk][tn];
+      cr = fmaf(a.x, bb.x, fmaf(a.y, bb.y, cr));
+      ci = fmaf(a.y, bb.x, fmaf(-a.x, bb.y, ci));
+    }
+    __syncthreads();
+  }
+  if (m0 + tm < M && n0 + tn < N) C[b * M * N + (n0 + tn) * M + m0 + tm] = make_float2(cr, ci);
 }
 
 }  // namespace alb
@@ -216,14 +305,7 @@ int alb_xcorr_all_ant(const void* x, int64_t M, int64_t K, int64_t F, void* out,
   ALB_REQUIRE(x && out, "alb_xcorr_all_ant: null pointer");
   ALB_REQUIRE(M >= 1 && K >= 0 && F >= 0, "alb_xcorr_all_ant: bad sizes");
   if (F == 0) return ALB_OK;
-  if (M <= 8) return alb_xcorr_all_ant_strided(x, M, K, F, 1, M, M * K, 0, out, scale, PREC_FP32, stream);
-  hipStream_t s = as_stream(stream);
-  ALB_REQUIRE(M * (M + 1) / 2 <= 4096, "alb_xcorr_all_ant: M too large");
-  ALB_REQUIRE(F <= 0x7fffffff, "alb_xcorr_all_ant: too many channels");
-  hipLaunchKernelGGL(xcorr_valu_kernel, dim3(unsigned(F)), dim3(256), 0, s, (const float2*)x, int(M), K, F,
-                     (float2*)out, scale);
-  ALB_LAUNCH_CHECK("xcorr_all_ant");
-  return ALB_OK;
+  return alb_xcorr_all_ant_strided(x, M, K, F, 1, M, M * K, 0, out, scale, PREC_FP32, stream);
 }
 
 // The same product in a chosen arithmetic: 0 FP32 (as alb_xcorr_all_ant), 1 BF16 hi/lo
@@ -244,12 +326,21 @@ int alb_xcorr_all_ant_strided(const void* x, int64_t M, int64_t K, int64_t F, in
   ALB_REQUIRE(x && out, "alb_xcorr_all_ant_strided: null pointer");
   ALB_REQUIRE(M >= 1 && K >= 0 && F >= 0 && S >= 1 && sk >= M && sf >= 0 && ss >= 0,
               "alb_xcorr_all_ant_strided: bad sizes");
-  if (M > 8) {
-    set_error("alb_xcorr_all_ant_strided: the matrix-core forms take M <= 8 (got %lld)", (long long)M);
+  if (M > 8 && mode != PREC_FP32) {
+    set_error("alb_xcorr_all_ant_strided: the reduced-precision forms take M <= 8 (got %lld)", (long long)M);
     return ALB_ENOTSUP;
   }
   if (F == 0) return ALB_OK;
   const int64_t nch = F * S;
+  if (M > 8) {
+    ALB_REQUIRE(M * (M + 1) / 2 <= 4096, "alb_xcorr_all_ant_strided: M too large (at most 90 inputs)");
+    ALB_REQUIRE(nch <= 0x7fffffff, "alb_xcorr_all_ant_strided: too many channels");
+    const XLayout L{sk, sf, ss, F};
+    hipLaunchKernelGGL(xcorr_valu_kernel, dim3(unsigned(nch)), dim3(256), sizeof(float2) * size_t(kVT) * size_t(M),
+                       as_stream(stream), (const float2*)x, int(M), K, L, (float2*)out, scale);
+    ALB_LAUNCH_CHECK("xcorr_valu_kernel");
+    return ALB_OK;
+  }
   const int64_t cpw = M <= 2 ? 4 : (M <= 4 ? 2 : 1);       // channels per wavefront (XPack)
   const int64_t nwave = (nch + cpw - 1) / cpw;
   ALB_REQUIRE((nwave + 3) / 4 <= 0x7fffffff, "alb_xcorr_all_ant_strided: too many channels");
@@ -273,6 +364,37 @@ int alb_xcorr_all_ant_strided(const void* x, int64_t M, int64_t K, int64_t F, in
 #undef CASE
   }
   ALB_LAUNCH_CHECK("xcorr_all_ant_strided");
+  return ALB_OK;
+}
+
+// correlations.py:115-151 avg_xcorr_all_ant_cpu (FP32 products, float64 sums and
+// division by K, complex64 out); strides in complex elements.
+int alb_xcorr_all_ant_f64acc(const void* x, int64_t M, int64_t K, int64_t F, int64_t sm, int64_t sk, int64_t sf,
+                             void* out, int64_t oi, int64_t oj, int64_t of, void* stream) {
+  ALB_REQUIRE(M >= 1 && K >= 0 && F >= 0, "alb_xcorr_all_ant_f64acc: bad sizes");
+  ALB_REQUIRE(M * (M + 1) / 2 <= 65535, "alb_xcorr_all_ant_f64acc: M too large (at most 361 inputs)");
+  if (F == 0) return ALB_OK;
+  ALB_REQUIRE(x && out, "alb_xcorr_all_ant_f64acc: null pointer");
+  ALB_REQUIRE((F + 255) / 256 <= 0x7fffffff, "alb_xcorr_all_ant_f64acc: too many channels");
+  const dim3 grid(unsigned((F + 255) / 256), unsigned(M * (M + 1) / 2));
+  hipLaunchKernelGGL(xcorr_f64acc_kernel, grid, dim3(256), 0, as_stream(stream), (const float2*)x, int(M), K, F, sm,
+                     sk, sf, (float2*)out, oi, oj, of);
+  ALB_LAUNCH_CHECK("xcorr_f64acc_kernel");
+  return ALB_OK;
+}
+
+// cgemm_batch.cu:8-54 cgemm_strided_batched: C = A B^H, column-major batches.
+int alb_cgemm_strided_batched(const void* A, const void* B, void* C, int64_t M, int64_t N, int64_t K,
+                              int64_t batch_count, void* stream) {
+  ALB_REQUIRE(M >= 0 && N >= 0 && K >= 0 && batch_count >= 0, "alb_cgemm_strided_batched: bad sizes");
+  if (M == 0 || N == 0 || batch_count == 0) return ALB_OK;
+  ALB_REQUIRE(A && B && C, "alb_cgemm_strided_batched: null pointer");
+  ALB_REQUIRE((M + 15) / 16 <= 65535 && (N + 15) / 16 <= 65535 && batch_count <= 0x7fffffff,
+              "alb_cgemm_strided_batched: shape too large for one launch");
+  const dim3 grid(unsigned(batch_count), unsigned((M + 15) / 16), unsigned((N + 15) / 16));
+  hipLaunchKernelGGL(cgemm_abh_kernel, grid, dim3(256), 0, as_stream(stream), (const float2*)A, (const float2*)B,
+                     (float2*)C, M, N, K);
+  ALB_LAUNCH_CHECK("cgemm_abh_kernel");
   return ALB_OK;
 }
 

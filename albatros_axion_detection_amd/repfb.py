@@ -350,6 +350,7 @@ class RePFBPipeline:
                 "stream": t.empty((rows, 4096), dtype=t.float32, device=dev),
                 "vis": t.empty((len(jobs), F, M, M), dtype=t.complex64, device=dev)}
         self._tile_buffer(min(self.tile_batch, nchunks))
+        self.prepare()
         if self.inputs_stage2_supported(nbt):
             self._workspace(nbt, M)
             # concurrent stage 1 of up to ``lanes`` inputs: a stream buffer and
@@ -365,11 +366,35 @@ class RePFBPipeline:
             self._workspace(nbt if self.fused_supported(nbt) else max(1, 512 // K) * K)
         return bufs
 
+    def prepare(self):
+        """Build every twiddle table this pipeline's kernels use (4096-point
+        rows, nslice-point or chirp-z columns) so that a pass captured into a
+        hipGraph finds them built: a first build allocates and synchronises,
+        which the library refuses on a capturing stream."""
+        _lib.call("alb_prepare_transform", 4096)
+        if self.pow2:
+            _lib.call("alb_prepare_transform", self.nslice)
+        else:
+            _lib.call("alb_prepare_transform", _pfb.bluestein_tables(self.nslice)[2])
+
+    def check_precision(self, precision):
+        """Raise before any launch if the correlator cannot run ``precision``
+        for this pipeline's input count (the reduced-precision matrix-core
+        forms take at most 8 inputs)."""
+        from .correlations import PRECISION_MODES
+        if precision not in PRECISION_MODES:
+            raise ValueError("precision must be one of %s" % sorted(PRECISION_MODES))
+        if precision != "fp32" and self.ninput > 8:
+            raise NotImplementedError("precision %r takes at most 8 correlator inputs (nant*npol = %d); use 'fp32'"
+                                      % (precision, self.ninput))
+
     def correlate_into(self, packed_list, rowmaps, nchunks, bufs, fused=True, precision="fp32"):
         """Launch the whole pass into ``bufs`` (no allocation, no host sync, so
-        it can be captured in a hipGraph).  Returns the ``(M, M, nfine, njobs)``
-        F-order view of ``bufs["vis"]``."""
+        it can be captured in a hipGraph once :meth:`buffers` has built the
+        transform tables).  Returns the ``(M, M, nfine, njobs)`` F-order view
+        of ``bufs["vis"]``."""
         from .correlations import PRECISION_MODES
+        self.check_precision(precision)
         M, K, F = self.ninput, self.nblock, self.nfine
         xin, stream, vis, nbt = bufs["xin"], bufs["stream"], bufs["vis"], bufs["nbt"]
         njobs = len(bufs["jobs"])

@@ -25,6 +25,26 @@ def _out(x, like_numpy):
     return _lib.host(x) if (like_numpy or _backend == "numpy") else x
 
 
+def _check_rows(d, rowstart, rowend, bytes_per_spectrum, name):
+    """Rows [rowstart, rowend) of spectra of ``bytes_per_spectrum`` bytes must
+    lie inside ``d`` (checked before any launch: the kernels index the raw
+    buffer by row and would otherwise read past it)."""
+    if rowstart < 0 or rowend < rowstart:
+        raise ValueError("%s: need 0 <= rowstart <= rowend (got %d, %d)" % (name, rowstart, rowend))
+    if rowend > rowstart and d.numel() < rowend * bytes_per_spectrum:
+        raise ValueError("%s: rowend %d needs %d bytes of data, the array holds %d" %
+                         (name, rowend, rowend * bytes_per_spectrum, d.numel()))
+
+
+def _check_channel_indices(ch, length_channels, name):
+    """Column indices must lie in [0, length_channels)."""
+    if ch.numel() == 0:
+        return
+    lo, hi = int(ch.min().item()), int(ch.max().item())
+    if lo < 0 or hi >= length_channels:
+        raise ValueError("%s: channel indices must lie in [0, %d) (got %d..%d)" % (name, length_channels, lo, hi))
+
+
 def unpack_4bit(data, rowstart, rowend, channels, length_channels):
     """unpacking.py:193-245 (GPU form :140-155; C form unpacking.c:155-205).
 
@@ -37,6 +57,8 @@ def unpack_4bit(data, rowstart, rowend, channels, length_channels):
     d = _lib.to_device(data, t.uint8)
     ch = _lib.to_device(np.asarray(channels, dtype=np.int64) if not isinstance(channels, t.Tensor) else channels,
                         t.int64)
+    _check_rows(d, int(rowstart), int(rowend), 2 * int(length_channels), "unpack_4bit")
+    _check_channel_indices(ch, int(length_channels), "unpack_4bit")
     nrows = int(rowend - rowstart)
     ncols = int(ch.numel())
     pol0 = t.empty((nrows, ncols), dtype=t.complex64, device=d.device)
@@ -58,8 +80,11 @@ def unpack_1bit(data, rowstart, rowend, channels, length_channels):
     chanend = int(ch[-1]) + 1
     if np.any(np.diff(ch) != 1):
         raise ValueError("A continuous range of channels must be provided in 1-bit mode.")
+    if chanstart < 0 or chanend > length_channels:
+        raise ValueError("unpack_1bit: channels must lie in [0, %d)" % length_channels)
     isnp = _lib.is_numpy(data)
     d = _lib.to_device(data, t.uint8)
+    _check_rows(d, int(rowstart), int(rowend), int(length_channels) // 2, "unpack_1bit")
     nrows = int(rowend - rowstart)
     ncols = chanend - chanstart
     pol0 = t.empty((nrows, ncols), dtype=t.complex64, device=d.device)
@@ -89,8 +114,12 @@ def sortpols(data, length_channels, bit_mode, rowstart, rowend, chanstart, chane
         ncols = int(np.ceil((chanend - chanstart) / 4))
     else:
         raise ValueError("bit_mode must be 1 or 4")
+    if chanstart < 0 or chanend <= chanstart:
+        raise ValueError("sortpols: need 0 <= chanstart < chanend (got %d, %d)" % (chanstart, chanend))
     isnp = _lib.is_numpy(data)
     d = _lib.to_device(data, t.uint8)
+    _check_rows(d, int(rowstart), int(rowend), 2 * int(length_channels) if bit_mode == 4 else int(length_channels) // 2,
+                "sortpols")
     pol0 = t.empty((nrows, ncols), dtype=t.uint8, device=d.device)
     pol1 = t.empty((nrows, ncols), dtype=t.uint8, device=d.device)
     _lib.call("alb_sortpols", _lib.ptr(d), _lib.ptr(pol0), _lib.ptr(pol1), int(rowstart), int(rowend), int(ncols),
@@ -106,6 +135,8 @@ def hist(data, rowstart, rowend, length_channels, bit_depth, mode):
     assert mode in (0, 1, -1), f"mode set to {mode}, only takes values 0,1,-1"
     isnp = _lib.is_numpy(data)
     d = _lib.to_device(data, t.uint8)
+    _check_rows(d, int(rowstart), int(rowend), 2 * int(length_channels) if bit_depth == 4 else int(length_channels) // 2,
+                "hist")
     nbins = 2 ** bit_depth
     h = t.empty((nbins, int(length_channels)), dtype=t.int64, device=d.device)
     _lib.call("alb_hist", _lib.ptr(d), _lib.ptr(h), int(rowstart), int(rowend), int(length_channels), int(bit_depth),

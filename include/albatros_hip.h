@@ -13,7 +13,9 @@
  *     supported; alb_last_error() gives the message of the calling thread;
  *   - no entry point keeps a pointer to caller memory after it returns; the only
  *     library-owned allocations are read-only twiddle tables built in float64 on
- *     first use of a transform size (one per device and size).
+ *     first use of a transform size (one per device and size), or ahead of time
+ *     by alb_prepare_transform -- which code captured into a hipGraph must do,
+ *     since a first use on a capturing stream is refused (ALB_EINVAL).
  *
  * Each declaration cites the reference interface it replaces
  * (paths relative to albatros_analysis/ of PhilJ0ly/albatros_axion_detection).
@@ -32,6 +34,11 @@ int alb_version(void);
 int alb_stream_synchronize(void* stream);
 /* Zero nbytes of device memory with a kernel (safe inside graph capture). */
 int alb_fill_zero(void* p, int64_t nbytes, void* stream);
+/* Build the twiddle table of an N-point transform (N a power of two) on the current device
+ * (no counterpart in the reference, whose cuFFT plans, pycufft.py:35-76, hold their own tables):
+ * the row passes use 4096, the column passes nslice (or the chirp-z length P), the fused column
+ * pass nslice and 4096.  Synchronises the device; call it before capturing a pass into a graph. */
+int alb_prepare_transform(int64_t N);
 
 /* ---- unpack (src/correlations/unpacking.c, unpacking.py) --------------- */
 /* unpacking.c:155 unpack_4bit_float / unpacking.py:140 _unpack_4bit_float_gpu.
@@ -81,8 +88,9 @@ int alb_xcorr_1bit_allpairs(const uint8_t* const* inputs, int ninput, int64_t nr
                             int64_t blocklen, int64_t* out, void* stream);
 
 /* ---- float correlator (src/correlations/correlations_gpu.py, cgemm_batch.cu) -- */
-/* correlations_gpu.py:95 avg_xcorr_all_ant_gpu / cgemm_batch.cu:8 cgemm_strided_batched (C = A B^H):
- * x complex64 F-order (M, K, F); out complex64 F-order (M, M, F) = x x^H * scale. */
+/* correlations_gpu.py:95-121 avg_xcorr_all_ant_gpu, whose cgemm_strided_batched call (:114-119) passes
+ * x as both operands: x complex64 F-order (M, K, F); out complex64 F-order (M, M, F) = x x^H * scale
+ * (any M: the matrix cores for M <= 8, a VALU form above). */
 int alb_xcorr_all_ant(const void* x, int64_t M, int64_t K, int64_t F, void* out, float scale, void* stream);
 /* The same product in a chosen arithmetic on the matrix cores, FP32 accumulation:
  * mode 0 FP32 (= alb_xcorr_all_ant), 1 BF16 hi/lo split (hi hi^T + hi lo^T + lo hi^T),
@@ -92,10 +100,22 @@ int alb_xcorr_all_ant_prec(const void* x, int64_t M, int64_t K, int64_t F, void*
 /* The matrix-core correlator on a strided input (no counterpart in the reference, whose
  * cgemm_strided_batched takes the F-order (M, K, F) block per job): S jobs of F channels,
  * element (input m, time k, channel f, job s) at x[s*ss + f*sf + k*sk + m] (complex64),
- * out as alb_xcorr_all_ant over the F*S channels (job-major).  M <= 8; mode as
- * alb_xcorr_all_ant_prec.  The batched re-PFB writes [S][K][F][M] (sk = F*M, sf = M). */
+ * out as alb_xcorr_all_ant over the F*S channels (job-major).  Mode as alb_xcorr_all_ant_prec
+ * (modes 1-3 take M <= 8; mode 0 above 8 runs the VALU form).  The batched re-PFB writes [S][K][F][M] (sk = F*M, sf = M). */
 int alb_xcorr_all_ant_strided(const void* x, int64_t M, int64_t K, int64_t F, int64_t S, int64_t sk, int64_t sf,
                               int64_t ss, void* out, float scale, int mode, void* stream);
+
+/* correlations.py:115-151 avg_xcorr_all_ant_cpu: out(i, j, f) = sum_k x(i,k,f) conj(x(j,k,f)) / K with every
+ * product rounded to complex64 (FP32 multiplies and adds, no fused multiply-add) and the sum and the
+ * division by K in float64, as the reference's Numba loop does; lower triangle = conjugate.
+ * Element (m, k, f) of x at x[m*sm + k*sk + f*sf], output (i, j, f) at out[i*oi + j*oj + f*of]
+ * (complex64 element strides; the reference's C-order output is oi = M*F, oj = F, of = 1). */
+int alb_xcorr_all_ant_f64acc(const void* x, int64_t M, int64_t K, int64_t F, int64_t sm, int64_t sk, int64_t sf,
+                             void* out, int64_t oi, int64_t oj, int64_t of, void* stream);
+/* cgemm_batch.cu:8-54 cgemm_strided_batched: C = A B^H for batch_count column-major batches,
+ * A (M, K), B (N, K), C (M, N) complex64 with batch strides M*K, N*K, M*N (alpha 1, beta 0). */
+int alb_cgemm_strided_batched(const void* A, const void* B, void* C, int64_t M, int64_t N, int64_t K,
+                              int64_t batch_count, void* stream);
 
 /* ---- PFB / inverse PFB (src/utils/pfb_gpu_utils.py, pycufft.py/.cu) ---- */
 /* pfb_gpu_utils.py:123 get_matft + :29 calculate_filter in float64: win float32 [ntap*lblock];

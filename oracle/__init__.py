@@ -42,6 +42,12 @@ PINNING = {
                           "property, oracle/_ref",
     "corr.avg_xcorr_all_ant": "reference constants test_gpu_all_ant_complex_xcorr.py:35-68, "
                               "numpy einsum in complex128",
+    "corr.avg_xcorr_all_ant_c64_products": "reference constants test_gpu_all_ant_complex_xcorr.py:35-68 "
+                                           "exactly, complex128 form to product rounding, Hermitian",
+    "corr.van_vleck_correction": "arcsine law of 1-bit Gaussian correlations (recovers the correlation "
+                                 "coefficient), closed-form special values",
+    "stream.missing_fraction": "hand-traced worked cases of helper_gpu_stream.py:157-233 "
+                               "(uneven chunks, a chunk spanning jobs, carried remainders past a job)",
     "pfb.sinc_hamming": "numpy.hamming * numpy.sinc closed form (library), symmetry",
     "pfb.matft_fp64": "float64 numpy.fft.rfft of the window matrix (the reference's "
                       "own construction) vs the 4-term closed form",

@@ -133,6 +133,30 @@ def avg_xcorr_all_ant(x, ntime):
     return np.einsum("ikf,jkf->ijf", xx, np.conj(xx)) / ntime
 
 
+def avg_xcorr_all_ant_c64_products(x, ntime):
+    """correlations.py:134-149 ``avg_xcorr_all_ant_cpu`` arithmetic step by
+    step: each product ``x[i,k,f] * conj(x[j,k,f])`` of complex64 values is
+    formed in float32 (``ar*br + ai*bi`` and ``ai*br - ar*bi``, each multiply
+    and add rounded, as the complex64 product of the Numba loop), the products
+    are summed in float64 (``corr_sum = 0.0 + 0.0j``), divided by K in
+    float64 and stored as complex64; the lower triangle is the conjugate.
+    Returns complex64 ``(M, M, F)``."""
+    xx = np.asarray(x, dtype=np.complex64)
+    M, K, F = xx.shape
+    re = np.ascontiguousarray(xx.real, dtype=np.float32)
+    im = np.ascontiguousarray(xx.imag, dtype=np.float32)
+    out = np.empty((M, M, F), dtype=np.complex64)
+    for i in range(M):
+        for j in range(i, M):
+            pr = np.add(np.multiply(re[i], re[j]), np.multiply(im[i], im[j]), dtype=np.float32)
+            pi = np.subtract(np.multiply(im[i], re[j]), np.multiply(re[i], im[j]), dtype=np.float32)
+            sr = pr.astype(np.float64).sum(axis=0) / K
+            si = pi.astype(np.float64).sum(axis=0) / K
+            out[i, j] = (sr + 1j * si).astype(np.complex64)
+            out[j, i] = np.conj(out[i, j])
+    return out
+
+
 def pair_list(ninput):
     """Pairs (i, j) with i <= j in row-major order (autos included)."""
     return [(i, j) for i in range(ninput) for j in range(i, ninput)]

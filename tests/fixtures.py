@@ -165,3 +165,24 @@ def write_raw(path, chans, spp, specnums, bits=4, missing=(), payload=None):
                 body = payload[p]
             np.asarray((int(s) % 2 ** 32, body),
                        dtype=[("spec_num", ">I"), ("spectra", "%dB" % (bpp - 4))]).tofile(f)
+
+
+# missing fraction: hand-traced cases of helper_gpu_stream.py:157-233.
+# Units are abstract sample counts (lblock, lchunk); acclen = 100, so a chunk
+# with n present spectra is (1 - n/100) * 100 % missing.  One antenna, so the
+# reference's shared list (helper_gpu_stream.py:125) and the per-antenna lists
+# (helper_gpu_stream_clean.py:204) agree.  Each entry: (lchunk, nblock, lblock,
+# present counts, planner output, expected per-job percentages).
+MISSING_CASES = [
+    # szblock 60, stride 30: chunk 2 spills 15 samples into job 1; the list
+    # entries of chunks 0-2 reach jobs 0-1 (counters 3, 3, 2), chunk 3 jobs 1-2
+    (25, 3, 10, [100, 90, 80, 100, 50, 70], [(0, 3), (3, 4), (4, 5), (5, 6), (6, None)],
+     [10.0, 7.5, 15.0, 30.0, np.nan]),
+    # chunks of 95 samples: the carried remainder alone fills jobs 1, 3, 4, 6, 7
+    # (first branch, nothing appended); a job whose list is empty is 0/0 = nan
+    (95, 3, 10, [90, 80, 60], [(0, 1), (1, 1), (1, 2), (2, 2), (2, 2), (2, 3), (3, 3), (3, 3), (3, None)],
+     [10.0, np.nan, 20.0, 20.0, np.nan, 40.0, 40.0, np.nan, np.nan]),
+    # every chunk exactly two blocks: counters all 1, each job reports its own chunks
+    (20, 2, 10, [100, 90, 80, 100, 50, 100, 70], [(0, 3), (3, 4), (4, 5), (5, 6), (6, 7), (7, None)],
+     [10.0, 0.0, 50.0, 0.0, 30.0, np.nan]),
+]

@@ -727,3 +727,118 @@ def test_autotune_keeps_results_and_leaves_its_choice_in_force():
         for k in keys:
             lib.alb_tune_set(k.encode(), 0)
     assert rel_err(after, before) < 1e-6
+
+
+def test_stream_xcorr_five_antennas(rng):
+    # nant*npol = 10 correlator inputs: the batched pass's strided correlator
+    # takes its VALU form above 8 inputs; against the float64 oracle stream.
+    # The reduced-precision forms refuse more than 8 inputs before any launch.
+    from albatros_axion_detection_amd import repfb
+    from oracle import unpack as ounpack
+    acclen, cut, osamp, nblock, nant, nchunks = 120, 4, 2, 4, 5, 3
+    chan = np.arange(300, 316)
+    chunks = []
+    for j in range(nant):
+        cj = _packed_chunks(rng, nchunks, acclen, len(chan), loss=0.02 * (j + 1), start=700)
+        for c in cj:
+            r1 = rng.integers(0, 256, size=c["pol0"].shape, dtype=np.uint8)
+            r1[len(c["specnums"]):] = 0
+            c["pol1"] = r1
+        chunks.append(cj)
+    vis, miss, freqs, _, _ = repfb.repfb_xcorr_avg_chunks(chunks, [700] * nant, chan, 300 * osamp, 316 * osamp,
+                                                          acclen, nchunks, nblock, osamp, cut=cut)
+    dec = [[{"pol0": (lambda r: r[0] + 1j * r[1])(ounpack.decode_4bit_bytes(c["pol0"])),
+             "pol1": (lambda r: r[0] + 1j * r[1])(ounpack.decode_4bit_bytes(c["pol1"])),
+             "specnums": c["specnums"]} for c in cj] for cj in chunks]
+    filt64 = opfb.wiener_filter(opfb.matft_fp64(acclen + 2 * cut), 0.45)
+    rvis, rmiss, _ = ostream.repfb_xcorr_avg(dec, [700] * nant, chan, 300 * osamp, 316 * osamp, acclen, nchunks,
+                                             nblock, osamp, cut, filt64, opfb.window_f32(4, 4096 * osamp))
+    assert vis.shape == rvis.shape == (10, 10, 16 * osamp, rvis.shape[3])
+    d = np.sqrt(np.abs(np.einsum("iifs->ifs", rvis)))
+    assert np.max(np.abs(np.asarray(vis) - rvis) / (d[:, None] * d[None, :])) < 1e-4
+    assert np.allclose(miss, rmiss, equal_nan=True)
+    pipe = repfb.RePFBPipeline(10, chan, acclen, cut, osamp, nblock, 300 * osamp, 316 * osamp)
+    with pytest.raises(NotImplementedError):
+        pipe.correlate_into(None, None, 0, None, precision="fp16x3")
+
+
+def test_batched_stream_300_channels_against_oracle():
+    # the batched pass at the streaming workload's channel count (300 coarse
+    # channels, 19 200 fine) over two jobs (one full, one over the zero-filled
+    # tail), uneven arrival chunks with runs of lost spectra, against the
+    # float64 oracle at the path's 1e-5 (max |d| / rms, and diagonal-normalised
+    # on in-band channels)
+    from albatros_axion_detection_amd import _lib, repfb, synth
+    nant, nchan, acclen, cut, osamp, nblock = 2, 300, 4088, 4, 64, 64
+    packed, specnums, lengths = synth.batched_stream(nant=nant, nchan=nchan, nchunks=3, lmin=3000, lmax=4000,
+                                                     loss=0.005, run=5, seed=5)
+    nfixed = int(lengths.sum()) // acclen
+    assert nfixed == 2
+    chan = np.arange(1000, 1000 + nchan)
+    M = 2 * nant
+    pipe = repfb.RePFBPipeline(M, chan, acclen, cut, osamp, nblock, 1000 * osamp, (1000 + nchan) * osamp)
+    rm = repfb.rowmap_stream(specnums, 0, nfixed, acclen)
+    plist = [_lib.to_device(packed[m]) for m in range(M)]
+    bufs = pipe.buffers(nfixed)
+    assert len(bufs["jobs"]) == 2
+    got = pipe.correlate_into(plist, [rm] * M, nfixed, bufs).cpu().numpy().copy()
+    print("[gpu-case] batched 300 channels: GPU done, oracle running", flush=True)
+    dec = _fixed_chunks_from_stream(packed, specnums, nfixed, acclen, nant)
+    filt64 = opfb.wiener_filter(opfb.matft_fp64(acclen + 2 * cut), 0.45)
+    rvis, _, _ = ostream.repfb_xcorr_avg(dec, [0] * nant, chan, 1000 * osamp, (1000 + nchan) * osamp, acclen, nfixed,
+                                         nblock, osamp, cut, filt64, opfb.window_f32(4, 4096 * osamp))
+    assert got.shape == rvis.shape == (M, M, nchan * osamp, 2)
+    d = np.sqrt(np.abs(np.einsum("iifs->ifs", rvis)))
+    norm = d[:, None] * d[None, :]
+    band = d.min(axis=0) > 1e-2 * np.median(d)
+    assert band.mean() > 0.5
+    e_rms, e_band = rel_err(got, rvis), float(np.max((np.abs(got - rvis) / norm)[:, :, band]))
+    print("[gpu-case] batched 300 channels: error vs float64 %.3g (rms), %.3g (in-band normalised)" % (e_rms, e_band),
+          flush=True)
+    assert e_rms < TOL and e_band < TOL
+
+
+def test_batched_pass_captured_before_any_eager_pass():
+    # buffers() builds the transform tables, so the first pass of a fresh
+    # process can be captured into a hipGraph (a lazy table build would have
+    # to allocate and sync inside the capture); replays equal the eager pass
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    code = r'''
+import sys
+sys.path.insert(0, %r)
+import numpy as np, torch
+from albatros_axion_detection_amd import _lib, repfb, synth
+torch.cuda.set_device(0)
+for acclen, osamp, nblock, nch in ((4088, 64, 16, 24), (120, 2, 4, 16)):
+    packed, specnums, lengths = synth.batched_stream(nant=1, nchan=nch, nchunks=3, lmin=acclen, lmax=acclen + 200,
+                                                     loss=0.01, run=5, seed=2)
+    nfixed = int(lengths.sum()) // acclen
+    pipe = repfb.RePFBPipeline(2, np.arange(500, 500 + nch), acclen, 4, osamp, nblock, 500 * osamp,
+                               (500 + nch) * osamp)
+    rm = repfb.rowmap_stream(specnums, 0, nfixed, acclen)
+    plist = [_lib.to_device(packed[m]) for m in range(2)]
+    bufs = pipe.buffers(nfixed)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        vis = pipe.correlate_into(plist, [rm, rm], nfixed, bufs)
+    g.replay()
+    torch.cuda.synchronize()
+    first = vis.clone()
+    eager = pipe.correlate_into(plist, [rm, rm], nfixed, bufs).clone()
+    torch.cuda.synchronize()
+    assert torch.equal(first, eager), acclen
+    bufs["vis"].zero_()
+    torch.cuda.synchronize()
+    g.replay()
+    torch.cuda.synchronize()
+    assert torch.equal(vis, eager), acclen
+    print("captured first pass ok", acclen, flush=True)
+print("CAPTURE_FIRST_OK")
+''' % root
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=240)
+    print(r.stdout[-2000:], r.stderr[-4000:], flush=True)
+    assert r.returncode == 0 and "CAPTURE_FIRST_OK" in r.stdout

@@ -457,3 +457,56 @@ def test_unpack_row_tiled_forms(unpk, rng):
         g = unpk.sortpols(raw1, nchan, 1, 2, nrows + 2, cs, ce)
         e = ounpack.sortpols(raw1, nchan, 1, 2, nrows + 2, cs, ce)
         assert np.array_equal(g[0], e[0]) and np.array_equal(g[1], e[1]), (cs, ce)
+
+
+def test_dropped_packets_more_than_65536_packets(rng):
+    # unpacking_gpu.cu:24 takes any num_packets: 70 000 packets, jumps after
+    # packets 3, 65 540 and 69 998 (the last pair)
+    import torch
+    from albatros_axion_detection_amd import _lib
+    spp, nchan, npk = 2, 3, 70000
+    bpp = 2 * nchan * spp
+    data = rng.integers(1, 256, size=(npk, bpp), dtype=np.uint8)
+    spec = spp * np.arange(npk, dtype=np.int64)
+    jumps = [3, 65540, 69998]
+    for j in jumps:
+        spec[j + 1:] += 7
+    d = _lib.to_device(data)
+    s = _lib.to_device(spec)
+    nd = torch.empty(1, dtype=torch.int64, device=d.device)
+    _lib.call("alb_dropped_packets", _lib.ptr(d), _lib.ptr(s), npk, spp, nchan, 4, _lib.ptr(nd), _lib.stream())
+    exp = data.copy()
+    exp[jumps] = 0
+    assert int(nd.item()) == len(jumps)
+    assert np.array_equal(_lib.host(d), exp)
+
+
+def test_unpack_rejects_out_of_range_rows_and_channels(unpk, rng):
+    # host checks before any launch: a channel index outside the spectrum, or
+    # rows past the end of the raw data, raise instead of reading beyond it
+    nchan = 8
+    data = rng.integers(0, 256, size=(10, 2 * nchan), dtype=np.uint8)
+    with pytest.raises(ValueError):
+        unpk.unpack_4bit(data, 0, 10, [0, 8], nchan)
+    with pytest.raises(ValueError):
+        unpk.unpack_4bit(data, 0, 10, [-1, 2], nchan)
+    with pytest.raises(ValueError):
+        unpk.unpack_4bit(data, 0, 11, [0, 1], nchan)
+    with pytest.raises(ValueError):
+        unpk.unpack_4bit(data, 5, 4, [0, 1], nchan)
+    with pytest.raises(ValueError):
+        unpk.sortpols(data, nchan, 4, 0, 11, 0, nchan)
+    with pytest.raises(ValueError):
+        unpk.hist(data, 0, 11, nchan, 4, -1)
+    d1 = rng.integers(0, 256, size=(10, nchan // 2), dtype=np.uint8)
+    with pytest.raises(ValueError):
+        unpk.unpack_1bit(d1, 0, 11, np.arange(0, nchan), nchan)
+    with pytest.raises(ValueError):
+        unpk.unpack_1bit(d1, 0, 10, np.arange(2, nchan + 2), nchan)
+    with pytest.raises(ValueError):
+        unpk.sortpols(d1, nchan, 1, 0, 11, 0, nchan)
+    # the full valid ranges still work
+    p0, p1 = unpk.unpack_4bit(data, 0, 10, [0, nchan - 1], nchan)
+    assert p0.shape == (10, 2)
+    q0, _ = unpk.unpack_1bit(d1, 0, 10, np.arange(0, nchan), nchan)
+    assert q0.shape == (10, nchan)

@@ -64,3 +64,13 @@ def test_bench_reads_committed_traffic_summaries():
         v = bench.read_traffic(k + "_rechannelizer")
         assert v is not None and v > 1e6, k
     assert bench.read_traffic("no_such_kernel") is None
+
+
+def test_missing_fraction_hand_traced_cases():
+    # the package's bookkeeping against the literal hand-traced percentages
+    from albatros_axion_detection_amd import repfb
+    for lchunk, nblock, lblock, present, jobs_expected, expected in fx.MISSING_CASES:
+        jobs = repfb.get_chunkier(len(present), lchunk, nblock, lblock, 4)
+        assert jobs == jobs_expected
+        got = repfb.missing_fraction([present], 100, nblock, lblock, lchunk, 4, jobs)
+        assert np.allclose(got[0], expected, equal_nan=True, rtol=0, atol=1e-12)

@@ -389,3 +389,85 @@ def test_make_continuous_repeated_channel_keeps_last_column():
     out = ostream.make_continuous(spec, [10, 11], 10, [5, 7, 5], 4, nchans=9)
     assert out[0, 5] == 3 + 3j and out[1, 5] == 6 + 6j and out[0, 7] == 2 + 2j
     assert np.count_nonzero(out) == 4
+
+
+# ---------------------------------------------------------------------------
+# missing fraction: hand-traced cases (tests/fixtures.py MISSING_CASES)
+MISSING_CASES = fx.MISSING_CASES
+
+
+@pytest.mark.parametrize("case", range(len(MISSING_CASES)))
+def test_missing_fraction_hand_traced(case):
+    lchunk, nblock, lblock, present, jobs_expected, expected = MISSING_CASES[case]
+    nchunks = len(present)
+    jobs = ostream.get_chunkier(nchunks, lchunk, nblock, lblock, 4)
+    assert jobs == jobs_expected
+    chunks = [[{"specnums": np.arange(n)} for n in present]]
+    got = ostream.missing_fraction(chunks, 100, nchunks, nblock, lblock, lchunk, 4, jobs)
+    assert got.shape == (1, len(jobs))
+    assert np.allclose(got[0], expected, equal_nan=True, rtol=0, atol=1e-12)
+
+
+def test_missing_fraction_antennas_keep_their_own_lists():
+    # two antennas: each row equals the one-antenna result of that antenna
+    # (the per-antenna lists of helper_gpu_stream_clean.py:204)
+    lchunk, nblock, lblock, present, _, exp0 = MISSING_CASES[0]
+    other = [50, 100, 100, 80, 100, 100]
+    jobs = ostream.get_chunkier(len(present), lchunk, nblock, lblock, 4)
+    chunks = [[{"specnums": np.arange(n)} for n in present], [{"specnums": np.arange(n)} for n in other]]
+    got = ostream.missing_fraction(chunks, 100, len(present), nblock, lblock, lchunk, 4, jobs)
+    alone = ostream.missing_fraction(chunks[1:], 100, len(present), nblock, lblock, lchunk, 4, jobs)
+    assert np.allclose(got[0], exp0, equal_nan=True)
+    assert np.allclose(got[1], alone[0], equal_nan=True)
+    # antenna 1 by hand: job 0 (0, 50, 0)/3, job 1 (0, 50, 0, 20)/4, job 2 (0, 50, 20, 0)/4
+    assert np.allclose(got[1][:3], [50 / 3, 17.5, 17.5])
+
+
+def test_van_vleck_correction_closed_forms():
+    # sin(pi/2 * R/n): R = n -> 1, R = n/3 -> 1/2, R = 0 -> 0; imag = I1 - I0 terms
+    n = 3000
+    v = ocorr.van_vleck_correction(np.float32(n), np.float32(n / 3), np.float32(0), np.float32(n), n)
+    assert abs(v - (1.5 + 1j * (1.0 - 0.0))) < 1e-12
+    v = ocorr.van_vleck_correction(0.0, 0.0, np.float32(-n), np.float32(n / 3), n)
+    assert abs(v - (0.0 + 1j * (0.5 + 1.0))) < 1e-12
+
+
+def test_van_vleck_correction_recovers_gaussian_correlation():
+    # arcsine law: sign-bit products of unit-variance Gaussians with
+    # correlation rho average to (2/pi) asin(rho); the correction returns the
+    # complex correlation E[x0 conj(x1)] = 2 a e^{i phi} of the generator
+    rng = np.random.default_rng(7)
+    n, a, phi = 400000, 0.3, 0.7
+    z = (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    w0 = (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    w1 = (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    x0 = np.sqrt(a) * z + np.sqrt(1 - a) * w0
+    x1 = np.sqrt(a) * np.exp(-1j * phi) * z + np.sqrt(1 - a) * w1
+    s = lambda v: np.sign(v)  # noqa: E731
+    R0 = np.sum(s(x0.real) * s(x1.real))
+    R1 = np.sum(s(x0.imag) * s(x1.imag))
+    I0 = np.sum(s(x0.real) * s(x1.imag))
+    I1 = np.sum(s(x1.real) * s(x0.imag))
+    v = ocorr.van_vleck_correction(R0, R1, I0, I1, n)
+    assert abs(v - 2 * a * np.exp(1j * phi)) < 0.01
+    # the uncorrected sign correlation is biased low (2/pi asin < identity)
+    raw = (R0 + R1 + 1j * (I1 - I0)) / n
+    assert abs(raw) < abs(v) - 0.02
+
+
+def test_complex64_product_oracle():
+    # the step-by-step arithmetic of avg_xcorr_all_ant_cpu: the reference
+    # constants exactly, the complex128 form to product rounding, Hermitian
+    x, nant, npol, ntime, nfreq, exp = fx.xcorr_constants()
+    got = ocorr.avg_xcorr_all_ant_c64_products(x, ntime)
+    for (i, j, f), v in exp.items():
+        assert got[i, j, f] == v
+    rng = np.random.default_rng(3)
+    y = (rng.standard_normal((5, 300, 7)) + 1j * rng.standard_normal((5, 300, 7))).astype(np.complex64)
+    a = ocorr.avg_xcorr_all_ant_c64_products(y, 300)
+    b = ocorr.avg_xcorr_all_ant(y, 300)
+    assert a.dtype == np.complex64
+    assert np.max(np.abs(a - b)) < 1e-6 * np.max(np.abs(b))
+    assert np.array_equal(a, np.conj(a.transpose(1, 0, 2)))
+    # the products are rounded: not bit-identical to complex128 products
+    assert not np.array_equal(a, b.astype(np.complex64))
