@@ -57,6 +57,8 @@ SIGNATURES = {
     "alb_ipfb_rows_coeff": (_i32, [_vp, _i64, _vp, _vp]),
     "alb_repfb": (_i32, [_vp, _vp, _i32, _i64, _i64, _vp, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _vp]),
     "alb_repfb_workspace_bytes": (_i64, [_i64, _i64]),
+    "alb_repfb_noalloc": (_i32, [_vp, _vp, _i32, _i64, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _vp]),
+    "alb_repfb_noalloc_max_bytes": (_i64, []),
     "alb_repfb_stage1": (_i32, [_vp, _vp, _i32, _i64, _i64, _vp, _vp]),
     "alb_repfb_stage2": (_i32, [_vp, _i64, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _vp]),
     "alb_repfb_stage2_inputs": (_i32, [_vp, _i64, _i64, _i64, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _i64, _i64,

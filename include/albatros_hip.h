@@ -204,6 +204,15 @@ int alb_repfb_stage2_inputs(const void* A, int64_t a_stride, int64_t ninput, int
 int alb_ipfb_columns_repfb(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, int64_t nslice,
                            const void* filt, int64_t ldf, int cut, float* ts, int64_t row0, int64_t rows_per_tile,
                            const float* win, int ntap, int64_t osamp, int64_t nblocks, void* A, void* stream);
+/* alb_repfb with its stage-1 workspace in a fixed device array of the library instead of A: allocates
+ * nothing (no counterpart in the reference; for callers that cannot call hipMalloc -- code under hipGraph
+ * capture, tight serving loops, harnesses that forbid allocation).  Requires
+ * alb_repfb_workspace_bytes(nblock, osamp) <= alb_repfb_noalloc_max_bytes() (64 MiB: 64 blocks at
+ * osamp 64), else ALB_EINVAL before any launch.  Each device holds its own array until the process
+ * ends; calls that overlap on one device share it and race. */
+int alb_repfb_noalloc(const float* ts, const float* win, int ntap, int64_t nblock, int64_t osamp, int64_t c_lo,
+                      int64_t c_hi, void* out, int64_t sb, int64_t sc, int64_t bpj, int64_t sj, void* stream);
+int64_t alb_repfb_noalloc_max_bytes(void);
 /* Interior blocks [*lo, *hi) of a tile whose kept rows are stream rows [g0, g0 + nkeep). */
 int alb_repfb_interior(int64_t g0, int64_t nkeep, int64_t osamp, int ntap, int64_t nblocks, int64_t* lo, int64_t* hi);
 /* alb_repfb_stage1 for the n blocks listed in the host array blocks (block b reads ts rows
