@@ -366,6 +366,15 @@ class RePFBPipeline:
             self._workspace(nbt if self.fused_supported(nbt) else max(1, 512 // K) * K)
         return bufs
 
+    @staticmethod
+    def _valid(nchunks, nvalid):
+        """Chunks holding data: ``nvalid`` (at most ``nchunks``) or all."""
+        if nvalid is None:
+            return nchunks
+        if not 0 <= nvalid <= nchunks:
+            raise ValueError("nvalid must lie in [0, nchunks]")
+        return int(nvalid)
+
     def prepare(self):
         """Build every twiddle table this pipeline's kernels use (4096-point
         rows, nslice-point or chirp-z columns) so that a pass captured into a
@@ -388,13 +397,17 @@ class RePFBPipeline:
             raise NotImplementedError("precision %r takes at most 8 correlator inputs (nant*npol = %d); use 'fp32'"
                                       % (precision, self.ninput))
 
-    def correlate_into(self, packed_list, rowmaps, nchunks, bufs, fused=True, precision="fp32"):
+    def correlate_into(self, packed_list, rowmaps, nchunks, bufs, fused=True, precision="fp32", nvalid=None):
         """Launch the whole pass into ``bufs`` (no allocation, no host sync, so
         it can be captured in a hipGraph once :meth:`buffers` has built the
-        transform tables).  Returns the ``(M, M, nfine, njobs)`` F-order view
-        of ``bufs["vis"]``."""
+        transform tables).  Jobs are planned for ``nchunks`` chunks of which
+        the first ``nvalid`` (default all) hold data; the time stream past them
+        is zero, as the reference's zero-filled incomplete buffer
+        (helper_gpu_stream.py:215-219).  Returns the ``(M, M, nfine, njobs)``
+        F-order view of ``bufs["vis"]``."""
         from .correlations import PRECISION_MODES
         self.check_precision(precision)
+        nchunks = self._valid(nchunks, nvalid)
         M, K, F = self.ninput, self.nblock, self.nfine
         xin, stream, vis, nbt = bufs["xin"], bufs["stream"], bufs["vis"], bufs["nbt"]
         njobs = len(bufs["jobs"])
@@ -509,11 +522,12 @@ class RePFBPipeline:
         best["tried"] = tried
         return best
 
-    def correlate_per_chunk(self, packed_list, rowmaps, nchunks, bufs):
+    def correlate_per_chunk(self, packed_list, rowmaps, nchunks, bufs, nvalid=None):
         """The per-chunk launch form in plain FP32, following the reference's
         loop structure (helper_gpu_stream.py:157-236): per input and chunk one
         row pass and one column pass, per job and input one re-PFB launch, per
         job one correlator launch.  Same result as :meth:`correlate_into`."""
+        nchunks = self._valid(nchunks, nvalid)
         M, K, F = self.ninput, self.nblock, self.nfine
         stream, vis = bufs["stream"], bufs["vis"]
         njobs = len(bufs["jobs"])
@@ -545,11 +559,11 @@ class RePFBPipeline:
             _lib.call("alb_xcorr_all_ant", _lib.ptr(xin[j]), M, K, F, _lib.ptr(vis[j]), float(scale), s)
         return vis.permute(3, 2, 1, 0)
 
-    def correlate(self, packed_list, rowmaps, nchunks, return_xin=False, fused=True):
+    def correlate(self, packed_list, rowmaps, nchunks, return_xin=False, fused=True, precision="fp32", nvalid=None):
         """All inputs: ``vis`` complex64 F-order ``(M, M, nfine, njobs)`` of
         ``avg_xcorr_all_ant_gpu`` per job (helper_gpu_stream.py:236)."""
         bufs = self.buffers(nchunks)
-        res = self.correlate_into(packed_list, rowmaps, nchunks, bufs, fused=fused)
+        res = self.correlate_into(packed_list, rowmaps, nchunks, bufs, fused=fused, precision=precision, nvalid=nvalid)
         if return_xin:
             return res, bufs["xin"].permute(3, 1, 2, 0)
         return res
@@ -628,3 +642,97 @@ def repfb_xcorr_avg_chunks(chunks, start_specnums, chan_numbers, repfb_chanstart
     if return_xin:
         return res + (out[1],)
     return res
+
+
+def repfb_xcorr_avg(idxs, files, acclen, nchunks, nblock, chanstart, chanend, osamp, cut=10, filt_thresh=0.45,
+                    cupy_win_big=None, filt=None, verbose=False, precision="fp32", **pipeline_kw):
+    """helper_gpu_stream.py:62-254: re-PFB + cross-correlate ``nchunks`` iPFB
+    chunks of ``acclen`` spectra read from baseband files.
+
+    ``idxs[j]`` is antenna j's start offset (spectra past the first spectrum
+    of ``files[j][0]``), ``files[j]`` its list of 4-bit ``.raw`` files;
+    coarse channel indices ``[chanstart, chanend)`` of the files' channel
+    list are selected, and the kept fine channels are
+    ``[channels[chanstart]*osamp, channels[chanend]*osamp)`` in channel
+    numbers (helper_gpu_stream.py:104-109).  Returns
+    ``(vis, missing_fraction, freqs, win, filt)`` as the reference:
+    ``vis`` a masked complex64 F-order ``(nant*npol, nant*npol, nchan, njobs)``
+    array (input row ``ant*npol + pol``, helper_gpu_stream_clean.py:375),
+    ``missing_fraction`` float64 ``(nant, njobs)`` percentages,
+    ``freqs`` the kept fine channel numbers, ``win`` and ``filt`` the window
+    and Wiener filter used (pass them back to the next call, as
+    xcorr_gpu_list.py:70-82 does).
+
+    The files are read through :class:`BasebandFileIterator` (packed form)
+    straight into one stacked device buffer per input; if an antenna runs
+    out of files, every antenna stops at that chunk (the reference's
+    ``zip``) and the stream past it is zero.  ``precision`` selects the
+    correlator arithmetic (``correlations.PRECISION_MODES``); other keywords
+    go to :class:`RePFBPipeline`.
+    """
+    import time
+    from . import baseband_data_classes as bdc
+    t = _lib.torch()
+    t0 = time.perf_counter()
+    nant, npol, ntap = len(idxs), 2, 4
+    its = [bdc.BasebandFileIterator(files[j], 0, idxs[j], acclen, nchunks=nchunks, chanstart=chanstart,
+                                    chanend=chanend, type="packed") for j in range(nant)]
+    obj = its[-1].obj
+    if obj.bit_mode != 4:
+        raise NotImplementedError("repfb_xcorr_avg reads 4-bit baseband files (bit mode %d)" % obj.bit_mode)
+    channels = np.asarray(obj.channels, dtype=np.int64)
+    nchan = int(obj.chanend - obj.chanstart) * osamp
+    if obj.chanend >= len(channels):
+        raise ValueError("chanend must index a channel of the file (the kept band ends at channels[chanend]*osamp)")
+    repfb_chanstart = int(channels[obj.chanstart]) * osamp
+    repfb_chanend = int(channels[obj.chanend]) * osamp
+    if repfb_chanend - repfb_chanstart != nchan:
+        raise ValueError("the selected coarse channels must have consecutive channel numbers")
+    chan_numbers = channels[its[-1].channel_idxs]
+    ncols = its[0].ncols
+    start_specnums = [it.spec_num_start for it in its]
+    dev = _lib.device()
+    packed = [t.empty((nchunks, acclen, ncols), dtype=t.uint8, device=dev) for _ in range(nant * npol)]
+    present = [[] for _ in range(nant)]
+    nread = 0
+    for c in range(nchunks):
+        try:
+            got = [it.read_into(packed[npol * j][c], packed[npol * j + 1][c]) for j, it in enumerate(its)]
+        except StopIteration:
+            break
+        for j in range(nant):
+            present[j].append(got[j])
+        nread += 1
+    t1 = time.perf_counter()
+    pipe = RePFBPipeline(nant * npol, chan_numbers, acclen, cut, osamp, nblock, repfb_chanstart, repfb_chanend,
+                         filt_thresh=filt_thresh, ntap=ntap, filt=filt, win=cupy_win_big, **pipeline_kw)
+    rmaps = []
+    for j in range(nant):
+        rm = rowmaps_from_specnums(present[j], start_specnums[j], acclen) if nread else \
+            t.full((nchunks, acclen), -1, dtype=t.int32, device=dev)
+        rmaps += [rm] * npol
+    vis_dev = pipe.correlate(packed, rmaps, nchunks, precision=precision, nvalid=nread)
+    jobs, _, _ = pipe.plan(nchunks)
+    lblock = 4096 * osamp
+    npres = [[len(s) for s in present[j]] for j in range(nant)]
+    miss = missing_fraction(npres, acclen, nblock, lblock, 4096 * acclen, ntap, jobs)
+    vis = np.asfortranarray(_lib.host(vis_dev))
+    if verbose:
+        print("repfb_xcorr_avg: %d antennas, %d of %d chunks read in %.3f s, %d jobs correlated in %.3f s"
+              % (nant, nread, nchunks, t1 - t0, len(jobs), time.perf_counter() - t1))
+    return (np.ma.masked_invalid(vis), np.asfortranarray(miss), np.arange(repfb_chanstart, repfb_chanend), pipe.win,
+            pipe.filt)
+
+
+def result_filename(init_t, acclen, osamp, nchunks, chanstart, chanend):
+    """xcorr_gpu_list.py:90: name of one processed interval's result file."""
+    return "stream_xcorr_all_ant_4bit_%s_%s_%s_%s_%s_%s.npz" % (init_t, acclen, osamp, nchunks, chanstart, chanend)
+
+
+def write_result(fpath, vis, missing_fraction, chans):
+    """xcorr_gpu_list.py:92: ``np.savez`` of ``data`` (the visibilities),
+    ``mask`` (their invalid-value mask), ``missing_fraction`` and ``chans``
+    (the fine channel numbers), as the reference's driver writes them."""
+    v = np.ma.masked_invalid(vis) if not isinstance(vis, np.ma.MaskedArray) else vis
+    np.savez(fpath, data=v.data, mask=v.mask, missing_fraction=np.asarray(missing_fraction), chans=np.asarray(chans))
+    return fpath

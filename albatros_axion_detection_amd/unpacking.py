@@ -36,6 +36,21 @@ def _check_rows(d, rowstart, rowend, bytes_per_spectrum, name):
                          (name, rowend, rowend * bytes_per_spectrum, d.numel()))
 
 
+def _outputs(out, nrows, ncols, dtype, device, name):
+    """(pol0, pol1): new device tensors, or the caller's ``out`` pair checked
+    for shape, dtype and row-contiguous memory."""
+    t = _lib.torch()
+    if out is None:
+        return (t.empty((nrows, ncols), dtype=dtype, device=device),
+                t.empty((nrows, ncols), dtype=dtype, device=device))
+    p0, p1 = out
+    for p in (p0, p1):
+        if tuple(p.shape) != (nrows, ncols) or p.dtype != dtype or not p.is_contiguous() or p.device != device:
+            raise ValueError("%s: out must be two contiguous %s device tensors of shape (%d, %d)"
+                             % (name, dtype, nrows, ncols))
+    return p0, p1
+
+
 def _check_channel_indices(ch, length_channels, name):
     """Column indices must lie in [0, length_channels)."""
     if ch.numel() == 0:
@@ -45,12 +60,13 @@ def _check_channel_indices(ch, length_channels, name):
         raise ValueError("%s: channel indices must lie in [0, %d) (got %d..%d)" % (name, length_channels, lo, hi))
 
 
-def unpack_4bit(data, rowstart, rowend, channels, length_channels):
+def unpack_4bit(data, rowstart, rowend, channels, length_channels, out=None):
     """unpacking.py:193-245 (GPU form :140-155; C form unpacking.c:155-205).
 
     ``data``: raw spectra, ``2*length_channels`` bytes each; ``channels``:
     column indices.  Returns complex64 (pol0, pol1) of shape
-    ``(rowend-rowstart, len(channels))``; nibble 8 decodes to +8.
+    ``(rowend-rowstart, len(channels))``; nibble 8 decodes to +8.  ``out``
+    (not in the reference): a pair of device tensors to write instead.
     """
     t = _lib.torch()
     isnp = _lib.is_numpy(data)
@@ -61,14 +77,15 @@ def unpack_4bit(data, rowstart, rowend, channels, length_channels):
     _check_channel_indices(ch, int(length_channels), "unpack_4bit")
     nrows = int(rowend - rowstart)
     ncols = int(ch.numel())
-    pol0 = t.empty((nrows, ncols), dtype=t.complex64, device=d.device)
-    pol1 = t.empty((nrows, ncols), dtype=t.complex64, device=d.device)
+    pol0, pol1 = _outputs(out, nrows, ncols, t.complex64, d.device, "unpack_4bit")
     _lib.call("alb_unpack_4bit_float", _lib.ptr(d), _lib.ptr(pol0), _lib.ptr(pol1), int(rowstart), int(rowend),
               _lib.ptr(ch), ncols, int(length_channels), _lib.stream())
+    if out is not None:
+        return pol0, pol1
     return _out(pol0, isnp), _out(pol1, isnp)
 
 
-def unpack_1bit(data, rowstart, rowend, channels, length_channels):
+def unpack_1bit(data, rowstart, rowend, channels, length_channels, out=None):
     """unpacking.py:248-302 (C form unpacking.c:207-246).
 
     ``channels``: a continuous range of channel indices starting at an even
@@ -87,14 +104,15 @@ def unpack_1bit(data, rowstart, rowend, channels, length_channels):
     _check_rows(d, int(rowstart), int(rowend), int(length_channels) // 2, "unpack_1bit")
     nrows = int(rowend - rowstart)
     ncols = chanend - chanstart
-    pol0 = t.empty((nrows, ncols), dtype=t.complex64, device=d.device)
-    pol1 = t.empty((nrows, ncols), dtype=t.complex64, device=d.device)
+    pol0, pol1 = _outputs(out, nrows, ncols, t.complex64, d.device, "unpack_1bit")
     _lib.call("alb_unpack_1bit_float", _lib.ptr(d), _lib.ptr(pol0), _lib.ptr(pol1), int(rowstart), int(rowend),
               chanstart, chanend, int(length_channels), _lib.stream())
+    if out is not None:
+        return pol0, pol1
     return _out(pol0, isnp), _out(pol1, isnp)
 
 
-def sortpols(data, length_channels, bit_mode, rowstart, rowend, chanstart, chanend):
+def sortpols(data, length_channels, bit_mode, rowstart, rowend, chanstart, chanend, out=None):
     """unpacking.py:305-377 / unpacking.c:248-368.
 
     4-bit: per-pol bytes ``(nrows, chanend-chanstart)``.  1-bit: per pol, four
@@ -120,10 +138,11 @@ def sortpols(data, length_channels, bit_mode, rowstart, rowend, chanstart, chane
     d = _lib.to_device(data, t.uint8)
     _check_rows(d, int(rowstart), int(rowend), 2 * int(length_channels) if bit_mode == 4 else int(length_channels) // 2,
                 "sortpols")
-    pol0 = t.empty((nrows, ncols), dtype=t.uint8, device=d.device)
-    pol1 = t.empty((nrows, ncols), dtype=t.uint8, device=d.device)
+    pol0, pol1 = _outputs(out, nrows, ncols, t.uint8, d.device, "sortpols")
     _lib.call("alb_sortpols", _lib.ptr(d), _lib.ptr(pol0), _lib.ptr(pol1), int(rowstart), int(rowend), int(ncols),
               int(length_channels), int(bit_mode), int(chanstart), int(chanend), _lib.stream())
+    if out is not None:
+        return pol0, pol1
     return _out(pol0, isnp), _out(pol1, isnp)
 
 
