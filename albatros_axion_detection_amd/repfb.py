@@ -144,6 +144,9 @@ class RePFBPipeline:
         self.filt = _lib.to_device(filt, t.complex64) if filt is not None else _pfb.get_filter(self.nslice, filt_thresh,
                                                                                                   ntap=ntap)
         self.win = _lib.to_device(win, t.float32) if win is not None else _pfb.window(ntap, self.lblock)
+        # optional per-launch hook timer(name, fn) (bench.py times each kernel
+        # with HIP events on the stream it is launched on)
+        self.timer = None
         self._tiles = {}
         self._ws = None
         self._xin_f = None
@@ -167,6 +170,13 @@ class RePFBPipeline:
             self.coeff = t.empty(nbytes, dtype=t.uint8, device=_lib.device())
             _lib.call("alb_ipfb_rows_coeff", _lib.ptr(self.chan), self.ncols, _lib.ptr(self.coeff), _lib.stream())
 
+    def _launch(self, name, entry, *args):
+        """Call library entry point ``entry``, through the timer hook when set."""
+        if self.timer is None:
+            _lib.call(entry, *args)
+        else:
+            self.timer(name, lambda: _lib.call(entry, *args))
+
     def _rows(self, packed, rowmap, t0, nt, y):
         """Row pass of tiles t0 .. t0+nt-1 into ``y``; returns the tile row
         stride argument of the column passes (4096 row-major, 8 blocked)."""
@@ -176,8 +186,8 @@ class RePFBPipeline:
                       self.ncols, self.acclen, self.cut, t0, nt, _lib.ptr(self.coeff), _lib.ptr(y), s)
             return 8
         if self.pow2:
-            _lib.call("alb_ipfb_rows_packed4_blocked", _lib.ptr(packed), _lib.ptr(rowmap), _lib.ptr(self.chan),
-                      self.ncols, self.acclen, self.cut, t0, nt, _lib.ptr(y), s)
+            self._launch("ipfb_rows_packed4", "alb_ipfb_rows_packed4_blocked", _lib.ptr(packed), _lib.ptr(rowmap),
+                         _lib.ptr(self.chan), self.ncols, self.acclen, self.cut, t0, nt, _lib.ptr(y), s)
             return 8
         _lib.call("alb_ipfb_rows_packed4", _lib.ptr(packed), _lib.ptr(rowmap), _lib.ptr(self.chan), self.ncols,
                   self.acclen, self.cut, t0, nt, _lib.ptr(y), s)
@@ -306,18 +316,19 @@ class RePFBPipeline:
         total = stream_out.shape[0]
         if total > rows_valid:
             tail = stream_out[rows_valid:]
-            _lib.call("alb_fill_zero", _lib.ptr(tail), tail.numel() * 4, s)
+            self._launch("fill_zero", "alb_fill_zero", _lib.ptr(tail), tail.numel() * 4, s)
         tb = self.tile_batch
         y = self._tile_buffer(min(tb, nchunks), lane)
         for t0 in range(0, nchunks, tb):
             nt = min(tb, nchunks - t0)
             ldy = self._rows(packed, rowmap, t0, nt, y)
-            _lib.call("alb_ipfb_columns_repfb", _lib.ptr(y), ldy, self.nslice * 4096, nt, self.nslice,
-                      _lib.ptr(self.filt), self.filt.shape[1], self.cut, _lib.ptr(stream_out), t0 * self.acclen,
-                      self.acclen, _lib.ptr(self.win), self.ntap, self.osamp, nblocks_total, _lib.ptr(ws), s)
+            self._launch("ipfb_columns_repfb", "alb_ipfb_columns_repfb", _lib.ptr(y), ldy, self.nslice * 4096, nt,
+                         self.nslice, _lib.ptr(self.filt), self.filt.shape[1], self.cut, _lib.ptr(stream_out),
+                         t0 * self.acclen, self.acclen, _lib.ptr(self.win), self.ntap, self.osamp, nblocks_total,
+                         _lib.ptr(ws), s)
         edges = self.edge_blocks(nchunks, nblocks_total)
-        _lib.call("alb_repfb_stage1_list", _lib.ptr(stream_out), _lib.ptr(self.win), self.ntap,
-                  edges.ctypes.data_as(ctypes.c_void_p), len(edges), self.osamp, _lib.ptr(ws), s)
+        self._launch("repfb_stage1_edges", "alb_repfb_stage1_list", _lib.ptr(stream_out), _lib.ptr(self.win),
+                     self.ntap, edges.ctypes.data_as(ctypes.c_void_p), len(edges), self.osamp, _lib.ptr(ws), s)
 
     # ------------------------------------------------------------ pipelines
     def spectra(self, packed, rowmap, nchunks, fused=True):
@@ -437,8 +448,8 @@ class RePFBPipeline:
                                            ws[m * per:], lane=lane)
                 for st in self._lane_streams[:nl]:
                     main.wait_stream(st)
-            _lib.call("alb_repfb_stage2_inputs", _lib.ptr(ws), per * 8, M, nbt, self.osamp, self.c_lo, self.c_hi,
-                      _lib.ptr(xin), 1, F * M, M, K, K * F * M, _lib.stream())
+            self._launch("repfb_stage2", "alb_repfb_stage2_inputs", _lib.ptr(ws), per * 8, M, nbt, self.osamp,
+                         self.c_lo, self.c_hi, _lib.ptr(xin), 1, F * M, M, K, K * F * M, _lib.stream())
         else:
             for m in range(M):
                 if fused:
@@ -448,8 +459,8 @@ class RePFBPipeline:
                     self.ipfb_stream(packed_list[m], rowmaps[m], nchunks, stream)
                     self.repfb(stream, nbt, xin.view(-1)[m:], F * M, M, K, K * F * M, block_batch=bb)
         scale = np.float32(1.0) / np.float32(K)
-        _lib.call("alb_xcorr_all_ant_strided", _lib.ptr(xin), M, K, F, njobs, F * M, M, K * F * M, _lib.ptr(vis),
-                  float(scale), PRECISION_MODES[precision], _lib.stream())
+        self._launch("xcorr_all_ant_strided", "alb_xcorr_all_ant_strided", _lib.ptr(xin), M, K, F, njobs, F * M, M,
+                     K * F * M, _lib.ptr(vis), float(scale), PRECISION_MODES[precision], _lib.stream())
         return vis.permute(3, 2, 1, 0)
 
     # launch-shape knobs of the library tried by autotune (0 = built-in rule)
@@ -584,6 +595,28 @@ def rowmaps_from_specnums(specnums_list, start_specnum, acclen, row_bases=None):
         base = c * acclen if row_bases is None else int(row_bases[c])
         _lib.call("alb_build_rowmap", _lib.ptr(d), len(sp), int(start_specnum), int(acclen), base, _lib.ptr(rm[c]), s)
     return rm
+
+
+class StreamRowmap:
+    """Row map of a stream whose present spectra are stored consecutively,
+    rebuilt on the device from its spec_nums by one ``alb_build_rowmap``
+    launch (so a captured pass can include it): the spec_nums of
+    ``[start, start + nchunks*acclen)`` are located once on the host, the
+    map ``[nchunks, acclen]`` int32 is filled by :meth:`build`."""
+
+    def __init__(self, specnums, start_specnum, nchunks, acclen):
+        t = _lib.torch()
+        sp = np.asarray(specnums if _lib.is_numpy(specnums) else specnums.cpu().numpy(), dtype=np.int64)
+        self.start, self.total = int(start_specnum), int(nchunks) * int(acclen)
+        self.lo = int(np.searchsorted(sp, self.start, side="left"))
+        self.hi = int(np.searchsorted(sp, self.start + self.total, side="left"))
+        self.spec = _lib.to_device(sp[self.lo:self.hi] if self.hi > self.lo else np.zeros(1, dtype=np.int64), t.int64)
+        self.rowmap = t.empty((nchunks, acclen), dtype=t.int32, device=_lib.device())
+
+    def build(self):
+        _lib.call("alb_build_rowmap", _lib.ptr(self.spec), self.hi - self.lo, self.start, self.total, self.lo,
+                  _lib.ptr(self.rowmap), _lib.stream())
+        return self.rowmap
 
 
 def rowmap_stream(specnums, start_specnum, nchunks, acclen):

@@ -1,20 +1,24 @@
 #!/usr/bin/env python3
 """Benchmark of the ALBATROS hot path on one MI355X.
 
-Headline workload (one "step"): the single-input rechannelizer -- 4 chunks of
-4088 coarse 4-bit spectra (100 coarse channels 64..163, nslice 4096 with
-2*cut = 8 carried rows) -> unpack + spec_num zero fill -> Wiener inverse PFB
-(thresh 0.45) -> re-PFB at 64x (ntap 4) -> the 6400 fine channels of every
-complete block.  Inputs are resident in HBM before the timed region.
+Headline workload (one "step"): the batched streaming pass -- 2 antennas x 2
+pols of 4-bit baseband, 300 coarse channels, 64 arrival chunks of uneven
+length (2^10..2^14 spectra) with runs of lost spectra, re-blocked by
+spec_num into 4088-spectrum iPFB tiles -> unpack + spec_num zero fill ->
+Wiener inverse PFB (thresh 0.45) -> re-PFB 64x (ntap 4, 19200 fine channels)
+-> 4x4 visibilities per 1024-spectrum job, as one hipGraph replay that also
+builds the row map from the spec_nums.  Inputs are resident in HBM before the
+timed region.  value = ADC-equivalent samples per second (inputs x coarse
+spectra x 4096 / s / 1e6).
 
-value = ADC-equivalent samples per second (coarse spectra x 4096 / s / 1e6).
-
-Also reported: live per-kernel HIP-event times over the timed steps, the
-roofline of the dominant kernel, the pipeline-level roofline, a CPU baseline
-(the reference CPU algorithm restated, timed on a bounded sample of the same
-workload) and secondary workloads (4-bit / 1-bit unpack and sortpols
-throughput, direct 4-bit and 1-bit correlators, the precision correlator,
-the batched streaming pass, autotuned, against its per-chunk launch form).
+Also reported: the same workload in the plain-FP32 per-chunk launch form and
+the speedup over it, live per-kernel HIP-event times, the roofline of the
+dominant kernel, the pipeline-level roofline, a CPU baseline (the reference
+CPU path restated, timed on a bounded sample of the same stream, all cores
+and one core) and secondary workloads (the single-input rechannelizer with
+its own roofline and CPU baseline, 4-bit / 1-bit unpack and sortpols, the
+direct 4-bit and 1-bit correlators with the reference's own OpenMP C timed
+beside them, the precision correlator).
 """
 import argparse
 import ctypes
@@ -354,7 +358,8 @@ def secondary(args, torch):
     del raw4, raw1, chans, c0, c1, s0, s1
     # direct 4-bit: 4 inputs x 2^17 spectra x 128 ch, 10 pairs, 4096-blocks
     lib = _lib.load()
-    inp = _lib.to_device(synth.direct4(T=2 ** 17, nchan=128, seed=0))
+    inp_np = synth.direct4(T=2 ** 17, nchan=128, seed=0)
+    inp = _lib.to_device(inp_np)
     ptrs = (_lib._vp * 4)(*[inp[i].data_ptr() for i in range(4)])
     b4 = 68091904
     forms, res4 = {}, {}
@@ -370,6 +375,9 @@ def secondary(args, torch):
     lib.alb_tune_set(b"xcorr_int_method", 0)
     out["direct4"] = {"workload": "4 inputs x 2^17 spectra x 128 ch, 10 pairs, 4096-blocks, int64 sums",
                       "forms": forms, "forms_bit_identical": bool(torch.equal(res4["mfma_i8"], res4["valu"]))}
+    if not args.no_cpu:
+        log("[bench] CPU reference: direct 4-bit ...")
+        out["direct4"]["cpu_baseline"] = cpu_reference_direct(4, inp_np, 128, 4096, 32)
     del inp, res4
     # direct 1-bit: 16 inputs x T spectra x 1024 ch (256 B), 136 pairs, 8192-blocks
     T1 = args.onebit_T
@@ -390,6 +398,9 @@ def secondary(args, torch):
     lib.alb_tune_set(b"xcorr_int_method", 0)
     out["direct1"] = {"workload": "16 inputs x %d spectra x 1024 ch, 136 pairs, 8192-blocks, int64 sums" % T1,
                       "forms": forms, "forms_bit_identical": bool(torch.equal(res1["mfma_i8"], res1["xor_valu"]))}
+    if not args.no_cpu:
+        log("[bench] CPU reference: direct 1-bit ...")
+        out["direct1"]["cpu_baseline"] = cpu_reference_direct(1, inp[:, :8192].cpu().numpy(), 1024, 8192, 1)
     del inp, res1
     # precision correlator: 8 inputs x 1024 fine spectra x 6400 channels, four
     # arithmetic forms on the matrix cores, each against a float64 product
@@ -418,32 +429,52 @@ def secondary(args, torch):
     out["xcorr_precision"] = {"workload": "8 inputs x 1024 x 6400 ch complex64 (F-order), full 8x8 per channel; "
                                           "error = max |C - C64| / sqrt(C64_ii C64_jj)", "forms": prec}
     del x, xd, vis, ref, norm
-    out["batched_stream"] = batched_stream_leg(args, torch)
     return out
 
 
-def batched_stream_leg(args, torch):
-    """Batched streaming: 2 antennas x 2 pols, 300 coarse channels of 4-bit
-    data, 64 arrival chunks of uneven length (uniform in [2^10, 2^14] spectra)
-    with runs of 5 lost spectra (~0.5%), re-blocked by spec_num into iPFB tiles
-    of 4088 (+8 carried) spectra -> iPFB -> re-PFB 64x (19200 fine channels)
-    -> 4x4 visibilities per 1024-spectrum job.  The batched form (fused column
-    pass, one stage 2 for every input, every job correlated at once, FP16 hi/lo
-    split correlator) is one hipGraph replay; the per-chunk form issues
-    per-chunk / per-job launches in plain FP32.  Unless --no-autotune, the
-    pipeline's autotuner first picks the batched form's tile batch, lanes and
-    launch shapes on these inputs (reported under "autotune"); the library's
-    knobs are restored afterwards."""
+def cpu_info():
+    """CPU model (/proc/cpuinfo) and the cores this process may use."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def batched(args, torch):
+    """Headline: the batched streaming workload.  2 antennas x 2 pols, 300
+    coarse channels of 4-bit data, 64 arrival chunks of uneven length (uniform
+    in [2^10, 2^14] spectra) with runs of 5 lost spectra (~0.5%), re-blocked by
+    spec_num into iPFB tiles of 4088 (+8 carried) spectra -> iPFB (Wiener
+    0.45) -> re-PFB 64x (ntap 4, 19200 fine channels) -> 4x4 visibilities per
+    1024-spectrum job.  One step = one hipGraph replay of the row-map build
+    (alb_build_rowmap from the stream's spec_nums) and the batched pass
+    (fused column pass + re-PFB stage 1, one stage 2 for every input, every
+    job correlated at once), inputs resident in HBM.  The plain-FP32 form
+    (per-chunk / per-job launches, FP32 correlator) is timed on the same
+    inputs.  With --autotune, the pipeline's autotuner first picks tile
+    batch, lanes and launch shapes on these inputs."""
     from albatros_axion_detection_amd import _lib, repfb, synth
     dev = _lib.device()
     acclen, cut, osamp, nblock, nchan, c0 = 4088, 4, 64, 1024, 300, 1000
+    log("[bench] generating the batched streaming input ...")
     packed, specnums, lengths = synth.batched_stream(nant=2, nchan=nchan, nchunks=64, lmin=2 ** 10, lmax=2 ** 14,
                                                      loss=0.005, run=5, seed=4, device=dev)
     total = int(lengths.sum())
     nfixed = total // acclen
     chan = np.arange(c0, c0 + nchan)
-    pipe = repfb.RePFBPipeline(4, chan, acclen, cut, osamp, nblock, c0 * osamp, (c0 + nchan) * osamp, tile_batch=16)
-    rm = repfb.rowmap_stream(specnums, 0, nfixed, acclen)
+    # one stage-1 lane: every launch runs alone, so the per-kernel HIP-event
+    # times and the profiler's dispatch durations are each kernel's own (two
+    # lanes overlap one input's edge blocks with the next input's passes,
+    # about 2% faster, but then no kernel can be timed by itself)
+    pipe = repfb.RePFBPipeline(4, chan, acclen, cut, osamp, nblock, c0 * osamp, (c0 + nchan) * osamp, tile_batch=16,
+                               lanes=args.lanes)
+    srm = repfb.StreamRowmap(specnums, 0, nfixed, acclen)
+    rm = srm.build()
     plist = [packed[m] for m in range(4)]
     rms = [rm] * 4
     bufs = pipe.buffers(nfixed)
@@ -452,32 +483,83 @@ def batched_stream_leg(args, torch):
     knob_keys = [k for k, _ in repfb.RePFBPipeline.TUNE_KNOBS]
     knobs_before = {k: int(lib.alb_tune_get(k.encode())) for k in knob_keys}
     tune = None
-    if not args.no_autotune:
+    if args.autotune:
         log("[bench] autotuning the batched pass ...")
-        tune = pipe.autotune(plist, rms, nfixed, bufs, precision="fp16x3", reps=2)
+        tune = pipe.autotune(plist, rms, nfixed, bufs, precision=args.precision, reps=2)
         log("[bench] autotune: %s" % json.dumps({k: v for k, v in tune.items() if k != "tried"}))
-    # batched form: the correlator in the FP16 hi/lo split (matrix cores, FP32
-    # accumulation, ~6e-7 vs FP64); the per-chunk form stays plain FP32
-    run_b = lambda: pipe.correlate_into(plist, rms, nfixed, bufs, precision="fp16x3")
-    run_p = lambda: pipe.correlate_per_chunk(plist, rms, nfixed, bufs)
-    run_p()
+    timer = Timer(torch)
+
+    def step():
+        timer("rowmap", srm.build)
+        return pipe.correlate_into(plist, rms, nfixed, bufs, precision=args.precision)
+
+    # plain-FP32 per-chunk form: the reference check of the batched result
+    pipe.correlate_per_chunk(plist, rms, nfixed, bufs)
     torch.cuda.synchronize()
     ref = bufs["vis"].clone()
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(side):
-        run_b()
+        step()
     torch.cuda.current_stream().wait_stream(side)
     torch.cuda.synchronize()
-    g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g):
-        run_b()
-    g.replay()
+    if args.no_graph:
+        replay = step
+    else:
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            step()
+        replay = g.replay
+    replay()
     torch.cuda.synchronize()
     vis = bufs["vis"]
     # max |difference| / rms over all visibilities (edge fine channels carry
     # only leakage, so per-channel ratios there measure FP32 rounding noise)
     err = float(torch.max(torch.abs(vis - ref)) / torch.sqrt(torch.mean(torch.abs(ref) ** 2)))
+    # clock settle (~100 ms of replays), warm-up, then K replays between two events
+    t_settle = time.perf_counter()
+    while time.perf_counter() - t_settle < 0.1:
+        replay()
+        torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        replay()
+    t0e = torch.cuda.Event(enable_timing=True)
+    t1e = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    t0e.record()
+    for _ in range(args.steps):
+        replay()
+    t1e.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = t0e.elapsed_time(t1e) / args.steps
+    # per-kernel durations: eager passes with a HIP event pair around every
+    # launch on the stream it goes to, with one stage-1 lane so that no two
+    # launches overlap (on two lanes an event pair would also time the other
+    # lane's kernels); the step time above is taken without them
+    nk = max(1, min(args.steps, 5))
+    lanes = pipe.lanes
+    pipe.lanes = 1
+    timer.on = True
+    pipe.timer = timer
+    for _ in range(nk):
+        step()
+    kern = timer.summary()
+    counts = {k: v // nk for k, v in timer.counts().items()}
+    timer.on = False
+    pipe.timer = None
+    pipe.lanes = lanes
+    eager0 = torch.cuda.Event(enable_timing=True)
+    eager1 = torch.cuda.Event(enable_timing=True)
+    eager0.record()
+    for _ in range(3):
+        step()
+    eager1.record()
+    torch.cuda.synchronize()
+    ms_eager = eager0.elapsed_time(eager1) / 3
+    log("[bench] batched: %.3f ms/step graph (wall %.3f), %.3f ms eager; kernels %s" % (
+        ms, 1e3 * wall / args.steps, ms_eager, json.dumps({k: round(v, 4) for k, v in kern.items()})))
 
     def timed(fn, n):
         a = torch.cuda.Event(enable_timing=True)
@@ -490,37 +572,203 @@ def batched_stream_leg(args, torch):
         torch.cuda.synchronize()
         return a.elapsed_time(b) / n
 
-    ms_graph = timed(g.replay, 3)
-    ms_eager = timed(run_b, 3)
-    # the per-chunk form at its own best column-pass width (it runs the plain
+    # the plain-FP32 form at its own best column-pass width (it runs the plain
     # column pass, whose fastest width differs from the fused pass's)
+    run_p = lambda: (srm.build(), pipe.correlate_per_chunk(plist, rms, nfixed, bufs))
     tuned_w = int(lib.alb_tune_get(b"ipfb_cols_width"))
     chunk_ms = {}
-    for w in (2, 4, 8):
+    for w in ((2, 4, 8) if not args.no_plain else ()):
         _lib.call("alb_tune_set", b"ipfb_cols_width", w)
         run_p()
         chunk_ms[w] = timed(run_p, 2)
     _lib.call("alb_tune_set", b"ipfb_cols_width", tuned_w)
-    ms_chunk = min(chunk_ms.values())
+    ms_chunk = min(chunk_ms.values()) if chunk_ms else None
     for k, v in knobs_before.items():
         _lib.call("alb_tune_set", k.encode(), v)
-    nsamp = 4 * nfixed * acclen * 4096
-    per_spec = 300 + 2 * 16384 + 2 * 19200 * 8 // 64       # raw + time stream + kept fine channels, per input
-    hbm = 4 * nfixed * acclen * per_spec
-    del g
-    return {"workload": "2 ant x 2 pol, 300 coarse ch 4-bit, 64 arrival chunks U[2^10, 2^14] with ~0.5%% loss "
-                        "(%d spectra) -> %d iPFB tiles of 4088 -> re-PFB 64x (19200 ch) -> 4x4 vis over %d jobs "
-                        "of 1024 spectra" % (total, nfixed, njobs),
-            "batched_graph_ms": round(ms_graph, 3), "batched_eager_ms": round(ms_eager, 3),
-            "per_chunk_ms": round(ms_chunk, 3),
-            "speedup_vs_per_chunk": round(ms_chunk / ms_graph, 3),
-            "Msamples_s": round(nsamp / (ms_graph * 1e-3) / 1e6, 1),
-            "per_chunk_Msamples_s": round(nsamp / (ms_chunk * 1e-3) / 1e6, 1),
-            "hbm_frac": round(hbm / (ms_graph * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "batched_correlator": "fp16x3", "per_chunk_correlator": "fp32",
-            "per_chunk_ms_by_cols_width": {str(w): round(v, 3) for w, v in chunk_ms.items()},
-            "max_abs_diff_over_rms_vs_per_chunk": err,
-            "autotune": tune}
+    del replay
+    n_coarse = nfixed * acclen                      # coarse spectra per input
+    nsamp = 4 * n_coarse * 4096                     # ADC-equivalent samples per step
+    # algorithmic bytes per coarse spectrum per input: raw 300 B in, time
+    # stream 16384 B written + 16384 B read, kept fine channels 2400 B written
+    # + 2400 B read (19200 channels x 8 B / 64 spectra per re-PFB block)
+    per_spec = 300 + 2 * 16384 + 2 * 19200 * 8 // 64
+    pipe_bytes = 4 * n_coarse * per_spec
+    # per-launch algorithmic bytes of each kernel (its terms of per_spec x the
+    # coarse spectra the launch covers): the fused column pass writes and reads
+    # the time stream; the row pass reads the raw bytes; stage 2 writes the
+    # kept fine channels and the correlator reads them; the edge blocks'
+    # stream re-reads are counted as what that launch reads
+    nedge = len(pipe.edge_blocks(nfixed, bufs["nbt"]))
+    per_pass = {
+        "ipfb_rows_packed4": 4 * n_coarse * 300,
+        "ipfb_columns_repfb": 4 * n_coarse * 2 * 16384,
+        "repfb_stage1_edges": 4 * nedge * osamp * 4 * 4096 * 4,
+        "repfb_stage2": 4 * n_coarse * 2400,
+        "xcorr_all_ant_strided": 4 * n_coarse * 2400,
+    }
+    tot = {k: kern[k] * counts.get(k, 1) for k in kern}
+    dom = max((k for k in tot if k in per_pass), key=lambda k: tot[k])
+    alg = per_pass[dom] / max(1, counts.get(dom, 1))
+    achieved = alg / (kern[dom] * 1e-3) / 1e9
+    roof = {"kernel": dom, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(dom + "_batched"),
+            "algorithmic_bytes_per_launch": int(alg), "avg_launch_ms": round(kern[dom], 5),
+            "launches_per_step": counts.get(dom, 1)}
+    pipe_roof = {"bound": "hbm", "algorithmic_bytes_per_step": pipe_bytes,
+                 "achieved": round(pipe_bytes / (ms * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(pipe_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    res = {
+        "value": nsamp / (ms * 1e-3) / 1e6, "ms_per_step": ms, "roofline": roof, "pipeline_roofline": pipe_roof,
+        "kernels_ms": {k: round(v, 5) for k, v in kern.items()}, "launches_per_step": counts,
+        "config": {"workload": "batched streaming: 2 ant x 2 pol, 300 coarse ch 4-bit, 64 arrival chunks "
+                               "U[2^10, 2^14] spectra with ~0.5%% loss in runs of 5 (%d spectra) -> %d iPFB tiles "
+                               "of 4088 (+8 carried), Wiener 0.45 -> re-PFB 64x ntap 4 (19200 fine ch) -> 4x4 vis "
+                               "over %d jobs of 1024 spectra; %s per step" % (
+                                   total, nfixed, njobs, "one eager pass" if args.no_graph else "one hipGraph replay"),
+                   "inputs": 4, "coarse_channels": nchan, "arrival_chunks": 64, "spectra_per_input": n_coarse,
+                   "acclen": acclen, "cut": cut, "osamp": osamp, "nblock": nblock, "jobs": njobs,
+                   "correlator": args.precision},
+        "plain_fp32": {"form": "per-chunk / per-job launches, FP32 correlator, fastest column-pass width",
+                       "ms_per_step": round(ms_chunk, 3),
+                       "Msamples_s": round(nsamp / (ms_chunk * 1e-3) / 1e6, 1),
+                       "ms_by_cols_width": {str(w): round(v, 3) for w, v in chunk_ms.items()}} if ms_chunk else None,
+        "speedup_vs_plain_fp32": round(ms_chunk / ms, 3) if ms_chunk else None,
+        "eager_ms_per_step": round(ms_eager, 3),
+        "max_abs_diff_over_rms_vs_plain_fp32": err,
+        "ref_unit_msps": 4 * n_coarse * 2049 * 8 / (ms * 1e-3) / 1e6,
+        "autotune": tune,
+    }
+    ctx = dict(packed=packed, specnums=specnums, acclen=acclen, cut=cut, osamp=osamp, c0=c0, nchan=nchan,
+               filt=pipe.filt, win=pipe.win)
+    return res, ctx
+
+
+def cpu_baseline_batched(ctx, budget_s, sample_chunks=2):
+    """The reference CPU path on a bounded sample of the same stream: the
+    first ``sample_chunks`` iPFB chunks of each of the 4 inputs, spec_num zero
+    filled, through cpu_ipfb (2*cut carried rows) and cpu_pfb at 64x
+    (pfb_cpu_utils.py:90-172, scipy.fft float32), then the visibilities of
+    the re-PFB blocks those chunks fill with the complex128 all-antenna
+    correlator (correlations.py:115-151).  All cores (median of up to 5 runs
+    within the budget) and one core (one run)."""
+    from albatros_axion_detection_amd import _lib
+    from oracle import cpu_port
+    acclen, cut, osamp, c0, nchan = ctx["acclen"], ctx["cut"], ctx["osamp"], ctx["c0"], ctx["nchan"]
+    sp = _lib.host(ctx["specnums"]) if not isinstance(ctx["specnums"], np.ndarray) else ctx["specnums"]
+    nrow = int(np.searchsorted(sp, sample_chunks * acclen))
+    raw = ctx["packed"][:, :nrow].cpu().numpy()          # [4][rows][300]
+    filt = _lib.host(ctx["filt"]).astype(np.complex64)
+    win = _lib.host(ctx["win"]).astype(np.float32)
+    model, cores = cpu_info()
+
+    def run(workers):
+        xin = []
+        for m in range(4):
+            b = raw[m]
+            hi = (b >> 4).astype(np.int8)
+            lo = (b & 15).astype(np.int8)
+            hi = np.where(hi > 8, hi - 16, hi)
+            lo = np.where(lo > 8, lo - 16, lo)
+            chunks = []
+            for c in range(sample_chunks):
+                sel = (sp[:nrow] >= c * acclen) & (sp[:nrow] < (c + 1) * acclen)
+                full = np.zeros((acclen, 2049), np.complex64)       # make_continuous_gpu zero fill
+                full[np.ix_(sp[:nrow][sel] - c * acclen, np.arange(c0, c0 + nchan))] = hi[sel] + 1j * lo[sel]
+                chunks.append(full)
+            xin.append(cpu_port.rechannelize(chunks, filt, win, acclen, cut, osamp, c0 * osamp,
+                                             (c0 + nchan) * osamp, workers))
+        x = np.stack(xin)                                             # (M, K, F)
+        K = x.shape[1]
+        return np.einsum("ikf,jkf->ijf", x.astype(np.complex128), np.conj(x.astype(np.complex128))) / K
+
+    run(cores)
+    times = []
+    t_all = time.perf_counter()
+    while len(times) < 5 and (time.perf_counter() - t_all) < budget_s:
+        t0 = time.perf_counter()
+        run(cores)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    t1c0 = time.perf_counter()
+    run(1)
+    t1 = time.perf_counter() - t1c0
+    nsamp = 4 * sample_chunks * acclen * 4096
+    return {"value": round(nsamp / t / 1e6, 2), "unit": "Msamples/s", "cores": cores, "kind": "port",
+            "cpu_model": model, "one_core_value": round(nsamp / t1 / 1e6, 2),
+            "sample": "%d iPFB chunks of 4088 spectra x 4 inputs of the same stream (%d coarse spectra): unpack + "
+                      "zero fill, cpu_ipfb, cpu_pfb 64x (scipy.fft float32, workers=%d), complex128 4x4 correlator "
+                      "of the %d re-PFB blocks; median of %d runs (one-core figure: workers=1, one run)"
+                      % (sample_chunks, 4 * sample_chunks * acclen, cores, sample_chunks * acclen // osamp - 3,
+                         len(times))}
+
+
+class _StdoutToStderr:
+    """Send file descriptor 1 to stderr while reference C code runs (its
+    printf output must not mix with the bench's one JSON line)."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+
+    def __exit__(self, *exc):
+        sys.stdout.flush()
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+
+
+def _omp_threads(n):
+    import ctypes.util
+    name = ctypes.util.find_library("gomp") or "libgomp.so.1"
+    try:
+        ctypes.CDLL(name).omp_set_num_threads(int(n))
+        return True
+    except OSError:
+        return False
+
+
+def cpu_reference_direct(kind, inputs, nchan, blocklen, nblocks):
+    """The reference's own OpenMP correlators (oracle/_ref, built from
+    correlations_cpu.c) over ``nblocks`` blocks of every pair: 4-bit
+    avg_autocorr_4bit / avg_xcorr_4bit (:39, :128) or 1-bit avg_xcorr_1bit
+    (:273), one call per pair per block as the reference's callers make
+    them.  All cores and one core."""
+    from oracle import ref as oref
+    if not oref.available():
+        return None
+    model, cores = cpu_info()
+    n = inputs.shape[0]
+    pairs = [(i, j) for i in range(n) for j in range(i, n)]
+
+    def run():
+        for b in range(nblocks):
+            blk = inputs[:, b * blocklen:(b + 1) * blocklen]
+            for i, j in pairs:
+                if kind == 4 and i == j:
+                    oref.avg_autocorr_4bit(blk[i], blocklen)
+                elif kind == 4:
+                    oref.avg_xcorr_4bit(blk[i], blk[j], blocklen)
+                else:
+                    oref.avg_xcorr_1bit(blk[i], blk[j], nchan, blocklen)
+
+    out = {}
+    with _StdoutToStderr():
+        for label, thr in (("all", cores), ("one", 1)):
+            _omp_threads(thr)
+            run()
+            ts = []
+            for _ in range(3 if label == "all" else 1):
+                t0 = time.perf_counter()
+                run()
+                ts.append(time.perf_counter() - t0)
+            out[label] = float(np.median(ts))
+        _omp_threads(cores)
+    nsamp = n * nblocks * blocklen * nchan
+    return {"value": round(nsamp / out["all"] / 1e6, 1), "unit": "Msamples/s", "cores": cores, "kind": "reference",
+            "cpu_model": model, "one_core_value": round(nsamp / out["one"] / 1e6, 1),
+            "sample": "%d of the workload's blocks of %d spectra, every pair (%d), one reference call per pair per "
+                      "block (OpenMP threads = %d; median of 3 runs, one-core: 1 run)"
+                      % (nblocks, blocklen, len(pairs), cores)}
 
 
 def copy_calibration(torch, nbytes=1 << 30, reps=5):
@@ -558,8 +806,18 @@ def main():
                     help="column-pass strips per XCD group (0: library default, 2 up to 4 tiles, 64 above)")
     ap.add_argument("--rows-group", type=int, default=0, choices=[0, 2, 4],
                     help="spectra per workgroup of the FFT row pass (0: library default, 2 up to 4 tiles, 4 above)")
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="skip the pipeline autotuner before the batched streaming leg")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the pipeline autotuner on the batched pass's inputs first and time its choice "
+                         "(default: the library's launch shapes, which the committed profiles were taken with)")
+    ap.add_argument("--no-autotune", action="store_true", help="accepted for compatibility (the default)")
+    ap.add_argument("--lanes", type=int, default=1, choices=[1, 2],
+                    help="concurrent stage-1 lanes of the batched pass (1: every kernel runs alone)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time eager passes instead of hipGraph replays (the counter-collection runs)")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="do not time the plain-FP32 per-chunk form (it still runs once as the result check)")
+    ap.add_argument("--precision", default="fp16x3", choices=["fp32", "bf16x3", "fp16", "fp16x3"],
+                    help="correlator arithmetic of the batched pass (FP16 hi/lo split by default)")
     ap.add_argument("--sweep", action="store_true",
                     help="time every column-pass strip width x row-pass group before timing and keep the fastest")
     ap.add_argument("--row-pass", default="auto", choices=["auto", "fft", "mfma"],
@@ -578,16 +836,24 @@ def main():
     if args.copy_calibration:
         copy_calibration(torch)
         return
-    res, ctx = rechannelizer(args, torch)
+    res, ctx = batched(args, torch)
     cpu = None
     if not args.no_cpu:
-        log("[bench] CPU baseline ...")
-        cpu = cpu_baseline(ctx, args.cpu_budget)
-        log("[bench] CPU baseline: %.1f Msamples/s on %d cores" % (cpu["value"], cpu["cores"]))
+        log("[bench] CPU baseline (batched stream sample) ...")
+        cpu = cpu_baseline_batched(ctx, args.cpu_budget)
+        log("[bench] CPU baseline: %.1f Msamples/s on %d cores, %.1f on one" % (cpu["value"], cpu["cores"],
+                                                                                cpu["one_core_value"]))
+    del ctx
     sec = None
     if not args.no_secondary:
+        log("[bench] rechannelizer ...")
+        rres, rctx = rechannelizer(args, torch)
+        if not args.no_cpu:
+            rres["cpu_baseline"] = cpu_baseline(rctx, args.cpu_budget)
+        rres["value_Msamples_s"] = round(rres.pop("value"), 2)
         log("[bench] secondary workloads ...")
-        sec = secondary(args, torch)
+        sec = {"rechannelizer": rres}
+        sec.update(secondary(args, torch))
     line = {
         "metric": METRIC,
         "value": round(res["value"], 2),
@@ -600,14 +866,19 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded noise + tone, coarse-PFB'd and 4-bit quantised)",
+        "data": "synthetic (seeded 4-bit Gaussian baseband, correlated antennas, runs of lost spectra)",
         "config": res["config"],
         "roofline": res["roofline"],
         "pipeline_roofline": res["pipeline_roofline"],
         "cpu_baseline": cpu,
+        "speedup_vs_plain_fp32": res["speedup_vs_plain_fp32"],
+        "plain_fp32": res["plain_fp32"],
+        "max_abs_diff_over_rms_vs_plain_fp32": res["max_abs_diff_over_rms_vs_plain_fp32"],
+        "eager_ms_per_step": res["eager_ms_per_step"],
         "kernels_ms": res["kernels_ms"],
+        "launches_per_step": res["launches_per_step"],
         "ref_unit_msps": round(res["ref_unit_msps"], 1),
-        "tuning": res["tuning"],
+        "autotune": res["autotune"],
         "secondary": sec,
     }
     print(json.dumps(line), flush=True)
