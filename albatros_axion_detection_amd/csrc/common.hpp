@@ -79,6 +79,15 @@ struct Twiddle {
 };
 int get_twiddle(int64_t N, Twiddle* out, hipStream_t s);
 
+// Re-PFB for osamp 128..65536 (repfb_large.hip): stage 1 into the Z[b][q][p]
+// workspace, band-limited stage 2 from it (argument meaning of alb_repfb_stage1
+// and alb_repfb_stage2_inputs, a_stride in complex elements).
+int repfb_large_stage1(const float* ts, const float* win, int ntap, int64_t nblock, int64_t R1, float2* Z,
+                       hipStream_t s);
+int repfb_large_stage2(const float2* Z, int64_t a_stride, int64_t ninput, int64_t nblock, int64_t R1, int64_t c_lo,
+                       int64_t c_hi, float2* out, int64_t sm, int64_t sb, int64_t sc, int64_t bpj, int64_t sj,
+                       hipStream_t s);
+
 // Tuning knob set through alb_tune_set (0 when unset).
 int64_t tune_get(const char* key);
 

@@ -53,6 +53,9 @@ int alb_repfb_noalloc(const float* ts, const float* win, int ntap, int64_t nbloc
                       int64_t c_hi, void* out, int64_t sb, int64_t sc, int64_t bpj, int64_t sj, void* stream) {
   ALB_REQUIRE(ts && win && out, "alb_repfb_noalloc: null pointer");
   ALB_REQUIRE(nblock >= 0 && osamp >= 1, "alb_repfb_noalloc: bad sizes");
+  // osamp above 64 takes transform tables that are built (allocated) on first use
+  ALB_REQUIRE(osamp <= 64, "alb_repfb_noalloc: osamp %lld above 64 is not allocation-free (use alb_repfb)",
+              (long long)osamp);
   const int64_t need = alb_repfb_workspace_bytes(nblock, osamp);
   ALB_REQUIRE(need <= alb::kStaticWorkspaceBytes,
               "alb_repfb_noalloc: %lld blocks at osamp %lld need %lld bytes of workspace, more than the %lld-byte "

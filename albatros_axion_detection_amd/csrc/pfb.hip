@@ -1764,7 +1764,8 @@ int alb_repfb_stage1(const float* ts, const float* win, int ntap, int64_t nblock
     CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) CASE(64)
 #undef CASE
     default:
-      set_error("alb_repfb: osamp %lld not supported (power of two 1..64)", (long long)osamp);
+      if (is_pow2(osamp) && osamp <= 65536) return repfb_large_stage1(ts, win, ntap, nblock, osamp, (float2*)A, s);
+      set_error("alb_repfb: osamp %lld not supported (power of two 1..65536)", (long long)osamp);
       return ALB_ENOTSUP;
   }
 }
@@ -1799,7 +1800,10 @@ int alb_repfb_stage2_inputs(const void* A, int64_t a_stride, int64_t ninput, int
     CASE(1) CASE(2) CASE(4) CASE(8) CASE(16) CASE(32) CASE(64)
 #undef CASE
     default:
-      set_error("alb_repfb: osamp %lld not supported (power of two 1..64)", (long long)osamp);
+      if (is_pow2(osamp) && osamp <= 65536)
+        return repfb_large_stage2((const float2*)A, as, ninput, nblock, osamp, c_lo, c_hi, (float2*)out, sm, sb, sc,
+                                  bpj, sj, s);
+      set_error("alb_repfb: osamp %lld not supported (power of two 1..65536)", (long long)osamp);
       return ALB_ENOTSUP;
   }
 }

@@ -26,10 +26,32 @@ def sinc_hanning(ntap, lblock):
     return np.hanning(ntap * lblock) * np.sinc(w / lblock)
 
 
+def _sinc_hamming_f32(ntap, lblock, chunk=1 << 24):
+    """``sinc_hamming(ntap, lblock).astype(float32)`` evaluated a chunk at a
+    time: numpy's hamming is ``0.54 + 0.46 cos(pi n / (N - 1))`` with
+    ``n = 1 - N, 3 - N, ...`` and its sinc ``sin(y) / y``, ``y = pi x`` (x = 0
+    as 1e-20), so the same float64 operations per element give the same
+    values without the full-length float64 temporaries (the osamp 65536 window
+    has 2^30 taps)."""
+    N = ntap * lblock
+    out = np.empty(N, dtype=np.float32)
+    if N == 1:
+        out[0] = np.float32(np.sinc(-0.5 / lblock))
+        return out
+    for a in range(0, N, chunk):
+        i = np.arange(a, min(N, a + chunk))
+        n = 1 - N + 2 * i
+        ham = 0.54 + 0.46 * np.cos(np.pi * n / (N - 1))
+        x = (i - N / 2) / lblock
+        y = np.pi * np.where(x == 0, 1.0e-20, x)
+        out[a:a + len(i)] = ham * (np.sin(y) / y)
+    return out
+
+
 def window(ntap, lblock, device_tensor=True):
     """float32 sinc-Hamming window as the reference hands it to the GPU
     (helper_gpu_stream.py:74-75)."""
-    w = sinc_hamming(ntap, lblock).astype(np.float32)
+    w = _sinc_hamming_f32(ntap, lblock)
     return _lib.to_device(w) if device_tensor else w
 
 

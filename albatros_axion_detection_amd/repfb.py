@@ -31,6 +31,8 @@ from . import pfb as _pfb
 FUSED_WORKSPACE_LIMIT = 16 << 30
 # stage-1 workspaces of every input kept at once for one multi-input stage 2
 INPUTS_WORKSPACE_LIMIT = 64 << 30
+# re-PFB workspace of one unfused alb_repfb call (blocks are batched up to it)
+BATCH_WORKSPACE_LIMIT = 8 << 30
 
 
 def get_chunkier(nchunks, lchunk, nblock, lblock, ntap):
@@ -245,9 +247,21 @@ class RePFBPipeline:
             _pfb.ipfb_columns(y, ldy, self.nslice * 4096, nt, self.nslice, 4096, self.filt, self.cut, out, 4096,
                               self.acclen * 4096)
 
-    def repfb(self, stream_rows, nblocks_total, out, sb, sc, bpj, sj, block_batch=512):
+    def block_batch(self, bpj, limit=512):
+        """Blocks per unfused ``alb_repfb`` call: a multiple of the job length
+        ``bpj``, at most ``limit`` blocks and (at least one job) whose
+        workspace stays within ``BATCH_WORKSPACE_LIMIT`` (one re-PFB block is
+        1 GiB of workspace at osamp 65536)."""
+        per = int(_lib.load().alb_repfb_workspace_bytes(1, self.osamp))
+        nb = min(int(limit), max(1, BATCH_WORKSPACE_LIMIT // per))
+        return max(1, nb // bpj) * bpj
+
+    def repfb(self, stream_rows, nblocks_total, out, sb, sc, bpj, sj, block_batch=None):
         """Re-PFB all ``nblocks_total`` spectra of one input's stream into
-        ``out`` (element addressing of ``alb_repfb``)."""
+        ``out`` (element addressing of ``alb_repfb``), ``block_batch`` blocks
+        per call (default :meth:`block_batch`)."""
+        if block_batch is None:
+            block_batch = self.block_batch(bpj)
         s = _lib.stream()
         for b0 in range(0, nblocks_total, block_batch):
             nb = min(block_batch, nblocks_total - b0)
@@ -297,7 +311,7 @@ class RePFBPipeline:
         column pass and re-PFB."""
         if not self.fused_supported(nblocks_total):
             self.ipfb_stream(packed, rowmap, nchunks, stream_out)
-            self.repfb(stream_out, nblocks_total, out, sb, sc, bpj, sj, block_batch=max(1, 512 // bpj) * bpj)
+            self.repfb(stream_out, nblocks_total, out, sb, sc, bpj, sj)
             return
         ws, _ = self._workspace(nblocks_total)
         self.stream_stage1(packed, rowmap, nchunks, stream_out, nblocks_total, ws)
@@ -342,7 +356,7 @@ class RePFBPipeline:
             self.stream_repfb(packed, rowmap, nchunks, stream, nbt, out, self.nfine, 1, nbt, 0)
         else:
             self.ipfb_stream(packed, rowmap, nchunks, stream)
-            self.repfb(stream, nbt, out, self.nfine, 1, nbt, 0, block_batch=nbt)
+            self.repfb(stream, nbt, out, self.nfine, 1, nbt, 0, block_batch=self.block_batch(nbt, nbt))
         return out
 
     def buffers(self, nchunks):
@@ -374,7 +388,7 @@ class RePFBPipeline:
             if nl > 1 and (self._lane_streams is None or len(self._lane_streams) < nl):
                 self._lane_streams = [t.cuda.Stream() for _ in range(nl)]
         else:
-            self._workspace(nbt if self.fused_supported(nbt) else max(1, 512 // K) * K)
+            self._workspace(nbt if self.fused_supported(nbt) else self.block_batch(K))
         return bufs
 
     @staticmethod
@@ -422,7 +436,7 @@ class RePFBPipeline:
         M, K, F = self.ninput, self.nblock, self.nfine
         xin, stream, vis, nbt = bufs["xin"], bufs["stream"], bufs["vis"], bufs["nbt"]
         njobs = len(bufs["jobs"])
-        bb = max(1, 512 // K) * K
+        bb = self.block_batch(K)
         if fused and self.inputs_stage2_supported(nbt):
             # stage 1 of every input into its own workspace, then one stage-2
             # launch writes all inputs' channels into the correlator layout.

@@ -169,7 +169,9 @@ int alb_ipfb_rows_packed4_mfma(const uint8_t* bytes, const int* rowmap, const in
                                int cut, int64_t tile0, int64_t ntiles, const void* coeff, float* out, void* stream);
 int64_t alb_ipfb_rows_coeff_bytes(int64_t ncols);
 int alb_ipfb_rows_coeff(const int* chan, int64_t ncols, void* coeff, void* stream);
-/* pfb_gpu_utils.py:105 cupy_pfb with lblock = 4096*osamp (re-PFB, helper_gpu_stream.py:222):
+/* pfb_gpu_utils.py:105 cupy_pfb with lblock = 4096*osamp (re-PFB, helper_gpu_stream.py:222), osamp a power
+ * of two up to 65536 (the reference's production runs use 65536, scripts/xcorr/config_axion_gpu.json:13;
+ * above 64 stage 2 sums each kept channel directly, for the few channels per bin such runs keep):
  * fine channels [c_lo, c_hi) of nblock spectra; spectrum b, channel c goes to
  * out[(b/bpj)*sj + (b%bpj)*sb + (c-c_lo)*sc] (bpj = blocks per re-PFB job, so all jobs of a
  * stream land in the correlator's F-order (M, nblock, nchan, njobs) layout); A is workspace of
@@ -206,7 +208,7 @@ int alb_ipfb_columns_repfb(const float* y, int64_t ldy, int64_t tile_stride, int
                            const float* win, int ntap, int64_t osamp, int64_t nblocks, void* A, void* stream);
 /* alb_repfb with its stage-1 workspace in a fixed device array of the library instead of A: allocates
  * nothing (no counterpart in the reference; for callers that cannot call hipMalloc -- code under hipGraph
- * capture, tight serving loops, harnesses that forbid allocation).  Requires
+ * capture, tight serving loops, harnesses that forbid allocation).  Requires osamp <= 64 and
  * alb_repfb_workspace_bytes(nblock, osamp) <= alb_repfb_noalloc_max_bytes() (64 MiB: 64 blocks at
  * osamp 64), else ALB_EINVAL before any launch.  Each device holds its own array until the process
  * ends; calls that overlap on one device share it and race. */

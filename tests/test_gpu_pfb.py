@@ -842,3 +842,102 @@ print("CAPTURE_FIRST_OK")
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=240)
     print(r.stdout[-2000:], r.stderr[-4000:], flush=True)
     assert r.returncode == 0 and "CAPTURE_FIRST_OK" in r.stdout
+
+
+@pytest.mark.parametrize("osamp,nblock,c_lo,c_hi", [(128, 6, 64 * 128 + 37, 74 * 128 + 5), (1024, 4, 300 * 1024, 302 * 1024),
+                                                     (8192, 3, 100 * 8192, 102 * 8192),
+                                                     (65536, 1, 47 * 65536, 48 * 65536)])
+def test_repfb_large_osamp(pu, osamp, nblock, c_lo, c_hi):
+    # re-PFB above osamp 64 (two-pass column FFTs + band-limited stage 2)
+    # against the float64 full-length rfft of each block's polyphase sum
+    # (pfb_gpu_utils.py:105-121); 8192 is the osamp of the reference's
+    # published GPU figure (pipeline_speed_test.out:88-91), 65536 with the one
+    # coarse channel 47 its production run (config_axion_gpu.json:13-16)
+    import torch
+    L = 4096 * osamp
+    g = torch.Generator(device="cuda")
+    g.manual_seed(osamp)
+    ts = torch.randn((nblock + 3) * L, generator=g, device="cuda", dtype=torch.float32)
+    win = pu.window(4, L)               # sinc_hamming(4, L) in float32 (bit-identical, tests/test_host_logic.py)
+    win_h = win.cpu().numpy()
+    got = pu.pfb_channels(ts, win, nchan=2048 * osamp + 1, ntap=4, c0=c_lo, c1=c_hi).cpu().numpy()
+    print("[gpu-case] re-PFB osamp %d: GPU done, float64 oracle running" % osamp, flush=True)
+    ts_h = ts.cpu().numpy()
+    del ts
+    errs = []
+    for b in range(nblock):
+        y = np.zeros(L)
+        for j in range(4):
+            y += ts_h[(b + j) * L:(b + j + 1) * L].astype(np.float64) * win_h[j * L:(j + 1) * L].astype(np.float64)
+        ref = np.fft.rfft(y)[c_lo:c_hi]
+        errs.append(rel_err(got[b], ref))
+    print("[gpu-case] re-PFB osamp %d: error vs float64 %s" % (osamp, ["%.2g" % e for e in errs]), flush=True)
+    assert got.shape == (nblock, c_hi - c_lo) and max(errs) < TOL
+
+
+def test_stream_large_osamp_against_oracle(rng):
+    # the streaming pipeline at osamp 1024 (unfused path: iPFB stream, then the
+    # large-osamp re-PFB per job batch) against the float64 oracle stream
+    from albatros_axion_detection_amd import repfb
+    acclen, cut, osamp, nblock, nchunks = 120, 4, 1024, 2, 96
+    chan = np.arange(500, 503)
+    chunks = [_packed_chunks(rng, nchunks, acclen, len(chan), loss=0.05, start=40)]
+    for c in chunks[0]:
+        r1 = rng.integers(0, 256, size=c["pol0"].shape, dtype=np.uint8)
+        r1[len(c["specnums"]):] = 0
+        c["pol1"] = r1
+    vis, miss, freqs, _, _ = repfb.repfb_xcorr_avg_chunks(chunks, [40], chan, 500 * osamp, 503 * osamp, acclen,
+                                                          nchunks, nblock, osamp, cut=cut)
+    from oracle import unpack as ounpack
+    dec = [[{"pol0": (lambda r: r[0] + 1j * r[1])(ounpack.decode_4bit_bytes(c["pol0"])),
+             "pol1": (lambda r: r[0] + 1j * r[1])(ounpack.decode_4bit_bytes(c["pol1"])),
+             "specnums": c["specnums"]} for c in chunks[0]]]
+    filt64 = opfb.wiener_filter(opfb.matft_fp64(acclen + 2 * cut), 0.45)
+    print("[gpu-case] stream osamp 1024: GPU done, oracle running", flush=True)
+    rvis, rmiss, _ = ostream.repfb_xcorr_avg(dec, [40], chan, 500 * osamp, 503 * osamp, acclen, nchunks, nblock, osamp,
+                                             cut, filt64, opfb.window_f32(4, 4096 * osamp))
+    assert vis.shape == rvis.shape and rvis.shape[3] >= 2
+    d = np.sqrt(np.abs(np.einsum("iifs->ifs", rvis)))
+    norm = d[:, None] * d[None, :]
+    band = d.min(axis=0) > 1e-2 * np.median(d)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        err = float(np.max((np.abs(np.asarray(vis) - rvis) / norm)[:, :, band]))
+    print("[gpu-case] stream osamp 1024: in-band normalised error vs float64 %.3g" % err, flush=True)
+    assert err < 1e-4
+    assert np.allclose(miss, rmiss, equal_nan=True)
+
+
+def test_production_shape_end_to_end():
+    # the reference's production configuration (config_axion_gpu.json:13-16:
+    # osamp 65536, ipfb_acclen 2048, cut 10 -> nslice 2068, nblock 3) on one
+    # coarse channel of one antenna's two pols, 195 chunks -> two jobs (one
+    # full, one over the zero-filled tail): runs end to end on the chirp-z
+    # iPFB and the large-osamp re-PFB; the batched pass equals the per-chunk
+    # launch form to FP32 rounding, autocorrelations are real and positive,
+    # cross terms Hermitian.  (Each stage is checked against the float64
+    # oracle separately: test_ipfb[2068], test_repfb_large_osamp[65536].)
+    import torch
+    from albatros_axion_detection_amd import _lib, repfb, synth
+    acclen, cut, osamp, nblock = 2048, 10, 65536, 3
+    packed, specnums, lengths = synth.batched_stream(nant=1, nchan=1, nchunks=2, lmin=200000, lmax=200000,
+                                                     loss=0.005, run=5, seed=8)
+    nfixed = int(lengths.sum()) // acclen
+    assert nfixed == 195
+    pipe = repfb.RePFBPipeline(2, np.array([47]), acclen, cut, osamp, nblock, 47 * osamp, 48 * osamp)
+    srm = repfb.StreamRowmap(specnums, 0, nfixed, acclen)
+    rm = srm.build()
+    plist = [_lib.to_device(packed[m]) for m in range(2)]
+    bufs = pipe.buffers(nfixed)
+    assert len(bufs["jobs"]) == 2
+    vis = pipe.correlate_into(plist, [rm, rm], nfixed, bufs).clone()
+    print("[gpu-case] production shape: batched pass done", flush=True)
+    per = pipe.correlate_per_chunk(plist, [rm, rm], nfixed, bufs).clone()
+    torch.cuda.synchronize()
+    v = vis.cpu().numpy()
+    assert v.shape == (2, 2, osamp, 2) and np.all(np.isfinite(v))
+    scale = np.sqrt(np.mean(np.abs(v) ** 2))
+    assert float(np.max(np.abs(v - per.cpu().numpy()))) < 5e-6 * scale
+    for i in range(2):
+        auto = v[i, i]
+        assert np.all(auto.real > 0) and np.max(np.abs(auto.imag)) < 1e-5 * np.max(auto.real)
+    assert np.max(np.abs(v[0, 1] - np.conj(v[1, 0]))) < 1e-6 * scale

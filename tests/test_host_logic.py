@@ -74,3 +74,14 @@ def test_missing_fraction_hand_traced_cases():
         assert jobs == jobs_expected
         got = repfb.missing_fraction([present], 100, nblock, lblock, lchunk, 4, jobs)
         assert np.allclose(got[0], expected, equal_nan=True, rtol=0, atol=1e-12)
+
+
+def test_window_chunked_matches_sinc_hamming():
+    # the float32 window is evaluated a chunk at a time (the osamp 65536
+    # window has 2^30 taps); it equals sinc_hamming(...).astype(float32) bit
+    # for bit whatever the chunking
+    from albatros_axion_detection_amd import pfb
+    for ntap, lblock, chunk in [(4, 4096, 1000), (4, 4096 * 64, 1 << 16), (6, 4096, 777), (4, 8, 3), (1, 1, 1),
+                                (4, 1, 2), (4, 4096 * 128, 1 << 24)]:
+        a = pfb._sinc_hamming_f32(ntap, lblock, chunk=chunk)
+        assert np.array_equal(a, pfb.sinc_hamming(ntap, lblock).astype(np.float32)), (ntap, lblock)
