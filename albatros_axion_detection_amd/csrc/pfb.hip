@@ -624,10 +624,13 @@ __device__ __forceinline__ void polyphase_walk(float2* Up, int xrow, const float
 // [R1 b, R1 (b + ntap)).  The blocks whose rows all lie in this tile,
 // [bi0, bi1) below, get their stage-1 result A[b][q][k] (q <= R1/2, StageA layout) straight
 // from LDS; only rows that other blocks read go to the stream buffer.
+// A tile may start before stream row 0 (g0 < 0: a window of the stream whose
+// row 0 is the first row of its first block); blocks start at 0.
+__host__ __device__ inline int64_t floor_div(int64_t a, int64_t b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
 __host__ __device__ inline void repfb_interior(int64_t g0, int64_t nkeep, int64_t R1, int64_t ntap, int64_t nblocks,
                                                int64_t* bi0, int64_t* bi1) {
-  const int64_t lo = (g0 + R1 - 1) / R1;
-  int64_t hi = (g0 + nkeep) / R1 - ntap + 1;
+  const int64_t lo = g0 > 0 ? (g0 + R1 - 1) / R1 : 0;
+  int64_t hi = floor_div(g0 + nkeep, R1) - ntap + 1;
   if (hi > nblocks) hi = nblocks;
   *bi0 = lo;
   *bi1 = hi > lo ? hi : lo;
@@ -675,6 +678,7 @@ __device__ __forceinline__ void cols_repfb_tile(const float* __restrict__ y, int
     for (int idx = tid; idx < nw; idx += T) {
       const int w = idx / NP, p = idx - w * NP;
       const int i = w < n1 ? w : w - n1 + n2;
+      if (g0 + i < 0) continue;   // before the window's first block: read by no block of this call
       const float2 v = U[p * PADN + pad16(i + cut)];
       reinterpret_cast<float2*>(ts + (g0 + i) * C + k0)[p] = make_float2(v.x, -v.y);
     }
@@ -1844,8 +1848,7 @@ int alb_repfb_stage1_list(const float* ts, const float* win, int ntap, const int
 // Interior re-PFB blocks of a tile whose kept rows are stream rows
 // [g0, g0 + nkeep): [*lo, *hi) (empty when *hi == *lo).
 int alb_repfb_interior(int64_t g0, int64_t nkeep, int64_t osamp, int ntap, int64_t nblocks, int64_t* lo, int64_t* hi) {
-  ALB_REQUIRE(lo && hi && g0 >= 0 && nkeep >= 0 && osamp >= 1 && ntap >= 1 && nblocks >= 0,
-              "alb_repfb_interior: bad arguments");
+  ALB_REQUIRE(lo && hi && nkeep >= 0 && osamp >= 1 && ntap >= 1 && nblocks >= 0, "alb_repfb_interior: bad arguments");
   repfb_interior(g0, nkeep, osamp, ntap, nblocks, lo, hi);
   return ALB_OK;
 }
@@ -1854,7 +1857,8 @@ int alb_repfb_interior(int64_t g0, int64_t nkeep, int64_t osamp, int ntap, int64
 int alb_ipfb_columns_repfb(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, int64_t nslice,
                            const void* filt, int64_t ldf, int cut, float* ts, int64_t row0, int64_t rows_per_tile,
                            const float* win, int ntap, int64_t osamp, int64_t nblocks, void* A, void* stream) {
-  ALB_REQUIRE(ntiles >= 0 && nblocks >= 0 && row0 >= 0 && rows_per_tile >= 0, "alb_ipfb_columns_repfb: bad sizes");
+  ALB_REQUIRE(ntiles >= 0 && nblocks >= 0 && rows_per_tile >= 0 && row0 + ntiles * rows_per_tile >= 0,
+              "alb_ipfb_columns_repfb: bad sizes");
   ALB_REQUIRE(cut >= 0 && 2 * cut < nslice, "alb_ipfb_columns_repfb: bad cut");
   ALB_REQUIRE(ntiles == 0 || ((ldy >= 4096 || ldy == 8) && ldf >= nslice / 2 + 1),
               "alb_ipfb_columns_repfb: bad strides (ldy >= 4096 row-major, or 8 for the blocked tile layout)");

@@ -30,7 +30,11 @@ from . import pfb as _pfb
 # largest re-PFB workspace (all blocks of one input at once) the fused path allocates
 FUSED_WORKSPACE_LIMIT = 16 << 30
 # stage-1 workspaces of every input kept at once for one multi-input stage 2
+# (a larger pass goes through in groups of jobs)
 INPUTS_WORKSPACE_LIMIT = 64 << 30
+# default time-stream window of the file-driven entry point: jobs go through in
+# groups whose stream rows fit in it
+STREAM_WINDOW_BYTES = 8 << 30
 # re-PFB workspace of one unfused alb_repfb call (blocks are batched up to it)
 BATCH_WORKSPACE_LIMIT = 8 << 30
 
@@ -221,29 +225,27 @@ class RePFBPipeline:
             self._ws = t.empty(need, dtype=t.complex64, device=_lib.device())
         return self._ws, per
 
-    def inputs_stage2_supported(self, nblocks_total):
-        """True when every input's stage-1 workspace fits in
-        ``INPUTS_WORKSPACE_LIMIT`` at once (one stage-2 launch for all inputs)."""
-        ws = int(_lib.load().alb_repfb_workspace_bytes(nblocks_total, self.osamp))
-        return self.fused_supported(nblocks_total) and ws * self.ninput <= INPUTS_WORKSPACE_LIMIT
-
     # --------------------------------------------------------------- stages
-    def ipfb_stream(self, packed, rowmap, nchunks, stream_out):
+    def ipfb_stream(self, packed, rowmap, nchunks, stream_out, tiles=None):
         """Fill ``stream_out`` ([rows, 4096] float32) with the iPFB time stream
         of one input: ``packed`` uint8 [nchunks, acclen, ncols] (present rows
-        first), ``rowmap`` int32 [nchunks, acclen]."""
+        first), ``rowmap`` int32 [nchunks, acclen].  ``tiles = (c_lo, c_hi)``
+        (default all ``nchunks``): only those tiles, tile t's kept rows at
+        rows ``(t - c_lo) * acclen`` of ``stream_out``; rows past them are
+        zeroed."""
+        c_lo, c_hi = tiles if tiles is not None else (0, nchunks)
         s = _lib.stream()
-        rows_valid = nchunks * self.acclen
+        rows_valid = (c_hi - c_lo) * self.acclen
         total = stream_out.shape[0]
         if total > rows_valid:
             tail = stream_out[rows_valid:]
-            _lib.call("alb_fill_zero", _lib.ptr(tail), tail.numel() * 4, s)
+            self._launch("fill_zero", "alb_fill_zero", _lib.ptr(tail), tail.numel() * 4, s)
         tb = self.tile_batch
-        y = self._tile_buffer(min(tb, nchunks))
-        for t0 in range(0, nchunks, tb):
-            nt = min(tb, nchunks - t0)
+        y = self._tile_buffer(min(tb, c_hi - c_lo))
+        for t0 in range(c_lo, c_hi, tb):
+            nt = min(tb, c_hi - t0)
             ldy = self._rows(packed, rowmap, t0, nt, y)
-            out = stream_out[t0 * self.acclen:]
+            out = stream_out[(t0 - c_lo) * self.acclen:]
             _pfb.ipfb_columns(y, ldy, self.nslice * 4096, nt, self.nslice, 4096, self.filt, self.cut, out, 4096,
                               self.acclen * 4096)
 
@@ -294,11 +296,15 @@ class RePFBPipeline:
                   ctypes.byref(lo), ctypes.byref(hi))
         return lo.value, hi.value
 
-    def edge_blocks(self, nchunks, nblocks_total):
-        """Blocks not interior to any of the ``nchunks`` tiles (sorted int32)."""
+    def edge_blocks(self, nchunks, nblocks_total, tiles=None, row_lo=0):
+        """Blocks not interior to any tile (sorted int32): tiles ``tiles =
+        (c_lo, c_hi)`` (default the ``nchunks`` tiles) of a stream window
+        whose row 0 is stream row ``row_lo`` (tile t's kept rows start at
+        window row ``t * acclen - row_lo``)."""
+        c_lo, c_hi = tiles if tiles is not None else (0, nchunks)
         mask = np.ones(nblocks_total, dtype=bool)
-        for c in range(nchunks):
-            lo, hi = self.interior(c * self.acclen, nblocks_total)
+        for c in range(c_lo, c_hi):
+            lo, hi = self.interior(c * self.acclen - row_lo, nblocks_total)
             mask[lo:hi] = False
         return np.nonzero(mask)[0].astype(np.int32)
 
@@ -318,29 +324,33 @@ class RePFBPipeline:
         _lib.call("alb_repfb_stage2", _lib.ptr(ws), nblocks_total, self.osamp, self.c_lo, self.c_hi, _lib.ptr(out),
                   sb, sc, bpj, sj, _lib.stream())
 
-    def stream_stage1(self, packed, rowmap, nchunks, stream_out, nblocks_total, ws, lane=0):
+    def stream_stage1(self, packed, rowmap, nchunks, stream_out, nblocks_total, ws, lane=0, tiles=None, row_lo=0):
         """Fused form up to re-PFB stage 1: every block's stage-1 result of one
         input in the workspace ``ws`` (row pass, column pass with stage 1 of
         the interior blocks, then the edge blocks from the stream buffer), on
-        the current stream with tile buffer ``lane``."""
-        if stream_out.shape[0] < max((nblocks_total + self.ntap - 1) * self.osamp, nchunks * self.acclen):
+        the current stream with tile buffer ``lane``.  ``tiles = (c_lo,
+        c_hi)`` and ``row_lo`` select a window of the stream (a group of
+        jobs): ``stream_out`` row 0 and block 0 start at stream row
+        ``row_lo``; rows of the window past the tiles are zeroed."""
+        c_lo, c_hi = tiles if tiles is not None else (0, nchunks)
+        rows_valid = max(0, c_hi * self.acclen - row_lo)
+        if stream_out.shape[0] < max((nblocks_total + self.ntap - 1) * self.osamp, rows_valid):
             raise ValueError("stream buffer too short for the re-PFB blocks")
         s = _lib.stream()
-        rows_valid = nchunks * self.acclen
         total = stream_out.shape[0]
         if total > rows_valid:
             tail = stream_out[rows_valid:]
             self._launch("fill_zero", "alb_fill_zero", _lib.ptr(tail), tail.numel() * 4, s)
         tb = self.tile_batch
-        y = self._tile_buffer(min(tb, nchunks), lane)
-        for t0 in range(0, nchunks, tb):
-            nt = min(tb, nchunks - t0)
+        y = self._tile_buffer(min(tb, c_hi - c_lo), lane)
+        for t0 in range(c_lo, c_hi, tb):
+            nt = min(tb, c_hi - t0)
             ldy = self._rows(packed, rowmap, t0, nt, y)
             self._launch("ipfb_columns_repfb", "alb_ipfb_columns_repfb", _lib.ptr(y), ldy, self.nslice * 4096, nt,
                          self.nslice, _lib.ptr(self.filt), self.filt.shape[1], self.cut, _lib.ptr(stream_out),
-                         t0 * self.acclen, self.acclen, _lib.ptr(self.win), self.ntap, self.osamp, nblocks_total,
-                         _lib.ptr(ws), s)
-        edges = self.edge_blocks(nchunks, nblocks_total)
+                         t0 * self.acclen - row_lo, self.acclen, _lib.ptr(self.win), self.ntap, self.osamp,
+                         nblocks_total, _lib.ptr(ws), s)
+        edges = self.edge_blocks(nchunks, nblocks_total, (c_lo, c_hi), row_lo)
         self._launch("repfb_stage1_edges", "alb_repfb_stage1_list", _lib.ptr(stream_out), _lib.ptr(self.win),
                      self.ntap, edges.ctypes.data_as(ctypes.c_void_p), len(edges), self.osamp, _lib.ptr(ws), s)
 
@@ -359,25 +369,78 @@ class RePFBPipeline:
             self.repfb(stream, nbt, out, self.nfine, 1, nbt, 0, block_batch=self.block_batch(nbt, nbt))
         return out
 
-    def buffers(self, nchunks):
+    def fused_shape(self):
+        """True when the column pass can carry re-PFB stage 1 (nslice 4096,
+        osamp 32 or 64)."""
+        lib = _lib.load()
+        return lib.alb_ipfb_columns_repfb(None, 4096, 0, 0, self.nslice, None, self.nslice // 2 + 1, self.cut, None, 0,
+                                          self.acclen, None, self.ntap, self.osamp, 0, None, None) == 0
+
+    def groups(self, njobs, group_jobs=None):
+        """Job groups ``[s0, s1)`` of one pass: ``group_jobs`` jobs each
+        (default all at once), in the fused form at most as many as keep every
+        input's stage-1 workspace within ``INPUTS_WORKSPACE_LIMIT``."""
+        if njobs <= 0:
+            return []
+        G = njobs if group_jobs is None else int(group_jobs)
+        if self.fused_shape():
+            per_job = int(_lib.load().alb_repfb_workspace_bytes(self.nblock, self.osamp)) * self.ninput
+            G = min(G, max(1, INPUTS_WORKSPACE_LIMIT // per_job))
+        G = max(1, min(G, njobs))
+        return [(s0, min(njobs, s0 + G)) for s0 in range(0, njobs, G)]
+
+    def window(self, s0, s1, nvalid):
+        """Stream rows and tiles of the jobs ``[s0, s1)``: their blocks read
+        stream rows ``[row_lo, row_hi)``, produced by tiles ``[c_lo, c_hi)``
+        of the ``nvalid`` tiles with data (helper_gpu_stream.py:148-214: each
+        job starts ``nblock`` blocks after the previous one and reads
+        ``ntap - 1`` blocks beyond its own)."""
+        K, R, a = self.nblock, self.osamp, self.acclen
+        row_lo = s0 * K * R
+        row_hi = s1 * K * R + (self.ntap - 1) * R
+        c_lo = min(nvalid, row_lo // a)
+        c_hi = min(nvalid, -(-row_hi // a))
+        return row_lo, row_hi, c_lo, max(c_lo, c_hi)
+
+    def _window_rows(self, s0, s1, nvalid):
+        """Rows of a window buffer that serves the jobs ``[s0, s1)`` both from
+        stream row ``row_lo`` (fused form) and from tile ``c_lo`` (unfused)."""
+        row_lo, row_hi, c_lo, c_hi = self.window(s0, s1, nvalid)
+        a = self.acclen
+        return max(row_hi - row_lo, c_hi * a - row_lo, row_hi - c_lo * a, (c_hi - c_lo) * a)
+
+    def group_jobs_for(self, window_bytes=STREAM_WINDOW_BYTES):
+        """Jobs per group whose time-stream window fits in ``window_bytes``
+        (at least one job)."""
+        rows = int(window_bytes) // (4096 * 4)
+        per = self.nblock * self.osamp
+        return max(1, (rows - (self.ntap - 1) * self.osamp - 2 * self.acclen) // per)
+
+    def buffers(self, nchunks, group_jobs=None):
         """Device buffers of one correlate pass over ``nchunks`` chunks,
         allocated once and reused (a captured hipGraph may not allocate).
-        Also sizes the tile and re-PFB work buffers."""
+        ``group_jobs`` (default: all jobs at once) bounds the memory: the time
+        stream window, the stage-1 workspace and the correlator input then
+        hold one group of jobs, whatever the length of the stream.  Also sizes
+        the tile and re-PFB work buffers."""
         t = _lib.torch()
-        jobs, nbt, rows = self.plan(nchunks)
+        jobs, nbt, _ = self.plan(nchunks)
+        grp = self.groups(len(jobs), group_jobs)
+        G = max([s1 - s0 for s0, s1 in grp] or [0])
+        rows = max([self._window_rows(s0, s1, nchunks) for s0, s1 in grp] or [0])
         M, K, F = self.ninput, self.nblock, self.nfine
         dev = _lib.device()
-        bufs = {"jobs": jobs, "nbt": nbt, "rows": rows,
-                # correlator input [S][K][F][M]: a re-PFB block's channels of all
-                # inputs are one contiguous run (whole-line stage-2 stores); the
-                # correlator takes it through its strided form
-                "xin": t.empty((len(jobs), K, F, M), dtype=t.complex64, device=dev),
+        bufs = {"jobs": jobs, "nbt": nbt, "rows": rows, "groups": grp,
+                # correlator input [S][K][F][M] of one group: a re-PFB block's
+                # channels of all inputs are one contiguous run (whole-line
+                # stage-2 stores); the correlator takes it through its strided form
+                "xin": t.empty((G, K, F, M), dtype=t.complex64, device=dev),
                 "stream": t.empty((rows, 4096), dtype=t.float32, device=dev),
                 "vis": t.empty((len(jobs), F, M, M), dtype=t.complex64, device=dev)}
         self._tile_buffer(min(self.tile_batch, nchunks))
         self.prepare()
-        if self.inputs_stage2_supported(nbt):
-            self._workspace(nbt, M)
+        if self.fused_shape():
+            self._workspace(G * K, M)
             # concurrent stage 1 of up to ``lanes`` inputs: a stream buffer and
             # a tile buffer per lane
             nl = min(self.lanes, M)
@@ -388,7 +451,7 @@ class RePFBPipeline:
             if nl > 1 and (self._lane_streams is None or len(self._lane_streams) < nl):
                 self._lane_streams = [t.cuda.Stream() for _ in range(nl)]
         else:
-            self._workspace(nbt if self.fused_supported(nbt) else self.block_batch(K))
+            self._workspace(self.block_batch(K))
         return bufs
 
     @staticmethod
@@ -428,53 +491,59 @@ class RePFBPipeline:
         transform tables).  Jobs are planned for ``nchunks`` chunks of which
         the first ``nvalid`` (default all) hold data; the time stream past them
         is zero, as the reference's zero-filled incomplete buffer
-        (helper_gpu_stream.py:215-219).  Returns the ``(M, M, nfine, njobs)``
-        F-order view of ``bufs["vis"]``."""
+        (helper_gpu_stream.py:215-219).  The jobs go through in the groups of
+        ``bufs`` (one group unless :meth:`buffers` was given ``group_jobs``).
+        Returns the ``(M, M, nfine, njobs)`` F-order view of ``bufs["vis"]``."""
         from .correlations import PRECISION_MODES
         self.check_precision(precision)
-        nchunks = self._valid(nchunks, nvalid)
+        nv = self._valid(nchunks, nvalid)
         M, K, F = self.ninput, self.nblock, self.nfine
-        xin, stream, vis, nbt = bufs["xin"], bufs["stream"], bufs["vis"], bufs["nbt"]
-        njobs = len(bufs["jobs"])
+        xin, stream, vis = bufs["xin"], bufs["stream"], bufs["vis"]
+        use_fused = fused and self.fused_shape()
         bb = self.block_batch(K)
-        if fused and self.inputs_stage2_supported(nbt):
-            # stage 1 of every input into its own workspace, then one stage-2
-            # launch writes all inputs' channels into the correlator layout.
-            # Inputs go round-robin to ``lanes`` HIP streams (default
-            # priority), each with its own stream and tile buffer, so one
-            # input's small edge-block launches overlap the next input's row
-            # and column passes; the current stream waits for all of them.
-            ws, per = self._workspace(nbt, M)
-            lane_bufs = bufs.get("lane_stream_bufs", [stream])[:self.lanes]
-            nl = len(lane_bufs)
-            if nl == 1:
-                for m in range(M):
-                    self.stream_stage1(packed_list[m], rowmaps[m], nchunks, stream, nbt, ws[m * per:])
-            else:
-                t = _lib.torch()
-                main = t.cuda.current_stream()
-                for st in self._lane_streams[:nl]:
-                    st.wait_stream(main)
-                for m in range(M):
-                    lane = m % nl
-                    with t.cuda.stream(self._lane_streams[lane]):
-                        self.stream_stage1(packed_list[m], rowmaps[m], nchunks, lane_bufs[lane], nbt,
-                                           ws[m * per:], lane=lane)
-                for st in self._lane_streams[:nl]:
-                    main.wait_stream(st)
-            self._launch("repfb_stage2", "alb_repfb_stage2_inputs", _lib.ptr(ws), per * 8, M, nbt, self.osamp,
-                         self.c_lo, self.c_hi, _lib.ptr(xin), 1, F * M, M, K, K * F * M, _lib.stream())
-        else:
-            for m in range(M):
-                if fused:
-                    self.stream_repfb(packed_list[m], rowmaps[m], nchunks, stream, nbt, xin.view(-1)[m:], F * M, M,
-                                      K, K * F * M)
-                else:
-                    self.ipfb_stream(packed_list[m], rowmaps[m], nchunks, stream)
-                    self.repfb(stream, nbt, xin.view(-1)[m:], F * M, M, K, K * F * M, block_batch=bb)
         scale = np.float32(1.0) / np.float32(K)
-        self._launch("xcorr_all_ant_strided", "alb_xcorr_all_ant_strided", _lib.ptr(xin), M, K, F, njobs, F * M, M,
-                     K * F * M, _lib.ptr(vis), float(scale), PRECISION_MODES[precision], _lib.stream())
+        a = self.acclen
+        for s0, s1 in bufs["groups"]:
+            row_lo, row_hi, c_lo, c_hi = self.window(s0, s1, nv)
+            ng = s1 - s0
+            nbg = ng * K
+            if use_fused:
+                # stage 1 of every input into its own workspace, then one
+                # stage-2 launch writes all inputs' channels into the
+                # correlator layout.  Inputs go round-robin to ``lanes`` HIP
+                # streams (default priority), each with its own stream window
+                # and tile buffer, so one input's small edge-block launches
+                # overlap the next input's row and column passes; the current
+                # stream waits for all of them.
+                ws, per = self._workspace(nbg, M)
+                lane_bufs = bufs.get("lane_stream_bufs", [stream])[:self.lanes]
+                nl = len(lane_bufs)
+                if nl == 1:
+                    for m in range(M):
+                        self.stream_stage1(packed_list[m], rowmaps[m], nchunks, stream, nbg, ws[m * per:],
+                                           tiles=(c_lo, c_hi), row_lo=row_lo)
+                else:
+                    t = _lib.torch()
+                    main = t.cuda.current_stream()
+                    for st in self._lane_streams[:nl]:
+                        st.wait_stream(main)
+                    for m in range(M):
+                        lane = m % nl
+                        with t.cuda.stream(self._lane_streams[lane]):
+                            self.stream_stage1(packed_list[m], rowmaps[m], nchunks, lane_bufs[lane], nbg,
+                                               ws[m * per:], lane=lane, tiles=(c_lo, c_hi), row_lo=row_lo)
+                    for st in self._lane_streams[:nl]:
+                        main.wait_stream(st)
+                self._launch("repfb_stage2", "alb_repfb_stage2_inputs", _lib.ptr(ws), per * 8, M, nbg, self.osamp,
+                             self.c_lo, self.c_hi, _lib.ptr(xin), 1, F * M, M, K, K * F * M, _lib.stream())
+            else:
+                # the window starts at tile c_lo; the group's blocks at row_lo
+                for m in range(M):
+                    self.ipfb_stream(packed_list[m], rowmaps[m], nchunks, stream, (c_lo, c_hi))
+                    self.repfb(stream[row_lo - c_lo * a:], nbg, xin.view(-1)[m:], F * M, M, K, K * F * M,
+                               block_batch=bb)
+            self._launch("xcorr_all_ant_strided", "alb_xcorr_all_ant_strided", _lib.ptr(xin), M, K, F, ng, F * M, M,
+                         K * F * M, _lib.ptr(vis[s0]), float(scale), PRECISION_MODES[precision], _lib.stream())
         return vis.permute(3, 2, 1, 0)
 
     # launch-shape knobs of the library tried by autotune (0 = built-in rule)
@@ -552,9 +621,12 @@ class RePFBPipeline:
         loop structure (helper_gpu_stream.py:157-236): per input and chunk one
         row pass and one column pass, per job and input one re-PFB launch, per
         job one correlator launch.  Same result as :meth:`correlate_into`."""
+        nchunks_planned = nchunks
         nchunks = self._valid(nchunks, nvalid)
         M, K, F = self.ninput, self.nblock, self.nfine
         stream, vis = bufs["stream"], bufs["vis"]
+        if stream.shape[0] < self.plan(nchunks_planned)[2]:
+            raise ValueError("the per-chunk form needs whole-stream buffers (buffers() without group_jobs)")
         njobs = len(bufs["jobs"])
         # the reference's per-job correlator input: F-order (M, K, F) per job
         t = _lib.torch()
@@ -584,10 +656,12 @@ class RePFBPipeline:
             _lib.call("alb_xcorr_all_ant", _lib.ptr(xin[j]), M, K, F, _lib.ptr(vis[j]), float(scale), s)
         return vis.permute(3, 2, 1, 0)
 
-    def correlate(self, packed_list, rowmaps, nchunks, return_xin=False, fused=True, precision="fp32", nvalid=None):
+    def correlate(self, packed_list, rowmaps, nchunks, return_xin=False, fused=True, precision="fp32", nvalid=None,
+                  group_jobs=None):
         """All inputs: ``vis`` complex64 F-order ``(M, M, nfine, njobs)`` of
-        ``avg_xcorr_all_ant_gpu`` per job (helper_gpu_stream.py:236)."""
-        bufs = self.buffers(nchunks)
+        ``avg_xcorr_all_ant_gpu`` per job (helper_gpu_stream.py:236).
+        ``return_xin`` (the last group's correlator input) needs one group."""
+        bufs = self.buffers(nchunks, group_jobs)
         res = self.correlate_into(packed_list, rowmaps, nchunks, bufs, fused=fused, precision=precision, nvalid=nvalid)
         if return_xin:
             return res, bufs["xin"].permute(3, 1, 2, 0)
@@ -692,7 +766,8 @@ def repfb_xcorr_avg_chunks(chunks, start_specnums, chan_numbers, repfb_chanstart
 
 
 def repfb_xcorr_avg(idxs, files, acclen, nchunks, nblock, chanstart, chanend, osamp, cut=10, filt_thresh=0.45,
-                    cupy_win_big=None, filt=None, verbose=False, precision="fp32", **pipeline_kw):
+                    cupy_win_big=None, filt=None, verbose=False, precision="fp32", group_jobs=None,
+                    window_bytes=STREAM_WINDOW_BYTES, **pipeline_kw):
     """helper_gpu_stream.py:62-254: re-PFB + cross-correlate ``nchunks`` iPFB
     chunks of ``acclen`` spectra read from baseband files.
 
@@ -713,7 +788,11 @@ def repfb_xcorr_avg(idxs, files, acclen, nchunks, nblock, chanstart, chanend, os
     The files are read through :class:`BasebandFileIterator` (packed form)
     straight into one stacked device buffer per input; if an antenna runs
     out of files, every antenna stops at that chunk (the reference's
-    ``zip``) and the stream past it is zero.  ``precision`` selects the
+    ``zip``) and the stream past it is zero.  The jobs go through in groups
+    of ``group_jobs`` (default: as many as keep one input's time-stream
+    window within ``window_bytes``), so device memory does not grow with the
+    length of the run, as the reference's rolling ``pfb_buf``
+    (helper_gpu_stream.py:88-89, 140-214).  ``precision`` selects the
     correlator arithmetic (``correlations.PRECISION_MODES``); other keywords
     go to :class:`RePFBPipeline`.
     """
@@ -758,7 +837,8 @@ def repfb_xcorr_avg(idxs, files, acclen, nchunks, nblock, chanstart, chanend, os
         rm = rowmaps_from_specnums(present[j], start_specnums[j], acclen) if nread else \
             t.full((nchunks, acclen), -1, dtype=t.int32, device=dev)
         rmaps += [rm] * npol
-    vis_dev = pipe.correlate(packed, rmaps, nchunks, precision=precision, nvalid=nread)
+    G = group_jobs if group_jobs is not None else pipe.group_jobs_for(window_bytes)
+    vis_dev = pipe.correlate(packed, rmaps, nchunks, precision=precision, nvalid=nread, group_jobs=G)
     jobs, _, _ = pipe.plan(nchunks)
     lblock = 4096 * osamp
     npres = [[len(s) for s in present[j]] for j in range(nant)]

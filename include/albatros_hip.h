@@ -200,7 +200,9 @@ int alb_repfb_stage2_inputs(const void* A, int64_t a_stride, int64_t ninput, int
  * [osamp*b, osamp*(b+ntap)) all lie in one tile (alb_repfb_interior) A[b] is produced in the
  * same launch; only stream rows read by the other blocks are written to ts ([rows][4096]).
  * Those other blocks go through alb_repfb_stage1_list once every tile that feeds them is done.
- * Tiles row-major (ldy >= 4096) or blocked (ldy = 8, alb_ipfb_rows_packed4_mfma).
+ * Tiles row-major (ldy >= 4096) or blocked (ldy = 8, alb_ipfb_rows_packed4_mfma).  row0 may be negative:
+ * ts is then a window of the stream whose row 0 is the first row of block 0, rows before it are not
+ * written, and ts must hold every row >= 0 of the tiles (a stream processed a group of jobs at a time).
  * nslice 4096 with osamp 32 or 64 (ENOTSUP otherwise: use alb_ipfb_columns + alb_repfb_stage1).
  * With ntiles = 0 only the shape (nslice, cut, ntap, osamp) is checked; pointers may be null. */
 int alb_ipfb_columns_repfb(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, int64_t nslice,
@@ -215,7 +217,8 @@ int alb_ipfb_columns_repfb(const float* y, int64_t ldy, int64_t tile_stride, int
 int alb_repfb_noalloc(const float* ts, const float* win, int ntap, int64_t nblock, int64_t osamp, int64_t c_lo,
                       int64_t c_hi, void* out, int64_t sb, int64_t sc, int64_t bpj, int64_t sj, void* stream);
 int64_t alb_repfb_noalloc_max_bytes(void);
-/* Interior blocks [*lo, *hi) of a tile whose kept rows are stream rows [g0, g0 + nkeep). */
+/* Interior blocks [*lo, *hi) of a tile whose kept rows are stream rows [g0, g0 + nkeep) (g0 may be
+ * negative; blocks start at 0). */
 int alb_repfb_interior(int64_t g0, int64_t nkeep, int64_t osamp, int ntap, int64_t nblocks, int64_t* lo, int64_t* hi);
 /* alb_repfb_stage1 for the n blocks listed in the host array blocks (block b reads ts rows
  * [osamp*b, osamp*(b+ntap)), writes A[b]); osamp 16, 32 or 64. */

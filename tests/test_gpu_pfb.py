@@ -941,3 +941,40 @@ def test_production_shape_end_to_end():
         auto = v[i, i]
         assert np.all(auto.real > 0) and np.max(np.abs(auto.imag)) < 1e-5 * np.max(auto.real)
     assert np.max(np.abs(v[0, 1] - np.conj(v[1, 0]))) < 1e-6 * scale
+
+
+@pytest.mark.parametrize("osamp,nblock,acclen,nchan,lmin,lmax", [(64, 16, 4088, 24, 2000, 4000),
+                                                                 (2, 4, 120, 16, 200, 400)])
+def test_job_groups_match_whole_stream(osamp, nblock, acclen, nchan, lmin, lmax):
+    # bounded memory: the pass in groups of 1, 3 and 5 jobs (one time-stream
+    # window, stage-1 workspace and correlator input per group, whatever the
+    # stream length) equals the whole-stream pass bit for bit, in the fused
+    # (osamp 64) and unfused (osamp 2) forms, also when the last chunks carry
+    # no data (nvalid < nchunks)
+    import torch
+    from albatros_axion_detection_amd import _lib, repfb, synth
+    cut = 4
+    packed, specnums, lengths = synth.batched_stream(nant=2, nchan=nchan, nchunks=6, lmin=lmin, lmax=lmax, loss=0.01,
+                                                     run=5, seed=osamp)
+    nfixed = int(lengths.sum()) // acclen
+    chan = np.arange(400, 400 + nchan)
+    pipe = repfb.RePFBPipeline(4, chan, acclen, cut, osamp, nblock, 400 * osamp, (400 + nchan) * osamp)
+    rm = repfb.rowmap_stream(specnums, 0, nfixed, acclen)
+    plist = [_lib.to_device(packed[m]) for m in range(4)]
+    rms = [rm] * 4
+    whole = pipe.buffers(nfixed)
+    assert len(whole["groups"]) == 1 and len(whole["jobs"]) >= 8
+    for nvalid in (nfixed, nfixed - 1):
+        ref = pipe.correlate_into(plist, rms, nfixed, whole, nvalid=nvalid).clone()
+        for G in (1, 3, 5):
+            bufs = pipe.buffers(nfixed, group_jobs=G)
+            print("[gpu-case] job groups osamp %d G %d nvalid %d: %d groups, window %d of %d rows"
+                  % (osamp, G, nvalid, len(bufs["groups"]), bufs["stream"].shape[0], whole["stream"].shape[0]),
+                  flush=True)
+            got = pipe.correlate_into(plist, rms, nfixed, bufs, nvalid=nvalid)
+            assert torch.equal(got, ref), (G, nvalid)
+    # the window and the correlator input of a group do not grow with the stream
+    a = pipe.buffers(nfixed // 2, group_jobs=2)
+    b = pipe.buffers(nfixed, group_jobs=2)
+    assert a["stream"].shape == b["stream"].shape and a["xin"].shape == b["xin"].shape
+    assert b["stream"].shape[0] < whole["stream"].shape[0]
