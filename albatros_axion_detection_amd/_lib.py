@@ -46,6 +46,9 @@ SIGNATURES = {
     "alb_irfft_rows": (_i32, [_vp, _i64, _vp, _i64, _i64, _i64, _vp]),
     "alb_rfft_rows": (_i32, [_vp, _i64, _vp, _i64, _i64, _i64, _i64, _i64, _vp]),
     "alb_fft_rows": (_i32, [_vp, _i64, _vp, _i64, _i64, _i64, _i32, _vp]),
+    "alb_fft_large": (_i32, [_vp, _vp, _vp, _i64, _i64, _i32, _vp]),
+    "alb_lag_xcorr": (_i32, [_vp, _vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp]),
+    "alb_apply_delay": (_i32, [_vp, _vp, _vp, _i64, _i64, _vp, _vp]),
     "alb_pfb_small": (_i32, [_vp, _i64, _i64, _vp, _i32, _i64, _i64, _vp, _i64, _vp]),
     "alb_ipfb_columns": (_i32, [_vp, _i64, _i64, _i64, _i64, _i64, _vp, _i64, _i32, _vp, _i64, _i64, _vp]),
     "alb_ipfb_columns_any": (_i32, [_vp, _i64, _i64, _i64, _i64, _i64, _vp, _i64, _i32, _vp, _i64, _i64, _vp, _vp,

@@ -141,7 +141,13 @@ def _fft(inp, axis=1, direction=-1):
     x = _lib.to_device(_lastaxis(inp, axis), t.complex64) if isnp else _lastaxis(inp, axis).to(t.complex64).contiguous()
     n = x.shape[-1]
     out = t.empty_like(x)
-    _lib.call("alb_fft_rows", _lib.ptr(x), n, _lib.ptr(out), n, x.numel() // n, n, int(direction), _lib.stream())
+    if n > 16384:
+        # four-step form above one workgroup's 16384 points (2^15..2^24)
+        work = t.empty_like(x)
+        _lib.call("alb_fft_large", _lib.ptr(x), _lib.ptr(work), _lib.ptr(out), n, x.numel() // n, int(direction),
+                  _lib.stream())
+    else:
+        _lib.call("alb_fft_rows", _lib.ptr(x), n, _lib.ptr(out), n, x.numel() // n, n, int(direction), _lib.stream())
     out = _lastaxis(out, axis)
     return _lib.host(out) if isnp else out
 

@@ -132,6 +132,21 @@ int alb_rfft_rows(const float* in, int64_t ldi, void* out, int64_t ldo, int64_t 
 /* pycufft.py:92 _fft / pycufft.cu:122 cufft_c2c along rows; direction -1 forward, +1 inverse (with 1/n). */
 int alb_fft_rows(const void* in, int64_t ldi, void* out, int64_t ldo, int64_t nrows, int64_t n, int direction,
                  void* stream);
+/* pycufft.py:92-131 _fft / pycufft.cu:122 cufft_c2c along rows of n = 2^15..2^24 points (longer than
+ * alb_fft_rows takes): x and out complex64 [nbatch][n] (they may be the same buffer), work a scratch
+ * buffer of the same size (aliasing neither); direction -1 forward, +1 inverse (with 1/n). */
+int alb_fft_large(const void* x, void* work, void* out, int64_t n, int64_t nbatch, int direction, void* stream);
+/* src/utils/orbcomm_utils_gpu.py:39-52 coarse_xcorr: FFT lag correlation per channel of f1, f2 complex64
+ * [N][nchan] (C order), zero-padded to P points (a power of two, N + dN <= P <= 2^24; the reference pads
+ * to 2N, and any P >= N + dN gives the same lags): out complex64 [nchan][2 dN] holds lag -dN + i at
+ * column i, sum_t f1[t + lag] conj(f2[t]) / (N - |lag|).  work: 3 * nchan * P complex64. */
+int alb_lag_xcorr(const void* f1, const void* f2, int64_t N, int64_t nchan, int64_t dN, int64_t P, void* work,
+                  void* out, void* stream);
+/* src/utils/orbcomm_utils_gpu.py:13-37 apply_delay: out[t][c] = arr[t][c] exp(-2 pi i freqs[c] delay[t])
+ * for arr complex64 [nspec][nchan], delay float64 [nspec], freqs float64 [nchan]: the phase and its
+ * cosine and sine in float64, the product rounded to complex64 (out may be arr). */
+int alb_apply_delay(const void* arr, const double* delay, const double* freqs, int64_t nspec, int64_t nchan, void* out,
+                    void* stream);
 /* pfb_gpu_utils.py:105 cupy_pfb for lblock <= 16384 (coarse PFB): channels [c0, c1) of
  * (nblock, lblock/2+1) written with row stride ldo. */
 int alb_pfb_small(const float* ts, int64_t nblock, int64_t lblock, const float* win, int ntap, int64_t c0, int64_t c1,

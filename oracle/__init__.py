@@ -16,6 +16,7 @@ pfb      sinc-Hamming window, window spectrum (matft), Wiener filter,
          inverse PFB and forward PFB, all in float64
 stream   the streaming re-PFB + correlate driver restated on in-memory
          chunks (chunk planner, zero fill, overlap bookkeeping)
+orbcomm  the FFT lag correlation and delay phase of the ORBCOMM delay search
 ref      ctypes bindings of the reference's own C sources compiled into
          ``oracle/_ref`` (built only where the reference tree is present)
 
@@ -48,6 +49,9 @@ PINNING = {
                                  "coefficient), closed-form special values",
     "stream.missing_fraction": "hand-traced worked cases of helper_gpu_stream.py:157-233 "
                                "(uneven chunks, a chunk spanning jobs, carried remainders past a job)",
+    "orbcomm.coarse_xcorr": "direct lag sums (lag_sums_brute) on small inputs; the lag of a "
+                            "delayed copy",
+    "orbcomm.apply_delay": "unit modulus, zero delay is the identity, linear phase in frequency",
     "pfb.sinc_hamming": "numpy.hamming * numpy.sinc closed form (library), symmetry",
     "pfb.matft_fp64": "float64 numpy.fft.rfft of the window matrix (the reference's "
                       "own construction) vs the 4-term closed form",

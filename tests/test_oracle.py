@@ -471,3 +471,24 @@ def test_complex64_product_oracle():
     assert np.array_equal(a, np.conj(a.transpose(1, 0, 2)))
     # the products are rounded: not bit-identical to complex128 products
     assert not np.array_equal(a, b.astype(np.complex64))
+
+
+def test_orbcomm_coarse_xcorr_equals_direct_lag_sums():
+    # the padded-FFT form of orbcomm_utils_gpu.py:39-52 against direct lag
+    # sums; a copy of f1 delayed by 7 spectra peaks at lag +7
+    from oracle import orbcomm as oorb
+    rng = np.random.default_rng(4)
+    N, nch, dN = 300, 3, 40
+    f1 = rng.standard_normal((N, nch)) + 1j * rng.standard_normal((N, nch))
+    f2 = rng.standard_normal((N, nch)) + 1j * rng.standard_normal((N, nch))
+    a = oorb.coarse_xcorr(f1, f2, dN)
+    assert a.shape == (nch, 2 * dN)
+    assert np.max(np.abs(a - oorb.lag_sums_brute(f1, f2, dN))) < 1e-12
+    g = np.zeros_like(f1)
+    g[7:] = f1[:-7]                                         # f2[t] = f1[t - 7] -> peak at lag -7
+    c = oorb.coarse_xcorr(f1, g, dN)
+    assert np.all(np.argmax(np.abs(c), axis=1) == dN - 7)
+    d = oorb.apply_delay(f1, np.zeros(N), np.linspace(1e6, 2e6, nch))
+    assert np.array_equal(d, f1)
+    e = oorb.apply_delay(np.ones((2, 4)), np.array([1e-6, 2e-6]), np.array([0.0, 1e5, 2e5, 3e5]))
+    assert np.allclose(np.abs(e), 1.0) and np.allclose(np.angle(e[1, 1]), -2 * np.pi * 0.2)
