@@ -96,6 +96,10 @@ int alb_tune_set(const char* key, int64_t value) {
     ALB_REQUIRE(value == 0 || value == 2 || value == 4, "alb_tune_set: ipfb_rows_group must be 0, 2 or 4");
   } else if (k == "xcorr_int_method") {
     ALB_REQUIRE(value >= 0 && value <= 2, "alb_tune_set: xcorr_int_method must be 0, 1 or 2");
+  } else if (k == "xcorr1_form") {
+    ALB_REQUIRE(value >= 0 && value <= 2, "alb_tune_set: xcorr1_form must be 0, 1 or 2");
+  } else if (k == "xcorr4_workgroups") {
+    ALB_REQUIRE(value >= 0 && value <= (int64_t(1) << 20), "alb_tune_set: xcorr4_workgroups must be 0..2^20");
   } else {
     ALB_REQUIRE(false, "alb_tune_set: unknown key '%s'", key);
   }

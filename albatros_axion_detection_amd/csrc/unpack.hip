@@ -182,6 +182,32 @@ __global__ void sortpols1_kernel(const uint8_t* __restrict__ data, uint8_t* __re
   }
 }
 
+// sortpols1_kernel for full byte pairs (channel span a multiple of 4), four
+// output bytes per lane: raw bytes 2k (even) and 2k + 1 (odd) of the lane's 8
+// make output byte k, so the even and odd bytes are gathered into one dword
+// each (v_perm_b32) and every bit move runs on all four bytes at once.  Block
+// (64, 4): x = quad of the row, y = row within the block's 4 rows.
+__global__ __launch_bounds__(256) void sortpols1_quad_kernel(const uint8_t* __restrict__ data,
+                                                             uint32_t* __restrict__ pol0, uint32_t* __restrict__ pol1,
+                                                             int64_t rowstart, int64_t nrows, int64_t nquad,
+                                                             int64_t nchan, int64_t chanstart) {
+  const int64_t q = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (q >= nquad) return;
+  const int64_t bps = nchan / 2;
+  for (int64_t r = int64_t(blockIdx.y) * blockDim.y + threadIdx.y; r < nrows; r += int64_t(gridDim.y) * blockDim.y) {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(data + bps * (r + rowstart) + chanstart / 2) + 2 * q;
+    const uint32_t w0 = src[0], w1 = src[1];
+    const uint32_t e = __builtin_amdgcn_perm(w1, w0, 0x06040200u);   // raw bytes 0, 2, 4, 6
+    const uint32_t o = __builtin_amdgcn_perm(w1, w0, 0x07050301u);   // raw bytes 1, 3, 5, 7
+    const uint32_t o0 = (e & 0xC0C0C0C0u) | ((e & 0x0C0C0C0Cu) << 2) | ((o & 0xC0C0C0C0u) >> 4) |
+                        ((o & 0x0C0C0C0Cu) >> 2);
+    const uint32_t o1 = ((e & 0x30303030u) << 2) | ((e & 0x03030303u) << 4) | ((o & 0x30303030u) >> 2) |
+                        (o & 0x03030303u);
+    pol0[r * nquad + q] = o0;
+    pol1[r * nquad + q] = o1;
+  }
+}
+
 // unpacking.c:7-74: raw nibble histogram hist[v * nchan + j] over rows; mode 0/1/-1.
 // One thread per (channel, row chunk), register counters, 64-bit atomics.
 __global__ void hist4_kernel(const uint8_t* __restrict__ data, unsigned long long* hist, int64_t rowstart, int64_t nrows,
@@ -326,6 +352,18 @@ int alb_sortpols(const uint8_t* data, uint8_t* pol0, uint8_t* pol1, int64_t rows
     ALB_REQUIRE(chanstart % 2 == 0, "ERROR: Start channel index must be even.");
     ALB_REQUIRE(ncols == (chanend - chanstart + 3) / 4, "alb_sortpols: 1-bit ncols must be ceil((chanend-chanstart)/4)");
     ALB_REQUIRE(ncols < (int64_t(1) << 31), "alb_sortpols: too many columns");
+    const int64_t bps = nchan / 2;
+    if ((chanend - chanstart) % 4 == 0 && ncols % 4 == 0 && bps % 4 == 0 && (chanstart / 2) % 4 == 0 &&
+        aligned(data, 4) && aligned(pol0, 4) && aligned(pol1, 4)) {
+      const int64_t nquad = ncols / 4;
+      int64_t gy = (nrows + 3) / 4;
+      if (gy > 4096) gy = 4096;
+      hipLaunchKernelGGL(sortpols1_quad_kernel, dim3(unsigned((nquad + 63) / 64), unsigned(gy)), dim3(64, 4), 0,
+                         as_stream(stream), data, (uint32_t*)pol0, (uint32_t*)pol1, rowstart, nrows, nquad, nchan,
+                         chanstart);
+      ALB_LAUNCH_CHECK("sortpols1_quad_kernel");
+      return ALB_OK;
+    }
     hipLaunchKernelGGL(sortpols1_kernel, grid_rows(ncols, nrows), dim3(256), 0, as_stream(stream), data, pol0, pol1,
                        rowstart, nrows, ncols, nchan, chanstart, int((chanend - chanstart) % 4));
   } else {

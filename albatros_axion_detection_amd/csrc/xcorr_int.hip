@@ -269,17 +269,23 @@ __device__ __forceinline__ void mfma_item(int64_t ngx, int64_t nblock, int64_t* 
   *gx = item - *blk * ngx;
 }
 
-// 4-bit: rows 0..7 = real parts of inputs 0..7, rows 8..15 = imaginary parts
-// (inputs >= NIN are zero); one channel (byte) per product.
+// 4-bit: rows 0..7 = real parts of inputs 0..7, rows 8..15 = imaginary parts;
+// one channel (byte) per product.
 // Re_ab = G[a][b] + G[8+a][8+b],  Im_ab = G[8+a][b] - G[a][8+b]  (x_a conj x_b).
+// A pair (a, b) reads only rows a, b, 8+a, 8+b, so rows of absent inputs
+// (u >= NIN) may hold anything: they read input 0's bytes or the zeroed slots.
 //
 // Workgroup: 4 waves, 4 dword columns (16 channels) of one block.  Each
 // 64-spectrum step the 16-byte row pieces of the inputs are loaded once into
-// LDS (double-buffered, [spectrum][dword][input], 33-dword spectrum stride so
-// the operand reads of 4 spectrum groups land in different banks; the real and
-// imaginary rows of an input read the same word).  Wave w takes dword w.
+// LDS (double-buffered, [spectrum][dword][input slot], 33-dword spectrum
+// stride so the operand reads of 4 spectrum groups land in different banks;
+// the real and imaginary rows of an input read the same word).  Wave w takes
+// dword w.  LS input slots are loaded (4 for up to 4 inputs, else 8), one
+// 16-byte load per (slot, spectrum); kPF4 steps of loads are in flight per
+// thread.  The output staging reuses the step buffers.
 constexpr int kDW4 = 4;          // dword columns per workgroup
 constexpr int kTS4 = 33;         // LDS dwords per spectrum
+constexpr int kPF4 = 4;          // 64-spectrum steps in flight per thread
 
 // With nsplit > 1 the spectra of a block are shared by nsplit workgroups whose
 // exact partial sums meet in the zero-filled output through int64 atomics.
@@ -287,8 +293,11 @@ template <int NIN>
 __global__ __launch_bounds__(256) void allpairs4_mfma_kernel(InputPtrs in, int64_t ncol, int64_t blocklen,
                                                              int64_t nblock, int nsplit, long long* __restrict__ out) {
   constexpr int NP = NIN * (NIN + 1) / 2;
+  constexpr int LS = NIN <= 4 ? 4 : 8;      // loaded input slots
+  constexpr int SP = 256 / LS;              // spectra per load pass
+  constexpr int LP = 64 / SP;               // 16-byte loads per thread per step
   __shared__ uint32_t buf[2][64 * kTS4];
-  __shared__ int S[4][4][256];
+  static_assert(4 * 4 * 256 <= 2 * 64 * kTS4, "output staging must fit the step buffers");
   const int tid = threadIdx.x;
   const int wv = tid / 64, lane = tid % 64;
   int64_t bs, gx;
@@ -296,67 +305,82 @@ __global__ __launch_bounds__(256) void allpairs4_mfma_kernel(InputPtrs in, int64
   const int64_t blk = bs / nsplit, sp = bs - blk * nsplit;
   const int r = lane % 16, g = lane / 16;
   const int u = r % 8, part = r / 8;
-  // loader: chunks c = tid + 256 e (e < 2): input lu = c % 8, spectrum lt = c / 8
-  const int lu = tid % 8, lt = tid / 8;             // lt in [0, 32); second chunk lt + 32
-  const bool lok = lu < NIN;
-  const uint8_t* lsrc = in.p[lok ? lu : 0] + 16 * gx;
+  const int lu = tid % LS, lt = tid / LS;
+  const uint8_t* lsrc = in.p[lu < NIN ? lu : 0] + 16 * gx;
   const int64_t nsteps = blocklen / 64 / nsplit;
   const int64_t t_begin = blk * blocklen + sp * nsteps * 64;
-  uint4 ld0 = make_uint4(0, 0, 0, 0), ld1 = ld0;
-  if (lok) {
-    ld0 = *reinterpret_cast<const uint4*>(lsrc + (t_begin + lt) * ncol);
-    ld1 = *reinterpret_cast<const uint4*>(lsrc + (t_begin + lt + 32) * ncol);
+  if (LS < 8) {
+    // slots LS..7 are read as operand rows of absent inputs: give them defined values
+    for (int i = tid; i < 2 * 64 * 4; i += 256) {
+      uint32_t* b = buf[i / 256] + ((i % 256) / 4) * kTS4 + (i % 4) * 8;
+#pragma unroll
+      for (int k = LS; k < 8; ++k) b[k] = 0;
+    }
   }
-  auto stash = [&](uint32_t* b) {
-    b[lt * kTS4 + 0 * 8 + lu] = ld0.x;
-    b[lt * kTS4 + 1 * 8 + lu] = ld0.y;
-    b[lt * kTS4 + 2 * 8 + lu] = ld0.z;
-    b[lt * kTS4 + 3 * 8 + lu] = ld0.w;
-    b[(lt + 32) * kTS4 + 0 * 8 + lu] = ld1.x;
-    b[(lt + 32) * kTS4 + 1 * 8 + lu] = ld1.y;
-    b[(lt + 32) * kTS4 + 2 * 8 + lu] = ld1.z;
-    b[(lt + 32) * kTS4 + 3 * 8 + lu] = ld1.w;
+  // ring slot q holds the steps st with st % kPF4 == q; unconditional loads
+  // (past the range they reload its last step) keep the wait counts exact
+  uint4 ld[kPF4][LP];
+  auto fetch = [&](int q, int64_t st) {
+    const uint8_t* p = lsrc + (t_begin + (st < nsteps ? st : nsteps - 1) * 64 + lt) * ncol;
+#pragma unroll
+    for (int e = 0; e < LP; ++e) ld[q][e] = *reinterpret_cast<const uint4*>(p + int64_t(SP) * e * ncol);
   };
-  stash(buf[0]);
+  auto stash = [&](uint32_t* b, int q) {
+#pragma unroll
+    for (int e = 0; e < LP; ++e) {
+      uint32_t* d = b + (lt + SP * e) * kTS4 + lu;
+      d[0 * 8] = ld[q][e].x;
+      d[1 * 8] = ld[q][e].y;
+      d[2 * 8] = ld[q][e].z;
+      d[3 * 8] = ld[q][e].w;
+    }
+  };
+#pragma unroll
+  for (int q = 0; q < kPF4; ++q) fetch(q, q);
+  stash(buf[0], 0);
+  fetch(0, kPF4);
   __syncthreads();
   intx4 acc[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) acc[j] = intx4{0, 0, 0, 0};
-  for (int64_t st = 0; st < nsteps; ++st) {
-    if (st + 1 < nsteps && lok) {
-      const int64_t t1 = t_begin + (st + 1) * 64;
-      ld0 = *reinterpret_cast<const uint4*>(lsrc + (t1 + lt) * ncol);
-      ld1 = *reinterpret_cast<const uint4*>(lsrc + (t1 + lt + 32) * ncol);
-    }
-    const uint32_t* b = buf[st & 1];
-    uint32_t x[16];
+  for (int64_t s0 = 0; s0 < nsteps; s0 += kPF4) {
 #pragma unroll
-    for (int i = 0; i < 16; ++i) x[i] = b[(16 * g + i) * kTS4 + wv * 8 + u];
+    for (int q = 0; q < kPF4; ++q) {
+      const int64_t st = s0 + q;
+      if (st < nsteps) {   // uniform
+        const uint32_t* b = buf[q & 1];
+        uint32_t x[16];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      intx4 v;
+        for (int i = 0; i < 16; ++i) x[i] = b[(16 * g + i) * kTS4 + wv * 8 + u];
 #pragma unroll
-      for (int d = 0; d < 4; ++d) {
-        const uint32_t pd = gather_byte(x[4 * d], x[4 * d + 1], x[4 * d + 2], x[4 * d + 3], j);
-        const uint32_t nib = part ? (pd & 0x0F0F0F0Fu) : ((pd >> 4) & 0x0F0F0F0Fu);
-        // v > 8 -> v - 16 per byte: set the high nibble of those bytes
-        uint32_t sgn = ((nib + 0x07070707u) >> 4) & 0x01010101u;
-        sgn <<= 4;
-        sgn |= sgn << 1;
-        sgn |= sgn << 2;
-        v[d] = int(nib | sgn);
+        for (int j = 0; j < 4; ++j) {
+          intx4 v;
+#pragma unroll
+          for (int d = 0; d < 4; ++d) {
+            const uint32_t pd = gather_byte(x[4 * d], x[4 * d + 1], x[4 * d + 2], x[4 * d + 3], j);
+            const uint32_t nib = part ? (pd & 0x0F0F0F0Fu) : ((pd >> 4) & 0x0F0F0F0Fu);
+            // v > 8 -> v - 16 per byte: set the high nibble of those bytes
+            uint32_t sgn = ((nib + 0x07070707u) >> 4) & 0x01010101u;
+            sgn <<= 4;
+            sgn |= sgn << 1;
+            sgn |= sgn << 2;
+            v[d] = int(nib | sgn);
+          }
+          acc[j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(v, v, acc[j], 0, 0, 0);
+        }
       }
-      // rows of absent inputs hold zeros (their loads were skipped)
-      acc[j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(v, v, acc[j], 0, 0, 0);
+      // buffer (q + 1) & 1 was last read one step ago; the ring slot is free after the stash
+      if (st + 1 < nsteps) stash(buf[(q + 1) & 1], (q + 1) % kPF4);
+      fetch((q + 1) % kPF4, st + 1 + kPF4);
+      __syncthreads();
     }
-    if (st + 1 < nsteps) stash(buf[(st + 1) & 1]);   // that buffer was last read one step ago
-    __syncthreads();
   }
-  // D layout: lane l holds G[4 (l / 16) + v][l % 16]
+  // D layout: lane l holds G[4 (l / 16) + v][l % 16]; staging [wave][j][256]
+  int* S = reinterpret_cast<int*>(&buf[0][0]);
 #pragma unroll
   for (int j = 0; j < 4; ++j)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) S[wv][j][(4 * g + v) * 16 + r] = acc[j][v];
+    for (int v = 0; v < 4; ++v) S[(wv * 4 + j) * 256 + (4 * g + v) * 16 + r] = acc[j][v];
   __syncthreads();
   const int64_t w = gx * kDW4 + wv;
   for (int it = lane; it < NP * 4; it += 64) {
@@ -364,7 +388,7 @@ __global__ __launch_bounds__(256) void allpairs4_mfma_kernel(InputPtrs in, int64
     int a = 0, rem = p;
     while (rem >= NIN - a) { rem -= NIN - a; ++a; }
     const int bb = a + rem;
-    const int* G = S[wv][j];
+    const int* G = S + (wv * 4 + j) * 256;
     const long long re = (long long)G[a * 16 + bb] + G[(8 + a) * 16 + 8 + bb];
     const long long im = (long long)G[(8 + a) * 16 + bb] - G[a * 16 + 8 + bb];
     const int64_t ch = 4 * w + j;
@@ -390,12 +414,23 @@ __global__ __launch_bounds__(256) void allpairs4_mfma_kernel(InputPtrs in, int64
 // double-buffered, laid out [spectrum][dword][input] with a 65-dword spectrum
 // stride so the matrix-operand reads (16 inputs x 4 spectrum groups per
 // instruction) hit 64 different banks.  Wave w takes bytes 2 (w % 2), +1 of
-// dword w / 2: 8 channels, 24 accumulator tiles.
+// dword w / 2: 8 channels, 24 accumulator tiles.  The row pieces of kPF1
+// steps are in flight per thread (a register ring), which covers the HBM
+// latency at one workgroup per CU.
+//
+// MASKED: the operand bytes are the raw bytes ANDed with the bit's mask (one
+// operation per 4 spectra), so a set bit weighs its place value s (-128 for
+// bit 7, signed int8) and each product sum is s_a s_b times the bit count:
+// an exact power-of-two multiple, removed by a shift.  The largest weight is
+// 128^2 = 2^14, so the int32 sums stay exact while blocklen <= 2^16.
+// Otherwise each bit is shifted to place 0 first.
 constexpr int kDW1 = 4;          // dword columns per workgroup
 constexpr int kTS1 = 65;         // LDS dwords per spectrum
 constexpr int kT1 = 512;         // threads per workgroup
+constexpr int kPF1 = 4;          // 64-spectrum steps in flight per thread
+constexpr int64_t kMasked1Max = int64_t(1) << 16;
 
-template <int NIN>
+template <int NIN, bool MASKED>
 __global__ __launch_bounds__(kT1) void allpairs1_mfma_kernel(InputPtrs in, int64_t ncol, int64_t nchan,
                                                              int64_t blocklen, int64_t nblock,
                                                              long long* __restrict__ out) {
@@ -408,26 +443,35 @@ __global__ __launch_bounds__(kT1) void allpairs1_mfma_kernel(InputPtrs in, int64
   const int dw = wv / 2, j0 = (wv % 2) * 2;
   const int u = lane % 16, g = lane / 16;
   // loader: chunks c = tid + 512 e (e < 2): input lu = c % 16, spectrum lt = c / 16
+  // Rows of absent inputs (lu >= NIN) load input 0's bytes: a pair (a, b) reads
+  // only rows a and b of the products, so those rows never reach an output.
   const int lu = tid % 16, lt = tid / 16;           // lt in [0, 32); second chunk lt + 32
-  const bool lok = lu < NIN;
-  const uint8_t* lsrc = in.p[lok ? lu : 0] + 16 * gx;
+  const uint8_t* lsrc = in.p[lu < NIN ? lu : 0] + 16 * gx;
   const int64_t t_begin = blk * blocklen, nsteps = blocklen / 64;
-  uint4 ld0 = make_uint4(0, 0, 0, 0), ld1 = ld0;
-  if (lok) {
-    ld0 = *reinterpret_cast<const uint4*>(lsrc + (t_begin + lt) * ncol);
-    ld1 = *reinterpret_cast<const uint4*>(lsrc + (t_begin + lt + 32) * ncol);
-  }
-  auto stash = [&](uint32_t* b) {
-    b[lt * kTS1 + 0 * 16 + lu] = ld0.x;
-    b[lt * kTS1 + 1 * 16 + lu] = ld0.y;
-    b[lt * kTS1 + 2 * 16 + lu] = ld0.z;
-    b[lt * kTS1 + 3 * 16 + lu] = ld0.w;
-    b[(lt + 32) * kTS1 + 0 * 16 + lu] = ld1.x;
-    b[(lt + 32) * kTS1 + 1 * 16 + lu] = ld1.y;
-    b[(lt + 32) * kTS1 + 2 * 16 + lu] = ld1.z;
-    b[(lt + 32) * kTS1 + 3 * 16 + lu] = ld1.w;
+  // Ring slot q holds the row pieces of the steps st with st % kPF1 == q.  The
+  // loads are unconditional (steps past the block reload its last step, an L2
+  // hit), so the compiler counts the loads in flight exactly and each stash
+  // waits only for its own step.
+  uint4 ld[kPF1][2];
+  auto fetch = [&](int q, int64_t st) {
+    const uint8_t* p = lsrc + (t_begin + (st < nsteps ? st : nsteps - 1) * 64 + lt) * ncol;
+    ld[q][0] = *reinterpret_cast<const uint4*>(p);
+    ld[q][1] = *reinterpret_cast<const uint4*>(p + 32 * ncol);
   };
-  stash(buf[0]);
+  auto stash = [&](uint32_t* b, int q) {
+    b[lt * kTS1 + 0 * 16 + lu] = ld[q][0].x;
+    b[lt * kTS1 + 1 * 16 + lu] = ld[q][0].y;
+    b[lt * kTS1 + 2 * 16 + lu] = ld[q][0].z;
+    b[lt * kTS1 + 3 * 16 + lu] = ld[q][0].w;
+    b[(lt + 32) * kTS1 + 0 * 16 + lu] = ld[q][1].x;
+    b[(lt + 32) * kTS1 + 1 * 16 + lu] = ld[q][1].y;
+    b[(lt + 32) * kTS1 + 2 * 16 + lu] = ld[q][1].z;
+    b[(lt + 32) * kTS1 + 3 * 16 + lu] = ld[q][1].w;
+  };
+#pragma unroll
+  for (int q = 0; q < kPF1; ++q) fetch(q, q);
+  stash(buf[0], 0);
+  fetch(0, kPF1);
   __syncthreads();
   intx4 acc[2][4][3];
 #pragma unroll
@@ -436,37 +480,46 @@ __global__ __launch_bounds__(kT1) void allpairs1_mfma_kernel(InputPtrs in, int64
     for (int c = 0; c < 4; ++c)
 #pragma unroll
       for (int m = 0; m < 3; ++m) acc[j][c][m] = intx4{0, 0, 0, 0};
-  for (int64_t st = 0; st < nsteps; ++st) {
-    if (st + 1 < nsteps && lok) {
-      const int64_t t1 = t_begin + (st + 1) * 64;
-      ld0 = *reinterpret_cast<const uint4*>(lsrc + (t1 + lt) * ncol);
-      ld1 = *reinterpret_cast<const uint4*>(lsrc + (t1 + lt + 32) * ncol);
-    }
-    const uint32_t* b = buf[st & 1];
-    uint32_t x[16];
+  // s0 is a multiple of kPF1 (even): step s0 + q reads buffer q & 1 and ring slot q
+  for (int64_t s0 = 0; s0 < nsteps; s0 += kPF1) {
 #pragma unroll
-    for (int i = 0; i < 16; ++i) x[i] = b[(16 * g + i) * kTS1 + dw * 16 + u];
+    for (int q = 0; q < kPF1; ++q) {
+      const int64_t st = s0 + q;
+      if (st < nsteps) {   // uniform
+        const uint32_t* b = buf[q & 1];
+        uint32_t x[16];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      uint32_t pd[4];
+        for (int i = 0; i < 16; ++i) x[i] = b[(16 * g + i) * kTS1 + dw * 16 + u];
 #pragma unroll
-      for (int d = 0; d < 4; ++d)
-        pd[d] = gather_byte(x[4 * d], x[4 * d + 1], x[4 * d + 2], x[4 * d + 3], j0 + j);
+        for (int j = 0; j < 2; ++j) {
+          uint32_t pd[4];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        intx4 re, im;
+          for (int d = 0; d < 4; ++d)
+            pd[d] = gather_byte(x[4 * d], x[4 * d + 1], x[4 * d + 2], x[4 * d + 3], j0 + j);
 #pragma unroll
-        for (int d = 0; d < 4; ++d) {
-          re[d] = int((pd[d] >> (7 - 2 * c)) & 0x01010101u);
-          im[d] = int((pd[d] >> (6 - 2 * c)) & 0x01010101u);
+          for (int c = 0; c < 4; ++c) {
+            intx4 re, im;
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+              if (MASKED) {
+                re[d] = int(pd[d] & (0x01010101u << (7 - 2 * c)));
+                im[d] = int(pd[d] & (0x01010101u << (6 - 2 * c)));
+              } else {
+                re[d] = int((pd[d] >> (7 - 2 * c)) & 0x01010101u);
+                im[d] = int((pd[d] >> (6 - 2 * c)) & 0x01010101u);
+              }
+            }
+            acc[j][c][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(re, re, acc[j][c][0], 0, 0, 0);
+            acc[j][c][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(im, im, acc[j][c][1], 0, 0, 0);
+            acc[j][c][2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(re, im, acc[j][c][2], 0, 0, 0);
+          }
         }
-        acc[j][c][0] = __builtin_amdgcn_mfma_i32_16x16x64_i8(re, re, acc[j][c][0], 0, 0, 0);
-        acc[j][c][1] = __builtin_amdgcn_mfma_i32_16x16x64_i8(im, im, acc[j][c][1], 0, 0, 0);
-        acc[j][c][2] = __builtin_amdgcn_mfma_i32_16x16x64_i8(re, im, acc[j][c][2], 0, 0, 0);
       }
+      // buffer (q + 1) & 1 was last read one step ago; the ring slot is free after the stash
+      if (st + 1 < nsteps) stash(buf[(q + 1) & 1], (q + 1) % kPF1);
+      fetch((q + 1) % kPF1, st + 1 + kPF1);
+      __syncthreads();
     }
-    if (st + 1 < nsteps) stash(buf[(st + 1) & 1]);   // that buffer was last read one step ago
-    __syncthreads();
   }
   // outputs: per channel, the wave's three 16 x 16 products go through its LDS
   // slot (768 dwords of the staging buffers) and lanes form the pairs
@@ -477,9 +530,19 @@ __global__ __launch_bounds__(kT1) void allpairs1_mfma_kernel(InputPtrs in, int64
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
 #pragma unroll
-      for (int m = 0; m < 3; ++m)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) S[m * 256 + (4 * g + v) * 16 + u] = acc[j][c][m][v];
+      for (int v = 0; v < 4; ++v) {
+        // bit counts: MASKED sums carry the weights 2^(14-4c) (re re), 2^(12-4c)
+        // (im im) and 2^(13-4c) (re im, negative for c = 0: bit 7 weighs -128)
+        int rr = acc[j][c][0][v], ii = acc[j][c][1][v], ri = acc[j][c][2][v];
+        if (MASKED) {
+          rr >>= 14 - 4 * c;
+          ii >>= 12 - 4 * c;
+          ri = c == 0 ? -(ri >> 13) : ri >> (13 - 4 * c);
+        }
+        S[0 * 256 + (4 * g + v) * 16 + u] = rr;
+        S[1 * 256 + (4 * g + v) * 16 + u] = ii;
+        S[2 * 256 + (4 * g + v) * 16 + u] = ri;
+      }
       __syncthreads();
       const int64_t ch = 4 * (4 * (gx * kDW1 + dw) + j0 + j) + c;
       if (ch < nchan) {
@@ -502,6 +565,180 @@ __global__ __launch_bounds__(kT1) void allpairs1_mfma_kernel(InputPtrs in, int64
       }
       __syncthreads();
     }
+  }
+}
+
+// 1-bit, one 16 x 16 tile per channel on the FP4 matrix cores.  With
+// x = +-1 coded as the e2m1 nibble 0x2 | (b << 3) (+1 or -1: the sign of x
+// flipped, which leaves every product below unchanged), P = x_re, Q = x_im,
+// and k concatenating two copies of a step's 64 spectra,
+//   MFMA(A = [Q | P], B = [Q | P])  = Q Q^T + P P^T
+//   MFMA(A = [Q | P], B = [P | -Q]) = Q P^T - P Q^T
+// sum to T = Re + Im with Re_ab = sum x_re,a x_re,b + x_im,a x_im,b
+// (symmetric) and Im_ab = sum x_im,a x_re,b - x_re,a x_im,b (antisymmetric),
+// so Re = (T + T^T) / 2 and Im = (T - T^T) / 2.  Every product is +-1 and
+// every sum an integer with |T| <= 4 blocklen <= 2^24: the FP32 accumulation
+// is exact while blocklen <= 2^22.  One tile per channel (a third of the
+// accumulators of the three-product form) lets one workgroup own 64 bytes of
+// every row (256 channels): each 64-byte line is fetched once and used whole.
+//
+// Workgroup: 8 waves, 64 bytes of columns of one block.  Each 64-spectrum step
+// the 16-byte row pieces (4 lanes per 64-byte line) go through registers into
+// LDS [spectrum][input][16 dwords] (strides kSS1t, kUS1t), double-buffered,
+// with the next step's loads in flight.  Wave w takes dwords 2w and 2w + 1:
+// 8 bytes, 32 channels, 32 tiles; lane (u, g) holds spectra 16 g .. 16 g + 15
+// of input u, 8 per operand dword.  Rows of absent inputs (u >= NIN) read
+// input 0's bytes and never reach an output.
+constexpr int kW1t = 64;                 // row bytes per workgroup
+constexpr int kUS1t = 18;                // LDS dwords per (spectrum, input)
+constexpr int kSS1t = 16 * kUS1t + 6;    // LDS dwords per spectrum
+constexpr int64_t kTile1Max = int64_t(1) << 22;
+
+typedef int intx8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// D += A B^T over 128 e2m1 elements per row (unit block scales; the e2m1
+// operands are the low 4 dwords of a and b)
+__device__ __forceinline__ floatx4 mfma_fp4(intx8 a, intx8 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b, c, 4, 4, 0, 0, 0, 0);
+}
+
+// shift left by s (right for s < 0)
+__device__ __forceinline__ uint32_t shl(uint32_t v, int s) { return s >= 0 ? v << s : v >> -s; }
+
+// e2m1 sign nibbles of 8 spectra.  pa, pb: one byte column of 4 spectra each.
+// hi holds the high nibbles of pa (high) and pb (low) of every byte, lo the
+// low nibbles; the channel bit at place p of both lands on bits 7 and 3 of
+// (p >= 4 ? hi : lo) << (3 - p % 4), which ORed with 0x2 per nibble gives
+// 0x2 | (bit << 3).  v_bitop3 truth tables over (s0, s1, s2) = (0xF0, 0xCC,
+// 0xAA): 0xE4 = s2 ? s0 : s1, 0xEA = (s0 & s1) | s2.
+__device__ __forceinline__ void nibble_halves(uint32_t pa, uint32_t pb, uint32_t* hi, uint32_t* lo) {
+  *hi = __builtin_amdgcn_bitop3_b32(pa, pb >> 4, 0xF0F0F0F0u, 0xE4);
+  *lo = __builtin_amdgcn_bitop3_b32(pa << 4, pb, 0xF0F0F0F0u, 0xE4);
+}
+__device__ __forceinline__ uint32_t fp4_signs(uint32_t src, int s) {
+  return __builtin_amdgcn_bitop3_b32(src << s, 0x88888888u, 0x22222222u, 0xEA);
+}
+
+template <int NIN>
+__global__ __launch_bounds__(512) void allpairs1_tile_kernel(InputPtrs in, int64_t ncol, int64_t nchan,
+                                                             int64_t blocklen, int64_t nblock,
+                                                             long long* __restrict__ out) {
+  constexpr int NP = NIN * (NIN + 1) / 2;
+  __shared__ uint32_t buf[2][64 * kSS1t];
+  static_assert(8 * 4 * 256 <= 2 * 64 * kSS1t, "output staging must fit the step buffers");
+  const int tid = threadIdx.x;
+  const int wv = tid / 64, lane = tid % 64;
+  int64_t blk, gx;
+  mfma_item(ncol / kW1t, nblock, &blk, &gx);
+  // loader: wave wv loads inputs 2 wv and 2 wv + 1; lane = (16-byte piece q, spectrum r of 16)
+  const int q = lane % 4, r = lane / 4;
+  const uint8_t* src[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) src[h] = in.p[2 * wv + h < NIN ? 2 * wv + h : 0] + kW1t * gx + 16 * q;
+  const int64_t t_begin = blk * blocklen, nsteps = blocklen / 64;
+  uint4 ld[8];
+  auto fetch = [&](int64_t st) {   // piece e: input 2 wv + (e & 1), spectrum r + 16 (e >> 1)
+    const int64_t row = t_begin + (st < nsteps ? st : nsteps - 1) * 64 + r;
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      ld[e] = *reinterpret_cast<const uint4*>(src[e & 1] + (row + 16 * (e >> 1)) * ncol);
+  };
+  auto stash = [&](uint32_t* b) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      uint32_t* d = b + (r + 16 * (e >> 1)) * kSS1t + (2 * wv + (e & 1)) * kUS1t + 4 * q;
+      *reinterpret_cast<uint2*>(d) = make_uint2(ld[e].x, ld[e].y);
+      *reinterpret_cast<uint2*>(d + 2) = make_uint2(ld[e].z, ld[e].w);
+    }
+  };
+  fetch(0);
+  stash(buf[0]);
+  fetch(1);
+  __syncthreads();
+  const int u = lane % 16, g = lane / 16;
+  floatx4 acc[8][4];
+#pragma unroll
+  for (int jj = 0; jj < 8; ++jj)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) acc[jj][c] = floatx4{0, 0, 0, 0};
+  for (int64_t s0 = 0; s0 < nsteps; s0 += 2) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int64_t st = s0 + h;
+      if (st < nsteps) {   // uniform
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+          // dword 2 wv + half of the 16 spectra 16 g .. 16 g + 15 of input u
+          const uint32_t* b = buf[h] + (16 * g) * kSS1t + u * kUS1t + 2 * wv + half;
+          uint32_t x[16];
+#pragma unroll
+          for (int i = 0; i < 16; ++i) x[i] = b[i * kSS1t];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int jj = 4 * half + j;
+            uint32_t pd[4];
+#pragma unroll
+            for (int d = 0; d < 4; ++d) pd[d] = gather_byte(x[4 * d], x[4 * d + 1], x[4 * d + 2], x[4 * d + 3], j);
+            // operands of the byte's 4 channels first, w = [P0 P1 Q0 Q1 -P0 -P1]:
+            //   MFMA(A = w[0:4], B = w[0:4]) = P P^T + Q Q^T
+            //   MFMA(A = w[2:6], B = w[0:4]) = Q P^T - P Q^T
+            // (every operand a window of w), then the products channel-interleaved;
+            // channel c's re bit is at place 7 - 2c, its im bit at 6 - 2c
+            uint32_t hi[2], lo[2];
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) nibble_halves(pd[2 * hh], pd[2 * hh + 1], &hi[hh], &lo[hh]);
+            intx8 w[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+#pragma unroll
+              for (int hh = 0; hh < 2; ++hh) {
+                const uint32_t src = c < 2 ? hi[hh] : lo[hh];
+                w[c][hh] = int(fp4_signs(src, 2 * (c % 2)));           // x_re
+                w[c][2 + hh] = int(fp4_signs(src, 2 * (c % 2) + 1));   // x_im
+                w[c][4 + hh] = w[c][hh] ^ int(0x88888888u);            // -x_re
+              }
+              w[c][6] = w[c][7] = 0;
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) acc[jj][c] = mfma_fp4(w[c], w[c], acc[jj][c]);
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+              acc[jj][c] = mfma_fp4(__builtin_shufflevector(w[c], w[c], 2, 3, 4, 5, 6, 7, 6, 7), w[c], acc[jj][c]);
+          }
+        }
+      }
+      // buffer h ^ 1 was last read one step ago; then the registers take step st + 2
+      if (st + 1 < nsteps) stash(buf[h ^ 1]);
+      fetch(st + 2);
+      __syncthreads();
+    }
+  }
+  // outputs, 4 channels (one byte column) at a time: the wave's 4 tiles go
+  // through its 4 KB of the staging buffers; lanes take (pair, channel) with
+  // the channel fastest, so each pair's 4 channels are 64 contiguous bytes
+  int* S = reinterpret_cast<int*>(&buf[0][0]) + wv * 1024;
+#pragma unroll
+  for (int jj = 0; jj < 8; ++jj) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) S[c * 256 + (4 * g + v) * 16 + u] = int(acc[jj][c][v]);   // exact integers
+    __syncthreads();
+    const int64_t ch0 = 4 * (kW1t * gx + 8 * wv + jj);
+    for (int it = lane; it < NP * 4; it += 64) {
+      const int c = it % 4, p = it / 4;
+      int a = 0, rem = p;
+      while (rem >= NIN - a) { rem -= NIN - a; ++a; }
+      const int bb = a + rem;
+      const long long tab = S[c * 256 + a * 16 + bb], tba = S[c * 256 + bb * 16 + a];
+      if (ch0 + c < nchan) {
+        long long* o = out + ((blk * NP + p) * nchan + ch0 + c) * 2;
+        o[0] = (tab + tba) >> 1;   // even: T + T^T = 2 Re
+        o[1] = (tab - tba) >> 1;
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -592,7 +829,8 @@ int alb_xcorr_4bit_allpairs(const uint8_t* const* inputs, int ninput, int64_t nr
     // combined by atomics into the zero-filled output)
     int nsplit = 1;
     const int64_t ngx = (ncol / 4) / kDW4;
-    while (ngx * nblock * nsplit < 2048 && (blocklen / 64) % (2 * nsplit) == 0) nsplit *= 2;
+    const int64_t target = tune_get("xcorr4_workgroups") > 0 ? tune_get("xcorr4_workgroups") : 2048;
+    while (ngx * nblock * nsplit < target && (blocklen / 64) % (2 * nsplit) == 0) nsplit *= 2;
     const int64_t total = ngx * nblock * nsplit;
     ALB_REQUIRE(total < (int64_t(1) << 31), "alb_xcorr_4bit_allpairs: too many blocks");
     if (nsplit > 1)
@@ -641,11 +879,33 @@ int alb_xcorr_1bit_allpairs(const uint8_t* const* inputs, int ninput, int64_t nr
   ALB_REQUIRE(method != 2 || mfma_ok,
               "alb_xcorr_1bit_allpairs: the matrix-core form needs blocklen %% 64 == 0, ncol %% 16 == 0 and "
               "16-byte aligned inputs");
+  // matrix-core forms: one tile per channel (64-byte column groups, blocks of
+  // at most 2^16 spectra) or three products per channel (any other shape)
+  const int64_t form = tune_get("xcorr1_form");
+  const bool tile_ok = ncol % kW1t == 0 && blocklen <= kTile1Max;
+  ALB_REQUIRE(form != 1 || (mfma_ok && tile_ok),
+              "alb_xcorr_1bit_allpairs: the one-tile form needs ncol %% 64 == 0 and blocklen <= 2^22");
+  if (method != 1 && mfma_ok && tile_ok && form != 2) {
+    const int64_t total = (ncol / kW1t) * nblock;
+    ALB_REQUIRE(total < (int64_t(1) << 31), "alb_xcorr_1bit_allpairs: too many blocks");
+    switch (ninput) {
+#define CASE(n) case n: hipLaunchKernelGGL(allpairs1_tile_kernel<n>, dim3(unsigned(total)), dim3(512), 0, s, ip, ncol, nchan, blocklen, nblock, (long long*)out); break;
+      CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
+      CASE(14) CASE(15) CASE(16)
+#undef CASE
+    }
+    ALB_LAUNCH_CHECK("allpairs1_tile_kernel");
+    return ALB_OK;
+  }
   if (method != 1 && mfma_ok) {
     const int64_t total = ((ncol / 4) / kDW1) * nblock;
     ALB_REQUIRE(total < (int64_t(1) << 31), "alb_xcorr_1bit_allpairs: too many blocks");
+    const bool masked = blocklen <= kMasked1Max;
     switch (ninput) {
-#define CASE(n) case n: hipLaunchKernelGGL(allpairs1_mfma_kernel<n>, dim3(unsigned(total)), dim3(kT1), 0, s, ip, ncol, nchan, blocklen, nblock, (long long*)out); break;
+#define CASE(n) case n: \
+      if (masked) hipLaunchKernelGGL((allpairs1_mfma_kernel<n, true>), dim3(unsigned(total)), dim3(kT1), 0, s, ip, ncol, nchan, blocklen, nblock, (long long*)out); \
+      else hipLaunchKernelGGL((allpairs1_mfma_kernel<n, false>), dim3(unsigned(total)), dim3(kT1), 0, s, ip, ncol, nchan, blocklen, nblock, (long long*)out); \
+      break;
       CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) CASE(9) CASE(10) CASE(11) CASE(12) CASE(13)
       CASE(14) CASE(15) CASE(16)
 #undef CASE

@@ -151,6 +151,76 @@ def test_allpairs_1bit_full_size(cr):
     assert np.array_equal(a[:1, :, :64].cpu().numpy(), exp)
 
 
+def _with_tune(key, value, fn):
+    from albatros_axion_detection_amd import _lib
+    lib = _lib.load()
+    old = lib.alb_tune_get(key)
+    lib.alb_tune_set(key, value)
+    try:
+        return fn()
+    finally:
+        lib.alb_tune_set(key, old)
+
+
+def test_allpairs_1bit_matrix_core_forms(cr, rng):
+    # Two matrix-core forms: one 16 x 16 tile per channel (x = +-1 as e2m1
+    # nibbles on the FP4 matrix cores, T = Re + Im; 64-byte column groups) and
+    # three int8 products per channel on the bits (any other shape; the bits
+    # weighed by their place value up to 2^16 spectra, shifted to place 0
+    # above).  At 2^16 and 2^17 spectra per block both agree with the XOR form
+    # everywhere and with the oracle on the first 16 channels; constant bytes
+    # give closed forms (x = +-1 +- i: 0xFF -> 1 + i, 0x00 -> -1 - i, 0xAA -> 1 - i).
+    T = 2 ** 17
+    inp = rng.integers(0, 256, size=(3, T, 64), dtype=np.uint8)
+    for blocklen in (2 ** 16, 2 ** 17):
+        ref = cr.xcorr_1bit_allpairs(inp, 256, blocklen, method="valu")
+        exp = ocorr.allpairs_1bit_block_sums(inp[:, :, :4], 16, blocklen)
+        assert np.array_equal(ref[:, :, :16], exp)
+        for form in (0, 1, 2):
+            got = _with_tune(b"xcorr1_form", form, lambda: cr.xcorr_1bit_allpairs(inp, 256, blocklen, method="mfma"))
+            print("[gpu-case] 1-bit allpairs blocklen", blocklen, "form", form, flush=True)
+            assert np.array_equal(got, ref), (blocklen, form)
+    with pytest.raises(ValueError):   # the one-tile form takes 64-byte column groups only
+        _with_tune(b"xcorr1_form", 1, lambda: cr.xcorr_1bit_allpairs(inp[:, :, :48], 192, 2 ** 16, method="mfma"))
+    const = np.empty((3, T, 64), dtype=np.uint8)
+    const[0], const[1], const[2] = 0xFF, 0x00, 0xAA
+    x = np.array([1 + 1j, -1 - 1j, 1 - 1j])
+    for blocklen, form in ((2 ** 16, 1), (2 ** 16, 2), (2 ** 17, 1), (2 ** 17, 2)):
+        got = _with_tune(b"xcorr1_form", form, lambda: cr.xcorr_1bit_allpairs(const, 256, blocklen, method="mfma"))
+        for p, (i, j) in enumerate(ocorr.pair_list(3)):
+            v = x[i] * np.conj(x[j]) * blocklen
+            assert np.all(got[:, p, :, 0] == v.real) and np.all(got[:, p, :, 1] == v.imag), (blocklen, form, i, j)
+
+
+def test_allpairs_1bit_one_tile_inputs_and_ragged_channels(cr):
+    # the one-tile form for 1, 5 and 16 inputs, 250 of 256 channels, two
+    # 64-byte column groups, against the oracle
+    from albatros_axion_detection_amd import synth
+    for n in (1, 5, 16):
+        inp = synth.direct1(ninput=n, T=4096, nchan=512, seed=10 + n)
+        exp = ocorr.allpairs_1bit_block_sums(inp, 500, 2048)
+        got = _with_tune(b"xcorr1_form", 1, lambda: cr.xcorr_1bit_allpairs(inp, 500, 2048, method="mfma"))
+        assert got.shape == exp.shape and np.array_equal(got, exp), n
+
+
+def test_allpairs_4bit_every_input_count(cr, rng):
+    # 1..8 inputs (4 loaded slots up to 4 inputs, 8 above), with the
+    # workgroup target swept so the block splits change (1 .. 16 workgroups
+    # per block, down to one 64-spectrum step each); all exact against the oracle
+    from albatros_axion_detection_amd import _lib
+    lib = _lib.load()
+    for n in range(1, 9):
+        raw = rng.integers(0, 256, size=(n, 2048, 32), dtype=np.uint8)
+        exp = ocorr.allpairs_4bit_block_sums(raw, 1024)
+        for target in (0, 1, 8, 4096):
+            lib.alb_tune_set(b"xcorr4_workgroups", target)
+            try:
+                got = cr.xcorr_4bit_allpairs(raw, 1024, method="mfma")
+            finally:
+                lib.alb_tune_set(b"xcorr4_workgroups", 0)
+            assert np.array_equal(got, exp), (n, target)
+
+
 def test_allpairs_1bit_ragged_columns(cr):
     # ncol not a multiple of 4: the VALU form serves "auto", the matrix-core form refuses
     from albatros_axion_detection_amd import synth

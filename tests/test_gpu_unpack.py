@@ -77,7 +77,8 @@ def test_sortpols_all_tails(unpk, rng):
     p0, p1 = unpk.sortpols(a2, 8, 1, 9, 10, 4, 7)
     assert p0[0, 0] == 168 and p1[0, 0] == 168
     raw = rng.integers(0, 256, size=(200, 512), dtype=np.uint8)
-    for cs, ce in [(0, 1024), (2, 1023), (4, 1022), (6, 1021), (0, 1), (10, 13)]:
+    # (0, 1024), (8, 1000), (16, 528): four output bytes per lane; the rest byte-wise
+    for cs, ce in [(0, 1024), (8, 1000), (16, 528), (2, 1023), (4, 1022), (6, 1021), (0, 1), (10, 13), (4, 1020)]:
         g = unpk.sortpols(raw, 1024, 1, 7, 190, cs, ce)
         e = ounpack.sortpols(raw, 1024, 1, 7, 190, cs, ce)
         assert np.array_equal(g[0], e[0]) and np.array_equal(g[1], e[1]), (cs, ce)
