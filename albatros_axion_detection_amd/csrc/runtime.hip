@@ -98,6 +98,8 @@ int alb_tune_set(const char* key, int64_t value) {
     ALB_REQUIRE(value >= 0 && value <= 2, "alb_tune_set: xcorr_int_method must be 0, 1 or 2");
   } else if (k == "xcorr1_form") {
     ALB_REQUIRE(value >= 0 && value <= 2, "alb_tune_set: xcorr1_form must be 0, 1 or 2");
+  } else if (k == "xcorr4_segment") {
+    ALB_REQUIRE(value == 0 || value == 64 || value == 128, "alb_tune_set: xcorr4_segment must be 0, 64 or 128");
   } else if (k == "xcorr4_workgroups") {
     ALB_REQUIRE(value >= 0 && value <= (int64_t(1) << 20), "alb_tune_set: xcorr4_workgroups must be 0..2^20");
   } else {

@@ -403,6 +403,157 @@ __global__ __launch_bounds__(256) void allpairs4_mfma_kernel(InputPtrs in, int64
   }
 }
 
+// 4-bit, 4 SEGD bytes of every row (64 or 128 channels) per workgroup, so
+// each line is fetched once and used whole; a wave's load instruction covers
+// 4 SEGD contiguous bytes of each of its rows (128-byte segments reach about
+// 5.7 TB/s of reads, 64-byte segments about 3.4).  Up to 4 inputs (LS = 4 loaded slots)
+// a product carries two channels: row r = 8 h + 4 part + u (channel half h,
+// part 0 = real, 1 = imaginary, input u); the two diagonal 8 x 8 blocks of G
+// are the two channels' Grams (the off-diagonal blocks mix channels and are
+// not read).  With 5..8 inputs (LS = 8) a product carries one channel: row
+// r = 8 part + u.  Rows of absent inputs read input 0's bytes (or duplicate
+// slots) and never reach an output.  Per channel half:
+//   Re_ab = G[a][b] + G[P + a][P + b],  Im_ab = G[P + a][b] - G[a][P + b]
+// with P = 4 (two channels per product) or 8, indices within the half's block.
+//
+// Workgroup: SEGD / 4 waves; wave w takes dwords 4w .. 4w + 3 (16 channels).
+// Each 64-spectrum step the 16-byte row pieces (SEGD / 4 lanes per segment) go
+// through registers into LDS [spectrum][SEGD dwords][LS slots] (spectrum stride
+// SEGD LS + 1: the operand reads of 4 spectrum groups land in different banks;
+// the lanes of one word broadcast), double-buffered, with kPF4w steps of loads
+// in flight per thread.
+constexpr int kPF4w = 2;
+
+template <int NIN, int SEGD>
+__global__ __launch_bounds__(16 * SEGD) void allpairs4_wide_kernel(InputPtrs in, int64_t ncol, int64_t blocklen,
+                                                                   int64_t nblock, int nsplit,
+                                                                   long long* __restrict__ out) {
+  constexpr int NP = NIN * (NIN + 1) / 2;
+  constexpr int NWAVE = SEGD / 4;            // waves (4 dwords each)
+  constexpr int LPR = SEGD / 4;              // loader lanes per row segment
+  constexpr int RPW = 64 / LPR;              // rows per wave load instruction
+  constexpr int LS = NIN <= 4 ? 4 : 8;       // loaded input slots
+  constexpr int PR = LS;                     // rows per part within a channel block
+  constexpr int SS = SEGD * LS + 1;          // LDS dwords per spectrum
+  constexpr int E = LS;                      // 16-byte pieces per thread per step
+  constexpr int CH = LS == 4 ? 2 : 1;        // channels per product
+  __shared__ uint32_t buf[2][64 * SS];
+  static_assert(NWAVE * 4 * 256 <= 2 * 64 * SS, "output staging must fit the step buffers");
+  static_assert(NWAVE * RPW == 64, "the waves' rows cover a step");
+  const int tid = threadIdx.x;
+  const int wv = __builtin_amdgcn_readfirstlane(tid / 64), lane = tid % 64;
+  int64_t bs, gx;
+  mfma_item(ncol / (4 * SEGD), nblock * nsplit, &bs, &gx);
+  const int64_t blk = bs / nsplit, sp = bs - blk * nsplit;
+  const int64_t nsteps = blocklen / 64 / nsplit;
+  const int64_t t_begin = blk * blocklen + sp * nsteps * 64;
+  // loader: lane = (16-byte piece q, row r of RPW); piece e: input slot e; wave wv takes
+  // spectra RPW wv .. RPW wv + RPW - 1 of each step
+  const int q = lane % LPR, r = lane / LPR;
+  uint4 ld[kPF4w][E];
+  auto fetch = [&](int slot, int64_t st) {
+    const int64_t row = t_begin + (st < nsteps ? st : nsteps - 1) * 64 + RPW * wv + r;
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+      ld[slot][e] = *reinterpret_cast<const uint4*>(in.p[e < NIN ? e : 0] + row * ncol + 4 * SEGD * gx + 16 * q);
+  };
+  auto stash = [&](uint32_t* b, int slot) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      uint32_t* d = b + (RPW * wv + r) * SS + (4 * q) * LS + e;
+      d[0 * LS] = ld[slot][e].x;
+      d[1 * LS] = ld[slot][e].y;
+      d[2 * LS] = ld[slot][e].z;
+      d[3 * LS] = ld[slot][e].w;
+    }
+  };
+#pragma unroll
+  for (int x = 0; x < kPF4w; ++x) fetch(x, x);
+  stash(buf[0], 0);
+  fetch(0, kPF4w);
+  __syncthreads();
+  const int rr = lane % 16, g = lane / 16;
+  // operand row: LS = 4: h = rr / 8, part = (rr / 4) % 2, u = rr % 4; LS = 8: h = 0, part = rr / 8, u = rr % 8
+  const int h = LS == 4 ? rr / 8 : 0;
+  const int part = LS == 4 ? (rr / 4) % 2 : rr / 8;
+  const int u = LS == 4 ? rr % 4 : rr % 8;
+  constexpr int NPROD = 16 / CH;             // products per wave per step (16 channels)
+  intx4 acc[NPROD];
+#pragma unroll
+  for (int k = 0; k < NPROD; ++k) acc[k] = intx4{0, 0, 0, 0};
+  for (int64_t s0 = 0; s0 < nsteps; s0 += 2) {   // s0 even and a multiple of kPF4w: static buffer and slot
+#pragma unroll
+    for (int hb = 0; hb < 2; ++hb) {
+      const int64_t st = s0 + hb;
+      if (st < nsteps) {   // uniform
+#pragma unroll
+        for (int dk = 0; dk < 4 / CH; ++dk) {
+          // the lane's dword: 4 wv + dk (+ 2 for channel half 1 when two channels share a product)
+          const int dw = 4 * wv + dk + (CH == 2 ? 2 * h : 0);
+          if (dk > 0) asm volatile("" ::: "memory");
+          const uint32_t* b = buf[hb] + (16 * g) * SS + dw * LS + u;
+          uint32_t x[16];
+#pragma unroll
+          for (int i = 0; i < 16; ++i) x[i] = b[i * SS];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            intx4 v;
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+              const uint32_t pd = gather_byte(x[4 * d], x[4 * d + 1], x[4 * d + 2], x[4 * d + 3], j);
+              const uint32_t nib = part ? (pd & 0x0F0F0F0Fu) : ((pd >> 4) & 0x0F0F0F0Fu);
+              // v > 8 -> v - 16 per byte: set the high nibble of those bytes
+              uint32_t sgn = ((nib + 0x07070707u) >> 4) & 0x01010101u;
+              sgn <<= 4;
+              sgn |= sgn << 1;
+              sgn |= sgn << 2;
+              v[d] = int(nib | sgn);
+            }
+            acc[4 * dk + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(v, v, acc[4 * dk + j], 0, 0, 0);
+          }
+        }
+      }
+      // buffer hb ^ 1 was last read one step ago; its ring slot then takes step st + 1 + kPF4w
+      if (st + 1 < nsteps) stash(buf[hb ^ 1], int((st + 1) % kPF4w));
+      fetch(int((st + 1) % kPF4w), st + 1 + kPF4w);
+      __syncthreads();
+    }
+  }
+  // D layout: lane l holds G[4 (l / 16) + v][l % 16]; staging [wave][4 products][256]
+  int* S = reinterpret_cast<int*>(&buf[0][0]) + wv * 1024;
+#pragma unroll
+  for (int k0 = 0; k0 < NPROD; k0 += 4) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+      for (int vv = 0; vv < 4; ++vv) S[k * 256 + (4 * g + vv) * 16 + rr] = acc[k0 + k][vv];
+    __syncthreads();
+    // 4 products x CH channels, each NP pairs
+    for (int it = lane; it < 4 * CH * NP; it += 64) {
+      const int p = it % NP, kc = it / NP, k = kc / CH, hc = kc % CH;
+      int a = 0, rem = p;
+      while (rem >= NIN - a) { rem -= NIN - a; ++a; }
+      const int bb = a + rem;
+      const int* G = S + k * 256;
+      const int o0 = 8 * hc;   // channel block
+      const long long re = (long long)G[(o0 + a) * 16 + o0 + bb] + G[(o0 + PR + a) * 16 + o0 + PR + bb];
+      const long long im = (long long)G[(o0 + PR + a) * 16 + o0 + bb] - G[(o0 + a) * 16 + o0 + PR + bb];
+      // product k0 + k: dword 4 wv + dk (+ 2 hc), byte j
+      const int kk = k0 + k, dk = kk / 4, j = kk % 4;
+      const int64_t ch = 4 * SEGD * gx + 4 * (4 * wv + dk + (CH == 2 ? 2 * hc : 0)) + j;
+      long long* o = out + ((blk * NP + p) * ncol + ch) * 2;
+      if (nsplit > 1) {
+        if (re) atomic_add_i64(o, re);
+        if (im) atomic_add_i64(o + 1, im);
+      } else {
+        o[0] = re;
+        o[1] = im;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // 1-bit: bit b -> x = 2b - 1.  The products run on the bits themselves
 // (0/1 int8): with G' = B B^T,  sum x_a x_b = 4 G'_ab - 2 G'_aa - 2 G'_bb + K
 // (b^2 = b, so the diagonal holds the bit counts).  Row u = input u; three
@@ -625,52 +776,62 @@ __global__ __launch_bounds__(512) void allpairs1_tile_kernel(InputPtrs in, int64
                                                              int64_t blocklen, int64_t nblock,
                                                              long long* __restrict__ out) {
   constexpr int NP = NIN * (NIN + 1) / 2;
+  constexpr int NW = 8;                // waves
+  constexpr int BW = kW1t / NW;        // byte columns per wave
+  constexpr int IW = 16 / NW;          // inputs loaded per wave
+  constexpr int E = 4 * IW;            // 16-byte pieces per thread per step
+  constexpr int R = 1;                 // steps of pieces in registers (a deeper ring gains nothing:
+                                       // the loads are bound by the requests in flight per CU)
   __shared__ uint32_t buf[2][64 * kSS1t];
-  static_assert(8 * 4 * 256 <= 2 * 64 * kSS1t, "output staging must fit the step buffers");
+  static_assert(NW * 4 * 256 <= 2 * 64 * kSS1t, "output staging must fit the step buffers");
   const int tid = threadIdx.x;
-  const int wv = tid / 64, lane = tid % 64;
+  const int wv = __builtin_amdgcn_readfirstlane(tid / 64), lane = tid % 64;   // wave index: uniform
   int64_t blk, gx;
   mfma_item(ncol / kW1t, nblock, &blk, &gx);
-  // loader: wave wv loads inputs 2 wv and 2 wv + 1; lane = (16-byte piece q, spectrum r of 16)
+  // loader: wave wv loads inputs IW wv .. IW wv + IW - 1; lane = (16-byte piece q, spectrum r of 16)
   const int q = lane % 4, r = lane / 4;
-  const uint8_t* src[2];
+  const uint8_t* src[IW];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) src[h] = in.p[2 * wv + h < NIN ? 2 * wv + h : 0] + kW1t * gx + 16 * q;
+  for (int h = 0; h < IW; ++h) src[h] = in.p[IW * wv + h < NIN ? IW * wv + h : 0] + kW1t * gx + 16 * q;
   const int64_t t_begin = blk * blocklen, nsteps = blocklen / 64;
-  uint4 ld[8];
-  auto fetch = [&](int64_t st) {   // piece e: input 2 wv + (e & 1), spectrum r + 16 (e >> 1)
+  // ring slot x % R holds step x from its loads to its stash, R steps later
+  uint4 ld[R][E];
+  auto fetch = [&](int slot, int64_t st) {   // piece e: input IW wv + e % IW, spectrum r + 16 (e / IW)
     const int64_t row = t_begin + (st < nsteps ? st : nsteps - 1) * 64 + r;
 #pragma unroll
-    for (int e = 0; e < 8; ++e)
-      ld[e] = *reinterpret_cast<const uint4*>(src[e & 1] + (row + 16 * (e >> 1)) * ncol);
+    for (int e = 0; e < E; ++e)
+      ld[slot][e] = *reinterpret_cast<const uint4*>(src[e % IW] + (row + 16 * (e / IW)) * ncol);
   };
-  auto stash = [&](uint32_t* b) {
+  auto stash = [&](uint32_t* b, int slot) {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      uint32_t* d = b + (r + 16 * (e >> 1)) * kSS1t + (2 * wv + (e & 1)) * kUS1t + 4 * q;
-      *reinterpret_cast<uint2*>(d) = make_uint2(ld[e].x, ld[e].y);
-      *reinterpret_cast<uint2*>(d + 2) = make_uint2(ld[e].z, ld[e].w);
+    for (int e = 0; e < E; ++e) {
+      uint32_t* d = b + (r + 16 * (e / IW)) * kSS1t + (IW * wv + e % IW) * kUS1t + 4 * q;
+      *reinterpret_cast<uint2*>(d) = make_uint2(ld[slot][e].x, ld[slot][e].y);
+      *reinterpret_cast<uint2*>(d + 2) = make_uint2(ld[slot][e].z, ld[slot][e].w);
     }
   };
-  fetch(0);
-  stash(buf[0]);
-  fetch(1);
+#pragma unroll
+  for (int x = 0; x < R; ++x) fetch(x, x);
+  stash(buf[0], 0);
+  fetch(0, R);
   __syncthreads();
   const int u = lane % 16, g = lane / 16;
-  floatx4 acc[8][4];
+  floatx4 acc[BW][4];
 #pragma unroll
-  for (int jj = 0; jj < 8; ++jj)
+  for (int jj = 0; jj < BW; ++jj)
 #pragma unroll
     for (int c = 0; c < 4; ++c) acc[jj][c] = floatx4{0, 0, 0, 0};
-  for (int64_t s0 = 0; s0 < nsteps; s0 += 2) {
+  for (int64_t s0 = 0; s0 < nsteps; s0 += 2) {   // s0 even and a multiple of R: static buffer and slot
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int64_t st = s0 + h;
       if (st < nsteps) {   // uniform
 #pragma unroll
-        for (int half = 0; half < 2; ++half) {
-          // dword 2 wv + half of the 16 spectra 16 g .. 16 g + 15 of input u
-          const uint32_t* b = buf[h] + (16 * g) * kSS1t + u * kUS1t + 2 * wv + half;
+        for (int half = 0; half < BW / 4; ++half) {
+          // dword BW / 4 wv + half of the 16 spectra 16 g .. 16 g + 15 of input u;
+          // read after the previous half's products (16 live registers, not 32)
+          if (half > 0) asm volatile("" ::: "memory");
+          const uint32_t* b = buf[h] + (16 * g) * kSS1t + u * kUS1t + (BW / 4) * wv + half;
           uint32_t x[16];
 #pragma unroll
           for (int i = 0; i < 16; ++i) x[i] = b[i * kSS1t];
@@ -680,37 +841,41 @@ __global__ __launch_bounds__(512) void allpairs1_tile_kernel(InputPtrs in, int64
             uint32_t pd[4];
 #pragma unroll
             for (int d = 0; d < 4; ++d) pd[d] = gather_byte(x[4 * d], x[4 * d + 1], x[4 * d + 2], x[4 * d + 3], j);
-            // operands of the byte's 4 channels first, w = [P0 P1 Q0 Q1 -P0 -P1]:
+            // operands of two channels first, w = [P0 P1 Q0 Q1 -P0 -P1]:
             //   MFMA(A = w[0:4], B = w[0:4]) = P P^T + Q Q^T
             //   MFMA(A = w[2:6], B = w[0:4]) = Q P^T - P Q^T
-            // (every operand a window of w), then the products channel-interleaved;
-            // channel c's re bit is at place 7 - 2c, its im bit at 6 - 2c
+            // (every operand a window of w), then the products of the two
+            // interleaved; channel c's re bit is at place 7 - 2c, its im bit at 6 - 2c
             uint32_t hi[2], lo[2];
 #pragma unroll
             for (int hh = 0; hh < 2; ++hh) nibble_halves(pd[2 * hh], pd[2 * hh + 1], &hi[hh], &lo[hh]);
-            intx8 w[4];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
+            for (int cp = 0; cp < 2; ++cp) {   // channels 2 cp, 2 cp + 1: from hi (cp = 0) or lo
+              intx8 w[2];
 #pragma unroll
-              for (int hh = 0; hh < 2; ++hh) {
-                const uint32_t src = c < 2 ? hi[hh] : lo[hh];
-                w[c][hh] = int(fp4_signs(src, 2 * (c % 2)));           // x_re
-                w[c][2 + hh] = int(fp4_signs(src, 2 * (c % 2) + 1));   // x_im
-                w[c][4 + hh] = w[c][hh] ^ int(0x88888888u);            // -x_re
+              for (int k = 0; k < 2; ++k) {
+#pragma unroll
+                for (int hh = 0; hh < 2; ++hh) {
+                  const uint32_t src = cp == 0 ? hi[hh] : lo[hh];
+                  w[k][hh] = int(fp4_signs(src, 2 * k));             // x_re
+                  w[k][2 + hh] = int(fp4_signs(src, 2 * k + 1));     // x_im
+                  w[k][4 + hh] = w[k][hh] ^ int(0x88888888u);        // -x_re
+                }
+                w[k][6] = w[k][7] = 0;
               }
-              w[c][6] = w[c][7] = 0;
+#pragma unroll
+              for (int k = 0; k < 2; ++k) acc[jj][2 * cp + k] = mfma_fp4(w[k], w[k], acc[jj][2 * cp + k]);
+#pragma unroll
+              for (int k = 0; k < 2; ++k)
+                acc[jj][2 * cp + k] = mfma_fp4(__builtin_shufflevector(w[k], w[k], 2, 3, 4, 5, 6, 7, 6, 7), w[k],
+                                               acc[jj][2 * cp + k]);
             }
-#pragma unroll
-            for (int c = 0; c < 4; ++c) acc[jj][c] = mfma_fp4(w[c], w[c], acc[jj][c]);
-#pragma unroll
-            for (int c = 0; c < 4; ++c)
-              acc[jj][c] = mfma_fp4(__builtin_shufflevector(w[c], w[c], 2, 3, 4, 5, 6, 7, 6, 7), w[c], acc[jj][c]);
           }
         }
       }
-      // buffer h ^ 1 was last read one step ago; then the registers take step st + 2
-      if (st + 1 < nsteps) stash(buf[h ^ 1]);
-      fetch(st + 2);
+      // buffer h ^ 1 was last read one step ago; its ring slot then takes step st + 1 + R
+      if (st + 1 < nsteps) stash(buf[h ^ 1], int((st + 1) % R));
+      fetch(int((st + 1) % R), st + 1 + R);
       __syncthreads();
     }
   }
@@ -719,13 +884,13 @@ __global__ __launch_bounds__(512) void allpairs1_tile_kernel(InputPtrs in, int64
   // the channel fastest, so each pair's 4 channels are 64 contiguous bytes
   int* S = reinterpret_cast<int*>(&buf[0][0]) + wv * 1024;
 #pragma unroll
-  for (int jj = 0; jj < 8; ++jj) {
+  for (int jj = 0; jj < BW; ++jj) {
 #pragma unroll
     for (int c = 0; c < 4; ++c)
 #pragma unroll
       for (int v = 0; v < 4; ++v) S[c * 256 + (4 * g + v) * 16 + u] = int(acc[jj][c][v]);   // exact integers
     __syncthreads();
-    const int64_t ch0 = 4 * (kW1t * gx + 8 * wv + jj);
+    const int64_t ch0 = 4 * (kW1t * gx + BW * wv + jj);
     for (int it = lane; it < NP * 4; it += 64) {
       const int c = it % 4, p = it / 4;
       int a = 0, rem = p;
@@ -824,6 +989,29 @@ int alb_xcorr_4bit_allpairs(const uint8_t* const* inputs, int ninput, int64_t nr
   ALB_REQUIRE(method != 2 || mfma_ok,
               "alb_xcorr_4bit_allpairs: the matrix-core form needs blocklen %% 64 == 0, ncol %% 16 == 0 and "
               "16-byte aligned inputs");
+  if (method != 1 && mfma_ok && ncol % 64 == 0) {
+    // 128- or 64-byte column groups; enough workgroups to fill the chip: split
+    // blocks (exact partial sums, combined by atomics into the zero-filled output)
+    int nsplit = 1;
+    const int segd = ncol % 128 == 0 && tune_get("xcorr4_segment") == 128 ? 32 : 16;
+    const int64_t ngx = ncol / (4 * segd);
+    const int64_t target = tune_get("xcorr4_workgroups") > 0 ? tune_get("xcorr4_workgroups") : 256;
+    while (ngx * nblock * nsplit < target && (blocklen / 64) % (2 * nsplit) == 0) nsplit *= 2;
+    const int64_t total = ngx * nblock * nsplit;
+    ALB_REQUIRE(total < (int64_t(1) << 31), "alb_xcorr_4bit_allpairs: too many blocks");
+    if (nsplit > 1)
+      hipLaunchKernelGGL(fill_i64, grid1(nblock * npair * ncol * 2), dim3(256), 0, s, out, nblock * npair * ncol * 2);
+    switch (ninput) {
+#define CASE(n) case n: \
+      if (segd == 32) hipLaunchKernelGGL((allpairs4_wide_kernel<n, 32>), dim3(unsigned(total)), dim3(512), 0, s, ip, ncol, blocklen, nblock, nsplit, (long long*)out); \
+      else hipLaunchKernelGGL((allpairs4_wide_kernel<n, 16>), dim3(unsigned(total)), dim3(256), 0, s, ip, ncol, blocklen, nblock, nsplit, (long long*)out); \
+      break;
+      CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+#undef CASE
+    }
+    ALB_LAUNCH_CHECK("allpairs4_wide_kernel");
+    return ALB_OK;
+  }
   if (method != 1 && mfma_ok) {
     // enough workgroups to fill the chip: split blocks (exact partial sums,
     // combined by atomics into the zero-filled output)

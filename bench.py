@@ -367,11 +367,12 @@ def secondary(args, torch):
         o4 = torch.empty((32, 10, 128, 2), dtype=torch.int64, device=dev)
         lib.alb_tune_set(b"xcorr_int_method", m)
         ms = timeit(lambda: _lib.call("alb_xcorr_4bit_allpairs", ptrs, 4, 2 ** 17, 128, 4096, _lib.ptr(o4),
-                                      _lib.stream()))
+                                      _lib.stream()), n=50)
         res4[name] = o4
         forms[name] = {"ms": round(ms, 5), "Msamples_s": round(4 * 2 ** 17 * 128 / (ms * 1e-3) / 1e6, 1),
                        "hbm_frac": round(b4 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-    forms["mfma_i8"]["mfma_frac"] = round(32 * 128 * 64 * 32768 / (forms["mfma_i8"]["ms"] * 1e-3) / 5e15, 5)
+    # two channels per 16 x 16 x 64 int8 product (4 inputs): 32 blocks x 64 channel pairs x 64 steps
+    forms["mfma_i8"]["mfma_frac"] = round(32 * 64 * 64 * 32768 / (forms["mfma_i8"]["ms"] * 1e-3) / 5e15, 5)
     lib.alb_tune_set(b"xcorr_int_method", 0)
     out["direct4"] = {"workload": "4 inputs x 2^17 spectra x 128 ch, 10 pairs, 4096-blocks, int64 sums",
                       "forms": forms, "forms_bit_identical": bool(torch.equal(res4["mfma_i8"], res4["valu"]))}
@@ -386,18 +387,26 @@ def secondary(args, torch):
     nb1 = T1 // 8192
     b1 = 16 * T1 * 256 + nb1 * 136 * 1024 * 2 * 8
     forms, res1 = {}, {}
-    for name, m, n in (("mfma_i8", 2, 5), ("xor_valu", 1, 2)):
+    # the matrix-core forms: one FP4 tile per channel (the default here) and
+    # three int8 products per channel; then the XOR form
+    for name, m, form, n in (("mfma_fp4", 2, 1, 10), ("mfma_i8", 2, 2, 5), ("xor_valu", 1, 0, 2)):
         o1 = torch.empty((nb1, 136, 1024, 2), dtype=torch.int64, device=dev)
         lib.alb_tune_set(b"xcorr_int_method", m)
+        lib.alb_tune_set(b"xcorr1_form", form)
         ms = timeit(lambda: _lib.call("alb_xcorr_1bit_allpairs", ptrs, 16, T1, 256, 1024, 8192, _lib.ptr(o1),
                                       _lib.stream()), n=n)
         res1[name] = o1
         forms[name] = {"ms": round(ms, 4), "Msamples_s": round(16 * T1 * 1024 / (ms * 1e-3) / 1e6, 1),
                        "hbm_frac": round(b1 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    lib.alb_tune_set(b"xcorr1_form", 0)
+    # FP4: two 16 x 16 x 128 products per channel per 64 spectra, against the dense FP4 peak
+    forms["mfma_fp4"]["mfma_frac"] = round(2 * 1024 * (T1 // 64) * 65536 / (forms["mfma_fp4"]["ms"] * 1e-3) / 10.1e15, 4)
     forms["mfma_i8"]["mfma_frac"] = round(3 * 1024 * (T1 // 64) * 32768 / (forms["mfma_i8"]["ms"] * 1e-3) / 5e15, 4)
     lib.alb_tune_set(b"xcorr_int_method", 0)
     out["direct1"] = {"workload": "16 inputs x %d spectra x 1024 ch, 136 pairs, 8192-blocks, int64 sums" % T1,
-                      "forms": forms, "forms_bit_identical": bool(torch.equal(res1["mfma_i8"], res1["xor_valu"]))}
+                      "forms": forms,
+                      "forms_bit_identical": bool(torch.equal(res1["mfma_fp4"], res1["xor_valu"]) and
+                                                  torch.equal(res1["mfma_i8"], res1["xor_valu"]))}
     if not args.no_cpu:
         log("[bench] CPU reference: direct 1-bit ...")
         out["direct1"]["cpu_baseline"] = cpu_reference_direct(1, inp[:, :8192].cpu().numpy(), 1024, 8192, 1)
@@ -826,6 +835,9 @@ def main():
                     help="time the separate column pass + re-PFB stage 1 instead of the fused kernel")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain device copies of 1 GiB (HBM counter calibration) and exit")
+    ap.add_argument("--secondary-only", action="store_true",
+                    help="only run the secondary workloads (unpack, direct correlators, precision correlator) "
+                         "and print their JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X (asked for %d)" % args.gpus,
@@ -835,6 +847,9 @@ def main():
     sys.path.insert(0, HERE)
     if args.copy_calibration:
         copy_calibration(torch)
+        return
+    if args.secondary_only:
+        print(json.dumps(secondary(args, torch)), flush=True)
         return
     res, ctx = batched(args, torch)
     cpu = None

@@ -84,10 +84,11 @@ def test_allpairs_4bit(cr, rng, method):
     got = cr.xcorr_4bit_allpairs(inp, 4096, method=method)
     exp = ocorr.allpairs_4bit_block_sums(inp, 4096)
     assert got.shape == (2, 10, 128, 2) and np.array_equal(got, exp)
-    # edge: nibble 8 (+8) everywhere, the largest products
-    eights = np.full((3, 4096, 16), 0x88, dtype=np.uint8)
-    got = cr.xcorr_4bit_allpairs(eights, 1024, method=method)
-    assert np.all(got[..., 0] == 128 * 1024) and np.all(got[..., 1] == 0)
+    # edge: nibble 8 (+8) everywhere, the largest products (16- and 64-byte column groups)
+    for nc in (16, 64):
+        eights = np.full((3, 4096, nc), 0x88, dtype=np.uint8)
+        got = cr.xcorr_4bit_allpairs(eights, 1024, method=method)
+        assert np.all(got[..., 0] == 128 * 1024) and np.all(got[..., 1] == 0)
     # every byte value, 8 inputs (a full 16-row product), 32 channels
     raw = rng.integers(0, 256, size=(8, 640, 32), dtype=np.uint8)
     got = cr.xcorr_4bit_allpairs(raw, 320, method=method)
@@ -204,21 +205,24 @@ def test_allpairs_1bit_one_tile_inputs_and_ragged_channels(cr):
 
 
 def test_allpairs_4bit_every_input_count(cr, rng):
-    # 1..8 inputs (4 loaded slots up to 4 inputs, 8 above), with the
-    # workgroup target swept so the block splits change (1 .. 16 workgroups
-    # per block, down to one 64-spectrum step each); all exact against the oracle
+    # 1..8 inputs (4 loaded slots and two channels per product up to 4
+    # inputs, 8 slots above), 128 channels (64-byte column groups) and 32
+    # (16-byte groups), with the workgroup target swept so the block splits
+    # change (1 .. 16 workgroups per block, down to one 64-spectrum step
+    # each); all exact against the oracle
     from albatros_axion_detection_amd import _lib
     lib = _lib.load()
-    for n in range(1, 9):
-        raw = rng.integers(0, 256, size=(n, 2048, 32), dtype=np.uint8)
-        exp = ocorr.allpairs_4bit_block_sums(raw, 1024)
-        for target in (0, 1, 8, 4096):
-            lib.alb_tune_set(b"xcorr4_workgroups", target)
-            try:
-                got = cr.xcorr_4bit_allpairs(raw, 1024, method="mfma")
-            finally:
-                lib.alb_tune_set(b"xcorr4_workgroups", 0)
-            assert np.array_equal(got, exp), (n, target)
+    for ncol in (128, 32):
+        for n in range(1, 9):
+            raw = rng.integers(0, 256, size=(n, 2048, ncol), dtype=np.uint8)
+            exp = ocorr.allpairs_4bit_block_sums(raw, 1024)
+            for target in (0, 1, 8, 4096):
+                lib.alb_tune_set(b"xcorr4_workgroups", target)
+                try:
+                    got = cr.xcorr_4bit_allpairs(raw, 1024, method="mfma")
+                finally:
+                    lib.alb_tune_set(b"xcorr4_workgroups", 0)
+                assert np.array_equal(got, exp), (ncol, n, target)
 
 
 def test_allpairs_1bit_ragged_columns(cr):
