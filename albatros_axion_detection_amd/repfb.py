@@ -108,12 +108,17 @@ class RePFBPipeline:
     ``chan_numbers`` = coarse channel number of each packed column
     (helper_gpu_stream.py:196).  ``tile_batch`` iPFB tiles go through one row
     and column launch; ``lanes`` inputs run their stage 1 concurrently in
-    :meth:`correlate_into` (one HIP stream and stream buffer each).
+    :meth:`correlate_into` (one HIP stream and stream buffer each).  With
+    ``stage2_overlap`` each input's re-PFB stage 2 runs on a stream of its
+    own as soon as that input's stage 1 is done, beside the next inputs'
+    stage 1 (off by default: on MI355X the overlapped kernels evict each
+    other's working sets, 67.4 against 63.4 ms on the batched workload;
+    :meth:`autotune` tries both).
     """
 
     def __init__(self, ninput, chan_numbers, acclen, cut, osamp, nblock, repfb_chanstart, repfb_chanend,
                  filt_thresh=0.45, ntap=4, filt=None, win=None, tile_batch=4, nchan=2049, row_pass="auto",
-                 lanes=2):
+                 lanes=2, stage2_overlap=False):
         t = _lib.torch()
         self.ninput = ninput
         self.acclen = int(acclen)
@@ -130,6 +135,7 @@ class RePFBPipeline:
         # inputs whose stage 1 runs concurrently in correlate_into (one HIP
         # stream, tile buffer and stream buffer each)
         self.lanes = max(1, int(lanes))
+        self.stage2_overlap = bool(stage2_overlap)
         if nchan != 2049:
             raise NotImplementedError("the fused iPFB handles 2049-channel spectra (lblock 4096)")
         chan_numbers = np.asarray(chan_numbers)
@@ -157,6 +163,8 @@ class RePFBPipeline:
         self._ws = None
         self._xin_f = None
         self._lane_streams = None
+        self._s2_stream = None   # stage-2 stream and per-input events (stage2_overlap)
+        self._s2_events = []
         # row pass: "fft" (2048-point FFT per spectrum) or "mfma" (matrix-core
         # synthesis from the ncols non-zero channels); with a power-of-two
         # nslice both write the blocked tile layout, which the column pass reads
@@ -450,6 +458,11 @@ class RePFBPipeline:
                 self._tile_buffer(min(self.tile_batch, nchunks), lane)
             if nl > 1 and (self._lane_streams is None or len(self._lane_streams) < nl):
                 self._lane_streams = [t.cuda.Stream() for _ in range(nl)]
+            # made here, before any capture: stream and event creation are not captured work
+            if self._s2_stream is None:
+                self._s2_stream = t.cuda.Stream()
+            while len(self._s2_events) < M:
+                self._s2_events.append(t.cuda.Event())
         else:
             self._workspace(self.block_batch(K))
         return bufs
@@ -518,24 +531,45 @@ class RePFBPipeline:
                 ws, per = self._workspace(nbg, M)
                 lane_bufs = bufs.get("lane_stream_bufs", [stream])[:self.lanes]
                 nl = len(lane_bufs)
-                if nl == 1:
+                s2 = self.stage2_overlap and M > 1 and self._s2_stream is not None and len(self._s2_events) >= M
+                if nl == 1 and not s2:
                     for m in range(M):
                         self.stream_stage1(packed_list[m], rowmaps[m], nchunks, stream, nbg, ws[m * per:],
                                            tiles=(c_lo, c_hi), row_lo=row_lo)
                 else:
                     t = _lib.torch()
                     main = t.cuda.current_stream()
-                    for st in self._lane_streams[:nl]:
-                        st.wait_stream(main)
+                    lanes = self._lane_streams[:nl] if nl > 1 else [main]
+                    if nl > 1:
+                        for st in lanes:
+                            st.wait_stream(main)
+                    if s2:
+                        self._s2_stream.wait_stream(main)
                     for m in range(M):
                         lane = m % nl
-                        with t.cuda.stream(self._lane_streams[lane]):
+                        with t.cuda.stream(lanes[lane]):
                             self.stream_stage1(packed_list[m], rowmaps[m], nchunks, lane_bufs[lane], nbg,
                                                ws[m * per:], lane=lane, tiles=(c_lo, c_hi), row_lo=row_lo)
-                    for st in self._lane_streams[:nl]:
-                        main.wait_stream(st)
-                self._launch("repfb_stage2", "alb_repfb_stage2_inputs", _lib.ptr(ws), per * 8, M, nbg, self.osamp,
-                             self.c_lo, self.c_hi, _lib.ptr(xin), 1, F * M, M, K, K * F * M, _lib.stream())
+                        if s2:
+                            # input m's stage 2 (its own workspace, its own channels of
+                            # xin) once its stage 1 is done, beside the next inputs' stage 1
+                            ev = self._s2_events[m]
+                            ev.record(lanes[lane])
+                            self._s2_stream.wait_event(ev)
+                            with t.cuda.stream(self._s2_stream):
+                                self._launch("repfb_stage2", "alb_repfb_stage2_inputs",
+                                             _lib._vp(ws.data_ptr() + m * per * 8), per * 8, 1, nbg, self.osamp,
+                                             self.c_lo, self.c_hi, _lib._vp(xin.data_ptr() + m * 8), 1, F * M, M, K,
+                                             K * F * M, _lib.stream())
+                    if nl > 1:
+                        for st in lanes:
+                            main.wait_stream(st)
+                    if s2:
+                        main.wait_stream(self._s2_stream)
+                if not s2:
+                    self._launch("repfb_stage2", "alb_repfb_stage2_inputs", _lib.ptr(ws), per * 8, M, nbg,
+                                 self.osamp, self.c_lo, self.c_hi, _lib.ptr(xin), 1, F * M, M, K, K * F * M,
+                                 _lib.stream())
             else:
                 # the window starts at tile c_lo; the group's blocks at row_lo
                 for m in range(M):
@@ -558,7 +592,8 @@ class RePFBPipeline:
         Coordinate descent, one setting at a time with the others at their
         best so far: iPFB tiles per row / column launch (``tile_batch``, up
         to ``nchunks``), concurrent stage-1 lanes (up to the lane buffers
-        ``bufs`` holds), then the library's launch-shape knobs in
+        ``bufs`` holds), stage 2 beside stage 1 or after it
+        (``stage2_overlap``), then the library's launch-shape knobs in
         ``TUNE_KNOBS`` (``alb_tune_set``; process-wide, so they also apply
         to other pipelines in this process).  Each candidate runs one untimed
         pass and ``reps`` timed ones (HIP events on the current stream, which
@@ -594,12 +629,16 @@ class RePFBPipeline:
         def set_lanes(v):
             self.lanes = v
 
+        def set_overlap(v):
+            self.stage2_overlap = bool(v)
+
         def knob_setter(key):
             return lambda v: _lib.call("alb_tune_set", key.encode(), int(v))
 
         space = [("tile_batch", sorted({min(int(v), nchunks) for v in tile_batches} | {min(self.tile_batch, nchunks)}),
                   self.tile_batch, set_tile_batch),
-                 ("lanes", sorted({1, nl_max}), min(self.lanes, nl_max), set_lanes)]
+                 ("lanes", sorted({1, nl_max}), min(self.lanes, nl_max), set_lanes),
+                 ("stage2_overlap", [0, 1], int(self.stage2_overlap), set_overlap)]
         space += [(key, vals, int(lib.alb_tune_get(key.encode())), knob_setter(key)) for key, vals in self.TUNE_KNOBS]
         best = {}
         for name, vals, cur, setter in space:

@@ -481,7 +481,7 @@ def batched(args, torch):
     # lanes overlap one input's edge blocks with the next input's passes,
     # about 2% faster, but then no kernel can be timed by itself)
     pipe = repfb.RePFBPipeline(4, chan, acclen, cut, osamp, nblock, c0 * osamp, (c0 + nchan) * osamp, tile_batch=16,
-                               lanes=args.lanes)
+                               lanes=args.lanes, stage2_overlap=not args.no_stage2_overlap)
     srm = repfb.StreamRowmap(specnums, 0, nfixed, acclen)
     rm = srm.build()
     plist = [packed[m] for m in range(4)]
@@ -548,8 +548,8 @@ def batched(args, torch):
     # launches overlap (on two lanes an event pair would also time the other
     # lane's kernels); the step time above is taken without them
     nk = max(1, min(args.steps, 5))
-    lanes = pipe.lanes
-    pipe.lanes = 1
+    lanes, overlap = pipe.lanes, pipe.stage2_overlap
+    pipe.lanes, pipe.stage2_overlap = 1, False
     timer.on = True
     pipe.timer = timer
     for _ in range(nk):
@@ -558,7 +558,7 @@ def batched(args, torch):
     counts = {k: v // nk for k, v in timer.counts().items()}
     timer.on = False
     pipe.timer = None
-    pipe.lanes = lanes
+    pipe.lanes, pipe.stage2_overlap = lanes, overlap
     eager0 = torch.cuda.Event(enable_timing=True)
     eager1 = torch.cuda.Event(enable_timing=True)
     eager0.record()
@@ -636,7 +636,8 @@ def batched(args, torch):
                                    total, nfixed, njobs, "one eager pass" if args.no_graph else "one hipGraph replay"),
                    "inputs": 4, "coarse_channels": nchan, "arrival_chunks": 64, "spectra_per_input": n_coarse,
                    "acclen": acclen, "cut": cut, "osamp": osamp, "nblock": nblock, "jobs": njobs,
-                   "correlator": args.precision},
+                   "correlator": args.precision, "stage1_lanes": args.lanes,
+                   "stage2_overlap": not args.no_stage2_overlap},
         "plain_fp32": {"form": "per-chunk / per-job launches, FP32 correlator, fastest column-pass width",
                        "ms_per_step": round(ms_chunk, 3),
                        "Msamples_s": round(nsamp / (ms_chunk * 1e-3) / 1e6, 1),
@@ -819,6 +820,8 @@ def main():
                     help="run the pipeline autotuner on the batched pass's inputs first and time its choice "
                          "(default: the library's launch shapes, which the committed profiles were taken with)")
     ap.add_argument("--no-autotune", action="store_true", help="accepted for compatibility (the default)")
+    ap.add_argument("--no-stage2-overlap", action="store_true",
+                    help="run the re-PFB stage 2 of all inputs as one launch after their stage 1 (no second stream)")
     ap.add_argument("--lanes", type=int, default=1, choices=[1, 2],
                     help="concurrent stage-1 lanes of the batched pass (1: every kernel runs alone)")
     ap.add_argument("--no-graph", action="store_true",

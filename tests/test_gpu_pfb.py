@@ -333,6 +333,24 @@ def test_batched_stream_graph_and_per_chunk_forms():
         torch.cuda.synchronize()
         print("[gpu-case] graph replay", rep, flush=True)
         assert torch.equal(bufs["vis"], raw), rep
+    # the other schedules (stage 2 of each input on its own stream beside the
+    # next inputs' stage 1; one stage-1 lane) give the same sums bit for bit,
+    # eager and captured
+    for lanes, overlap in ((2, True), (1, True), (1, False)):
+        pipe.lanes, pipe.stage2_overlap = lanes, overlap
+        bufs["vis"].zero_()
+        assert torch.equal(pipe.correlate_into(plist, rms, nfixed, bufs), raw.permute(3, 2, 1, 0)), (lanes, overlap)
+        g2 = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g2):
+            pipe.correlate_into(plist, rms, nfixed, bufs)
+        for rep in range(2):
+            bufs["vis"].zero_()
+            bufs["xin"].zero_()
+            torch.cuda.synchronize()
+            g2.replay()
+            torch.cuda.synchronize()
+            print("[gpu-case] schedule", lanes, overlap, "replay", rep, flush=True)
+            assert torch.equal(bufs["vis"], raw), (lanes, overlap, rep)
 
 
 @pytest.mark.parametrize("nslice,chans", [(256, [0, 5, 2048, 1000, 1001, 1700, 7, 8, 9, 10, 11, 12, 13, 14, 15,
