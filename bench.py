@@ -481,7 +481,7 @@ def batched(args, torch):
     # lanes overlap one input's edge blocks with the next input's passes,
     # about 2% faster, but then no kernel can be timed by itself)
     pipe = repfb.RePFBPipeline(4, chan, acclen, cut, osamp, nblock, c0 * osamp, (c0 + nchan) * osamp, tile_batch=16,
-                               lanes=args.lanes, stage2_overlap=not args.no_stage2_overlap)
+                               lanes=args.lanes, stage2_overlap=args.stage2_overlap)
     srm = repfb.StreamRowmap(specnums, 0, nfixed, acclen)
     rm = srm.build()
     plist = [packed[m] for m in range(4)]
@@ -637,7 +637,7 @@ def batched(args, torch):
                    "inputs": 4, "coarse_channels": nchan, "arrival_chunks": 64, "spectra_per_input": n_coarse,
                    "acclen": acclen, "cut": cut, "osamp": osamp, "nblock": nblock, "jobs": njobs,
                    "correlator": args.precision, "stage1_lanes": args.lanes,
-                   "stage2_overlap": not args.no_stage2_overlap},
+                   "stage2_overlap": args.stage2_overlap},
         "plain_fp32": {"form": "per-chunk / per-job launches, FP32 correlator, fastest column-pass width",
                        "ms_per_step": round(ms_chunk, 3),
                        "Msamples_s": round(nsamp / (ms_chunk * 1e-3) / 1e6, 1),
@@ -820,8 +820,9 @@ def main():
                     help="run the pipeline autotuner on the batched pass's inputs first and time its choice "
                          "(default: the library's launch shapes, which the committed profiles were taken with)")
     ap.add_argument("--no-autotune", action="store_true", help="accepted for compatibility (the default)")
-    ap.add_argument("--no-stage2-overlap", action="store_true",
-                    help="run the re-PFB stage 2 of all inputs as one launch after their stage 1 (no second stream)")
+    ap.add_argument("--stage2-overlap", action="store_true",
+                    help="run each input's re-PFB stage 2 on a second stream beside the next inputs' stage 1 "
+                         "(default: one stage-2 launch for all inputs after their stage 1, which measured faster)")
     ap.add_argument("--lanes", type=int, default=1, choices=[1, 2],
                     help="concurrent stage-1 lanes of the batched pass (1: every kernel runs alone)")
     ap.add_argument("--no-graph", action="store_true",
