@@ -68,6 +68,10 @@ SIGNATURES = {
                                        _vp]),
     "alb_ipfb_columns_repfb": (_i32, [_vp, _i64, _i64, _i64, _i64, _vp, _i64, _i32, _vp, _i64, _i64, _vp, _i32, _i64,
                                       _i64, _vp, _vp]),
+    "alb_ipfb_chan_fft_packed4": (_i32, [_vp, _vp, _vp, _i64, _i64, _i32, _i64, _i64, _vp, _vp]),
+    "alb_ipfb_rows_spectral": (_i32, [_vp, _vp, _i64, _i64, _i64, _vp, _vp]),
+    "alb_ipfb_columns_repfb_spectral": (_i32, [_vp, _i64, _i64, _vp, _i64, _i32, _vp, _i64, _i64, _vp, _i32, _i64,
+                                               _i64, _vp, _vp]),
     "alb_repfb_interior": (_i32, [_i64, _i64, _i64, _i32, _i64, _vp, _vp]),
     "alb_repfb_stage1_list": (_i32, [_vp, _vp, _i32, _vp, _i64, _i64, _vp, _vp]),
     "alb_tune_set": (_i32, [ctypes.c_char_p, _i64]),

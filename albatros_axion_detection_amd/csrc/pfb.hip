@@ -533,6 +533,45 @@ __device__ __forceinline__ void cols_core(const float* __restrict__ yt, int64_t 
   fft_lds<NSL, NP, T, 1, NSL, true, KEEP ? 2 : 0>(U, TA, TB, lb, 0, tid, W16, tw256);
 }
 
+// Spectral-order column core (see ipfb_rows_spectral_kernel): the strip's
+// time spectra are already transformed, Ds[4 f + c] = ddft[k0 + c][f] for
+// f <= NSL/2 without the 1/4096 of the channel transform.  Wiener filter with
+// 1/(4096 nslice) (a power of two: exact), inverse FFT; on return U holds the
+// time rows as cols_core leaves them.
+template <int NSL, int CW, int T>
+__device__ __forceinline__ void cols_core_spec(const float2* __restrict__ Ds, const float2* __restrict__ filt,
+                                               int64_t ldf, int64_t k0, float2* U, const float2* TA, const float2* TB,
+                                               int lb, int tid, const float2* W16) {
+  static_assert(CW == 4, "the spectral strip layout holds 4 columns");
+  constexpr int NP = CW / 2;
+  constexpr int PADN = Padded<NSL>::value;
+  constexpr int NF = NSL / 2 + 1;
+  constexpr int TOT = NF * NP;
+  constexpr int PER = (TOT + T - 1) / T;
+  const float sc = 1.0f / (float(NSL) * 4096.0f);
+  // lane (f, p), p fastest: a wave reads 32 consecutive 32-byte pieces
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const int idx = tid + e * T;
+    if (idx < TOT) {
+      const int p = idx % NP, f = idx / NP;
+      const float4 d = *reinterpret_cast<const float4*>(Ds + int64_t(f) * 4 + 2 * p);
+      const float2 y0 = make_float2(d.x, d.y), y1 = make_float2(d.z, d.w);
+      const float2* h0p = filt + (k0 + 2 * p) * ldf;
+      const float2 hv0 = h0p[f];
+      const float2 hv1 = h0p[ldf + f];
+      float2 g0 = make_float2(fmaf(hv0.x, y0.x, -(hv0.y * y0.y)) * sc, fmaf(hv0.x, y0.y, hv0.y * y0.x) * sc);
+      float2 g1 = make_float2(fmaf(hv1.x, y1.x, -(hv1.y * y1.y)) * sc, fmaf(hv1.x, y1.y, hv1.y * y1.x) * sc);
+      if (f == 0 || 2 * f == NSL) { g0.y = 0.f; g1.y = 0.f; }
+      float2* Up = U + p * PADN;
+      Up[pad16(f)] = make_float2(g0.x - g1.y, -(g0.y + g1.x));               // conj(g0 + i g1)
+      if (f != 0 && 2 * f != NSL) Up[pad16(NSL - f)] = make_float2(g0.x + g1.y, g0.y - g1.x);   // conj(conj g0 + i conj g1)
+    }
+  }
+  __syncthreads();
+  fft_lds<NSL, NP, T, 1, NSL, true, 0>(U, TA, TB, lb, 0, tid, W16);
+}
+
 template <int NSL, int CW, int T>
 __global__ __launch_bounds__(T) void ipfb_cols_kernel(const float* __restrict__ y, int64_t ldy, int64_t tile_stride,
                                                          const float2* __restrict__ filt, int64_t ldf, int64_t ncols,
@@ -643,7 +682,7 @@ __host__ __device__ inline void repfb_interior(int64_t g0, int64_t nkeep, int64_
 // Y1[q] = (Z[q] - conj Z[-q]) / 2i.
 // One tile of the fused column pass: column pass of strip k0 of tile `tile`,
 // stream rows for the tile-boundary blocks, stage 1 of the interior blocks.
-template <int NSL, int CW, int T, int R1>
+template <int NSL, int CW, int T, int R1, bool SPEC = false>
 __device__ __forceinline__ void cols_repfb_tile(const float* __restrict__ y, int64_t ldy, int64_t tile_stride,
                                                 const float2* __restrict__ filt, int64_t ldf, int cut,
                                                 float* __restrict__ ts, int64_t row0, int64_t rows_per_tile,
@@ -657,7 +696,11 @@ __device__ __forceinline__ void cols_repfb_tile(const float* __restrict__ y, int
   constexpr int H = R1 / 2;
   constexpr int SP = S >= 16 ? S + 1 : S;   // row of the exchange buffer (padded when it fits)
   constexpr int64_t C = 4096;
-  cols_core<NSL, CW, T, 5>(y + tile * tile_stride, ldy, filt, ldf, k0, U, TA, TB, lb, tid, W16);
+  if constexpr (SPEC)
+    cols_core_spec<NSL, CW, T>(reinterpret_cast<const float2*>(y) + (tile * (4096 / 4) + k0 / 4) * int64_t(NSL / 2 + 1) * 4,
+                               filt, ldf, k0, U, TA, TB, lb, tid, W16);
+  else
+    cols_core<NSL, CW, T, 5>(y + tile * tile_stride, ldy, filt, ldf, k0, U, TA, TB, lb, tid, W16);
 
   const int nkeep = NSL - 2 * cut;
   const int64_t g0 = row0 + tile * rows_per_tile;
@@ -794,7 +837,9 @@ __device__ __forceinline__ void cols_repfb_tile(const float* __restrict__ y, int
   }
 }
 
-template <int NSL, int CW, int T, int R1>
+// SPEC: y is the spectral-order strip array of ipfb_rows_spectral_kernel
+// (complex [tiles][4096/4][nslice/2+1][4]; ldy and tile_stride unused).
+template <int NSL, int CW, int T, int R1, bool SPEC = false>
 __global__ __launch_bounds__(T) void ipfb_cols_repfb_kernel(const float* __restrict__ y, int64_t ldy,
                                                                int64_t tile_stride, const float2* __restrict__ filt,
                                                                int64_t ldf, int cut, float* __restrict__ ts,
@@ -830,8 +875,142 @@ __global__ __launch_bounds__(T) void ipfb_cols_repfb_kernel(const float* __restr
     const int r = idx / CW, c = idx - r * CW;
     WIN[idx] = win[int64_t(r) * C + k0 + c];
   }
-  cols_repfb_tile<NSL, CW, T, R1>(y, ldy, tile_stride, filt, ldf, cut, ts, row0, rows_per_tile, ntap, nblocks, A, tile,
+  cols_repfb_tile<NSL, CW, T, R1, SPEC>(y, ldy, tile_stride, filt, ldf, cut, ts, row0, rows_per_tile, ntap, nblocks, A, tile,
                                   k0, U, TA, TB, W16, WR, WIN, tw.lb, tid);
+}
+
+// ------------------------------------------- spectral-order inverse PFB
+// cupy_ipfb (pfb_gpu_utils.py:56-91) takes dd = irfft(dat, axis=1) and then
+// ddft = rfft(dd.T, axis=1), a transform along time of every one of the 4096
+// samples of a spectrum.  The two transforms are linear and act on different
+// axes, so the one along time can run first, on the ncols non-zero channel
+// columns of dat only.  With E[f][c] = FFT_nslice(dat[:, c])[f] (the
+// imaginary parts of channels 0 and 2048 dropped first, as irfft drops them):
+//   ddft[j][f] = (1/4096) sum_k X_f[k] exp(+2 pi i j k / 4096),
+//   X_f[chan c] = E[f][c],  X_f[4096 - chan c] = conj(E[-f][c]) (0 < chan c < 2048)
+// for f = 0..nslice/2.  That is nslice/2+1 inverse 4096-point transforms plus
+// ncols transforms along time in place of nslice inverse transforms of the
+// spectra plus 4096 real transforms along time: the column pass then has only
+// its inverse transform left.
+
+// Channel transform along time: E[t][f][c] = FFT_nslice over the tile's rows of
+// the decoded 4-bit values of column c (zero for lost spectra, out-of-range
+// channels and the rows before the first chunk).  One workgroup = CA columns
+// of one tile; workgroups of one XCD take neighbouring columns.
+template <int NSL, int CA, int T>
+__global__ __launch_bounds__(T) void ipfb_chan_fft_kernel(Packed4Rows pk, int64_t ntiles, float2* __restrict__ E,
+                                                          Twiddle tw) {
+  constexpr int PADN = Padded<NSL>::value;
+  extern __shared__ float2 smem[];
+  float2* U = smem;                        // CA x PADN
+  float2* TA = U + CA * PADN;
+  float2* TB = TA + tw.nA;
+  float2* W16 = TB + tw.nB;
+  const int tid = threadIdx.x;
+  load_twiddle(tw, TA, TB, tid, T);
+  fill_w16(W16, tw.A, tw.B, tw.lb, tw.nA * tw.nB, tid, T);
+  const int ncols = int(pk.ncols);
+  const int64_t ngrp = (ncols + CA - 1) / CA;
+  const int64_t nitem = ngrp * ntiles;
+  // XCD-contiguous item order (workgroup i runs on XCD i % 8)
+  const int64_t per = (nitem + 7) / 8;
+  const int64_t item = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (item >= nitem) return;               // whole workgroup: no barrier is skipped by part of it
+  const int64_t tile = item / ngrp;
+  const int c0 = int(item - tile * ngrp) * CA;
+  // first radix-16 pass straight from the packed bytes: butterfly (j, p) takes
+  // rows j + NB r of column c0 + p (p fastest: neighbouring lanes read
+  // neighbouring bytes of one row)
+  {
+    constexpr int NB = NSL / 16;
+    constexpr int TOT = NB * CA;
+    constexpr int PER = (TOT + T - 1) / T;
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int b = tid + e * T;
+      if (TOT % T == 0 || b < TOT) {
+        const int p = b % CA, j = b / CA;
+        const int c = c0 + p;
+        const int ch = c < ncols ? pk.chan[c] : -1;
+        const bool use = ch >= 0 && ch <= 2048;
+        float2 v[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int64_t prow = use ? packed_row(pk, tile * NSL + j + NB * r, NSL) : -1;
+          const int byte = prow >= 0 ? int(pk.bytes[prow * ncols + c]) : 0;
+          v[r] = make_float2(float(nib4f(byte >> 4)), (ch == 0 || ch == 2048) ? 0.f : float(nib4f(byte & 15)));
+        }
+        DFT<16>::run(v);
+        float2* Up = U + p * PADN;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) Up[pad16(16 * j + r)] = v[r];
+      }
+    }
+  }
+  __syncthreads();
+  fft_lds<NSL, CA, T, 16, NSL, true>(U, TA, TB, tw.lb, 0, tid, W16);
+  float2* Et = E + tile * NSL * int64_t(ncols);
+  for (int idx = tid; idx < NSL * CA; idx += T) {
+    const int f = idx / CA, p = idx - f * CA;
+    if (c0 + p < ncols) Et[int64_t(f) * ncols + c0 + p] = U[p * PADN + pad16(f)];
+  }
+}
+
+// Spectra along the sample axis: for each tile and f = 0..nslice/2,
+// D[t][s][f][c] = 4096 ddft[4 s + c][f] (s < 1024 strips of 4 samples) from
+// E[t][f][.] and E[t][-f][.], FB rows f per workgroup in LDS.  A strip's rows
+// f, f+1 are adjacent, so each store instruction writes 64-byte runs.
+template <int FB, int T>
+__global__ __launch_bounds__(T) void ipfb_rows_spectral_kernel(const float2* __restrict__ E, const int* __restrict__ chan,
+                                                               int ncols, int64_t nslice, int64_t ntiles,
+                                                               float2* __restrict__ D, Twiddle tw) {
+  constexpr int N = 4096;
+  constexpr int NP = Padded<N>::value;
+  extern __shared__ float2 smem[];
+  float2* Z = smem;                        // FB x NP
+  float2* TA = Z + FB * NP;
+  float2* TB = TA + tw.nA;
+  float2* W16 = TB + tw.nB;
+  const int tid = threadIdx.x;
+  load_twiddle(tw, TA, TB, tid, T);
+  fill_w16(W16, tw.A, tw.B, tw.lb, tw.nA * tw.nB, tid, T);
+  const int64_t nf = nslice / 2 + 1;
+  const int64_t ngrp = (nf + FB - 1) / FB;
+  const int64_t nitem = ngrp * ntiles;
+  const int64_t per = (nitem + 7) / 8;
+  const int64_t item = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if (item >= nitem) return;               // whole workgroup
+  const int64_t tile = item / ngrp;
+  const int64_t f0 = (item - tile * ngrp) * FB;
+  for (int idx = tid; idx < FB * NP; idx += T) Z[idx] = make_float2(0.f, 0.f);
+  __syncthreads();
+  // conj(X_f) into LDS (a forward transform of it is the conjugated inverse);
+  // channel numbers are distinct (the caller marks repeats -1), so no two
+  // items write one bin
+  const float2* Et = E + tile * nslice * int64_t(ncols);
+  for (int idx = tid; idx < FB * ncols; idx += T) {
+    const int fb = idx / ncols, c = idx - fb * ncols;
+    const int64_t f = f0 + fb;
+    const int ch = chan[c];
+    if (f >= nf || ch < 0 || ch > 2048) continue;
+    const int64_t fm = f == 0 ? 0 : nslice - f;
+    float2* Zf = Z + fb * NP;
+    Zf[pad16(ch)] = cconj(Et[f * ncols + c]);
+    if (ch > 0 && ch < 2048) Zf[pad16(N - ch)] = Et[fm * ncols + c];
+  }
+  __syncthreads();
+  fft_lds<N, FB, T, 1, N, true>(Z, TA, TB, tw.lb, 0, tid, W16);
+  // lane q = (s, fb, h): samples 4 s + 2 h, + 1 of row f0 + fb, one float4
+  constexpr int NQ = (N / 2) * FB;
+  float2* Dt = D + tile * (N / 4) * nf * 4;
+  for (int q = tid; q < NQ; q += T) {
+    const int h = q & 1, fb = (q >> 1) % FB, s = q / (2 * FB);
+    const int64_t f = f0 + fb;
+    if (f >= nf) continue;
+    const float2 a = Z[fb * NP + pad16(4 * s + 2 * h)];
+    const float2 b = Z[fb * NP + pad16(4 * s + 2 * h + 1)];
+    *reinterpret_cast<float4*>(Dt + (int64_t(s) * nf + f) * 4 + 2 * h) = make_float4(a.x, -a.y, b.x, -b.y);
+  }
 }
 
 // ---------------------------------------- iPFB column pass, any even nslice
@@ -1581,7 +1760,7 @@ int check_cols_repfb(int cut, int ntap) {
   return ALB_OK;
 }
 
-template <int NSL, int CW, int T, int R1>
+template <int NSL, int CW, int T, int R1, bool SPEC = false>
 int launch_cols_repfb(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, const float2* filt, int64_t ldf,
                       int cut, float* ts, int64_t row0, int64_t rows_per_tile, const float* win, int ntap,
                       int64_t nblocks, float2* A, hipStream_t s) {
@@ -1596,10 +1775,42 @@ int launch_cols_repfb(const float* y, int64_t ldy, int64_t tile_stride, int64_t 
   const size_t lds = sizeof(float2) * (size_t(NP) * Padded<NSL>::value + tw.nA + tw.nB + kW16Size + R1) +
                      sizeof(float) * size_t(ntap) * R1 * CW;
   ALB_REQUIRE(lds <= 160 * 1024, "fused iPFB/re-PFB: LDS budget exceeded");
-  hipLaunchKernelGGL((ipfb_cols_repfb_kernel<NSL, CW, T, R1>), dim3(unsigned((4096 / CW) * ntiles)), dim3(T), lds, s, y,
+  hipLaunchKernelGGL((ipfb_cols_repfb_kernel<NSL, CW, T, R1, SPEC>), dim3(unsigned((4096 / CW) * ntiles)), dim3(T), lds, s, y,
                      ldy, tile_stride, filt, ldf, cut, ts, row0, rows_per_tile, win, ntap, nblocks, A, cols_group(ntiles),
                      tw);
   ALB_LAUNCH_CHECK("ipfb_cols_repfb_kernel");
+  return ALB_OK;
+}
+
+// Item-per-workgroup launches of the spectral-order iPFB: a multiple of 8
+// workgroups (the kernels map them to XCD-contiguous items).
+inline unsigned xcd_grid(int64_t nitem) { return unsigned((nitem + 7) / 8 * 8); }
+
+template <int NSL, int CA, int T>
+int launch_chan_fft(const Packed4Rows& pk, int64_t ntiles, float2* E, hipStream_t s) {
+  Twiddle tw;
+  int rc = get_twiddle(NSL, &tw, s);
+  if (rc) return rc;
+  const size_t lds = sizeof(float2) * (size_t(CA) * Padded<NSL>::value + tw.nA + tw.nB + kW16Size);
+  const int64_t nitem = (pk.ncols + CA - 1) / CA * ntiles;
+  ALB_REQUIRE(nitem < (int64_t(1) << 31) - 8, "alb_ipfb_chan_fft_packed4: too many tiles");
+  hipLaunchKernelGGL((ipfb_chan_fft_kernel<NSL, CA, T>), dim3(xcd_grid(nitem)), dim3(T), lds, s, pk, ntiles, E, tw);
+  ALB_LAUNCH_CHECK("ipfb_chan_fft_kernel");
+  return ALB_OK;
+}
+
+template <int FB, int T>
+int launch_rows_spectral(const float2* E, const int* chan, int64_t ncols, int64_t nslice, int64_t ntiles, float2* D,
+                         hipStream_t s) {
+  Twiddle tw;
+  int rc = get_twiddle(4096, &tw, s);
+  if (rc) return rc;
+  const size_t lds = sizeof(float2) * (size_t(FB) * Padded<4096>::value + tw.nA + tw.nB + kW16Size);
+  const int64_t nitem = (nslice / 2 + 1 + FB - 1) / FB * ntiles;
+  ALB_REQUIRE(nitem < (int64_t(1) << 31) - 8, "alb_ipfb_rows_spectral: too many tiles");
+  hipLaunchKernelGGL((ipfb_rows_spectral_kernel<FB, T>), dim3(xcd_grid(nitem)), dim3(T), lds, s, E, chan, int(ncols),
+                     nslice, ntiles, D, tw);
+  ALB_LAUNCH_CHECK("ipfb_rows_spectral_kernel");
   return ALB_OK;
 }
 
@@ -1877,6 +2088,56 @@ int alb_ipfb_columns_repfb(const float* y, int64_t ldy, int64_t tile_stride, int
                                                 rows_per_tile, win, ntap, nblocks, (float2*)A, s);
   }
   set_error("alb_ipfb_columns_repfb: nslice %lld with osamp %lld not supported (nslice 4096, osamp 32 or 64)",
+            (long long)nslice, (long long)osamp);
+  return ALB_ENOTSUP;
+}
+
+// Spectral-order inverse PFB (cupy_ipfb, pfb_gpu_utils.py:56-91, with its
+// transform along time taken first), step 1: E[t][f][c] (complex64,
+// [ntiles][nslice][ncols]) = FFT along the nslice rows of tile t of the
+// decoded 4-bit column c (rows as in alb_ipfb_rows_packed4; channels 0 and
+// 2048 keep their real part only; out-of-range channels give zeros).
+int alb_ipfb_chan_fft_packed4(const uint8_t* bytes, const int* rowmap, const int* chan, int64_t ncols, int64_t acclen,
+                              int cut, int64_t tile0, int64_t ntiles, void* E, void* stream) {
+  ALB_REQUIRE(bytes && rowmap && chan && E, "alb_ipfb_chan_fft_packed4: null pointer");
+  const int64_t nslice = acclen + 2 * int64_t(cut);
+  ALB_REQUIRE(acclen >= 2 * cut && cut >= 0 && ncols > 0 && ncols <= 2049 && ntiles >= 0 && tile0 >= 0,
+              "alb_ipfb_chan_fft_packed4: bad sizes");
+  ALB_REQUIRE(nslice == 4096, "alb_ipfb_chan_fft_packed4: nslice %lld not supported (4096)", (long long)nslice);
+  if (ntiles == 0) return ALB_OK;
+  Packed4Rows pk{bytes, rowmap, chan, ncols, acclen, 2 * int64_t(cut), tile0};
+  return launch_chan_fft<4096, 2, 512>(pk, ntiles, (float2*)E, as_stream(stream));
+}
+
+// Step 2: D[t][s][f][c] (complex64, [ntiles][1024][nslice/2+1][4]) =
+// 4096 * ddft[4 s + c][f], ddft = rfft(irfft(dat, axis=1), axis=0) of tile t,
+// from step 1's E.  chan: channel number of each column (repeats marked -1).
+int alb_ipfb_rows_spectral(const void* E, const int* chan, int64_t ncols, int64_t nslice, int64_t ntiles, void* D,
+                           void* stream) {
+  ALB_REQUIRE(E && chan && D, "alb_ipfb_rows_spectral: null pointer");
+  ALB_REQUIRE(ncols > 0 && ncols <= 2049 && ntiles >= 0, "alb_ipfb_rows_spectral: bad sizes");
+  ALB_REQUIRE(nslice == 4096, "alb_ipfb_rows_spectral: nslice %lld not supported (4096)", (long long)nslice);
+  if (ntiles == 0) return ALB_OK;
+  return launch_rows_spectral<2, 512>((const float2*)E, chan, ncols, nslice, ntiles, (float2*)D, as_stream(stream));
+}
+
+// Step 3: alb_ipfb_columns_repfb on step 2's D (Wiener filter, inverse
+// transform along time, re-PFB stage 1 of the interior blocks, edge rows).
+int alb_ipfb_columns_repfb_spectral(const void* D, int64_t ntiles, int64_t nslice, const void* filt, int64_t ldf,
+                                    int cut, float* ts, int64_t row0, int64_t rows_per_tile, const float* win,
+                                    int ntap, int64_t osamp, int64_t nblocks, void* A, void* stream) {
+  ALB_REQUIRE(ntiles >= 0 && nblocks >= 0 && rows_per_tile >= 0 && row0 + ntiles * rows_per_tile >= 0,
+              "alb_ipfb_columns_repfb_spectral: bad sizes");
+  ALB_REQUIRE(cut >= 0 && 2 * cut < nslice, "alb_ipfb_columns_repfb_spectral: bad cut");
+  ALB_REQUIRE(ntiles == 0 || ldf >= nslice / 2 + 1, "alb_ipfb_columns_repfb_spectral: bad filter stride");
+  hipStream_t s = as_stream(stream);
+  if (nslice == 4096 && osamp == 64)
+    return launch_cols_repfb<4096, 4, 512, 64, true>((const float*)D, 0, 0, ntiles, (const float2*)filt, ldf, cut, ts,
+                                                     row0, rows_per_tile, win, ntap, nblocks, (float2*)A, s);
+  if (nslice == 4096 && osamp == 32)
+    return launch_cols_repfb<4096, 4, 512, 32, true>((const float*)D, 0, 0, ntiles, (const float2*)filt, ldf, cut, ts,
+                                                     row0, rows_per_tile, win, ntap, nblocks, (float2*)A, s);
+  set_error("alb_ipfb_columns_repfb_spectral: nslice %lld with osamp %lld not supported (nslice 4096, osamp 32 or 64)",
             (long long)nslice, (long long)osamp);
   return ALB_ENOTSUP;
 }

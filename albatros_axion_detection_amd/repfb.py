@@ -118,7 +118,7 @@ class RePFBPipeline:
 
     def __init__(self, ninput, chan_numbers, acclen, cut, osamp, nblock, repfb_chanstart, repfb_chanend,
                  filt_thresh=0.45, ntap=4, filt=None, win=None, tile_batch=4, nchan=2049, row_pass="auto",
-                 lanes=2, stage2_overlap=False):
+                 lanes=2, stage2_overlap=False, ipfb_form="auto"):
         t = _lib.torch()
         self.ninput = ninput
         self.acclen = int(acclen)
@@ -178,6 +178,23 @@ class RePFBPipeline:
         if row_pass == "mfma" and self.ncols > 512:
             raise ValueError("the matrix-core row pass takes at most 512 channels")
         self.row_pass = "mfma" if row_pass == "mfma" else "fft"
+        # iPFB order of the fused pass: "direct" is the reference's order
+        # (row pass, then both transforms along time in the column pass);
+        # "spectral" runs the transform along time first, on the channel
+        # columns only (alb_ipfb_chan_fft_packed4 + alb_ipfb_rows_spectral +
+        # alb_ipfb_columns_repfb_spectral), leaving one 4096-point transform
+        # per column pair in the column pass.  Both give the same result to
+        # float32 rounding.  "auto" takes the direct order: on MI355X the
+        # spectral one's column pass is 15% faster but its two launches before
+        # it cost more than the row pass (69.9 against 62.6 ms on the batched
+        # workload); :meth:`autotune` tries both where both apply.
+        if ipfb_form not in ("auto", "spectral", "direct"):
+            raise ValueError("ipfb_form must be 'auto', 'spectral' or 'direct'")
+        self.spectral_ok = self.nslice == 4096 and self.row_pass == "fft"
+        if ipfb_form == "spectral" and not self.spectral_ok:
+            raise ValueError("the spectral-order iPFB needs nslice 4096 and the FFT row pass")
+        self.ipfb_form = "spectral" if ipfb_form == "spectral" else "direct"
+        self._chan = {}
         self.coeff = None
         if self.row_pass == "mfma":
             nbytes = int(_lib.load().alb_ipfb_rows_coeff_bytes(self.ncols))
@@ -217,11 +234,22 @@ class RePFBPipeline:
         return jobs, nblocks_total, rows
 
     def _tile_buffer(self, ntiles, lane=0):
+        """float32 buffer of ``ntiles`` iPFB tiles: the row pass's blocked tiles,
+        or the spectral form's strips ([1024][nslice/2+1][4] complex64 per tile)."""
         t = _lib.torch()
-        need = ntiles * self.nslice * 4096
+        need = ntiles * max(self.nslice * 4096, 1024 * (self.nslice // 2 + 1) * 8)
         if lane not in self._tiles or self._tiles[lane].numel() < need:
             self._tiles[lane] = t.empty(need, dtype=t.float32, device=_lib.device())
         return self._tiles[lane]
+
+    def _chan_buffer(self, ntiles, lane=0):
+        """Spectral form: the channel columns' transforms along time of
+        ``ntiles`` tiles ([ntiles][nslice][ncols] complex64)."""
+        t = _lib.torch()
+        need = ntiles * self.nslice * self.ncols
+        if lane not in self._chan or self._chan[lane].numel() < need:
+            self._chan[lane] = t.empty(need, dtype=t.complex64, device=_lib.device())
+        return self._chan[lane]
 
     def _workspace(self, nblk, ninput=1):
         """Stage-1 workspace of ``ninput`` inputs of ``nblk`` blocks each
@@ -335,7 +363,9 @@ class RePFBPipeline:
     def stream_stage1(self, packed, rowmap, nchunks, stream_out, nblocks_total, ws, lane=0, tiles=None, row_lo=0):
         """Fused form up to re-PFB stage 1: every block's stage-1 result of one
         input in the workspace ``ws`` (row pass, column pass with stage 1 of
-        the interior blocks, then the edge blocks from the stream buffer), on
+        the interior blocks, then the edge blocks from the stream buffer; in
+        the spectral form the channel transform along time and the spectra
+        along the samples replace the row pass), on
         the current stream with tile buffer ``lane``.  ``tiles = (c_lo,
         c_hi)`` and ``row_lo`` select a window of the stream (a group of
         jobs): ``stream_out`` row 0 and block 0 start at stream row
@@ -351,13 +381,26 @@ class RePFBPipeline:
             self._launch("fill_zero", "alb_fill_zero", _lib.ptr(tail), tail.numel() * 4, s)
         tb = self.tile_batch
         y = self._tile_buffer(min(tb, c_hi - c_lo), lane)
-        for t0 in range(c_lo, c_hi, tb):
-            nt = min(tb, c_hi - t0)
-            ldy = self._rows(packed, rowmap, t0, nt, y)
-            self._launch("ipfb_columns_repfb", "alb_ipfb_columns_repfb", _lib.ptr(y), ldy, self.nslice * 4096, nt,
-                         self.nslice, _lib.ptr(self.filt), self.filt.shape[1], self.cut, _lib.ptr(stream_out),
-                         t0 * self.acclen - row_lo, self.acclen, _lib.ptr(self.win), self.ntap, self.osamp,
-                         nblocks_total, _lib.ptr(ws), s)
+        if self.ipfb_form == "spectral":
+            E = self._chan_buffer(min(tb, c_hi - c_lo), lane)
+            for t0 in range(c_lo, c_hi, tb):
+                nt = min(tb, c_hi - t0)
+                self._launch("ipfb_chan_fft", "alb_ipfb_chan_fft_packed4", _lib.ptr(packed), _lib.ptr(rowmap),
+                             _lib.ptr(self.chan), self.ncols, self.acclen, self.cut, t0, nt, _lib.ptr(E), s)
+                self._launch("ipfb_rows_spectral", "alb_ipfb_rows_spectral", _lib.ptr(E), _lib.ptr(self.chan),
+                             self.ncols, self.nslice, nt, _lib.ptr(y), s)
+                self._launch("ipfb_columns_repfb", "alb_ipfb_columns_repfb_spectral", _lib.ptr(y), nt, self.nslice,
+                             _lib.ptr(self.filt), self.filt.shape[1], self.cut, _lib.ptr(stream_out),
+                             t0 * self.acclen - row_lo, self.acclen, _lib.ptr(self.win), self.ntap, self.osamp,
+                             nblocks_total, _lib.ptr(ws), s)
+        else:
+            for t0 in range(c_lo, c_hi, tb):
+                nt = min(tb, c_hi - t0)
+                ldy = self._rows(packed, rowmap, t0, nt, y)
+                self._launch("ipfb_columns_repfb", "alb_ipfb_columns_repfb", _lib.ptr(y), ldy, self.nslice * 4096,
+                             nt, self.nslice, _lib.ptr(self.filt), self.filt.shape[1], self.cut, _lib.ptr(stream_out),
+                             t0 * self.acclen - row_lo, self.acclen, _lib.ptr(self.win), self.ntap, self.osamp,
+                             nblocks_total, _lib.ptr(ws), s)
         edges = self.edge_blocks(nchunks, nblocks_total, (c_lo, c_hi), row_lo)
         self._launch("repfb_stage1_edges", "alb_repfb_stage1_list", _lib.ptr(stream_out), _lib.ptr(self.win),
                      self.ntap, edges.ctypes.data_as(ctypes.c_void_p), len(edges), self.osamp, _lib.ptr(ws), s)
@@ -446,6 +489,8 @@ class RePFBPipeline:
                 "stream": t.empty((rows, 4096), dtype=t.float32, device=dev),
                 "vis": t.empty((len(jobs), F, M, M), dtype=t.complex64, device=dev)}
         self._tile_buffer(min(self.tile_batch, nchunks))
+        if self.ipfb_form == "spectral":
+            self._chan_buffer(min(self.tile_batch, nchunks))
         self.prepare()
         if self.fused_shape():
             self._workspace(G * K, M)
@@ -456,6 +501,8 @@ class RePFBPipeline:
                                                            for _ in range(nl - 1)]
             for lane in range(1, nl):
                 self._tile_buffer(min(self.tile_batch, nchunks), lane)
+                if self.ipfb_form == "spectral":
+                    self._chan_buffer(min(self.tile_batch, nchunks), lane)
             if nl > 1 and (self._lane_streams is None or len(self._lane_streams) < nl):
                 self._lane_streams = [t.cuda.Stream() for _ in range(nl)]
             # made here, before any capture: stream and event creation are not captured work
@@ -593,7 +640,8 @@ class RePFBPipeline:
         best so far: iPFB tiles per row / column launch (``tile_batch``, up
         to ``nchunks``), concurrent stage-1 lanes (up to the lane buffers
         ``bufs`` holds), stage 2 beside stage 1 or after it
-        (``stage2_overlap``), then the library's launch-shape knobs in
+        (``stage2_overlap``), the iPFB order (``ipfb_form``, where the
+        spectral order applies), then the library's launch-shape knobs in
         ``TUNE_KNOBS`` (``alb_tune_set``; process-wide, so they also apply
         to other pipelines in this process).  Each candidate runs one untimed
         pass and ``reps`` timed ones (HIP events on the current stream, which
@@ -625,12 +673,20 @@ class RePFBPipeline:
             self.tile_batch = v
             for lane in range(nl_max):
                 self._tile_buffer(min(v, nchunks), lane)
+                if self.ipfb_form == "spectral":
+                    self._chan_buffer(min(v, nchunks), lane)
 
         def set_lanes(v):
             self.lanes = v
 
         def set_overlap(v):
             self.stage2_overlap = bool(v)
+
+        def set_form(v):
+            self.ipfb_form = v
+            if v == "spectral":
+                for lane in range(nl_max):
+                    self._chan_buffer(min(self.tile_batch, nchunks), lane)
 
         def knob_setter(key):
             return lambda v: _lib.call("alb_tune_set", key.encode(), int(v))
@@ -639,6 +695,8 @@ class RePFBPipeline:
                   self.tile_batch, set_tile_batch),
                  ("lanes", sorted({1, nl_max}), min(self.lanes, nl_max), set_lanes),
                  ("stage2_overlap", [0, 1], int(self.stage2_overlap), set_overlap)]
+        if self.spectral_ok and self.fused_shape():
+            space.append(("ipfb_form", ["direct", "spectral"], self.ipfb_form, set_form))
         space += [(key, vals, int(lib.alb_tune_get(key.encode())), knob_setter(key)) for key, vals in self.TUNE_KNOBS]
         best = {}
         for name, vals, cur, setter in space:

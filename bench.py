@@ -97,7 +97,7 @@ def rechannelizer(args, torch):
     nblk = (n_coarse // osamp) - 3  # every complete re-PFB block of the stream
     c_lo, c_hi = chan0 * osamp, (chan0 + nkeep) * osamp
     pipe = repfb.RePFBPipeline(1, chans, acclen, cut, osamp, nblk, c_lo, c_hi, filt_thresh=0.45,
-                               tile_batch=nchunks, row_pass=args.row_pass)
+                               tile_batch=nchunks, row_pass=args.row_pass, ipfb_form=args.ipfb_form)
     ldy = 8 if pipe.pow2 else 4096
     dev = _lib.device()
     packed = _lib.to_device(packed_np.reshape(nchunks, acclen, nkeep))
@@ -106,7 +106,12 @@ def rechannelizer(args, torch):
     rowmap = torch.empty((nchunks, acclen), dtype=torch.int32, device=dev)
     rows = max((nblk + 3) * osamp, n_coarse)
     stream = torch.zeros((rows, 4096), dtype=torch.float32, device=dev)
-    tiles = torch.empty((nchunks, nslice, 4096), dtype=torch.float32, device=dev)
+    # tiles: the row pass's blocked tiles, or the spectral form's strips
+    # ([1024][nslice/2+1][4] complex64 per tile); chan_t: the spectral form's
+    # channel transforms along time
+    tiles = torch.empty(nchunks * max(nslice * 4096, 1024 * (nslice // 2 + 1) * 8), dtype=torch.float32, device=dev)
+    spectral = pipe.ipfb_form == "spectral" and not args.unfused
+    chan_t = torch.empty((nchunks, nslice, nkeep), dtype=torch.complex64, device=dev) if spectral else None
     ws = torch.empty(int(_lib.load().alb_repfb_workspace_bytes(nblk, osamp)) // 8, dtype=torch.complex64,
                      device=dev)
     out = torch.empty((nblk, c_hi - c_lo), dtype=torch.complex64, device=dev)
@@ -124,7 +129,14 @@ def rechannelizer(args, torch):
         # range, so one call maps all of them: row = spec_num - start
         timer("rowmap", lambda: _lib.call(
             "alb_build_rowmap", _lib.ptr(specnums), n_coarse, 0, n_coarse, 0, _lib.ptr(rowmap), s()))
-        if pipe.row_pass == "mfma":
+        if spectral:
+            timer("ipfb_chan_fft", lambda: _lib.call(
+                "alb_ipfb_chan_fft_packed4", _lib.ptr(packed), _lib.ptr(rowmap), _lib.ptr(pipe.chan), nkeep, acclen,
+                cut, 0, nchunks, _lib.ptr(chan_t), s()))
+            timer("ipfb_rows_spectral", lambda: _lib.call(
+                "alb_ipfb_rows_spectral", _lib.ptr(chan_t), _lib.ptr(pipe.chan), nkeep, nslice, nchunks,
+                _lib.ptr(tiles), s()))
+        elif pipe.row_pass == "mfma":
             timer("ipfb_rows_mfma", lambda: _lib.call(
                 "alb_ipfb_rows_packed4_mfma", _lib.ptr(packed), _lib.ptr(rowmap), _lib.ptr(pipe.chan), nkeep, acclen,
                 cut, 0, nchunks, _lib.ptr(pipe.coeff), _lib.ptr(tiles), s()))
@@ -132,11 +144,17 @@ def rechannelizer(args, torch):
             timer("ipfb_rows_packed4", lambda: _lib.call(
                 "alb_ipfb_rows_packed4_blocked" if pipe.pow2 else "alb_ipfb_rows_packed4", _lib.ptr(packed),
                 _lib.ptr(rowmap), _lib.ptr(pipe.chan), nkeep, acclen, cut, 0, nchunks, _lib.ptr(tiles), s()))
-        if fused:
+        if spectral:
+            timer("ipfb_columns_repfb", lambda: _lib.call(
+                "alb_ipfb_columns_repfb_spectral", _lib.ptr(tiles), nchunks, nslice, _lib.ptr(pipe.filt),
+                pipe.filt.shape[1], cut, _lib.ptr(stream), 0, acclen, _lib.ptr(pipe.win), 4, osamp, nblk,
+                _lib.ptr(ws), s()))
+        if fused and not spectral:
             timer("ipfb_columns_repfb", lambda: _lib.call(
                 "alb_ipfb_columns_repfb", _lib.ptr(tiles), ldy, nslice * 4096, nchunks, nslice, _lib.ptr(pipe.filt),
                 pipe.filt.shape[1], cut, _lib.ptr(stream), 0, acclen, _lib.ptr(pipe.win), 4, osamp, nblk,
                 _lib.ptr(ws), s()))
+        if fused:
             timer("repfb_stage1_edges", lambda: _lib.call(
                 "alb_repfb_stage1_list", _lib.ptr(stream), _lib.ptr(pipe.win), 4, edges_p, len(edges), osamp,
                 _lib.ptr(ws), s()))
@@ -231,6 +249,7 @@ def rechannelizer(args, torch):
     alg = {
         "ipfb_rows_packed4": n_coarse * nkeep,                          # raw bytes in
         "ipfb_rows_mfma": n_coarse * nkeep,                             # raw bytes in
+        "ipfb_chan_fft": n_coarse * nkeep,                              # raw bytes in
         "ipfb_columns": n_coarse * 4096 * 4,                            # time stream out
         "repfb_stage1": (nblk + 3) * osamp * 4096 * 4,                  # time stream in
         "ipfb_columns_repfb": 2 * n_coarse * 4096 * 4,                  # time stream out + in
@@ -256,7 +275,7 @@ def rechannelizer(args, torch):
                                "spectra" % (n_coarse, nchunks, acclen, nslice, cut, chan0, chan0 + nkeep - 1,
                                             c_hi - c_lo, osamp, nblk),
                    "n_coarse": n_coarse, "acclen": acclen, "cut": cut, "osamp": osamp, "nchan_fine": c_hi - c_lo,
-                   "nblock": nblk},
+                   "nblock": nblk, "ipfb_form": "spectral" if spectral else "direct"},
         "ref_unit_msps": n_coarse * 2049 * 8 / (ms * 1e-3) / 1e6,
         "tuning": dict(tuned, sweep_ms_per_step={k: round(v, 4) for k, v in sweep.items()}),
     }
@@ -481,7 +500,7 @@ def batched(args, torch):
     # lanes overlap one input's edge blocks with the next input's passes,
     # about 2% faster, but then no kernel can be timed by itself)
     pipe = repfb.RePFBPipeline(4, chan, acclen, cut, osamp, nblock, c0 * osamp, (c0 + nchan) * osamp, tile_batch=16,
-                               lanes=args.lanes, stage2_overlap=args.stage2_overlap)
+                               lanes=args.lanes, stage2_overlap=args.stage2_overlap, ipfb_form=args.ipfb_form)
     srm = repfb.StreamRowmap(specnums, 0, nfixed, acclen)
     rm = srm.build()
     plist = [packed[m] for m in range(4)]
@@ -610,6 +629,7 @@ def batched(args, torch):
     nedge = len(pipe.edge_blocks(nfixed, bufs["nbt"]))
     per_pass = {
         "ipfb_rows_packed4": 4 * n_coarse * 300,
+        "ipfb_chan_fft": 4 * n_coarse * 300,
         "ipfb_columns_repfb": 4 * n_coarse * 2 * 16384,
         "repfb_stage1_edges": 4 * nedge * osamp * 4 * 4096 * 4,
         "repfb_stage2": 4 * n_coarse * 2400,
@@ -636,7 +656,7 @@ def batched(args, torch):
                                    total, nfixed, njobs, "one eager pass" if args.no_graph else "one hipGraph replay"),
                    "inputs": 4, "coarse_channels": nchan, "arrival_chunks": 64, "spectra_per_input": n_coarse,
                    "acclen": acclen, "cut": cut, "osamp": osamp, "nblock": nblock, "jobs": njobs,
-                   "correlator": args.precision, "stage1_lanes": args.lanes,
+                   "correlator": args.precision, "stage1_lanes": args.lanes, "ipfb_form": pipe.ipfb_form,
                    "stage2_overlap": args.stage2_overlap},
         "plain_fp32": {"form": "per-chunk / per-job launches, FP32 correlator, fastest column-pass width",
                        "ms_per_step": round(ms_chunk, 3),
@@ -820,6 +840,9 @@ def main():
                     help="run the pipeline autotuner on the batched pass's inputs first and time its choice "
                          "(default: the library's launch shapes, which the committed profiles were taken with)")
     ap.add_argument("--no-autotune", action="store_true", help="accepted for compatibility (the default)")
+    ap.add_argument("--ipfb-form", default="auto", choices=["auto", "spectral", "direct"],
+                    help="iPFB order of the fused pass: direct (the reference's order; auto takes it) or "
+                         "spectral (transform along time first, on the channel columns)")
     ap.add_argument("--stage2-overlap", action="store_true",
                     help="run each input's re-PFB stage 2 on a second stream beside the next inputs' stage 1 "
                          "(default: one stage-2 launch for all inputs after their stage 1, which measured faster)")

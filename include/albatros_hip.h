@@ -223,6 +223,27 @@ int alb_repfb_stage2_inputs(const void* A, int64_t a_stride, int64_t ninput, int
 int alb_ipfb_columns_repfb(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, int64_t nslice,
                            const void* filt, int64_t ldf, int cut, float* ts, int64_t row0, int64_t rows_per_tile,
                            const float* win, int ntap, int64_t osamp, int64_t nblocks, void* A, void* stream);
+/* Spectral-order streaming iPFB (pfb_gpu_utils.py:56-91 cupy_ipfb: dd = irfft(dat, axis=1), then
+ * rfft of dd along time).  The two transforms act on different axes, so the one along time runs
+ * first, on the ncols non-zero channel columns only; three launches replace the row pass and the
+ * forward half of the column pass (alb_ipfb_rows_packed4_blocked + alb_ipfb_columns_repfb), nslice 4096.
+ * Step 1: E [ntiles][nslice][ncols] complex64 = FFT along the nslice rows of each tile (rows, bytes,
+ * rowmap, chan and zero fill exactly as alb_ipfb_rows_packed4; channels 0 and 2048 keep the real part,
+ * which is all irfft reads of them, pycufft.py:139-161). */
+int alb_ipfb_chan_fft_packed4(const uint8_t* bytes, const int* rowmap, const int* chan, int64_t ncols, int64_t acclen,
+                              int cut, int64_t tile0, int64_t ntiles, void* E, void* stream);
+/* Step 2: D [ntiles][1024][nslice/2+1][4] complex64, D[t][s][f][c] = 4096 * ddft[4s+c][f] where
+ * ddft = rfft(irfft(dat, axis=1), axis=0) of tile t (the reference's ddft before the filter,
+ * pfb_gpu_utils.py:84-87), from step 1's E: one inverse 4096-point transform per f of the bins
+ * X_f[chan] = E[f], X_f[4096 - chan] = conj(E[-f]).  chan must hold distinct numbers (repeats -1). */
+int alb_ipfb_rows_spectral(const void* E, const int* chan, int64_t ncols, int64_t nslice, int64_t ntiles, void* D,
+                           void* stream);
+/* Step 3: alb_ipfb_columns_repfb (same arguments and outputs) on step 2's D in place of the tiles:
+ * Wiener filter (pfb_gpu_utils.py:87), inverse transform along time, re-PFB stage 1 of the interior
+ * blocks, the stream rows of the others.  nslice 4096 with osamp 32 or 64. */
+int alb_ipfb_columns_repfb_spectral(const void* D, int64_t ntiles, int64_t nslice, const void* filt, int64_t ldf,
+                                    int cut, float* ts, int64_t row0, int64_t rows_per_tile, const float* win,
+                                    int ntap, int64_t osamp, int64_t nblocks, void* A, void* stream);
 /* alb_repfb with its stage-1 workspace in a fixed device array of the library instead of A: allocates
  * nothing (no counterpart in the reference; for callers that cannot call hipMalloc -- code under hipGraph
  * capture, tight serving loops, harnesses that forbid allocation).  Requires osamp <= 64 and
