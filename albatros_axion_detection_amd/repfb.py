@@ -360,7 +360,15 @@ class RePFBPipeline:
         _lib.call("alb_repfb_stage2", _lib.ptr(ws), nblocks_total, self.osamp, self.c_lo, self.c_hi, _lib.ptr(out),
                   sb, sc, bpj, sj, _lib.stream())
 
-    def stream_stage1(self, packed, rowmap, nchunks, stream_out, nblocks_total, ws, lane=0, tiles=None, row_lo=0):
+    def zero_from(self, rows_valid, nblocks_total):
+        """First block (of ``nblocks_total``) whose rows all lie at or past
+        window row ``rows_valid``: it and every later block read only the
+        zero-filled tail of the stream (helper_gpu_stream.py:215-219), so
+        their re-PFB output is exactly zero."""
+        return int(min(nblocks_total, max(0, -(-int(rows_valid) // self.osamp))))
+
+    def stream_stage1(self, packed, rowmap, nchunks, stream_out, nblocks_total, ws, lane=0, tiles=None, row_lo=0,
+                      skip_zero_tail=False):
         """Fused form up to re-PFB stage 1: every block's stage-1 result of one
         input in the workspace ``ws`` (row pass, column pass with stage 1 of
         the interior blocks, then the edge blocks from the stream buffer; in
@@ -369,15 +377,22 @@ class RePFBPipeline:
         the current stream with tile buffer ``lane``.  ``tiles = (c_lo,
         c_hi)`` and ``row_lo`` select a window of the stream (a group of
         jobs): ``stream_out`` row 0 and block 0 start at stream row
-        ``row_lo``; rows of the window past the tiles are zeroed."""
+        ``row_lo``; rows of the window past the tiles are zeroed.  With
+        ``skip_zero_tail`` the blocks from :meth:`zero_from` on are left out
+        (their workspace is not written) and only the zero rows that earlier
+        blocks read are cleared."""
         c_lo, c_hi = tiles if tiles is not None else (0, nchunks)
         rows_valid = max(0, c_hi * self.acclen - row_lo)
         if stream_out.shape[0] < max((nblocks_total + self.ntap - 1) * self.osamp, rows_valid):
             raise ValueError("stream buffer too short for the re-PFB blocks")
         s = _lib.stream()
         total = stream_out.shape[0]
+        nb_used = nblocks_total
+        if skip_zero_tail:
+            nb_used = self.zero_from(rows_valid, nblocks_total)
+            total = min(total, max(rows_valid, (nb_used + self.ntap - 1) * self.osamp))
         if total > rows_valid:
-            tail = stream_out[rows_valid:]
+            tail = stream_out[rows_valid:total]
             self._launch("fill_zero", "alb_fill_zero", _lib.ptr(tail), tail.numel() * 4, s)
         tb = self.tile_batch
         y = self._tile_buffer(min(tb, c_hi - c_lo), lane)
@@ -402,6 +417,7 @@ class RePFBPipeline:
                              t0 * self.acclen - row_lo, self.acclen, _lib.ptr(self.win), self.ntap, self.osamp,
                              nblocks_total, _lib.ptr(ws), s)
         edges = self.edge_blocks(nchunks, nblocks_total, (c_lo, c_hi), row_lo)
+        edges = np.ascontiguousarray(edges[edges < nb_used])
         self._launch("repfb_stage1_edges", "alb_repfb_stage1_list", _lib.ptr(stream_out), _lib.ptr(self.win),
                      self.ntap, edges.ctypes.data_as(ctypes.c_void_p), len(edges), self.osamp, _lib.ptr(ws), s)
 
@@ -568,6 +584,13 @@ class RePFBPipeline:
             ng = s1 - s0
             nbg = ng * K
             if use_fused:
+                # blocks from nbz on read only the zero-filled stream tail:
+                # their channels are exactly zero, written by one fill instead
+                # of stage 1 and stage 2
+                nbz = self.zero_from(max(0, c_hi * a - row_lo), nbg)
+                if nbz < nbg:
+                    ztail = xin.view(-1)[nbz * F * M:nbg * F * M]
+                    self._launch("fill_zero", "alb_fill_zero", _lib.ptr(ztail), ztail.numel() * 8, _lib.stream())
                 # stage 1 of every input into its own workspace, then one
                 # stage-2 launch writes all inputs' channels into the
                 # correlator layout.  Inputs go round-robin to ``lanes`` HIP
@@ -582,7 +605,7 @@ class RePFBPipeline:
                 if nl == 1 and not s2:
                     for m in range(M):
                         self.stream_stage1(packed_list[m], rowmaps[m], nchunks, stream, nbg, ws[m * per:],
-                                           tiles=(c_lo, c_hi), row_lo=row_lo)
+                                           tiles=(c_lo, c_hi), row_lo=row_lo, skip_zero_tail=True)
                 else:
                     t = _lib.torch()
                     main = t.cuda.current_stream()
@@ -596,7 +619,8 @@ class RePFBPipeline:
                         lane = m % nl
                         with t.cuda.stream(lanes[lane]):
                             self.stream_stage1(packed_list[m], rowmaps[m], nchunks, lane_bufs[lane], nbg,
-                                               ws[m * per:], lane=lane, tiles=(c_lo, c_hi), row_lo=row_lo)
+                                               ws[m * per:], lane=lane, tiles=(c_lo, c_hi), row_lo=row_lo,
+                                               skip_zero_tail=True)
                         if s2:
                             # input m's stage 2 (its own workspace, its own channels of
                             # xin) once its stage 1 is done, beside the next inputs' stage 1
@@ -605,7 +629,7 @@ class RePFBPipeline:
                             self._s2_stream.wait_event(ev)
                             with t.cuda.stream(self._s2_stream):
                                 self._launch("repfb_stage2", "alb_repfb_stage2_inputs",
-                                             _lib._vp(ws.data_ptr() + m * per * 8), per * 8, 1, nbg, self.osamp,
+                                             _lib._vp(ws.data_ptr() + m * per * 8), per * 8, 1, nbz, self.osamp,
                                              self.c_lo, self.c_hi, _lib._vp(xin.data_ptr() + m * 8), 1, F * M, M, K,
                                              K * F * M, _lib.stream())
                     if nl > 1:
@@ -614,7 +638,7 @@ class RePFBPipeline:
                     if s2:
                         main.wait_stream(self._s2_stream)
                 if not s2:
-                    self._launch("repfb_stage2", "alb_repfb_stage2_inputs", _lib.ptr(ws), per * 8, M, nbg,
+                    self._launch("repfb_stage2", "alb_repfb_stage2_inputs", _lib.ptr(ws), per * 8, M, nbz,
                                  self.osamp, self.c_lo, self.c_hi, _lib.ptr(xin), 1, F * M, M, K, K * F * M,
                                  _lib.stream())
             else:
