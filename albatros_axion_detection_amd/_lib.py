@@ -25,6 +25,7 @@ SIGNATURES = {
     "alb_stream_synchronize": (_i32, [_vp]),
     "alb_fill_zero": (_i32, [_vp, _i64, _vp]),
     "alb_prepare_transform": (_i32, [_i64]),
+    "alb_prepare_repfb": (_i32, [_i64]),
     "alb_unpack_4bit_float": (_i32, [_vp, _vp, _vp, _i64, _i64, _vp, _i64, _i64, _vp]),
     "alb_unpack_1bit_float": (_i32, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i64, _vp]),
     "alb_sortpols": (_i32, [_vp, _vp, _vp, _i64, _i64, _i64, _i64, _i32, _i64, _i64, _vp]),

@@ -68,8 +68,9 @@ __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2
 // Two-level table of W_N^m = exp(-2 pi i m / N):  W_N^m = A[m >> lb] * B[m & (2^lb - 1)]
 // with A[i] = W_N^(i << lb) (N >> lb entries) and B[i] = W_N^i (2^lb entries),
 // both rounded from float64.  One table per (device, N), built on first use
-// (or ahead of time by alb_prepare_transform); a first use on a stream that is
-// being captured into a graph is refused, since building allocates and syncs.
+// (or ahead of time by alb_prepare_transform / alb_prepare_repfb) in a fixed
+// device array; a first use on a stream that is being captured into a graph is
+// refused, since building synchronises.
 struct Twiddle {
   const float2* A;
   const float2* B;

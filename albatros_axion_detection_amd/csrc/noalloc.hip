@@ -12,6 +12,7 @@ extern "C" {
 int alb_repfb(const float* ts, const float* win, int ntap, int64_t nblock, int64_t osamp, void* A, int64_t c_lo,
               int64_t c_hi, void* out, int64_t sb, int64_t sc, int64_t bpj, int64_t sj, void* stream);
 int64_t alb_repfb_workspace_bytes(int64_t nblock, int64_t osamp);
+int alb_prepare_repfb(int64_t osamp);
 }
 
 namespace alb {
@@ -62,6 +63,10 @@ int alb_repfb_noalloc(const float* ts, const float* win, int ntap, int64_t nbloc
               "static workspace (use alb_repfb with a caller workspace)",
               (long long)nblock, (long long)osamp, (long long)need, (long long)alb::kStaticWorkspaceBytes);
   if (nblock == 0) return ALB_OK;
+  // transform tables: built on a device's first call from the library's fixed
+  // table array (no allocation; alb_prepare_repfb builds them ahead of time)
+  const int rp = alb_prepare_repfb(osamp);
+  if (rp != ALB_OK) return rp;
   void* A = nullptr;
   const int rc = alb::static_workspace(&A);
   if (rc != ALB_OK) return rc;

@@ -1,5 +1,9 @@
 // Library runtime: per-thread error string, device twiddle-table registry and
 // the small C-ABI housekeeping entry points (version, last error, sync).
+// Transform tables live in a fixed device array (kTablePool float2 per
+// device): building one launches a kernel and synchronises but allocates
+// nothing, so the allocation-free entry points stay so on their first call.
+// Tables beyond the pool (only the largest transforms) fall back to hipMalloc.
 #include <mutex>
 #include <map>
 #include <string>
@@ -38,6 +42,34 @@ struct TwKey {
 static std::mutex g_tw_mu;
 static std::map<TwKey, Twiddle> g_tw;
 
+constexpr int64_t kTablePool = int64_t(1) << 18;   // float2 per device (2 MiB)
+constexpr int kPoolDevices = 64;
+static __device__ __align__(256) float2 table_pool[kTablePool];
+static void* g_pool_ptr[kPoolDevices];
+static int64_t g_pool_used[kPoolDevices];
+
+// n float2 of table storage on device dev (caller holds g_tw_mu): from the
+// pool while it lasts (address of this device's copy from
+// hipGetSymbolAddress, looked up once with the device current), else hipMalloc.
+static int table_alloc(int dev, int64_t n, float2** out) {
+  n = (n + 31) / 32 * 32;   // 256-byte aligned pieces
+  if (dev >= 0 && dev < kPoolDevices) {
+    if (g_pool_ptr[dev] == nullptr) {
+      void* p = nullptr;
+      ALB_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(table_pool)));
+      ALB_REQUIRE(p != nullptr, "no device address for the transform-table pool on device %d", dev);
+      g_pool_ptr[dev] = p;
+    }
+    if (g_pool_used[dev] + n <= kTablePool) {
+      *out = reinterpret_cast<float2*>(g_pool_ptr[dev]) + g_pool_used[dev];
+      g_pool_used[dev] += n;
+      return ALB_OK;
+    }
+  }
+  ALB_HIP(hipMalloc(out, sizeof(float2) * size_t(n)));
+  return ALB_OK;
+}
+
 int get_twiddle(int64_t N, Twiddle* out, hipStream_t s) {
   int dev = 0;
   ALB_HIP(hipGetDevice(&dev));
@@ -58,7 +90,8 @@ int get_twiddle(int64_t N, Twiddle* out, hipStream_t s) {
   int nA = int(N >> lb);
   if (nA < 1) nA = 1;
   float2* buf = nullptr;
-  ALB_HIP(hipMalloc(&buf, sizeof(float2) * size_t(nA + nB)));
+  const int rc = table_alloc(dev, nA + nB, &buf);
+  if (rc != ALB_OK) return rc;
   hipLaunchKernelGGL(twiddle_kernel, dim3((nA + 255) / 256), dim3(256), 0, 0, buf, nA, N, int64_t(nB));
   hipLaunchKernelGGL(twiddle_kernel, dim3((nB + 255) / 256), dim3(256), 0, 0, buf + nA, nB, N, int64_t(1));
   ALB_LAUNCH_CHECK("twiddle_kernel");
@@ -68,6 +101,7 @@ int get_twiddle(int64_t N, Twiddle* out, hipStream_t s) {
   *out = t;
   return ALB_OK;
 }
+
 
 // Tuning knobs (0 = the built-in choice).
 static std::mutex g_tune_mu;

@@ -540,11 +540,13 @@ class RePFBPipeline:
         return int(nvalid)
 
     def prepare(self):
-        """Build every twiddle table this pipeline's kernels use (4096-point
-        rows, nslice-point or chirp-z columns) so that a pass captured into a
-        hipGraph finds them built: a first build allocates and synchronises,
-        which the library refuses on a capturing stream."""
+        """Build every transform table this pipeline's kernels use (4096-point
+        rows, nslice-point or chirp-z columns, the re-PFB stage-2 phases) so
+        that a pass captured into a hipGraph finds them built: a first build
+        synchronises, which the library refuses on a capturing stream."""
         _lib.call("alb_prepare_transform", 4096)
+        if self.osamp <= 64:
+            _lib.call("alb_prepare_repfb", self.osamp)
         if self.pow2:
             _lib.call("alb_prepare_transform", self.nslice)
         else:

@@ -11,11 +11,13 @@
  *     nothing synchronises the host unless stated;
  *   - return 0 on success, 1 invalid argument, 2 HIP runtime error, 3 size not
  *     supported; alb_last_error() gives the message of the calling thread;
- *   - no entry point keeps a pointer to caller memory after it returns; the only
- *     library-owned allocations are read-only twiddle tables built in float64 on
- *     first use of a transform size (one per device and size), or ahead of time
- *     by alb_prepare_transform -- which code captured into a hipGraph must do,
- *     since a first use on a capturing stream is refused (ALB_EINVAL).
+ *   - no entry point keeps a pointer to caller memory after it returns; the
+ *     library's own device memory is read-only transform tables built in float64
+ *     on first use of a transform size (one per device and size) in a fixed 2 MiB
+ *     device array (hipMalloc only for tables beyond it: transforms above 2^24
+ *     points), or ahead of time by alb_prepare_transform / alb_prepare_repfb --
+ *     which code captured into a hipGraph must do, since a build synchronises and
+ *     a first use on a capturing stream is refused (ALB_EINVAL).
  *
  * Each declaration cites the reference interface it replaces
  * (paths relative to albatros_analysis/ of PhilJ0ly/albatros_axion_detection).
@@ -39,6 +41,10 @@ int alb_fill_zero(void* p, int64_t nbytes, void* stream);
  * the row passes use 4096, the column passes nslice (or the chirp-z length P), the fused column
  * pass nslice and 4096.  Synchronises the device; call it before capturing a pass into a graph. */
 int alb_prepare_transform(int64_t N);
+/* Build the transform tables of the re-PFB at osamp (a power of two 1..64; stage 2's 4096-point
+ * twiddles) on the current device (no counterpart in the reference, whose cupy_pfb,
+ * pfb_gpu_utils.py:105-121, hands the FFT to cuFFT).  Synchronises the device; allocates nothing. */
+int alb_prepare_repfb(int64_t osamp);
 
 /* ---- unpack (src/correlations/unpacking.c, unpacking.py) --------------- */
 /* unpacking.c:155 unpack_4bit_float / unpacking.py:140 _unpack_4bit_float_gpu.
