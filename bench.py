@@ -862,6 +862,8 @@ def main():
                     help="time the separate column pass + re-PFB stage 1 instead of the fused kernel")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain device copies of 1 GiB (HBM counter calibration) and exit")
+    ap.add_argument("--rechannelizer-only", action="store_true",
+                    help="only run the single-input rechannelizer workload and print its JSON (its profiles)")
     ap.add_argument("--secondary-only", action="store_true",
                     help="only run the secondary workloads (unpack, direct correlators, precision correlator) "
                          "and print their JSON")
@@ -877,6 +879,12 @@ def main():
         return
     if args.secondary_only:
         print(json.dumps(secondary(args, torch)), flush=True)
+        return
+    if args.rechannelizer_only:
+        rres, rctx = rechannelizer(args, torch)
+        if not args.no_cpu:
+            rres["cpu_baseline"] = cpu_baseline(rctx, args.cpu_budget)
+        print(json.dumps(rres), flush=True)
         return
     res, ctx = batched(args, torch)
     cpu = None
