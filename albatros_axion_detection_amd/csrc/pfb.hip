@@ -976,12 +976,20 @@ __global__ __launch_bounds__(T) void ipfb_chan_fft_kernel(Packed4Rows pk, int64_
 
 // Spectra along the sample axis: for each tile and f = 0..nslice/2,
 // D[t][s][f][c] = 4096 ddft[4 s + c][f] (s < 1024 strips of 4 samples) from
-// E[t][f][.] and E[t][-f][.], FB rows f per workgroup in LDS.  A strip's rows
-// f, f+1 are adjacent, so each store instruction writes 64-byte runs.
-template <int FB, int T>
-__global__ __launch_bounds__(T) void ipfb_rows_spectral_kernel(const float2* __restrict__ E, const int* __restrict__ chan,
-                                                               int ncols, int64_t nslice, int64_t ntiles,
-                                                               float2* __restrict__ D, Twiddle tw) {
+// E[t][f][.] and E[t][-f][.], FB rows f per group in LDS.  A strip's rows f,
+// f+1 are adjacent, so each store instruction writes 64-byte runs.
+//
+// Persistent: workgroup w runs on XCD w % 8 and walks that XCD's contiguous
+// range of (tile, row group) items, so neighbouring groups (whose stores share
+// lines) run on one XCD.  Thread item k is column (tid + k T) % ncols of row
+// (tid + k T) / ncols of the group: its channel number is fixed, and the E
+// values of the next group are fetched while the current one is transformed.
+// The store reads Z and clears it, so the next scatter starts from zeros.
+template <int FB, int T, int KI>
+__global__ __launch_bounds__(T, 2) void ipfb_rows_spectral_kernel(const float2* __restrict__ E,
+                                                                  const int* __restrict__ chan, int ncols,
+                                                                  int64_t nslice, int64_t ntiles,
+                                                                  float2* __restrict__ D, Twiddle tw) {
   constexpr int N = 4096;
   constexpr int NP = Padded<N>::value;
   extern __shared__ float2 smem[];
@@ -996,38 +1004,84 @@ __global__ __launch_bounds__(T) void ipfb_rows_spectral_kernel(const float2* __r
   const int64_t ngrp = (nf + FB - 1) / FB;
   const int64_t nitem = ngrp * ntiles;
   const int64_t per = (nitem + 7) / 8;
-  const int64_t item = (blockIdx.x % 8) * per + blockIdx.x / 8;
-  if (item >= nitem) return;               // whole workgroup
-  const int64_t tile = item / ngrp;
-  const int64_t f0 = (item - tile * ngrp) * FB;
-  for (int idx = tid; idx < FB * NP; idx += T) Z[idx] = make_float2(0.f, 0.f);
-  __syncthreads();
-  // conj(X_f) into LDS (a forward transform of it is the conjugated inverse);
-  // channel numbers are distinct (the caller marks repeats -1), so no two
-  // items write one bin
-  const float2* Et = E + tile * nslice * int64_t(ncols);
-  for (int idx = tid; idx < FB * ncols; idx += T) {
-    const int fb = idx / ncols, c = idx - fb * ncols;
-    const int64_t f = f0 + fb;
-    const int ch = chan[c];
-    if (f >= nf || ch < 0 || ch > 2048) continue;
-    const int64_t fm = f == 0 ? 0 : nslice - f;
-    float2* Zf = Z + fb * NP;
-    Zf[pad16(ch)] = cconj(Et[f * ncols + c]);
-    if (ch > 0 && ch < 2048) Zf[pad16(N - ch)] = Et[fm * ncols + c];
+  const int64_t S = gridDim.x / 8;
+  const int64_t xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+  auto item_of = [&](int64_t it) -> int64_t {   // item of iteration it, -1 past the end
+    const int64_t j = it * S + slot;
+    const int64_t g = xcd * per + j;
+    return (j < per && g < nitem) ? g : -1;
+  };
+  // the thread's items: row fb[k] of the group, column cc[k], channel ch[k] (-1: none)
+  int fbk[KI], cc[KI], ch[KI];
+#pragma unroll
+  for (int k = 0; k < KI; ++k) {
+    const int idx = tid + k * T;
+    fbk[k] = idx / ncols;
+    cc[k] = idx - fbk[k] * ncols;
+    const int cv = idx < FB * ncols ? chan[cc[k]] : -1;
+    ch[k] = (cv >= 0 && cv <= 2048) ? cv : -1;
   }
-  __syncthreads();
-  fft_lds<N, FB, T, 1, N, true>(Z, TA, TB, tw.lb, 0, tid, W16);
-  // lane q = (s, fb, h): samples 4 s + 2 h, + 1 of row f0 + fb, one float4
-  constexpr int NQ = (N / 2) * FB;
-  float2* Dt = D + tile * (N / 4) * nf * 4;
-  for (int q = tid; q < NQ; q += T) {
-    const int h = q & 1, fb = (q >> 1) % FB, s = q / (2 * FB);
-    const int64_t f = f0 + fb;
-    if (f >= nf) continue;
-    const float2 a = Z[fb * NP + pad16(4 * s + 2 * h)];
-    const float2 b = Z[fb * NP + pad16(4 * s + 2 * h + 1)];
-    *reinterpret_cast<float4*>(Dt + (int64_t(s) * nf + f) * 4 + 2 * h) = make_float4(a.x, -a.y, b.x, -b.y);
+  // E values of item k in group g: a = E[f][c], b = E[-f][c]; loads from a
+  // readable address when the item has none (kept apart by ch / f checks)
+  auto fetch = [&](int64_t g, float2* av, float2* bv) {
+#pragma unroll
+    for (int k = 0; k < KI; ++k) {
+      int64_t f = 0, t = 0;
+      if (g >= 0) {
+        t = g / ngrp;
+        f = (g - t * ngrp) * FB + fbk[k];
+      }
+      const bool ok = g >= 0 && ch[k] >= 0 && f < nf;
+      const int64_t fm = f == 0 ? 0 : nslice - f;
+      const float2* Et = E + t * nslice * int64_t(ncols) + cc[k];
+      av[k] = Et[(ok ? f : 0) * ncols];
+      bv[k] = Et[(ok ? fm : 0) * ncols];
+    }
+  };
+  for (int idx = tid; idx < FB * NP; idx += T) Z[idx] = make_float2(0.f, 0.f);
+  float2 aA[KI], bA[KI], aB[KI], bB[KI];
+  fetch(item_of(0), aA, bA);
+  // one group: conj(X_f) into LDS (a forward transform of it is the
+  // conjugated inverse; channel numbers are distinct, so no two items write
+  // one bin), transform, store (clearing Z)
+  auto transform = [&](int64_t g, const float2* av, const float2* bv) {
+    const int64_t t = g / ngrp;
+    const int64_t f0 = (g - t * ngrp) * FB;
+    __syncthreads();   // Z is all zeros: the initial fill, or the previous group's store cleared it
+#pragma unroll
+    for (int k = 0; k < KI; ++k) {
+      const int c = ch[k];
+      if (c >= 0 && f0 + fbk[k] < nf) {
+        float2* Zf = Z + fbk[k] * NP;
+        Zf[pad16(c)] = cconj(av[k]);
+        if (c > 0 && c < 2048) Zf[pad16(N - c)] = bv[k];
+      }
+    }
+    __syncthreads();
+    fft_lds<N, FB, T, 1, N, true>(Z, TA, TB, tw.lb, 0, tid, W16);
+    // lane q = (s, fb, h): samples 4 s + 2 h, + 1 of row f0 + fb, one float4
+    constexpr int NQ = (N / 2) * FB;
+    float2* Dt = D + t * (N / 4) * nf * 4;
+    for (int q = tid; q < NQ; q += T) {
+      const int h = q & 1, fb = (q >> 1) % FB, sI = q / (2 * FB);
+      const int64_t f = f0 + fb;
+      float2* za = Z + fb * NP + pad16(4 * sI + 2 * h);
+      const float2 va = za[0], vb = za[1];
+      za[0] = make_float2(0.f, 0.f);
+      za[1] = make_float2(0.f, 0.f);
+      if (f < nf)
+        *reinterpret_cast<float4*>(Dt + (int64_t(sI) * nf + f) * 4 + 2 * h) = make_float4(va.x, -va.y, vb.x, -vb.y);
+    }
+  };
+  for (int64_t it = 0;; it += 2) {
+    int64_t g = item_of(it);
+    if (g < 0) break;
+    fetch(item_of(it + 1), aB, bB);
+    transform(g, aA, bA);
+    g = item_of(it + 1);
+    if (g < 0) break;
+    fetch(item_of(it + 2), aA, bA);
+    transform(g, aB, bB);
   }
 }
 
@@ -1819,7 +1873,9 @@ int launch_chan_fft(const Packed4Rows& pk, int64_t ntiles, float2* E, hipStream_
   return ALB_OK;
 }
 
-template <int FB, int T>
+// Persistent launch: as many workgroups as fit on the device at once (a
+// multiple of 8, so every XCD gets the same count), at most one per item.
+template <int FB, int T, int KI>
 int launch_rows_spectral(const float2* E, const int* chan, int64_t ncols, int64_t nslice, int64_t ntiles, float2* D,
                          hipStream_t s) {
   Twiddle tw;
@@ -1828,8 +1884,22 @@ int launch_rows_spectral(const float2* E, const int* chan, int64_t ncols, int64_
   const size_t lds = sizeof(float2) * (size_t(FB) * Padded<4096>::value + tw.nA + tw.nB + kW16Size);
   const int64_t nitem = (nslice / 2 + 1 + FB - 1) / FB * ntiles;
   ALB_REQUIRE(nitem < (int64_t(1) << 31) - 8, "alb_ipfb_rows_spectral: too many tiles");
-  hipLaunchKernelGGL((ipfb_rows_spectral_kernel<FB, T>), dim3(xcd_grid(nitem)), dim3(T), lds, s, E, chan, int(ncols),
-                     nslice, ntiles, D, tw);
+  static std::atomic<int> resident[64];
+  int dev = 0;
+  ALB_HIP(hipGetDevice(&dev));
+  int nres = dev < 64 ? resident[dev].load() : 0;
+  if (nres == 0) {
+    int ncu = 0, occ = 0;
+    ALB_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    ALB_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, ipfb_rows_spectral_kernel<FB, T, KI>, T, lds));
+    nres = ncu * (occ > 0 ? occ : 1);
+    if (dev < 64) resident[dev].store(nres);
+  }
+  const int64_t per = (nitem + 7) / 8;
+  int64_t S = nres / 8 > 0 ? nres / 8 : 1;
+  if (S > per) S = per;
+  hipLaunchKernelGGL((ipfb_rows_spectral_kernel<FB, T, KI>), dim3(unsigned(8 * S)), dim3(T), lds, s, E, chan,
+                     int(ncols), nslice, ntiles, D, tw);
   ALB_LAUNCH_CHECK("ipfb_rows_spectral_kernel");
   return ALB_OK;
 }
@@ -2138,7 +2208,12 @@ int alb_ipfb_rows_spectral(const void* E, const int* chan, int64_t ncols, int64_
   ALB_REQUIRE(ncols > 0 && ncols <= 2049 && ntiles >= 0, "alb_ipfb_rows_spectral: bad sizes");
   ALB_REQUIRE(nslice == 4096, "alb_ipfb_rows_spectral: nslice %lld not supported (4096)", (long long)nslice);
   if (ntiles == 0) return ALB_OK;
-  return launch_rows_spectral<2, 512>((const float2*)E, chan, ncols, nslice, ntiles, (float2*)D, as_stream(stream));
+  // items per thread: every (row, column) of a group of two rows
+  hipStream_t s = as_stream(stream);
+  if (2 * ncols <= 512) return launch_rows_spectral<2, 512, 1>((const float2*)E, chan, ncols, nslice, ntiles, (float2*)D, s);
+  if (2 * ncols <= 1024) return launch_rows_spectral<2, 512, 2>((const float2*)E, chan, ncols, nslice, ntiles, (float2*)D, s);
+  if (2 * ncols <= 2048) return launch_rows_spectral<2, 512, 4>((const float2*)E, chan, ncols, nslice, ntiles, (float2*)D, s);
+  return launch_rows_spectral<2, 512, 9>((const float2*)E, chan, ncols, nslice, ntiles, (float2*)D, s);
 }
 
 // Step 3: alb_ipfb_columns_repfb on step 2's D (Wiener filter, inverse

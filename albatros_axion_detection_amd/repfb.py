@@ -185,9 +185,10 @@ class RePFBPipeline:
         # alb_ipfb_columns_repfb_spectral), leaving one 4096-point transform
         # per column pair in the column pass.  Both give the same result to
         # float32 rounding.  "auto" takes the direct order: on MI355X the
-        # spectral one's column pass is 15% faster but its two launches before
-        # it cost more than the row pass (69.9 against 62.6 ms on the batched
-        # workload); :meth:`autotune` tries both where both apply.
+        # spectral one's column pass is 10% faster but its two launches before
+        # it cost more than the row pass (iPFB launches 1.18 against 1.07 ms per
+        # 16 tiles on the batched workload); :meth:`autotune` tries both where
+        # both apply.
         if ipfb_form not in ("auto", "spectral", "direct"):
             raise ValueError("ipfb_form must be 'auto', 'spectral' or 'direct'")
         self.spectral_ok = self.nslice == 4096 and self.row_pass == "fft"
