@@ -137,13 +137,13 @@ __device__ __forceinline__ constexpr int pad_step() { return K % 16 == 0 ? K + K
 // TWC 1: also keep this pass's twiddles w[1..R-1] in twc; TWC 2: take them from
 // twc (a later transform of the same size on the same thread mapping, one
 // butterfly per thread).
-template <int N, int R, int NS, int F, int T, bool TAB = false, int TWC = 0>
+template <int N, int R, int NS, int F, int T, bool TAB = false, int TWC = 0, int RS = Padded<N>::value>
 __device__ __forceinline__ void fft_pass(float2* lds, const float2* A, const float2* B, int lb, int tsh, int tid,
                                          const float2* W16 = nullptr, float2* twc = nullptr) {
   constexpr int NB = N / R;
   constexpr int TOT = NB * F;
   constexpr int PER = (TOT + T - 1) / T;
-  constexpr int NP = Padded<N>::value;
+  constexpr int NP = RS;   // row stride of the F sequences (>= the padded length)
   constexpr bool RD_FAST = NB % 16 == 0;
   constexpr bool WR_FAST = NS % 16 == 0 || (NS == 1 && 16 % R == 0);
   float2 v[PER][R];
@@ -231,13 +231,13 @@ struct PickRadix {
 // With TAB the radix-16 pass at sub-transform size 16 takes its twiddles from
 // W16 (fill_w16).  TWC (see fft_pass) applies to the pass at sub-transform
 // size 256.
-template <int N, int F, int T, int NS = 1, int NSE = N, bool TAB = false, int TWC = 0>
+template <int N, int F, int T, int NS = 1, int NSE = N, bool TAB = false, int TWC = 0, int RS = Padded<N>::value>
 __device__ __forceinline__ void fft_lds(float2* lds, const float2* A, const float2* B, int lb, int tsh, int tid,
                                         const float2* W16 = nullptr, float2* twc = nullptr) {
   if constexpr (NS < NSE) {
     constexpr int R = PickRadix<N / NS>::value;
-    fft_pass<N, R, NS, F, T, TAB, NS == 256 ? TWC : 0>(lds, A, B, lb, tsh, tid, W16, twc);
-    fft_lds<N, F, T, NS * R, NSE, TAB, TWC>(lds, A, B, lb, tsh, tid, W16, twc);
+    fft_pass<N, R, NS, F, T, TAB, NS == 256 ? TWC : 0, RS>(lds, A, B, lb, tsh, tid, W16, twc);
+    fft_lds<N, F, T, NS * R, NSE, TAB, TWC, RS>(lds, A, B, lb, tsh, tid, W16, twc);
   }
 }
 

@@ -72,11 +72,10 @@ struct DenseRows {
 // (tile t = row / nslice, i = row % nslice).
 // ZERO: also clear every entry of Z it reads (rows past nrows included), so Z
 // is all zeros again for the next group's scatter.
-template <int M, int F, int TT, bool ZERO = false>
+template <int M, int F, int TT, bool ZERO = false, int MP = Padded<M>::value>
 __device__ __forceinline__ void irfft_store(float2* Z, int64_t row0, int64_t nrows, int64_t nslice,
                                             float* __restrict__ out, int64_t ldo, bool blocked, int tid) {
   constexpr int N = 2 * M;
-  constexpr int MP = Padded<M>::value;
   const float sc = 1.0f / float(N);
   if constexpr (M % TT == 0 && TT % (4 * F) == 0) {
     if (blocked) {
@@ -179,11 +178,17 @@ __global__ __launch_bounds__(TT) void irfft_rows_kernel(DenseRows dense, int64_t
 // packed bytes of the next group and the row-map entries of the group after
 // it are fetched while the current group is transformed.  KI = kRowItems
 // covers F * ncols <= KI * TT (100-channel spectra: 2; 300 channels: 4).
+// Row stride of the packed row pass's LDS spectra: the padded length, plus 8
+// complex entries with 4 rows per group, so that the rows start 16 banks
+// apart (the store reads all 4 rows in one instruction).
+template <int F>
+constexpr int packed4_row_stride() { return Padded<2048>::value + (F >= 4 ? 8 : 0); }
+
 template <int F, int TT, int kRowItems>
 __global__ __launch_bounds__(TT, 1024 / TT) void irfft_packed4_kernel(Packed4Rows pk, int64_t nrows, int64_t nslice,
                                                            float* __restrict__ out, int64_t ldo, Twiddle tw) {
   constexpr int M = 2048;
-  constexpr int MP = Padded<M>::value;
+  constexpr int MP = packed4_row_stride<F>();
   extern __shared__ float2 smem[];
   float2* Z = smem;                       // F x MP
   float2* TA = Z + F * MP;
@@ -332,8 +337,8 @@ __global__ __launch_bounds__(TT, 1024 / TT) void irfft_packed4_kernel(Packed4Row
       }
     }
     __syncthreads();
-    fft_lds<M, F, TT, 1, M, true, 2>(Z, TA, TB, tw.lb, 1, tl, W16, tw8);
-    irfft_store<M, F, TT, true>(Z, g * F, nrows, nslice, out, ldo, ldo == 8, tl);
+    fft_lds<M, F, TT, 1, M, true, 2, MP>(Z, TA, TB, tw.lb, 1, tl, W16, tw8);
+    irfft_store<M, F, TT, true, MP>(Z, g * F, nrows, nslice, out, ldo, ldo == 8, tl);
   };
   // Two sets of prefetch registers used in turn (no copies of values still in
   // flight): while group it is transformed, the bytes of group it + 1 and the
@@ -1617,7 +1622,7 @@ int launch_irfft_packed4(const Packed4Rows& pk, int64_t nrows, int64_t nslice, f
   Twiddle tw;
   int rc = get_twiddle(4096, &tw, s);
   if (rc) return rc;
-  const size_t lds = sizeof(float2) * (size_t(F) * Padded<2048>::value + tw.nA + tw.nB + kW16Size);
+  const size_t lds = sizeof(float2) * (size_t(F) * packed4_row_stride<F>() + tw.nA + tw.nB + kW16Size);
   const int64_t ngroups = (nrows + F - 1) / F;
   ALB_REQUIRE(nrows < (int64_t(1) << 31) - F && pk.tile0 >= 0 && pk.tile0 < (int64_t(1) << 30),
               "irfft: too many rows");
