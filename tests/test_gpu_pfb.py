@@ -999,3 +999,35 @@ def test_job_groups_match_whole_stream(osamp, nblock, acclen, nchan, lmin, lmax)
     b = pipe.buffers(nfixed, group_jobs=2)
     assert a["stream"].shape == b["stream"].shape and a["xin"].shape == b["xin"].shape
     assert b["stream"].shape[0] < whole["stream"].shape[0]
+
+
+def test_zero_tail_blocks_match_computed_tail():
+    # the batched pass fills the channels of blocks that read only the
+    # zero-filled stream tail instead of running stage 1 and stage 2 on them;
+    # the per-chunk form computes them.  With the data ending two chunks early
+    # (whole jobs past it) both agree to FP32 rounding, and every job past the
+    # data has exactly zero visibilities in both
+    import torch
+    from albatros_axion_detection_amd import _lib, repfb, synth
+    acclen, cut, osamp, nblock, nchan = 4088, 4, 64, 16, 24
+    packed, specnums, lengths = synth.batched_stream(nant=2, nchan=nchan, nchunks=6, lmin=2000, lmax=4000, loss=0.01,
+                                                     run=5, seed=21)
+    nfixed = int(lengths.sum()) // acclen
+    chan = np.arange(700, 700 + nchan)
+    pipe = repfb.RePFBPipeline(4, chan, acclen, cut, osamp, nblock, 700 * osamp, (700 + nchan) * osamp)
+    rm = repfb.rowmap_stream(specnums, 0, nfixed, acclen)
+    plist = [_lib.to_device(packed[m]) for m in range(4)]
+    bufs = pipe.buffers(nfixed)
+    nvalid = nfixed - 2
+    rows_valid = nvalid * acclen
+    nbz = pipe.zero_from(rows_valid, bufs["nbt"])
+    assert 0 < nbz < bufs["nbt"] - nblock          # at least one whole job past the data
+    got = pipe.correlate_into(plist, [rm] * 4, nfixed, bufs, nvalid=nvalid).clone().cpu().numpy()
+    ref = pipe.correlate_per_chunk(plist, [rm] * 4, nfixed, bufs, nvalid=nvalid).cpu().numpy()
+    scale = float(np.max(np.abs(ref)))
+    d = float(np.max(np.abs(got - ref)))
+    print("[gpu-case] zero tail from block %d of %d: max |diff| %.3g of max %.3g" % (nbz, bufs["nbt"], d, scale),
+          flush=True)
+    assert d < 2e-6 * scale
+    first_zero_job = -(-nbz // nblock)
+    assert np.all(got[..., first_zero_job:] == 0) and np.all(ref[..., first_zero_job:] == 0)
