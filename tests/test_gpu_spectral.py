@@ -85,6 +85,40 @@ def test_spectral_steps_against_fp64(rng):
     assert e1 < 1e-6 and e2 < 2e-6
 
 
+@pytest.mark.parametrize("ncols", [700, 2049])
+def test_spectral_steps_many_channels(rng, ncols):
+    # more channel items per thread than the small cases (the 4- and 9-item
+    # forms of the spectral row kernel): 700 distinct channels, and every
+    # channel 0..2048 in a shuffled order, one tile, against float64
+    import torch
+    from albatros_axion_detection_amd import _lib
+    acclen, cut, ntiles = 4088, 4, 1
+    nslice = acclen + 2 * cut
+    chans = rng.permutation(2049)[:ncols].astype(np.int32)
+    packed = rng.integers(0, 256, size=(ntiles * acclen, ncols), dtype=np.uint8)
+    rowmap = np.arange(ntiles * acclen, dtype=np.int32).reshape(ntiles, acclen)
+    rowmap[rng.random(rowmap.shape) < 0.05] = -1
+    dev = _lib.device()
+    pk, rm, ch = _lib.to_device(packed), _lib.to_device(rowmap), _lib.to_device(chans)
+    E = torch.empty((ntiles, nslice, ncols), dtype=torch.complex64, device=dev)
+    D = torch.empty((ntiles, 1024, nslice // 2 + 1, 4), dtype=torch.complex64, device=dev)
+    s = _lib.stream()
+    _lib.call("alb_ipfb_chan_fft_packed4", _lib.ptr(pk), _lib.ptr(rm), _lib.ptr(ch), ncols, acclen, cut, 0, ntiles,
+              _lib.ptr(E), s)
+    _lib.call("alb_ipfb_rows_spectral", _lib.ptr(E), _lib.ptr(ch), ncols, nslice, ntiles, _lib.ptr(D), s)
+    got_d = D.cpu().numpy().transpose(0, 2, 1, 3).reshape(ntiles, nslice // 2 + 1, 4096)
+    del D, E
+    spec = _tile_spectra(packed, rowmap, chans, acclen, cut, ntiles)
+    ref_d = 4096.0 * np.fft.rfft(np.fft.irfft(spec, n=4096, axis=2), axis=1)
+    del spec
+    e2 = max(rel_err(got_d[0, f], ref_d[0, f]) for f in range(nslice // 2 + 1))
+    print("[gpu-case] spectral steps, %d channels: D %.3g (per row) vs float64" % (ncols, e2), flush=True)
+    # dense spectra (every bin filled) go through two float32 4096-point
+    # transforms with no zero bins: 5e-6 per row, the bound of the dense
+    # row-pass test (test_gpu_pfb.py::test_row_pass_every_channel)
+    assert e2 < (5e-6 if ncols > 1024 else 2e-6)
+
+
 @pytest.mark.parametrize("osamp,nblock", [(64, 20), (32, 30)])
 def test_spectral_form_matches_direct_form_and_oracle(rng, osamp, nblock):
     # the fused streaming spectra of both iPFB orders: same stream rows, same
