@@ -2,22 +2,31 @@
 scripts/xcorr/helper_gpu_stream.py (``repfb_xcorr_avg``, ``get_chunkier``).
 
 The reference loops job -> antenna -> pol -> chunk with a handful of CuPy /
-cuFFT / cuBLAS launches per chunk and a host sync per job.  Here the whole
-stream of one input goes through four launches:
+cuFFT / cuBLAS launches per chunk and a host sync per job.  Here one pass over
+the whole stream (or over a group of jobs, when memory is bounded) is:
 
-1. ``alb_ipfb_rows_packed4``: 4-bit unpack + spec_num zero fill + channel
-   scatter + row inverse FFT for a batch of iPFB tiles (the ``2*cut`` rows
-   carried from the previous chunk are read in place, helper_gpu_stream.py:195-197);
-2. ``alb_ipfb_columns``: column FFT, Wiener filter, inverse column FFT; rows
-   ``[cut, nslice-cut)`` land at their place in the input's time stream
-   (helper_gpu_stream.py:199-214);
-3. ``alb_repfb``: all re-PFB jobs of the stream at once (job s covers stream
-   samples ``[s*nblock*lblock, s*nblock*lblock + (nblock+ntap-1)*lblock)``,
-   the zero-padded tail included, helper_gpu_stream.py:148-150, 217-222),
-   writing the kept fine channels straight into the correlator layout;
-4. ``alb_xcorr_all_ant``: every job's ``M x M`` visibilities in one launch.
+1. ``alb_build_rowmap``: spec_num -> row map of the fixed iPFB tiles, so
+   uneven arrival chunks with lost spectra are re-blocked without copies
+   (helper_gpu_stream.py:183-199; the ``2*cut`` rows carried from the
+   previous chunk are read in place);
+2. per input, per batch of tiles: ``alb_ipfb_rows_packed4_blocked`` (4-bit
+   unpack + zero fill + channel scatter + inverse FFT of each spectrum) and
+   ``alb_ipfb_columns_repfb`` (transform along time, Wiener filter, inverse
+   transform, and re-PFB stage 1 of every block inside the tile); or, in the
+   spectral order, ``alb_ipfb_chan_fft_packed4`` + ``alb_ipfb_rows_spectral``
+   + ``alb_ipfb_columns_repfb_spectral``;
+3. per input: ``alb_repfb_stage1_list`` for the blocks that straddle tiles
+   (job s covers stream samples ``[s*nblock*lblock, s*nblock*lblock +
+   (nblock+ntap-1)*lblock)``, helper_gpu_stream.py:148-150, 217-222); blocks
+   that read only the zero-filled stream tail get their channels from one
+   fill instead;
+4. ``alb_repfb_stage2_inputs``: the kept fine channels of every input and
+   block, written into the correlator layout;
+5. ``alb_xcorr_all_ant_strided``: every job's ``M x M`` visibilities.
 
-Row ``ant*npol + pol`` of the correlator input follows
+``RePFBPipeline.correlate_per_chunk`` keeps the reference's per-chunk,
+per-job launch structure in plain FP32 (the form the batched pass is
+measured against).  Row ``ant*npol + pol`` of the correlator input follows
 helper_gpu_stream_clean.py:375 (helper_gpu_stream.py:222 indexes ``j*nant+k``).
 """
 import ctypes
@@ -113,7 +122,8 @@ class RePFBPipeline:
     own as soon as that input's stage 1 is done, beside the next inputs'
     stage 1 (off by default: on MI355X the overlapped kernels evict each
     other's working sets, 67.4 against 63.4 ms on the batched workload;
-    :meth:`autotune` tries both).
+    :meth:`autotune` tries both).  ``ipfb_form`` picks the iPFB order of the
+    fused pass ("direct", the default, or "spectral"; see below).
     """
 
     def __init__(self, ninput, chan_numbers, acclen, cut, osamp, nblock, repfb_chanstart, repfb_chanend,
