@@ -495,10 +495,11 @@ def batched(args, torch):
     total = int(lengths.sum())
     nfixed = total // acclen
     chan = np.arange(c0, c0 + nchan)
-    # one stage-1 lane: every launch runs alone, so the per-kernel HIP-event
-    # times and the profiler's dispatch durations are each kernel's own (two
-    # lanes overlap one input's edge blocks with the next input's passes,
-    # about 2% faster, but then no kernel can be timed by itself)
+    # two stage-1 lanes: inputs alternate between two HIP streams, so one
+    # input's row, column and edge-block launches run beside the next one's
+    # (1.5% faster than one lane; four lanes measured the same as two).  The
+    # per-kernel times below come from passes with one lane, where every
+    # launch runs alone.
     pipe = repfb.RePFBPipeline(4, chan, acclen, cut, osamp, nblock, c0 * osamp, (c0 + nchan) * osamp, tile_batch=16,
                                lanes=args.lanes, stage2_overlap=args.stage2_overlap, ipfb_form=args.ipfb_form)
     srm = repfb.StreamRowmap(specnums, 0, nfixed, acclen)
@@ -846,8 +847,9 @@ def main():
     ap.add_argument("--stage2-overlap", action="store_true",
                     help="run each input's re-PFB stage 2 on a second stream beside the next inputs' stage 1 "
                          "(default: one stage-2 launch for all inputs after their stage 1, which measured faster)")
-    ap.add_argument("--lanes", type=int, default=1, choices=[1, 2],
-                    help="concurrent stage-1 lanes of the batched pass (1: every kernel runs alone)")
+    ap.add_argument("--lanes", type=int, default=2, choices=[1, 2],
+                    help="concurrent stage-1 lanes of the batched pass (2: two inputs' stage 1 side by side, "
+                         "1: every kernel runs alone; the per-kernel times are taken with one lane either way)")
     ap.add_argument("--no-graph", action="store_true",
                     help="time eager passes instead of hipGraph replays (the counter-collection runs)")
     ap.add_argument("--no-plain", action="store_true",
