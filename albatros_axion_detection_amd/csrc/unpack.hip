@@ -1,8 +1,8 @@
 // Baseband unpack / repack kernels: 4-bit and 1-bit decode to complex64,
 // per-pol repack (sortpols), raw-value histograms, spec_num zero-fill scatter
 // and the dropped-packet zero fill.  All memory-bound.  The unpack and sortpols
-// kernels are row-tiled (a lane owns a column or column group and walks rows
-// blockIdx.y + gridDim.y * m); the others are grid-stride loops with 64-bit
+// kernels are row-tiled (a lane owns a column or column group and walks the
+// rows of its workgroup); the others are grid-stride loops with 64-bit
 // indexing, every thread owning whole output elements.
 #include "common.hpp"
 
@@ -66,15 +66,78 @@ __global__ void unpack1_kernel(const uint8_t* __restrict__ data, float2* __restr
 }
 
 // Row-tiled forms of the three kernels above and below for aligned shapes:
-// x indexes a group of columns, rows run over blockIdx.y + gridDim.y * j, so no
-// 64-bit division per element, and each lane writes 16 bytes per pol (4-bit
-// and 1-bit unpack, two complex64 each) or 4 bytes per pol (sortpols, four
-// channels).
+// x indexes a group of columns, rows run over blockIdx.y + gridDim.y * j (the
+// sortpols forms) or over one contiguous block of rows per y (the unpack
+// forms: their workgroups then write long contiguous runs, 9% faster for
+// 1-bit unpack, 1% for the 4-bit pair form, while sortpols measured 6%
+// slower that way), so no 64-bit division per element, and each lane writes
+// 16 bytes per pol (4-bit and 1-bit unpack, two complex64 each) or 4 bytes
+// per pol (sortpols, four channels).
 // streaming 16-byte store: the unpacked pols are written once and not read back here
 typedef float nt_float4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ void store_nt(float4* p, float a, float b, float c, float d) {
   const nt_float4 v = {a, b, c, d};
   __builtin_nontemporal_store(v, reinterpret_cast<nt_float4*>(p));
+}
+
+// Rows [*rb, *re) of workgroup row block blockIdx.y: nrows split into
+// gridDim.y contiguous blocks.
+__device__ __forceinline__ void row_block(int64_t nrows, int64_t* rb, int64_t* re) {
+  const int64_t per = (nrows + gridDim.y - 1) / gridDim.y;
+  *rb = int64_t(blockIdx.y) * per;
+  *re = *rb + per < nrows ? *rb + per : nrows;
+}
+
+// 4-bit unpack through LDS: kRows4 consecutive raw spectra (2 nchan bytes
+// each, contiguous in memory) are read with one 16-byte load per lane into
+// LDS, and each lane gathers the two bytes of each of its two channels from
+// there; the next batch's load is issued before this batch's stores.  Against
+// the pair form's per-lane 32-bit loads: 1.076 -> 0.970 ms at 2^18 spectra x
+// 1024 channels, bit-identical.  One workgroup covers every column pair of a
+// row (npair <= 512), kRows4 spectra fit the 512 lanes' loads
+// (2 nchan kRows4 <= 8192 bytes).
+constexpr int kRows4 = 4;
+__global__ __launch_bounds__(512) void unpack4_lds_kernel(const uint8_t* __restrict__ data, float4* __restrict__ pol0,
+                                                         float4* __restrict__ pol1, int64_t rowstart, int64_t nrows,
+                                                         const int64_t* __restrict__ channels, int64_t npair,
+                                                         int64_t nchan) {
+  extern __shared__ uint4 rowbuf[];   // kRows4 spectra of raw bytes
+  const int tid = threadIdx.x;
+  const int64_t kp = blockIdx.x * int64_t(blockDim.x) + tid;
+  const bool valid = kp < npair;
+  const int o0 = valid ? int(2 * channels[2 * kp]) : 0, o1 = valid ? int(2 * channels[2 * kp + 1]) : 0;
+  const int64_t rowbytes = 2 * nchan;
+  int64_t rb, re;
+  row_block(nrows, &rb, &re);
+  if (rb >= re) return;   // the whole workgroup
+  const int n16 = int(kRows4 * rowbytes / 16);   // 16-byte pieces of a full batch (<= blockDim.x)
+  // lane tid's piece of the batch starting at spectrum r0 (zero past the block's rows)
+  auto fetch = [&](int64_t r0) -> uint4 {
+    const int64_t nr = re - r0 < kRows4 ? re - r0 : kRows4;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (tid < nr * rowbytes / 16) v = reinterpret_cast<const uint4*>(data + (rowstart + r0) * rowbytes)[tid];
+    return v;
+  };
+  uint4 nxt = fetch(rb);
+  for (int64_t r0 = rb; r0 < re; r0 += kRows4) {
+    if (tid < n16) rowbuf[tid] = nxt;
+    __syncthreads();
+    if (r0 + kRows4 < re) nxt = fetch(r0 + kRows4);
+    const int nr = int(re - r0 < kRows4 ? re - r0 : kRows4);
+    if (valid) {
+      const uint8_t* base = reinterpret_cast<const uint8_t*>(rowbuf);
+      for (int rr = 0; rr < nr; ++rr) {
+        const int a = *reinterpret_cast<const uint16_t*>(base + rr * rowbytes + o0);   // byte 0 pol0, byte 1 pol1
+        const int b = *reinterpret_cast<const uint16_t*>(base + rr * rowbytes + o1);
+        const int64_t r = r0 + rr;
+        store_nt(&pol0[r * npair + kp], float(nib((a >> 4) & 15)), float(nib(a & 15)), float(nib((b >> 4) & 15)),
+                 float(nib(b & 15)));
+        store_nt(&pol1[r * npair + kp], float(nib((a >> 12) & 15)), float(nib((a >> 8) & 15)),
+                 float(nib((b >> 12) & 15)), float(nib((b >> 8) & 15)));
+      }
+    }
+    __syncthreads();   // every lane has read the batch before the next one is stored
+  }
 }
 
 __global__ void unpack4_pair_kernel(const uint8_t* __restrict__ data, float4* __restrict__ pol0,
@@ -85,7 +148,9 @@ __global__ void unpack4_pair_kernel(const uint8_t* __restrict__ data, float4* __
   const int64_t o0 = 2 * channels[2 * kp], o1 = 2 * channels[2 * kp + 1];
   // consecutive channels starting on a 4-byte boundary: one 32-bit load
   const bool word = word_rows && o1 == o0 + 2 && (o0 & 3) == 0;
-  for (int64_t r = blockIdx.y; r < nrows; r += gridDim.y) {
+  int64_t rb, re;
+  row_block(nrows, &rb, &re);
+  for (int64_t r = rb; r < re; ++r) {
     const uint8_t* row = data + (rowstart + r) * 2 * nchan;
     int a, b;   // byte 0 pol0, byte 1 pol1 of each channel
     if (word) {
@@ -109,7 +174,9 @@ __global__ void unpack1_rows_kernel(const uint8_t* __restrict__ data, float4* __
   const int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (j >= nb) return;
   const int64_t bps = nchan / 2;
-  for (int64_t r = blockIdx.y; r < nrows; r += gridDim.y) {
+  int64_t rb, re;
+  row_block(nrows, &rb, &re);
+  for (int64_t r = rb; r < re; ++r) {
     const int b = data[(r + rowstart) * bps + chanstart / 2 + j];
     auto pm = [&](int s) { return float(2 * ((b >> s) & 1) - 1); };
     store_nt(&pol0[r * nb + j], pm(7), pm(6), pm(3), pm(2));
@@ -292,6 +359,13 @@ int alb_unpack_4bit_float(const uint8_t* data, void* pol0, void* pol1, int64_t r
   const int64_t nrows = rowend - rowstart;
   if (nrows * ncols == 0) return ALB_OK;
   ALB_REQUIRE(data && pol0 && pol1 && channels, "alb_unpack_4bit_float: null pointer");
+  if (ncols % 2 == 0 && ncols / 2 <= 512 && aligned(data, 16) && (2 * nchan) % 16 == 0 && 2 * nchan * kRows4 <= 512 * 16 &&
+      aligned(pol0, 16) && aligned(pol1, 16)) {
+    hipLaunchKernelGGL(unpack4_lds_kernel, grid_rows(ncols / 2, nrows, 512), dim3(512), size_t(2 * nchan * kRows4),
+                       as_stream(stream), data, (float4*)pol0, (float4*)pol1, rowstart, nrows, channels, ncols / 2, nchan);
+    ALB_LAUNCH_CHECK("unpack4_lds_kernel");
+    return ALB_OK;
+  }
   if (ncols % 2 == 0 && aligned(data, 2) && aligned(pol0, 16) && aligned(pol1, 16) && ncols / 2 < (int64_t(1) << 31)) {
     hipLaunchKernelGGL(unpack4_pair_kernel, grid_rows(ncols / 2, nrows), dim3(256), 0, as_stream(stream), data,
                        (float4*)pol0, (float4*)pol1, rowstart, nrows, channels, ncols / 2, nchan,

@@ -460,6 +460,27 @@ def test_unpack_row_tiled_forms(unpk, rng):
         assert np.array_equal(g[0], e[0]) and np.array_equal(g[1], e[1]), (cs, ce)
 
 
+def test_unpack_row_blocks_longer_than_a_batch(unpk, rng):
+    # 9 * 4096 + 5 rows: each of the 4096 row blocks holds 10 rows, so the
+    # 4-bit LDS form (4 spectra per batch) runs two full batches and a partial
+    # one, and the 1-bit form walks 10 rows per lane; the last blocks are
+    # short or empty.  Full, gathered (repeats, descending) and edge channel sets.
+    nrows, nchan = 9 * 4096 + 5, 256
+    raw4 = rng.integers(0, 256, size=(nrows + 1, 2 * nchan), dtype=np.uint8)
+    gathered = rng.choice(nchan, 90, replace=True)
+    for ch in (np.arange(0, 256), gathered, np.arange(255, -1, -2), np.array([0, 255])):
+        print("[gpu-case] unpack_4bit rows %d cols %d" % (nrows, len(ch)), flush=True)
+        p0, p1 = unpk.unpack_4bit(raw4, 1, nrows + 1, ch, nchan)
+        e0, e1 = ounpack.unpack_4bit(raw4, 1, nrows + 1, ch, nchan)
+        assert np.array_equal(p0, e0) and np.array_equal(p1, e1), len(ch)
+    raw1 = rng.integers(0, 256, size=(nrows + 1, nchan // 2), dtype=np.uint8)
+    for cs, ce in ((0, 256), (6, 200)):
+        print("[gpu-case] unpack_1bit rows %d channels %d..%d" % (nrows, cs, ce), flush=True)
+        p0, p1 = unpk.unpack_1bit(raw1, 1, nrows + 1, np.arange(cs, ce), nchan)
+        e0, e1 = ounpack.unpack_1bit(raw1, 1, nrows + 1, cs, ce, nchan)
+        assert np.array_equal(p0, e0) and np.array_equal(p1, e1), (cs, ce)
+
+
 def test_dropped_packets_more_than_65536_packets(rng):
     # unpacking_gpu.cu:24 takes any num_packets: 70 000 packets, jumps after
     # packets 3, 65 540 and 69 998 (the last pair)
