@@ -163,3 +163,42 @@ def hist(data, rowstart, rowend, length_channels, bit_depth, mode):
     if isnp or _backend == "numpy":
         return _lib.host(h).astype(np.uint64)
     return h
+
+
+def dropped_packets(data, spec_num, spectra_per_packet, nchan, bit_depth):
+    """unpacking_gpu.cu:24-72 ``dropped_packets``: zero every byte of packet i
+    when ``spec_num[i+1] - spec_num[i] != spectra_per_packet`` (i < num_packets
+    - 1, num_packets = len(spec_num)) and return how many packets that is.
+
+    ``data`` holds ``num_packets`` packets of ``2*nchan*spectra_per_packet``
+    (4-bit), ``nchan*spectra_per_packet`` (2-bit) or
+    ``nchan*spectra_per_packet/2`` (1-bit) bytes and is changed in place: a
+    numpy array as the reference's host buffer (copied to the device and
+    back), a device tensor on the device.  Like the reference, fewer than two
+    packets return 0 after a message, and another bit depth returns 0."""
+    t = _lib.torch()
+    spec = np.asarray(spec_num, dtype=np.int64) if not isinstance(spec_num, t.Tensor) else spec_num
+    npk = int(spec.shape[0])
+    if npk < 2:
+        print("There are fewer than 2 packets in the dropped packets function")
+        return 0
+    if bit_depth not in (1, 2, 4):
+        return 0
+    spp, nchan = int(spectra_per_packet), int(nchan)
+    bpp = {4: 2 * nchan * spp, 2: nchan * spp, 1: nchan * spp // 2}[bit_depth]
+    isnp = _lib.is_numpy(data)
+    if isnp and not (data.flags.c_contiguous and data.dtype == np.uint8):
+        raise ValueError("dropped_packets: data must be a C-contiguous uint8 array")
+    d = _lib.to_device(data, t.uint8)
+    if not isnp and d.data_ptr() != data.data_ptr():
+        raise ValueError("dropped_packets: data must be a contiguous uint8 device tensor (it is changed in place)")
+    if d.numel() < npk * bpp:
+        raise ValueError("dropped_packets: %d packets of %d bytes need %d bytes of data, the array holds %d"
+                         % (npk, bpp, npk * bpp, d.numel()))
+    s = _lib.to_device(spec, t.int64)
+    nd = t.empty(1, dtype=t.int64, device=d.device)
+    _lib.call("alb_dropped_packets", _lib.ptr(d), _lib.ptr(s), npk, spp, nchan, int(bit_depth), _lib.ptr(nd),
+              _lib.stream())
+    if isnp:
+        data.reshape(-1)[:] = _lib.host(d).reshape(-1)
+    return int(nd.item())

@@ -171,6 +171,19 @@ def test_dropped_packets(rng):
     exp = data.copy()
     exp[2] = 0
     assert np.array_equal(out, exp) and int(nd.item()) == 1
+    # the Python mirror: host array changed in place, device tensor in place
+    from albatros_axion_detection_amd import unpacking
+    h = data.copy()
+    assert unpacking.dropped_packets(h, spec, spp, nchan, 4) == 1 and np.array_equal(h, exp)
+    dd = _lib.to_device(data)
+    assert unpacking.dropped_packets(dd, s, spp, nchan, 4) == 1 and np.array_equal(_lib.host(dd), exp)
+    h1 = rng.integers(1, 256, size=(npk, nchan * spp // 2 + 1), dtype=np.uint8)[:, :nchan * spp // 2].copy()
+    exp1 = h1.copy()
+    exp1[2] = 0
+    assert unpacking.dropped_packets(h1, spec, spp, 2 * (nchan // 2), 1) == 1 and np.array_equal(h1, exp1)
+    assert unpacking.dropped_packets(h, spec[:1], spp, nchan, 4) == 0
+    with pytest.raises(ValueError):
+        unpacking.dropped_packets(data[:3].copy(), spec, spp, nchan, 4)
 
 
 @pytest.mark.gpu
