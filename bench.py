@@ -456,8 +456,38 @@ def secondary(args, torch):
                       "mfma_frac": round(nprod * flops / (ms * 1e-3) / 1e12 / peaks[name], 4)}
     out["xcorr_precision"] = {"workload": "8 inputs x 1024 x 6400 ch complex64 (F-order), full 8x8 per channel; "
                                           "error = max |C - C64| / sqrt(C64_ii C64_jj)", "forms": prec}
+    if not args.no_cpu:
+        log("[bench] CPU port: all-antenna correlator ...")
+        out["xcorr_precision"]["cpu_baseline"] = cpu_port_xcorr(x.cpu().numpy(), args.cpu_budget)
     del x, xd, vis, ref, norm
     return out
+
+
+def cpu_port_xcorr(x, budget_s):
+    """The reference's CPU correlator (avg_xcorr_all_ant_cpu,
+    correlations.py:115-151: upper-triangle loop, complex64 products summed in
+    complex128, parallel over channels) restated in oracle.cpu_port, on the
+    whole (M, K, F) workload: all cores (median of up to 5 runs within the
+    budget) and one core (one run)."""
+    from oracle import cpu_port
+    model, cores = cpu_info()
+    M, K, F = x.shape
+    cpu_port.avg_xcorr_all_ant(x, cores)
+    times = []
+    t_all = time.perf_counter()
+    while len(times) < 5 and (time.perf_counter() - t_all) < budget_s:
+        t0 = time.perf_counter()
+        cpu_port.avg_xcorr_all_ant(x, cores)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    t1c0 = time.perf_counter()
+    cpu_port.avg_xcorr_all_ant(x, 1)
+    t1 = time.perf_counter() - t1c0
+    nsamp = M * K * F
+    return {"value": round(nsamp / t / 1e6, 2), "unit": "Msamples/s", "cores": cores, "kind": "port",
+            "cpu_model": model, "one_core_value": round(nsamp / t1 / 1e6, 2),
+            "sample": "the whole workload (%d inputs x %d spectra x %d channels), numpy threads over channel "
+                      "chunks; median of %d runs (one-core figure: one thread, one run)" % (M, K, F, len(times))}
 
 
 def cpu_info():

@@ -58,3 +58,31 @@ def rechannelize(chunks_c64, filt, win, acclen, cut, osamp, c0, c1, workers=None
         parts.append(cpu_ipfb(tile, filt, cut, workers))
     ts = np.concatenate(parts)
     return cpu_pfb(ts, win, 2048 * osamp + 1, 4, c0, c1, workers)
+
+
+def avg_xcorr_all_ant(x, workers=None):
+    """correlations.py:115-151 ``avg_xcorr_all_ant_cpu``: for every channel f
+    and pair i <= j, the complex64 products ``x[i,k,f] * conj(x[j,k,f])``
+    summed over k in complex128, divided by K and stored as complex64, the
+    lower triangle the conjugate.  ``x`` is ``(M, K, F)`` complex64; returns
+    ``(M, M, F)`` complex64.  The Numba ``prange`` over channels becomes
+    channel chunks on a thread pool (numpy releases the GIL in its loops);
+    each chunk runs the step-by-step arithmetic of
+    ``oracle.corr.avg_xcorr_all_ant_c64_products``."""
+    from concurrent.futures import ThreadPoolExecutor
+    from . import corr
+    x = np.asarray(x, dtype=np.complex64)
+    M, K, F = x.shape
+    workers = int(workers or ncores())
+    out = np.empty((M, M, F), dtype=np.complex64)
+    nchunk = max(1, min(F, 4 * workers))
+    edges = np.linspace(0, F, nchunk + 1).astype(np.int64)
+
+    def one(c):
+        f0, f1 = int(edges[c]), int(edges[c + 1])
+        if f1 > f0:
+            out[:, :, f0:f1] = corr.avg_xcorr_all_ant_c64_products(x[:, :, f0:f1], K)
+
+    with ThreadPoolExecutor(workers) as ex:
+        list(ex.map(one, range(nchunk)))
+    return out

@@ -471,6 +471,12 @@ def test_complex64_product_oracle():
     assert np.array_equal(a, np.conj(a.transpose(1, 0, 2)))
     # the products are rounded: not bit-identical to complex128 products
     assert not np.array_equal(a, b.astype(np.complex64))
+    # the timed CPU port (channel chunks on threads) is the same arithmetic,
+    # for any chunking; F-order input as the correlator takes it
+    from oracle import cpu_port
+    yf = np.asfortranarray(y)
+    for workers in (1, 3):
+        assert np.array_equal(cpu_port.avg_xcorr_all_ant(yf, workers), a)
 
 
 def test_orbcomm_coarse_xcorr_equals_direct_lag_sums():
