@@ -636,16 +636,20 @@ struct StageA {
 };
 
 // Row walk of the fused polyphase: sums of rows xrow + R1 (bl + j), j < ntap
-// (taps j >= ntap have weight 0), for the nb <= kWalkSeg blocks bl of one
-// lane, with the window weights of the lane's row at wl[j R1 CW] (.x and .y
-// for the two columns of the pair; U holds conj for the second).  Rows past
-// the lane's last block feed only block sums that are never stored, so every
-// read is unconditional (clamped into the sequence).  Ends with the block
-// sums written to U[pad16((bl0 + b) R1 + rb)] after a workgroup barrier.
+// (taps j >= ntap have weight 0), for the nb <= SEG blocks bl of one lane,
+// with the window weights of the lane's row at wl[j R1 CW] (.x and .y for the
+// two columns of the pair; Up holds conj for the second).  Rows past the
+// lane's last block feed only block sums that are never stored, so every
+// read is unconditional (clamped into the sequence).  MASK: rows outside
+// [lo, hi) (outside the tile's kept rows) count as zero: reads clamp at both
+// ends, and the caller has zeroed the rows of Up outside [lo, hi) (the cut
+// rows, lo >= 1), so a clamped read is a zero.  Ends with the block sums
+// written to Yp[pad16((bl0 + b) R1 + rb)] after a workgroup barrier (Yp may
+// overlap Up).
 constexpr int kWalkSeg = 16;
-template <int NSL, int R1, int CW, int NT>
-__device__ __forceinline__ void polyphase_walk(float2* Up, int xrow, const float* wl, int ntap, int nb, int bl0,
-                                               int rb) {
+template <int NSL, int R1, int CW, int NT, int SEG = kWalkSeg, bool MASK = false>
+__device__ __forceinline__ void polyphase_walk(const float2* Up, float2* Yp, int xrow, const float* wl, int ntap,
+                                               int nb, int bl0, int rb, int lo = 0, int hi = NSL) {
   float w0[NT], w1[NT];
 #pragma unroll
   for (int j = 0; j < NT; ++j) {
@@ -653,17 +657,18 @@ __device__ __forceinline__ void polyphase_walk(float2* Up, int xrow, const float
     w0[j] = j < ntap ? w.x : 0.f;
     w1[j] = j < ntap ? w.y : 0.f;
   }
-  float2 acc[kWalkSeg];
+  float2 acc[SEG];
 #pragma unroll
-  for (int b = 0; b < kWalkSeg; ++b) acc[b] = make_float2(0.f, 0.f);
+  for (int b = 0; b < SEG; ++b) acc[b] = make_float2(0.f, 0.f);
 #pragma unroll
-  for (int m = 0; m < kWalkSeg + NT - 1; ++m) {
-    int x = xrow + R1 * m;
-    x = x < NSL - 1 ? x : NSL - 1;
-    const float2 v = Up[pad16(x)];
+  for (int m = 0; m < SEG + NT - 1; ++m) {
+    const int x = xrow + R1 * m;
+    int xc = x < NSL - 1 ? x : NSL - 1;
+    if constexpr (MASK) xc = xc > 0 ? xc : 0;
+    const float2 v = Up[pad16(xc)];
 #pragma unroll
     for (int j = 0; j < NT; ++j) {
-      if (m - j >= 0 && m - j < kWalkSeg) {
+      if (m - j >= 0 && m - j < SEG) {
         acc[m - j].x = fmaf(v.x, w0[j], acc[m - j].x);
         acc[m - j].y = fmaf(-v.y, w1[j], acc[m - j].y);
       }
@@ -671,9 +676,31 @@ __device__ __forceinline__ void polyphase_walk(float2* Up, int xrow, const float
   }
   __syncthreads();  // every lane has read its rows: U becomes Y
 #pragma unroll
-  for (int b = 0; b < kWalkSeg; ++b)
-    if (b < nb) Up[pad16((bl0 + b) * R1 + rb)] = acc[b];
+  for (int b = 0; b < SEG; ++b)
+    if (b < nb) Yp[pad16((bl0 + b) * R1 + rb)] = acc[b];
 }
+
+// Split form of the fused column pass (tile-boundary blocks in two halves):
+// each tile computes re-PFB stage 1 of every block that touches its kept rows,
+// over those rows only (the others count as zero).  The tile that holds a
+// block's first row writes the result to the workspace A; the next tile
+// writes its half to the boundary array P, slot
+// ntap * (that tile's index in the stream window - 1) + (b - first block that
+// straddles its first row); stage 2 adds the two halves.  Blocks span at most
+// two tiles (R1 ntap <= kept rows).  Up to 4 taps: the walk segments hold
+// kWalkSegSplit blocks and the block sums (Y) and the DFT exchange live in an
+// area of SplitArea<NSL, R1, NP>::value complex entries.
+constexpr int kWalkSegSplit = 17;
+template <int NSL, int R1, int NP>
+struct SplitArea {
+  static constexpr int kBlocks = NSL / R1 + 4;                        // blocks touching one tile, at most
+  static constexpr int kYStride = Padded<kBlocks * R1>::value;        // Y row of one column pair
+  static constexpr int kSP = R1 / 4 >= 16 ? R1 / 4 + 1 : R1 / 4;      // exchange row (as the column pass)
+  static constexpr int kExch = kBlocks * NP * 4 * kSP;
+  static constexpr int kFFT = NP * Padded<NSL>::value;
+  static constexpr int a = kFFT > NP * kYStride ? kFFT : NP * kYStride;
+  static constexpr int value = a > kExch ? a : kExch;
+};
 
 // ------------------------------------ iPFB column pass fused with re-PFB stage 1
 // Stream row g = row0 + tile * rows_per_tile + (i - cut) holds kept row i of a
@@ -700,14 +727,16 @@ __host__ __device__ inline void repfb_interior(int64_t g0, int64_t nkeep, int64_
 // Y1[q] = (Z[q] - conj Z[-q]) / 2i.
 // One tile of the fused column pass: column pass of strip k0 of tile `tile`,
 // stream rows for the tile-boundary blocks, stage 1 of the interior blocks.
-template <int NSL, int CW, int T, int R1, bool SPEC = false>
+// SPLIT: the split form above (P, tile0 = this launch's first tile's index
+// in the stream window; ts unused).
+template <int NSL, int CW, int T, int R1, bool SPEC = false, bool SPLIT = false>
 __device__ __forceinline__ void cols_repfb_tile(const float* __restrict__ y, int64_t ldy, int64_t tile_stride,
                                                 const float2* __restrict__ filt, int64_t ldf, int cut,
                                                 float* __restrict__ ts, int64_t row0, int64_t rows_per_tile,
                                                 int ntap, int64_t nblocks, float2* __restrict__ A, int64_t tile,
                                                 int64_t k0, float2* U, const float2* TA, const float2* TB,
                                                 const float2* W16, const float2* WR, const float* WIN, int lb,
-                                                int tid) {
+                                                int tid, float2* __restrict__ P = nullptr, int64_t tile0 = 0) {
   constexpr int NP = CW / 2;
   constexpr int PADN = Padded<NSL>::value;
   constexpr int S = R1 / 4;
@@ -723,11 +752,21 @@ __device__ __forceinline__ void cols_repfb_tile(const float* __restrict__ y, int
   const int nkeep = NSL - 2 * cut;
   const int64_t g0 = row0 + tile * rows_per_tile;
   int64_t bi0, bi1;
-  repfb_interior(g0, nkeep, R1, ntap, nblocks, &bi0, &bi1);
+  // first block straddling this tile's first row (split form)
+  const int64_t e0 = floor_div(g0, R1) - ntap + 1;
+  if constexpr (SPLIT) {
+    // every block that touches the kept rows [g0, g0 + nkeep)
+    bi0 = e0 > 0 ? e0 : 0;
+    bi1 = floor_div(g0 + nkeep - 1, R1) + 1;
+    if (bi1 > nblocks) bi1 = nblocks;
+    if (bi1 < bi0) bi1 = bi0;
+  } else {
+    repfb_interior(g0, nkeep, R1, ntap, nblocks, &bi0, &bi1);
+  }
   const int nint = int(bi1 - bi0);
   // Rows read by blocks outside [bi0, bi1) go to the stream buffer: local
   // rows [0, n1) and [n2, nkeep) (all rows when the two ranges meet).
-  {
+  if constexpr (!SPLIT) {
     int n1 = nkeep, n2 = nkeep;
     if (nint > 0) {
       n1 = int(R1 * (bi0 + ntap - 1) - g0);
@@ -755,16 +794,32 @@ __device__ __forceinline__ void cols_repfb_tile(const float* __restrict__ y, int
   // j = 0..ntap-1, as in the stage-1 kernels.
   constexpr int NSEG = T / (R1 * NP);        // segments per (rb, p)
   static_assert(NSEG * R1 * NP == T, "one walk lane per thread");
+  // Y (block sums) row stride of one column pair: the FFT rows' in the
+  // reference form, the longer split-area row in the split form
+  constexpr int YS = SPLIT ? SplitArea<NSL, R1, NP>::kYStride : PADN;
+  if constexpr (SPLIT) {
+    // the cut rows (outside the kept rows) read as zero in the walk
+    if (tid < 2 * cut * NP) {
+      const int p = tid / (2 * cut), c = tid % (2 * cut);
+      U[p * PADN + pad16(c < cut ? c : NSL - 2 * cut + c)] = make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+  }
   {
-    const int SL = (nint + NSEG - 1) / NSEG;  // <= kWalkSeg (checked by the host)
+    const int SL = (nint + NSEG - 1) / NSEG;  // <= the walk's segment (checked by the host)
     const int rb = tid % R1, pw = (tid / R1) % NP, seg = tid / (R1 * NP);
     const int bl0 = seg * SL;
     int nb = nint - bl0;
     nb = nb < 0 ? 0 : (nb > SL ? SL : nb);
     const int xrow = int(R1 * (bi0 + bl0) - g0) + cut + rb;   // local row of block row m = 0
     const float* wl = WIN + rb * CW + 2 * pw;
-    if (ntap <= 4) polyphase_walk<NSL, R1, CW, 4>(U + pw * PADN, xrow, wl, ntap, nb, bl0, rb);
-    else polyphase_walk<NSL, R1, CW, 8>(U + pw * PADN, xrow, wl, ntap, nb, bl0, rb);
+    if constexpr (SPLIT) {
+      polyphase_walk<NSL, R1, CW, 4, kWalkSegSplit, true>(U + pw * PADN, U + pw * YS, xrow, wl, ntap, nb, bl0, rb, cut,
+                                                          cut + nkeep);
+    } else {
+      if (ntap <= 4) polyphase_walk<NSL, R1, CW, 4>(U + pw * PADN, U + pw * PADN, xrow, wl, ntap, nb, bl0, rb);
+      else polyphase_walk<NSL, R1, CW, 8>(U + pw * PADN, U + pw * PADN, xrow, wl, ntap, nb, bl0, rb);
+    }
   }
   __syncthreads();
 
@@ -779,7 +834,7 @@ __device__ __forceinline__ void cols_repfb_tile(const float* __restrict__ y, int
     if (u < nlanes) {
       const int t = u & 3, task = u >> 2;
       const int p = task / nint, bl = task - p * nint;
-      const float2* Yp = U + p * PADN + bl * (R1 + R1 / 16) + t;   // pad16(bl R1 + 4 s + t), R1 a multiple of 16
+      const float2* Yp = U + p * YS + bl * (R1 + R1 / 16) + t;   // pad16(bl R1 + 4 s + t), R1 a multiple of 16
 #pragma unroll
       for (int sI = 0; sI < S; ++sI) z[e][sI] = Yp[4 * sI + sI / 4];
       DFT<S>::run(z[e]);
@@ -851,8 +906,14 @@ __device__ __forceinline__ void cols_repfb_tile(const float* __restrict__ y, int
       // tile (and, in the batched pass, every input): 6% faster column pass
       // on the batched workload, stage 2 unchanged
       typedef float f4v __attribute__((ext_vector_type(4)));
-      __builtin_nontemporal_store(f4v{o.x, o.y, o.z, o.w},
-                                  reinterpret_cast<f4v*>(A + StageA<R1>::at(bi0 + bl, q, k0 + 2 * p)));
+      float2* dst = A + StageA<R1>::at(bi0 + bl, q, k0 + 2 * p);
+      if constexpr (SPLIT) {
+        // a block whose first row lies before this tile: its second half
+        const int64_t b = bi0 + bl;
+        if (R1 * b < g0)
+          dst = P + (ntap * (tile0 + tile - 1) + (b - e0)) * StageA<R1>::kBlock + StageA<R1>::at(0, q, k0 + 2 * p);
+      }
+      __builtin_nontemporal_store(f4v{o.x, o.y, o.z, o.w}, reinterpret_cast<f4v*>(dst));
       q += QA;
       bl += BA;
       if (q >= H) { q -= H; ++bl; }
@@ -862,23 +923,25 @@ __device__ __forceinline__ void cols_repfb_tile(const float* __restrict__ y, int
 
 // SPEC: y is the spectral-order strip array of ipfb_rows_spectral_kernel
 // (complex [tiles][4096/4][nslice/2+1][4]; ldy and tile_stride unused).
-template <int NSL, int CW, int T, int R1, bool SPEC = false>
+// SPLIT: the split form of cols_repfb_tile (P, tile0).
+template <int NSL, int CW, int T, int R1, bool SPEC = false, bool SPLIT = false>
 __global__ __launch_bounds__(T) void ipfb_cols_repfb_kernel(const float* __restrict__ y, int64_t ldy,
                                                                int64_t tile_stride, const float2* __restrict__ filt,
                                                                int64_t ldf, int cut, float* __restrict__ ts,
                                                                int64_t row0, int64_t rows_per_tile,
                                                                const float* __restrict__ win, int ntap,
                                                                int64_t nblocks, float2* __restrict__ A, int group,
-                                                               Twiddle tw) {
+                                                               Twiddle tw, float2* __restrict__ P, int64_t tile0) {
   constexpr int NP = CW / 2;
   constexpr int PADN = Padded<NSL>::value;
   constexpr int S = R1 / 4;
   constexpr int H = R1 / 2;
   constexpr int SP = S >= 16 ? S + 1 : S;   // row of the exchange buffer (padded when it fits)
   constexpr int64_t C = 4096;
+  constexpr int UA = SPLIT ? SplitArea<NSL, R1, NP>::value : NP * PADN;   // transform / stage-1 area
   extern __shared__ float2 smem[];
   float2* U = smem;
-  float2* TA = U + NP * PADN;
+  float2* TA = U + UA;
   float2* TB = TA + tw.nA;
   float2* W16 = TB + tw.nB;                                 // W_256^(m r), fill_w16
   float2* WR = W16 + kW16Size;                                   // W_R1^m, m < R1
@@ -898,8 +961,8 @@ __global__ __launch_bounds__(T) void ipfb_cols_repfb_kernel(const float* __restr
     const int r = idx / CW, c = idx - r * CW;
     WIN[idx] = win[int64_t(r) * C + k0 + c];
   }
-  cols_repfb_tile<NSL, CW, T, R1, SPEC>(y, ldy, tile_stride, filt, ldf, cut, ts, row0, rows_per_tile, ntap, nblocks, A, tile,
-                                  k0, U, TA, TB, W16, WR, WIN, tw.lb, tid);
+  cols_repfb_tile<NSL, CW, T, R1, SPEC, SPLIT>(y, ldy, tile_stride, filt, ldf, cut, ts, row0, rows_per_tile, ntap,
+                                               nblocks, A, tile, k0, U, TA, TB, W16, WR, WIN, tw.lb, tid, P, tile0);
 }
 
 // ------------------------------------------- spectral-order inverse PFB
@@ -1413,11 +1476,38 @@ __device__ __forceinline__ void q2_range(int64_t q1, int64_t R1, int64_t c_lo, i
   *hi = z < 4095 ? z : 4095;
 }
 
-template <int R1>
+// Boundary halves of the split column pass, read by stage 2: block b of the
+// stream window whose rows straddle the boundary between the tile holding its
+// first row (window tile tr) and the next tile of the window also takes slot
+// ntap tr + (b - first block straddling that boundary) of P (input m at
+// P + m p_stride), added to its A before the transform.
+struct SplitRead {
+  const float2* P;     // nullptr: A alone
+  int64_t p_stride;    // complex elements per input
+  int64_t g_first;     // window row of the window's first tile's first kept row (<= 0)
+  int64_t rows;        // rows per tile
+  int64_t ntiles;      // tiles of the window
+  int ntap;
+};
+// 32-bit unsigned arithmetic: block rows and window rows are >= 0 and below
+// 2^31 (checked by the host), and the next tile's first row bt is > 0.
+__host__ __device__ inline int64_t split_slot(const SplitRead& sp, int64_t b, int64_t R1) {
+  const uint32_t r0 = uint32_t(R1 * b);
+  const uint32_t off = uint32_t(r0 - sp.g_first);             // row of the block's first row from tile 0's first row
+  const uint32_t tr = off / uint32_t(sp.rows);                 // tile holding the block's first row
+  if (int64_t(tr) + 1 >= sp.ntiles) return -1;                // no next tile: rows past the window's tiles are zero
+  const uint32_t bt = uint32_t(sp.g_first + (int64_t(tr) + 1) * sp.rows);   // the next tile's first row
+  if (r0 + uint32_t(R1 * sp.ntap) <= bt) return -1;            // the block lies in its first tile
+  return int64_t(sp.ntap) * tr + (b - (int64_t(bt / uint32_t(R1)) - sp.ntap + 1));
+}
+
+// SPLIT: add the tile-boundary halves of sp (a separate instantiation, so
+// the plain form keeps its register budget).
+template <int R1, bool SPLIT = false>
 __global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __restrict__ A, int64_t a_stride, int ninput,
                                                             int64_t nblock, int64_t c_lo, int64_t c_hi,
                                                             float2* __restrict__ out, int64_t sm, int64_t sb, int64_t sc,
-                                                            int64_t bpj, int64_t sj, Twiddle tw4) {
+                                                            int64_t bpj, int64_t sj, Twiddle tw4, SplitRead sp) {
   constexpr int N = 4096;
   constexpr int H = R1 / 2;
   constexpr int NPAD = Padded<N>::value;
@@ -1451,6 +1541,17 @@ __global__ __launch_bounds__(kTPB) void repfb_stage2_kernel(const float2* __rest
   float2 v[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) v[r] = src[r * (64 * StageA<R1>::QP * 4)];
+  if constexpr (SPLIT) {   // split column pass: add the block's second half (uniform per workgroup)
+    const int64_t e = split_slot(sp, b, R1);
+    if (e >= 0) {
+      const float2* ps = sp.P + m_in * sp.p_stride + e * StageA<R1>::kBlock + StageA<R1>::at(0, slot, tid);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const float2 w = ps[r * (64 * StageA<R1>::QP * 4)];
+        v[r] = make_float2(v[r].x + w.x, v[r].y + w.y);
+      }
+    }
+  }
   // W_L^(q1 k) = W_L^(q1 c) W_4096^(q1 a) for k = R1 a + c; c = tid mod R1 is
   // the same for all of this thread's k (kTPB is a multiple of R1).  The
   // float64 sincospi here measured 3-5% faster than reading the same value
@@ -1820,7 +1921,7 @@ int launch_repfb_stage1_list(const float* ts, const float* win, int ntap, const 
 
 // Shape limits of the fused kernel: lane slots (2 per thread), the exchange
 // buffer (reuses the strip's LDS) and the window strip.
-template <int NSL, int CW, int T, int R1>
+template <int NSL, int CW, int T, int R1, bool SPLIT = false>
 int check_cols_repfb(int cut, int ntap) {
   constexpr int NP = CW / 2, S = R1 / 4;
   constexpr int SP = S >= 16 ? S + 1 : S;
@@ -1829,6 +1930,19 @@ int check_cols_repfb(int cut, int ntap) {
   if (ntap < 1 || ntap > 8) {
     set_error("fused iPFB/re-PFB: ntap must be 1..8");
     return ALB_ENOTSUP;
+  }
+  if constexpr (SPLIT) {
+    // blocks touching one tile: at most ceil((nkeep - 1) / R1) + ntap (the split area holds NSL / R1 + 4)
+    const int64_t nblk_max = (nkeep + R1 - 2) / R1 + ntap;
+    if (ntap > 4 || cut < 1 || 2 * cut * NP > T || nkeep < int64_t(R1) * ntap ||
+        nblk_max > int64_t(T / (R1 * NP)) * kWalkSegSplit ||
+        nblk_max * NP * 4 > 2 * T || nblk_max > SplitArea<NSL, R1, NP>::kBlocks) {
+      set_error("fused iPFB/re-PFB split form: needs ntap <= 4, 1 <= cut <= %d and at most %d blocks per tile at osamp %d",
+                T / (2 * NP),
+                int(T / (R1 * NP)) * kWalkSegSplit, R1);
+      return ALB_ENOTSUP;
+    }
+    return ALB_OK;
   }
   // walk segments (T / (R1 NP) per row, at most 16 blocks each) and DFT lanes
   if (nint_max > int64_t(T / (R1 * NP)) * kWalkSeg || nint_max * NP * 4 > 2 * T ||
@@ -1839,24 +1953,28 @@ int check_cols_repfb(int cut, int ntap) {
   return ALB_OK;
 }
 
-template <int NSL, int CW, int T, int R1, bool SPEC = false>
+// SPLIT: the split form (P, tile0; ts unused).
+template <int NSL, int CW, int T, int R1, bool SPEC = false, bool SPLIT = false>
 int launch_cols_repfb(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, const float2* filt, int64_t ldf,
                       int cut, float* ts, int64_t row0, int64_t rows_per_tile, const float* win, int ntap,
-                      int64_t nblocks, float2* A, hipStream_t s) {
+                      int64_t nblocks, float2* A, hipStream_t s, float2* P = nullptr, int64_t tile0 = 0) {
   constexpr int NP = CW / 2;
-  int rc = check_cols_repfb<NSL, CW, T, R1>(cut, ntap);
+  constexpr int UA = SPLIT ? SplitArea<NSL, R1, NP>::value : NP * Padded<NSL>::value;
+  int rc = check_cols_repfb<NSL, CW, T, R1, SPLIT>(cut, ntap);
   if (rc || ntiles == 0) return rc;
-  ALB_REQUIRE(y && filt && ts && win && A, "alb_ipfb_columns_repfb: null pointer");
+  ALB_REQUIRE(y && filt && win && A && (SPLIT ? P != nullptr : ts != nullptr), "alb_ipfb_columns_repfb: null pointer");
   ALB_REQUIRE((4096 / CW) * ntiles < (int64_t(1) << 31), "fused iPFB/re-PFB: too many tiles");
+  ALB_REQUIRE(!SPLIT || (tile0 >= 0 && rows_per_tile == NSL - 2 * int64_t(cut)),
+              "fused iPFB/re-PFB split form: consecutive tiles must be nslice - 2 cut rows apart and tile0 >= 0");
   Twiddle tw;
   rc = get_twiddle(NSL, &tw, s);
   if (rc) return rc;
-  const size_t lds = sizeof(float2) * (size_t(NP) * Padded<NSL>::value + tw.nA + tw.nB + kW16Size + R1) +
+  const size_t lds = sizeof(float2) * (size_t(UA) + tw.nA + tw.nB + kW16Size + R1) +
                      sizeof(float) * size_t(ntap) * R1 * CW;
   ALB_REQUIRE(lds <= 160 * 1024, "fused iPFB/re-PFB: LDS budget exceeded");
-  hipLaunchKernelGGL((ipfb_cols_repfb_kernel<NSL, CW, T, R1, SPEC>), dim3(unsigned((4096 / CW) * ntiles)), dim3(T), lds, s, y,
-                     ldy, tile_stride, filt, ldf, cut, ts, row0, rows_per_tile, win, ntap, nblocks, A, cols_group(ntiles),
-                     tw);
+  hipLaunchKernelGGL((ipfb_cols_repfb_kernel<NSL, CW, T, R1, SPEC, SPLIT>), dim3(unsigned((4096 / CW) * ntiles)), dim3(T),
+                     lds, s, y, ldy, tile_stride, filt, ldf, cut, ts, row0, rows_per_tile, win, ntap, nblocks, A,
+                     cols_group(ntiles), tw, P, tile0);
   ALB_LAUNCH_CHECK("ipfb_cols_repfb_kernel");
   return ALB_OK;
 }
@@ -1911,15 +2029,20 @@ int launch_rows_spectral(const float2* E, const int* chan, int64_t ncols, int64_
 
 template <int R1>
 int launch_repfb_stage2(const float2* A, int64_t a_stride, int64_t ninput, int64_t nblock, int64_t c_lo, int64_t c_hi,
-                        float2* out, int64_t sm, int64_t sb, int64_t sc, int64_t bpj, int64_t sj, hipStream_t s) {
+                        float2* out, int64_t sm, int64_t sb, int64_t sc, int64_t bpj, int64_t sj, hipStream_t s,
+                        SplitRead sp = SplitRead{nullptr, 0, 0, 1, 0, 1}) {
   Twiddle tw4;
   int rc = get_twiddle(4096, &tw4, s);
   if (rc) return rc;
   const size_t lds2 = sizeof(float2) * (size_t(Padded<4096>::value) + tw4.nA + tw4.nB + kW16Size);
   const int64_t total = int64_t(R1 / 2 + 1) * nblock * ninput;
   ALB_REQUIRE(total < (int64_t(1) << 31) - 8, "alb_repfb: too many blocks");
-  hipLaunchKernelGGL((repfb_stage2_kernel<R1>), dim3(unsigned((total + 7) / 8 * 8)), dim3(kTPB), lds2, s, A, a_stride,
-                     int(ninput), nblock, c_lo, c_hi, out, sm, sb, sc, bpj, sj, tw4);
+  if (sp.P != nullptr)
+    hipLaunchKernelGGL((repfb_stage2_kernel<R1, true>), dim3(unsigned((total + 7) / 8 * 8)), dim3(kTPB), lds2, s, A,
+                       a_stride, int(ninput), nblock, c_lo, c_hi, out, sm, sb, sc, bpj, sj, tw4, sp);
+  else
+    hipLaunchKernelGGL((repfb_stage2_kernel<R1, false>), dim3(unsigned((total + 7) / 8 * 8)), dim3(kTPB), lds2, s, A,
+                       a_stride, int(ninput), nblock, c_lo, c_hi, out, sm, sb, sc, bpj, sj, tw4, sp);
   ALB_LAUNCH_CHECK("repfb_stage2_kernel");
   return ALB_OK;
 }
@@ -2241,6 +2364,97 @@ int alb_ipfb_columns_repfb_spectral(const void* D, int64_t ntiles, int64_t nslic
             (long long)nslice, (long long)osamp);
   return ALB_ENOTSUP;
 }
+
+// Split form of alb_ipfb_columns_repfb (nslice 4096, osamp 64, ntap <= 4,
+// strip width 4): stage 1 of every block touching a tile's kept rows, over
+// those rows; a block's first tile writes A, the next tile writes its half to
+// P slot ntap * (tile0 + t - 1) + (b - first block straddling tile t's first
+// row), where t is the tile within this launch and tile0 this launch's first
+// tile's index in the stream window (tiles are rows_per_tile = nslice - 2 cut
+// rows apart).  P holds p_slots slots of alb_repfb_workspace_bytes(1, osamp)
+// bytes.  No stream rows are written.
+static int check_split_slots(int64_t ntiles, int64_t tile0, int ntap, int64_t p_slots) {
+  ALB_REQUIRE(ntiles == 0 || (tile0 >= 0 && ntap >= 1 && int64_t(ntap) * (tile0 + ntiles - 1) <= p_slots),
+              "split column pass: P holds %lld slots, tiles %lld..%lld need %lld", (long long)p_slots,
+              (long long)tile0, (long long)(tile0 + ntiles - 1), (long long)(int64_t(ntap) * (tile0 + ntiles - 1)));
+  return ALB_OK;
+}
+
+int alb_ipfb_columns_repfb_split(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, int64_t nslice,
+                                 const void* filt, int64_t ldf, int cut, int64_t row0, int64_t rows_per_tile,
+                                 const float* win, int ntap, int64_t osamp, int64_t nblocks, void* A, void* P,
+                                 int64_t tile0, int64_t p_slots, void* stream) {
+  ALB_REQUIRE(ntiles >= 0 && nblocks >= 0 && rows_per_tile >= 0 && row0 + ntiles * rows_per_tile >= 0,
+              "alb_ipfb_columns_repfb_split: bad sizes");
+  ALB_REQUIRE(cut >= 0 && 2 * cut < nslice, "alb_ipfb_columns_repfb_split: bad cut");
+  ALB_REQUIRE(ntiles == 0 || ((ldy >= 4096 || ldy == 8) && ldf >= nslice / 2 + 1),
+              "alb_ipfb_columns_repfb_split: bad strides (ldy >= 4096 row-major, or 8 for the blocked tile layout)");
+  if (nslice != 4096 || osamp != 64) {
+    set_error("alb_ipfb_columns_repfb_split: nslice %lld with osamp %lld not supported (nslice 4096, osamp 64)",
+              (long long)nslice, (long long)osamp);
+    return ALB_ENOTSUP;
+  }
+  int rc = check_split_slots(ntiles, tile0, ntap, p_slots);
+  if (rc) return rc;
+  return launch_cols_repfb<4096, 4, 512, 64, false, true>(y, ldy, tile_stride, ntiles, (const float2*)filt, ldf, cut,
+                                                          nullptr, row0, rows_per_tile, win, ntap, nblocks, (float2*)A,
+                                                          as_stream(stream), (float2*)P, tile0);
+}
+
+// The same on the spectral-order strips of alb_ipfb_rows_spectral.
+int alb_ipfb_columns_repfb_spectral_split(const void* D, int64_t ntiles, int64_t nslice, const void* filt, int64_t ldf,
+                                          int cut, int64_t row0, int64_t rows_per_tile, const float* win, int ntap,
+                                          int64_t osamp, int64_t nblocks, void* A, void* P, int64_t tile0,
+                                          int64_t p_slots, void* stream) {
+  ALB_REQUIRE(ntiles >= 0 && nblocks >= 0 && rows_per_tile >= 0 && row0 + ntiles * rows_per_tile >= 0,
+              "alb_ipfb_columns_repfb_spectral_split: bad sizes");
+  ALB_REQUIRE(cut >= 0 && 2 * cut < nslice, "alb_ipfb_columns_repfb_spectral_split: bad cut");
+  ALB_REQUIRE(ntiles == 0 || ldf >= nslice / 2 + 1, "alb_ipfb_columns_repfb_spectral_split: bad filter stride");
+  if (nslice != 4096 || osamp != 64) {
+    set_error("alb_ipfb_columns_repfb_spectral_split: nslice %lld with osamp %lld not supported (nslice 4096, osamp 64)",
+              (long long)nslice, (long long)osamp);
+    return ALB_ENOTSUP;
+  }
+  int rc = check_split_slots(ntiles, tile0, ntap, p_slots);
+  if (rc) return rc;
+  return launch_cols_repfb<4096, 4, 512, 64, true, true>((const float*)D, 0, 0, ntiles, (const float2*)filt, ldf, cut,
+                                                         nullptr, row0, rows_per_tile, win, ntap, nblocks, (float2*)A,
+                                                         as_stream(stream), (float2*)P, tile0);
+}
+
+// Stage 2 after the split column pass: alb_repfb_stage2_inputs (osamp 64) with
+// each tile-boundary block's second half added from P (input m at P + m
+// p_stride bytes): the stream window's tiles start at window row g_first (<= 0;
+// tile c at g_first + c rows_per_tile), ntiles of them were processed.
+int alb_repfb_stage2_split(const void* A, int64_t a_stride, const void* P, int64_t p_stride, int64_t ninput,
+                           int64_t nblock, int64_t osamp, int64_t c_lo, int64_t c_hi, void* out, int64_t sm, int64_t sb,
+                           int64_t sc, int64_t bpj, int64_t sj, int64_t g_first, int64_t rows_per_tile, int64_t ntiles,
+                           int ntap, void* stream) {
+  ALB_REQUIRE(A && out && (P || ntiles <= 1), "alb_repfb_stage2_split: null pointer");
+  ALB_REQUIRE(nblock >= 0 && ninput >= 1 && ninput <= 1024 && c_lo >= 0 && c_hi > c_lo &&
+                  c_hi <= 2048 * osamp + 1 && bpj >= 1,
+              "alb_repfb_stage2_split: bad sizes");
+  ALB_REQUIRE(osamp == 64, "alb_repfb_stage2_split: osamp must be 64");
+  if (nblock == 0) return ALB_OK;   // e.g. a window of jobs past the tiles with data
+  ALB_REQUIRE(ntap >= 1 && ntap <= 4 && g_first <= 0 && rows_per_tile >= int64_t(osamp) * ntap && ntiles >= 0,
+              "alb_repfb_stage2_split: bad tile geometry");
+  const int64_t blk = alb_repfb_workspace_bytes(1, osamp);
+  ALB_REQUIRE(a_stride % int64_t(sizeof(float2)) == 0 && (ninput == 1 || a_stride >= alb_repfb_workspace_bytes(nblock, osamp)),
+              "alb_repfb_stage2_split: workspace stride must be a multiple of 8 bytes and hold nblock blocks");
+  ALB_REQUIRE(p_stride % int64_t(sizeof(float2)) == 0 && (ninput == 1 || p_stride >= blk * ntap * (ntiles > 1 ? ntiles - 1 : 0)),
+              "alb_repfb_stage2_split: boundary stride must be a multiple of 8 bytes and hold ntap (ntiles - 1) blocks");
+  // every block's reads lie inside the tiles: its first row in tile 0..ntiles-1
+  ALB_REQUIRE(nblock == 0 || ntiles == 0 || int64_t(osamp) * (nblock - 1) < g_first + ntiles * rows_per_tile,
+              "alb_repfb_stage2_split: blocks past the window's tiles");
+  ALB_REQUIRE(int64_t(osamp) * (nblock + ntap) - g_first < (int64_t(1) << 31) && g_first > -rows_per_tile,
+              "alb_repfb_stage2_split: window too long (rows below 2^31) or g_first <= -rows_per_tile");
+  const SplitRead sp{ntiles > 1 ? (const float2*)P : nullptr, p_stride / int64_t(sizeof(float2)), g_first, rows_per_tile,
+                     ntiles, ntap};
+  return launch_repfb_stage2<64>((const float2*)A, a_stride / int64_t(sizeof(float2)), ninput, nblock, c_lo, c_hi,
+                                 (float2*)out, sm, sb, sc, bpj, sj, as_stream(stream), sp);
+}
+
+int64_t alb_repfb_split_slot_bytes(int64_t osamp) { return alb_repfb_workspace_bytes(1, osamp); }
 
 // Build the transform tables of alb_repfb / alb_repfb_stage2 at this osamp
 // (1..64) on the current device now (stage 1 takes none; stage 2 the

@@ -123,12 +123,16 @@ class RePFBPipeline:
     stage 1 (off by default: on MI355X the overlapped kernels evict each
     other's working sets, 67.4 against 63.4 ms on the batched workload;
     :meth:`autotune` tries both).  ``ipfb_form`` picks the iPFB order of the
-    fused pass ("direct", the default, or "spectral"; see below).
+    fused pass ("direct", the default, or "spectral"; see below).  With
+    ``split_edges`` (the default, where the shape allows it: osamp 64, ntap
+    <= 4) :meth:`correlate_into` takes the re-PFB blocks that straddle two
+    iPFB tiles in two halves, one from each tile's column pass, added in
+    stage 2: no edge-block launch and no time-stream buffer per lane.
     """
 
     def __init__(self, ninput, chan_numbers, acclen, cut, osamp, nblock, repfb_chanstart, repfb_chanend,
                  filt_thresh=0.45, ntap=4, filt=None, win=None, tile_batch=4, nchan=2049, row_pass="auto",
-                 lanes=2, stage2_overlap=False, ipfb_form="auto"):
+                 lanes=2, stage2_overlap=False, ipfb_form="auto", split_edges=True):
         t = _lib.torch()
         self.ninput = ninput
         self.acclen = int(acclen)
@@ -146,6 +150,7 @@ class RePFBPipeline:
         # stream, tile buffer and stream buffer each)
         self.lanes = max(1, int(lanes))
         self.stage2_overlap = bool(stage2_overlap)
+        self.split_edges = bool(split_edges)
         if nchan != 2049:
             raise NotImplementedError("the fused iPFB handles 2049-channel spectra (lblock 4096)")
         chan_numbers = np.asarray(chan_numbers)
@@ -379,7 +384,7 @@ class RePFBPipeline:
         return int(min(nblocks_total, max(0, -(-int(rows_valid) // self.osamp))))
 
     def stream_stage1(self, packed, rowmap, nchunks, stream_out, nblocks_total, ws, lane=0, tiles=None, row_lo=0,
-                      skip_zero_tail=False):
+                      skip_zero_tail=False, halves=None, p_slots=0):
         """Fused form up to re-PFB stage 1: every block's stage-1 result of one
         input in the workspace ``ws`` (row pass, column pass with stage 1 of
         the interior blocks, then the edge blocks from the stream buffer; in
@@ -391,8 +396,34 @@ class RePFBPipeline:
         ``row_lo``; rows of the window past the tiles are zeroed.  With
         ``skip_zero_tail`` the blocks from :meth:`zero_from` on are left out
         (their workspace is not written) and only the zero rows that earlier
-        blocks read are cleared."""
+        blocks read are cleared.  With ``halves`` (a complex64 device tensor
+        of ``p_slots`` blocks) the column passes take the split form: the
+        second halves of the tile-boundary blocks go to ``halves`` (added by
+        :meth:`correlate_into`'s stage 2), ``stream_out`` is not written and
+        there is no edge-block launch."""
         c_lo, c_hi = tiles if tiles is not None else (0, nchunks)
+        if halves is not None:
+            s = _lib.stream()
+            tb = self.tile_batch
+            y = self._tile_buffer(min(tb, c_hi - c_lo), lane)
+            if self.ipfb_form == "spectral":
+                E = self._chan_buffer(min(tb, c_hi - c_lo), lane)
+            for t0 in range(c_lo, c_hi, tb):
+                nt = min(tb, c_hi - t0)
+                geo = (self.cut, t0 * self.acclen - row_lo, self.acclen, _lib.ptr(self.win), self.ntap, self.osamp,
+                       nblocks_total, _lib.ptr(ws), _lib.ptr(halves), t0 - c_lo, int(p_slots), s)
+                if self.ipfb_form == "spectral":
+                    self._launch("ipfb_chan_fft", "alb_ipfb_chan_fft_packed4", _lib.ptr(packed), _lib.ptr(rowmap),
+                                 _lib.ptr(self.chan), self.ncols, self.acclen, self.cut, t0, nt, _lib.ptr(E), s)
+                    self._launch("ipfb_rows_spectral", "alb_ipfb_rows_spectral", _lib.ptr(E), _lib.ptr(self.chan),
+                                 self.ncols, self.nslice, nt, _lib.ptr(y), s)
+                    self._launch("ipfb_columns_repfb", "alb_ipfb_columns_repfb_spectral_split", _lib.ptr(y), nt,
+                                 self.nslice, _lib.ptr(self.filt), self.filt.shape[1], *geo)
+                else:
+                    ldy = self._rows(packed, rowmap, t0, nt, y)
+                    self._launch("ipfb_columns_repfb", "alb_ipfb_columns_repfb_split", _lib.ptr(y), ldy,
+                                 self.nslice * 4096, nt, self.nslice, _lib.ptr(self.filt), self.filt.shape[1], *geo)
+            return
         rows_valid = max(0, c_hi * self.acclen - row_lo)
         if stream_out.shape[0] < max((nblocks_total + self.ntap - 1) * self.osamp, rows_valid):
             raise ValueError("stream buffer too short for the re-PFB blocks")
@@ -453,6 +484,14 @@ class RePFBPipeline:
         lib = _lib.load()
         return lib.alb_ipfb_columns_repfb(None, 4096, 0, 0, self.nslice, None, self.nslice // 2 + 1, self.cut, None, 0,
                                           self.acclen, None, self.ntap, self.osamp, 0, None, None) == 0
+
+    def split_shape(self):
+        """True when the fused column pass can take the tile-boundary blocks
+        in two halves (nslice 4096, osamp 64, ntap <= 4)."""
+        lib = _lib.load()
+        return self.fused_shape() and lib.alb_ipfb_columns_repfb_split(
+            None, 4096, 0, 0, self.nslice, None, self.nslice // 2 + 1, self.cut, 0, self.acclen, None, self.ntap,
+            self.osamp, 0, None, None, 0, 0, None) == 0
 
     def groups(self, njobs, group_jobs=None):
         """Job groups ``[s0, s1)`` of one pass: ``group_jobs`` jobs each
@@ -521,11 +560,22 @@ class RePFBPipeline:
         self.prepare()
         if self.fused_shape():
             self._workspace(G * K, M)
-            # concurrent stage 1 of up to ``lanes`` inputs: a stream buffer and
-            # a tile buffer per lane
             nl = min(self.lanes, M)
-            bufs["lane_stream_bufs"] = [bufs["stream"]] + [t.empty((rows, 4096), dtype=t.float32, device=dev)
-                                                           for _ in range(nl - 1)]
+            bufs["split"] = self.split_edges and self.split_shape()
+            if bufs["split"]:
+                # the second halves of the tile-boundary blocks: ntap slots per
+                # tile of the largest window, per input; no lane writes a stream
+                # buffer, so every lane shares the (unwritten) one
+                ntw = max([c_hi - c_lo for c_lo, c_hi in (self.window(s0, s1, nchunks)[2:] for s0, s1 in grp)] or [1])
+                slot = int(_lib.load().alb_repfb_split_slot_bytes(self.osamp)) // 8
+                bufs["p_slots"] = self.ntap * max(1, ntw)
+                bufs["halves"] = t.empty(M * bufs["p_slots"] * slot, dtype=t.complex64, device=dev)
+                bufs["lane_stream_bufs"] = [bufs["stream"]] * nl
+            else:
+                # concurrent stage 1 of up to ``lanes`` inputs: a stream buffer
+                # and a tile buffer per lane
+                bufs["lane_stream_bufs"] = [bufs["stream"]] + [t.empty((rows, 4096), dtype=t.float32, device=dev)
+                                                               for _ in range(nl - 1)]
             for lane in range(1, nl):
                 self._tile_buffer(min(self.tile_batch, nchunks), lane)
                 if self.ipfb_form == "spectral":
@@ -615,10 +665,26 @@ class RePFBPipeline:
                 lane_bufs = bufs.get("lane_stream_bufs", [stream])[:self.lanes]
                 nl = len(lane_bufs)
                 s2 = self.stage2_overlap and M > 1 and self._s2_stream is not None and len(self._s2_events) >= M
+                # split form: each input's tile-boundary halves, and stage 2 adds them
+                split = bool(bufs.get("split"))
+                p_slots = bufs.get("p_slots", 0)
+                pper = p_slots * (int(_lib.load().alb_repfb_split_slot_bytes(self.osamp)) // 8) if split else 0
+                halves = (lambda m: bufs["halves"][m * pper:]) if split else (lambda m: None)
+
+                def stage2(ws_ptr, half_ptr, nin, out_ptr):
+                    if split:
+                        self._launch("repfb_stage2", "alb_repfb_stage2_split", ws_ptr, per * 8, half_ptr, pper * 8, nin,
+                                     nbz, self.osamp, self.c_lo, self.c_hi, out_ptr, 1, F * M, M, K, K * F * M,
+                                     c_lo * a - row_lo, a, c_hi - c_lo, self.ntap, _lib.stream())
+                    else:
+                        self._launch("repfb_stage2", "alb_repfb_stage2_inputs", ws_ptr, per * 8, nin, nbz, self.osamp,
+                                     self.c_lo, self.c_hi, out_ptr, 1, F * M, M, K, K * F * M, _lib.stream())
+
                 if nl == 1 and not s2:
                     for m in range(M):
                         self.stream_stage1(packed_list[m], rowmaps[m], nchunks, stream, nbg, ws[m * per:],
-                                           tiles=(c_lo, c_hi), row_lo=row_lo, skip_zero_tail=True)
+                                           tiles=(c_lo, c_hi), row_lo=row_lo, skip_zero_tail=True, halves=halves(m),
+                                           p_slots=p_slots)
                 else:
                     t = _lib.torch()
                     main = t.cuda.current_stream()
@@ -633,7 +699,7 @@ class RePFBPipeline:
                         with t.cuda.stream(lanes[lane]):
                             self.stream_stage1(packed_list[m], rowmaps[m], nchunks, lane_bufs[lane], nbg,
                                                ws[m * per:], lane=lane, tiles=(c_lo, c_hi), row_lo=row_lo,
-                                               skip_zero_tail=True)
+                                               skip_zero_tail=True, halves=halves(m), p_slots=p_slots)
                         if s2:
                             # input m's stage 2 (its own workspace, its own channels of
                             # xin) once its stage 1 is done, beside the next inputs' stage 1
@@ -641,19 +707,16 @@ class RePFBPipeline:
                             ev.record(lanes[lane])
                             self._s2_stream.wait_event(ev)
                             with t.cuda.stream(self._s2_stream):
-                                self._launch("repfb_stage2", "alb_repfb_stage2_inputs",
-                                             _lib._vp(ws.data_ptr() + m * per * 8), per * 8, 1, nbz, self.osamp,
-                                             self.c_lo, self.c_hi, _lib._vp(xin.data_ptr() + m * 8), 1, F * M, M, K,
-                                             K * F * M, _lib.stream())
+                                stage2(_lib._vp(ws.data_ptr() + m * per * 8),
+                                       _lib._vp(bufs["halves"].data_ptr() + m * pper * 8) if split else None, 1,
+                                       _lib._vp(xin.data_ptr() + m * 8))
                     if nl > 1:
                         for st in lanes:
                             main.wait_stream(st)
                     if s2:
                         main.wait_stream(self._s2_stream)
                 if not s2:
-                    self._launch("repfb_stage2", "alb_repfb_stage2_inputs", _lib.ptr(ws), per * 8, M, nbz,
-                                 self.osamp, self.c_lo, self.c_hi, _lib.ptr(xin), 1, F * M, M, K, K * F * M,
-                                 _lib.stream())
+                    stage2(_lib.ptr(ws), _lib.ptr(bufs["halves"]) if split else None, M, _lib.ptr(xin))
             else:
                 # the window starts at tile c_lo; the group's blocks at row_lo
                 for m in range(M):

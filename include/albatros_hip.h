@@ -250,6 +250,34 @@ int alb_ipfb_rows_spectral(const void* E, const int* chan, int64_t ncols, int64_
 int alb_ipfb_columns_repfb_spectral(const void* D, int64_t ntiles, int64_t nslice, const void* filt, int64_t ldf,
                                     int cut, float* ts, int64_t row0, int64_t rows_per_tile, const float* win,
                                     int ntap, int64_t osamp, int64_t nblocks, void* A, void* stream);
+/* Split form of alb_ipfb_columns_repfb and its spectral step 3 (no counterpart call in the reference;
+ * it computes the same cupy_pfb input, helper_gpu_stream.py:183-222, with no stream buffer): each
+ * tile computes re-PFB stage 1 of every block that touches its kept rows, over those rows only.  The
+ * tile that holds block b's first row writes A[b]; the next tile writes its half to P slot
+ * ntap*(tile0 + t - 1) + (b - first block straddling tile t's first row), t the tile within this
+ * launch and tile0 this launch's first tile's index in the stream window (whose tile 0 starts at
+ * or before window row 0).  Consecutive tiles must be rows_per_tile = nslice - 2*cut rows apart.
+ * P holds p_slots slots of alb_repfb_split_slot_bytes(osamp) bytes; launches that would write past
+ * them fail with ALB_EINVAL.  nslice 4096, osamp 64, ntap <= 4, strip width 4 (ENOTSUP otherwise;
+ * with ntiles = 0 only the shape is checked). */
+int alb_ipfb_columns_repfb_split(const float* y, int64_t ldy, int64_t tile_stride, int64_t ntiles, int64_t nslice,
+                                 const void* filt, int64_t ldf, int cut, int64_t row0, int64_t rows_per_tile,
+                                 const float* win, int ntap, int64_t osamp, int64_t nblocks, void* A, void* P,
+                                 int64_t tile0, int64_t p_slots, void* stream);
+int alb_ipfb_columns_repfb_spectral_split(const void* D, int64_t ntiles, int64_t nslice, const void* filt, int64_t ldf,
+                                          int cut, int64_t row0, int64_t rows_per_tile, const float* win, int ntap,
+                                          int64_t osamp, int64_t nblocks, void* A, void* P, int64_t tile0,
+                                          int64_t p_slots, void* stream);
+/* alb_repfb_stage2_inputs (osamp 64) after the split column pass: each block that straddles two of
+ * the window's ntiles tiles gets its second half from P (input m at P + m*p_stride bytes) added to
+ * A before the transform.  The window's tile c starts at window row g_first + c*rows_per_tile
+ * (g_first <= 0); every block's first row must lie in its tiles. */
+int alb_repfb_stage2_split(const void* A, int64_t a_stride, const void* P, int64_t p_stride, int64_t ninput,
+                           int64_t nblock, int64_t osamp, int64_t c_lo, int64_t c_hi, void* out, int64_t sm, int64_t sb,
+                           int64_t sc, int64_t bpj, int64_t sj, int64_t g_first, int64_t rows_per_tile, int64_t ntiles,
+                           int ntap, void* stream);
+/* Bytes of one P slot (one block of stage-1 workspace). */
+int64_t alb_repfb_split_slot_bytes(int64_t osamp);
 /* alb_repfb with its stage-1 workspace in a fixed device array of the library instead of A: allocates
  * nothing (no counterpart in the reference; for callers that cannot call hipMalloc -- code under hipGraph
  * capture, tight serving loops, harnesses that forbid allocation).  Requires osamp <= 64 and

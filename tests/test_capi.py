@@ -98,3 +98,41 @@ def test_fused_column_pass_shape_probe():
     with pytest.raises(ValueError):
         _lib.call("alb_ipfb_columns_repfb", None, 4096, 0, 1, 4096, None, 2049, 4, None, 0, 4088, None, 4, 64, 10,
                   None, None)
+
+
+def test_split_column_pass_shape_probe_and_refusals():
+    # the split form (tile-boundary blocks in two halves): shapes it takes,
+    # and undersized boundary arrays / bad geometry refused before any launch
+    from albatros_axion_detection_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    lib = _lib.load()
+    probe = lambda nslice, cut, ntap, osamp: lib.alb_ipfb_columns_repfb_split(
+        None, 4096, 0, 0, nslice, None, nslice // 2 + 1, cut, 0, nslice - 2 * cut, None, ntap, osamp, 0, None, None,
+        0, 0, None)
+    assert probe(4096, 4, 4, 64) == 0
+    assert probe(4096, 10, 3, 64) == 0
+    assert probe(4096, 4, 4, 32) == 3      # ENOTSUP: osamp 64 only
+    assert probe(4096, 0, 4, 64) == 3      # needs cut >= 1 (the zeroed cut rows)
+    assert probe(4096, 4, 5, 64) == 3      # ntap <= 4
+    assert probe(2048, 4, 4, 64) == 3
+    assert lib.alb_repfb_split_slot_bytes(64) == lib.alb_repfb_workspace_bytes(1, 64)
+    fake = ctypes.c_void_p(1 << 20)        # never dereferenced: the checks fail first
+    # tiles 3..4 of a window need ntap * 4 = 16 slots, 15 given
+    with pytest.raises(ValueError):
+        _lib.call("alb_ipfb_columns_repfb_split", fake, 8, 4096 * 4096, 2, 4096, fake, 2049, 4, 3 * 4088, 4088, fake,
+                  4, 64, 1000, fake, fake, 3, 15, None)
+    # consecutive tiles must be nslice - 2 cut rows apart
+    with pytest.raises(ValueError):
+        _lib.call("alb_ipfb_columns_repfb_split", fake, 8, 4096 * 4096, 2, 4096, fake, 2049, 4, 0, 4000, fake,
+                  4, 64, 1000, fake, fake, 0, 16, None)
+    # stage 2: osamp 64 only, g_first <= 0, blocks inside the tiles
+    stage2 = lambda osamp, nblock, g_first, ntiles: _lib.call(
+        "alb_repfb_stage2_split", fake, 0, fake, 0, 1, nblock, osamp, 64000, 83200, fake, 1, 4, 4, 1024, 4096,
+        g_first, 4088, ntiles, 4, None)
+    with pytest.raises(ValueError):
+        stage2(32, 10, 0, 2)
+    with pytest.raises(ValueError):
+        stage2(64, 10, 5, 2)
+    with pytest.raises(ValueError):
+        stage2(64, 130, 0, 2)                # block 129 starts past two tiles of 4088 rows

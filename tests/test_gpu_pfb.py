@@ -1031,3 +1031,46 @@ def test_zero_tail_blocks_match_computed_tail():
     assert d < 2e-6 * scale
     first_zero_job = -(-nbz // nblock)
     assert np.all(got[..., first_zero_job:] == 0) and np.all(ref[..., first_zero_job:] == 0)
+
+
+@pytest.mark.gpu
+def test_split_boundary_blocks_match_stream_rows():
+    # The batched pass at osamp 64 takes the re-PFB blocks that straddle two
+    # iPFB tiles in two halves (one from each tile's column pass, added in
+    # stage 2) instead of stream rows and an edge-block launch.  Both forms
+    # compute the same blocks to float32 rounding: direct and spectral iPFB
+    # order, one and two lanes, a zero-filled stream tail, and job groups
+    # (bit for bit against the split form over the whole stream).
+    import torch
+    from albatros_axion_detection_amd import _lib, repfb, synth
+    acclen, cut, osamp, nblock, nchan = 4088, 4, 64, 16, 24
+    packed, specnums, lengths = synth.batched_stream(nant=2, nchan=nchan, nchunks=7, lmin=2000, lmax=4000, loss=0.01,
+                                                     run=5, seed=33)
+    nfixed = int(lengths.sum()) // acclen
+    chan = np.arange(600, 600 + nchan)
+    rm = repfb.rowmap_stream(specnums, 0, nfixed, acclen)
+    plist = [_lib.to_device(packed[m]) for m in range(4)]
+    for form in ("direct", "spectral"):
+        pipe = repfb.RePFBPipeline(4, chan, acclen, cut, osamp, nblock, 600 * osamp, (600 + nchan) * osamp,
+                                   tile_batch=3, ipfb_form=form)
+        assert pipe.split_shape()
+        pipe.split_edges = False
+        old = pipe.buffers(nfixed)
+        pipe.split_edges = True
+        new = pipe.buffers(nfixed)
+        assert new["split"] and not old.get("split")
+        for nvalid in (nfixed, nfixed - 2):
+            ref = pipe.correlate_into(plist, [rm] * 4, nfixed, old, nvalid=nvalid).clone()
+            scale = torch.sqrt(torch.mean(torch.abs(ref) ** 2))
+            for lanes in (1, 2):
+                pipe.lanes = lanes
+                got = pipe.correlate_into(plist, [rm] * 4, nfixed, new, nvalid=nvalid).clone()
+                d = float(torch.max(torch.abs(got - ref)) / scale)
+                print("[gpu-case] split vs stream rows: %s nvalid %d lanes %d max |d| / rms %.3g"
+                      % (form, nvalid, lanes, d), flush=True)
+                assert d < 2e-6, (form, nvalid, lanes)
+            grp = pipe.buffers(nfixed, group_jobs=2)
+            assert grp["split"] and len(grp["groups"]) > 1
+            g = pipe.correlate_into(plist, [rm] * 4, nfixed, grp, nvalid=nvalid)
+            print("[gpu-case] split job groups: %s nvalid %d" % (form, nvalid), flush=True)
+            assert torch.equal(g, got), (form, nvalid)
