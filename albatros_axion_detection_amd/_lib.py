@@ -78,6 +78,8 @@ SIGNATURES = {
     "alb_repfb_stage2_split": (_i32, [_vp, _i64, _vp, _i64, _i64, _i64, _i64, _i64, _i64, _vp, _i64, _i64, _i64, _i64,
                                       _i64, _i64, _i64, _i64, _i32, _vp]),
     "alb_repfb_split_slot_bytes": (_i64, [_i64]),
+    "alb_repfb_split_blocks": (_i32, [_i64, _i64, _i64, _i32, _i64, _vp, _vp, _vp]),
+    "alb_repfb_split_slot": (_i64, [_i64, _i64, _i64, _i64, _i32, _i64]),
     "alb_ipfb_columns_repfb_spectral": (_i32, [_vp, _i64, _i64, _vp, _i64, _i32, _vp, _i64, _i64, _vp, _i32, _i64,
                                                _i64, _vp, _vp]),
     "alb_repfb_interior": (_i32, [_i64, _i64, _i64, _i32, _i64, _vp, _vp]),

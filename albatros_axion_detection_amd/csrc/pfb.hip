@@ -720,6 +720,18 @@ __host__ __device__ inline void repfb_interior(int64_t g0, int64_t nkeep, int64_
   *bi1 = hi > lo ? hi : lo;
 }
 
+// Split form: the blocks [*lo, *hi) (of nblocks) that touch a tile's kept rows
+// [g0, g0 + nkeep), and *e0, the first block that straddles its first row g0.
+__host__ __device__ inline void split_blocks(int64_t g0, int64_t nkeep, int64_t R1, int64_t ntap, int64_t nblocks,
+                                             int64_t* lo, int64_t* hi, int64_t* e0) {
+  *e0 = floor_div(g0, R1) - ntap + 1;
+  const int64_t a = *e0 > 0 ? *e0 : 0;
+  int64_t b = floor_div(g0 + nkeep - 1, R1) + 1;
+  if (b > nblocks) b = nblocks;
+  *lo = a;
+  *hi = b > a ? b : a;
+}
+
 // Stage 1 for both columns of a pair at once: z[r] = y0[r] + i y1[r] (y0, y1
 // the polyphase sums of columns k0+2p and k0+2p+1), R1-point DFT split over 4
 // lanes as in the split kernel below (lane t: rows 4s + t, S-point DFT, radix-4
@@ -751,15 +763,9 @@ __device__ __forceinline__ void cols_repfb_tile(const float* __restrict__ y, int
 
   const int nkeep = NSL - 2 * cut;
   const int64_t g0 = row0 + tile * rows_per_tile;
-  int64_t bi0, bi1;
-  // first block straddling this tile's first row (split form)
-  const int64_t e0 = floor_div(g0, R1) - ntap + 1;
+  int64_t bi0, bi1, e0 = 0;   // e0: first block straddling this tile's first row (split form)
   if constexpr (SPLIT) {
-    // every block that touches the kept rows [g0, g0 + nkeep)
-    bi0 = e0 > 0 ? e0 : 0;
-    bi1 = floor_div(g0 + nkeep - 1, R1) + 1;
-    if (bi1 > nblocks) bi1 = nblocks;
-    if (bi1 < bi0) bi1 = bi0;
+    split_blocks(g0, nkeep, R1, ntap, nblocks, &bi0, &bi1, &e0);   // every block touching the kept rows
   } else {
     repfb_interior(g0, nkeep, R1, ntap, nblocks, &bi0, &bi1);
   }
@@ -2455,6 +2461,25 @@ int alb_repfb_stage2_split(const void* A, int64_t a_stride, const void* P, int64
 }
 
 int64_t alb_repfb_split_slot_bytes(int64_t osamp) { return alb_repfb_workspace_bytes(1, osamp); }
+
+// The split form's index functions, as the kernels use them (host-only: for
+// callers that plan their own launches, and for tests).
+int alb_repfb_split_blocks(int64_t g0, int64_t nkeep, int64_t osamp, int ntap, int64_t nblocks, int64_t* lo,
+                           int64_t* hi, int64_t* e0) {
+  ALB_REQUIRE(lo && hi && e0, "alb_repfb_split_blocks: null pointer");
+  ALB_REQUIRE(osamp >= 1 && ntap >= 1 && nkeep >= 1 && nblocks >= 0, "alb_repfb_split_blocks: bad sizes");
+  split_blocks(g0, nkeep, osamp, ntap, nblocks, lo, hi, e0);
+  return ALB_OK;
+}
+
+int64_t alb_repfb_split_slot(int64_t b, int64_t g_first, int64_t rows_per_tile, int64_t ntiles, int ntap,
+                             int64_t osamp) {
+  if (b < 0 || g_first > 0 || g_first <= -rows_per_tile || rows_per_tile < osamp * ntap || ntap < 1 ||
+      osamp * (b + ntap) - g_first >= (int64_t(1) << 31))
+    return -2;
+  const SplitRead sp{nullptr, 0, g_first, rows_per_tile, ntiles, ntap};
+  return split_slot(sp, b, osamp);
+}
 
 // Build the transform tables of alb_repfb / alb_repfb_stage2 at this osamp
 // (1..64) on the current device now (stage 1 takes none; stage 2 the

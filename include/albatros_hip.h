@@ -278,6 +278,13 @@ int alb_repfb_stage2_split(const void* A, int64_t a_stride, const void* P, int64
                            int ntap, void* stream);
 /* Bytes of one P slot (one block of stage-1 workspace). */
 int64_t alb_repfb_split_slot_bytes(int64_t osamp);
+/* The split form's index functions as the kernels use them (host-only, no GPU work): the blocks
+ * [*lo, *hi) of nblocks that touch a tile's kept rows [g0, g0 + nkeep) and *e0, the first block
+ * straddling row g0; and the P slot stage 2 adds to block b (-1: none; -2: arguments out of range). */
+int alb_repfb_split_blocks(int64_t g0, int64_t nkeep, int64_t osamp, int ntap, int64_t nblocks, int64_t* lo,
+                           int64_t* hi, int64_t* e0);
+int64_t alb_repfb_split_slot(int64_t b, int64_t g_first, int64_t rows_per_tile, int64_t ntiles, int ntap,
+                             int64_t osamp);
 /* alb_repfb with its stage-1 workspace in a fixed device array of the library instead of A: allocates
  * nothing (no counterpart in the reference; for callers that cannot call hipMalloc -- code under hipGraph
  * capture, tight serving loops, harnesses that forbid allocation).  Requires osamp <= 64 and

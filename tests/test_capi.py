@@ -136,3 +136,66 @@ def test_split_column_pass_shape_probe_and_refusals():
         stage2(64, 10, 5, 2)
     with pytest.raises(ValueError):
         stage2(64, 130, 0, 2)                # block 129 starts past two tiles of 4088 rows
+
+
+def test_split_halves_cover_every_block_once():
+    # The split column pass's block ranges and boundary slots, and stage 2's
+    # slot lookup (the library's own index functions, host-side), over random
+    # stream windows as the batched pass plans them: every block with data
+    # gets exactly its rows below the end of the data, as A (from the tile
+    # holding its first row) plus at most one boundary half (from the next
+    # tile); no destination is written twice and every written half is read.
+    from albatros_axion_detection_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    import numpy as np
+    rng = np.random.default_rng(11)
+    lo, hi, e0 = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    R1, nwin = 64, 0
+    for _ in range(120):
+        ntap = int(rng.integers(1, 5))
+        cut = int(rng.integers(1, 40))
+        acclen = 4096 - 2 * cut
+        nblock = int(rng.choice([3, 5, 16, 64, 1024]))
+        nchunks = int(rng.integers(2, 30))
+        njobs = max(1, (nchunks * acclen // R1 - (ntap - 1)) // nblock)
+        s0 = int(rng.integers(0, njobs))
+        s1 = int(rng.integers(s0 + 1, njobs + 1))
+        nvalid = int(rng.integers(1, nchunks + 1))
+        row_lo, row_hi = s0 * nblock * R1, s1 * nblock * R1 + (ntap - 1) * R1
+        c_lo = min(nvalid, row_lo // acclen)
+        c_hi = max(c_lo, min(nvalid, -(-row_hi // acclen)))
+        ntiles, nbg = c_hi - c_lo, (s1 - s0) * nblock
+        if ntiles == 0:
+            continue
+        g_first = c_lo * acclen - row_lo
+        rows_valid = c_hi * acclen - row_lo
+        nbz = min(nbg, -(-rows_valid // R1))
+        written, owner = {}, {}
+        for tt in range(ntiles):
+            g0 = g_first + tt * acclen
+            _lib.call("alb_repfb_split_blocks", g0, acclen, R1, ntap, nbg, ctypes.byref(lo), ctypes.byref(hi),
+                      ctypes.byref(e0))
+            for b in range(lo.value, hi.value):
+                rows = {r for r in range(R1 * b, R1 * (b + ntap)) if g0 <= r < g0 + acclen}
+                assert rows, (b, g0)
+                dst = ("A", b) if R1 * b >= g0 else ("P", ntap * (tt - 1) + b - e0.value)
+                if dst[0] == "P":
+                    assert tt >= 1 and 0 <= dst[1] < ntap * ntiles
+                assert dst not in written
+                written[dst] = rows
+                owner[dst] = b
+        read = set()
+        for b in range(nbz):
+            rows = set(written.get(("A", b), set()))
+            e = int(_lib.load().alb_repfb_split_slot(b, g_first, acclen, ntiles, ntap, R1))
+            assert e >= -1
+            if e >= 0:
+                assert ("P", e) in written and not rows & written[("P", e)]
+                rows |= written[("P", e)]
+                read.add(e)
+            assert rows == {r for r in range(R1 * b, R1 * (b + ntap)) if r < rows_valid}, (b, ntap, cut)
+        # every half written for a block with data is added by stage 2
+        assert {d[1] for d in written if d[0] == "P" and owner[d] < nbz} == read
+        nwin += 1
+    assert nwin > 50
