@@ -2379,10 +2379,17 @@ int alb_ipfb_columns_repfb_spectral(const void* D, int64_t ntiles, int64_t nslic
 // tile's index in the stream window (tiles are rows_per_tile = nslice - 2 cut
 // rows apart).  P holds p_slots slots of alb_repfb_workspace_bytes(1, osamp)
 // bytes.  No stream rows are written.
-static int check_split_slots(int64_t ntiles, int64_t tile0, int ntap, int64_t p_slots) {
+// The window's tile 0 must start at or before window row 0 (row0 - tile0
+// rows_per_tile <= 0): its column pass then has no block starting before its
+// first kept row, which would have no P slot.
+static int check_split_slots(int64_t ntiles, int64_t tile0, int ntap, int64_t p_slots, int64_t row0,
+                             int64_t rows_per_tile) {
   ALB_REQUIRE(ntiles == 0 || (tile0 >= 0 && ntap >= 1 && int64_t(ntap) * (tile0 + ntiles - 1) <= p_slots),
               "split column pass: P holds %lld slots, tiles %lld..%lld need %lld", (long long)p_slots,
               (long long)tile0, (long long)(tile0 + ntiles - 1), (long long)(int64_t(ntap) * (tile0 + ntiles - 1)));
+  ALB_REQUIRE(ntiles == 0 || row0 - tile0 * rows_per_tile <= 0,
+              "split column pass: the window's tile 0 starts at row %lld (row0 - tile0 rows_per_tile), after row 0",
+              (long long)(row0 - tile0 * rows_per_tile));
   return ALB_OK;
 }
 
@@ -2400,7 +2407,7 @@ int alb_ipfb_columns_repfb_split(const float* y, int64_t ldy, int64_t tile_strid
               (long long)nslice, (long long)osamp);
     return ALB_ENOTSUP;
   }
-  int rc = check_split_slots(ntiles, tile0, ntap, p_slots);
+  int rc = check_split_slots(ntiles, tile0, ntap, p_slots, row0, rows_per_tile);
   if (rc) return rc;
   return launch_cols_repfb<4096, 4, 512, 64, false, true>(y, ldy, tile_stride, ntiles, (const float2*)filt, ldf, cut,
                                                           nullptr, row0, rows_per_tile, win, ntap, nblocks, (float2*)A,
@@ -2421,7 +2428,7 @@ int alb_ipfb_columns_repfb_spectral_split(const void* D, int64_t ntiles, int64_t
               (long long)nslice, (long long)osamp);
     return ALB_ENOTSUP;
   }
-  int rc = check_split_slots(ntiles, tile0, ntap, p_slots);
+  int rc = check_split_slots(ntiles, tile0, ntap, p_slots, row0, rows_per_tile);
   if (rc) return rc;
   return launch_cols_repfb<4096, 4, 512, 64, true, true>((const float*)D, 0, 0, ntiles, (const float2*)filt, ldf, cut,
                                                          nullptr, row0, rows_per_tile, win, ntap, nblocks, (float2*)A,

@@ -255,8 +255,8 @@ int alb_ipfb_columns_repfb_spectral(const void* D, int64_t ntiles, int64_t nslic
  * tile computes re-PFB stage 1 of every block that touches its kept rows, over those rows only.  The
  * tile that holds block b's first row writes A[b]; the next tile writes its half to P slot
  * ntap*(tile0 + t - 1) + (b - first block straddling tile t's first row), t the tile within this
- * launch and tile0 this launch's first tile's index in the stream window (whose tile 0 starts at
- * or before window row 0).  Consecutive tiles must be rows_per_tile = nslice - 2*cut rows apart.
+ * launch and tile0 this launch's first tile's index in the stream window, whose tile 0 (first kept
+ * row row0 - tile0*rows_per_tile) must start at or before window row 0 (ALB_EINVAL otherwise).  Consecutive tiles must be rows_per_tile = nslice - 2*cut rows apart.
  * P holds p_slots slots of alb_repfb_split_slot_bytes(osamp) bytes; launches that would write past
  * them fail with ALB_EINVAL.  nslice 4096, osamp 64, ntap <= 4, strip width 4 (ENOTSUP otherwise;
  * with ntiles = 0 only the shape is checked). */

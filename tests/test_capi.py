@@ -122,6 +122,14 @@ def test_split_column_pass_shape_probe_and_refusals():
     with pytest.raises(ValueError):
         _lib.call("alb_ipfb_columns_repfb_split", fake, 8, 4096 * 4096, 2, 4096, fake, 2049, 4, 3 * 4088, 4088, fake,
                   4, 64, 1000, fake, fake, 3, 15, None)
+    # the window's tile 0 must start at or before row 0: tile0 0 at row 100
+    # would put the blocks before row 100 in a slot before P
+    with pytest.raises(ValueError, match="after row 0"):
+        _lib.call("alb_ipfb_columns_repfb_split", fake, 8, 4096 * 4096, 2, 4096, fake, 2049, 4, 100, 4088, fake,
+                  4, 64, 1000, fake, fake, 0, 16, None)
+    with pytest.raises(ValueError, match="after row 0"):
+        _lib.call("alb_ipfb_columns_repfb_spectral_split", fake, 2, 4096, fake, 2049, 4, 2 * 4088 + 1, 4088, fake,
+                  4, 64, 1000, fake, fake, 2, 16, None)
     # consecutive tiles must be nslice - 2 cut rows apart
     with pytest.raises(ValueError):
         _lib.call("alb_ipfb_columns_repfb_split", fake, 8, 4096 * 4096, 2, 4096, fake, 2049, 4, 0, 4000, fake,
