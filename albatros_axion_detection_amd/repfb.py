@@ -657,10 +657,10 @@ class RePFBPipeline:
                 # stage 1 of every input into its own workspace, then one
                 # stage-2 launch writes all inputs' channels into the
                 # correlator layout.  Inputs go round-robin to ``lanes`` HIP
-                # streams (default priority), each with its own stream window
-                # and tile buffer, so one input's small edge-block launches
-                # overlap the next input's row and column passes; the current
-                # stream waits for all of them.
+                # streams (default priority), each with its own tile buffer
+                # (and, without the split form, its own stream window), so one
+                # input's launches overlap the next input's row and column
+                # passes; the current stream waits for all of them.
                 ws, per = self._workspace(nbg, M)
                 lane_bufs = bufs.get("lane_stream_bufs", [stream])[:self.lanes]
                 nl = len(lane_bufs)
