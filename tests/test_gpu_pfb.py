@@ -1034,16 +1034,19 @@ def test_zero_tail_blocks_match_computed_tail():
 
 
 @pytest.mark.gpu
-def test_split_boundary_blocks_match_stream_rows():
+@pytest.mark.parametrize("acclen,cut", [(4088, 4), (4076, 10)])
+def test_split_boundary_blocks_match_stream_rows(acclen, cut):
     # The batched pass at osamp 64 takes the re-PFB blocks that straddle two
     # iPFB tiles in two halves (one from each tile's column pass, added in
     # stage 2) instead of stream rows and an edge-block launch.  Both forms
     # compute the same blocks to float32 rounding: direct and spectral iPFB
     # order, one and two lanes, a zero-filled stream tail, and job groups
-    # (bit for bit against the split form over the whole stream).
+    # (bit for bit against the split form over the whole stream); cut 4 and
+    # the reference's production cut of 10 (scripts/xcorr/config_axion_gpu.json:15), both at
+    # nslice 4096.
     import torch
     from albatros_axion_detection_amd import _lib, repfb, synth
-    acclen, cut, osamp, nblock, nchan = 4088, 4, 64, 16, 24
+    osamp, nblock, nchan = 64, 16, 24
     packed, specnums, lengths = synth.batched_stream(nant=2, nchan=nchan, nchunks=7, lmin=2000, lmax=4000, loss=0.01,
                                                      run=5, seed=33)
     nfixed = int(lengths.sum()) // acclen
@@ -1066,8 +1069,8 @@ def test_split_boundary_blocks_match_stream_rows():
                 pipe.lanes = lanes
                 got = pipe.correlate_into(plist, [rm] * 4, nfixed, new, nvalid=nvalid).clone()
                 d = float(torch.max(torch.abs(got - ref)) / scale)
-                print("[gpu-case] split vs stream rows: %s nvalid %d lanes %d max |d| / rms %.3g"
-                      % (form, nvalid, lanes, d), flush=True)
+                print("[gpu-case] split vs stream rows: cut %d %s nvalid %d lanes %d max |d| / rms %.3g"
+                      % (cut, form, nvalid, lanes, d), flush=True)
                 assert d < 2e-6, (form, nvalid, lanes)
             grp = pipe.buffers(nfixed, group_jobs=2)
             assert grp["split"] and len(grp["groups"]) > 1
